@@ -1,0 +1,143 @@
+/*
+ * ticit_hip: C ABI of the MI355X sampling backend for ticit's compiled
+ * Clifford+T sampler.
+ *
+ * The entry points replace, for a `hip` backend, the surfaces a ticit host
+ * binds today:
+ *   - Circuit::compile(SamplerOptions) -> Sampler
+ *       (src/circuit/mod.rs:75-77, src/sampler/prepared.rs:492-626)
+ *   - Sampler::sample_with_seed(shots, seed, bit_packed) -> SampleResult
+ *       (src/sampler/prepared.rs:663-670, 692-859)
+ *   - Sampler::info() -> SamplerInfo (src/sampler/prepared.rs:628-632)
+ *   - the free GPU functions ticit::gpu::sample_circuit* (src/gpu/mod.rs:235-590)
+ * All arguments are plain pointers and sizes; no C++ or torch types cross
+ * the boundary and no exception escapes it. Error kinds mirror TicitError
+ * (src/errors.rs:9-57). A program handle is single-threaded, like the
+ * `&mut self` Sampler (src/sampler/prepared.rs:444-448); distinct handles may
+ * be used from different threads.
+ */
+#ifndef TICIT_HIP_H
+#define TICIT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum ticit_hip_status {
+  TICIT_HIP_OK = 0,
+  TICIT_HIP_INVALID_INPUT = 1, /* TicitError::InvalidInput */
+  TICIT_HIP_PARSE = 2,         /* TicitError::Parse */
+  TICIT_HIP_UNSUPPORTED = 3,   /* TicitError::Unsupported */
+  TICIT_HIP_INTERNAL = 4,      /* TicitError::Internal */
+  TICIT_HIP_OUT_OF_DEVICE_MEMORY = 5,
+  TICIT_HIP_DEVICE_ERROR = 6
+} ticit_hip_status;
+
+typedef struct ticit_hip_program ticit_hip_program;
+
+/* Mirrors SamplerOptions (src/sampler/prepared.rs:104-142) plus the backend
+ * knobs of the hip path. Zero-initialize for defaults. */
+typedef struct ticit_hip_options {
+  uint32_t observable;   /* observable counted as a logical error */
+  uint32_t precision;    /* 0: FP64 amplitudes; 1: FP32 amplitudes, FP64 reductions */
+  uint32_t planner_mode; /* 0: heuristic tier choice; 1: learned planner */
+  uint32_t tier;         /* 0: automatic; 1: lane tier; 2: group tier; 3: wide tier */
+  uint32_t reserved[4];
+} ticit_hip_options;
+
+/* Mirrors SampleResult (src/sampler/prepared.rs:197-219). Buffers are owned
+ * by the library until ticit_hip_result_free. Rows are row-major; packed
+ * rows hold ceil(columns / 8) bytes in little bit order
+ * (src/sampler/prepared.rs:909-941). */
+typedef struct ticit_hip_result {
+  uint64_t shots, discarded, accepted, logical_errors;
+  uint64_t record_rows;
+  int32_t bit_packed;
+  int32_t reserved0;
+  uint64_t nmeasurements, ndetectors, nobservables; /* column counts */
+  uint8_t* measurements;
+  uint8_t* detectors;
+  uint8_t* observables;
+  uint64_t* observable_ones;
+  double sample_s; /* wall time of the sampling call */
+  double kernel_s; /* device time of the sampling kernels */
+} ticit_hip_result;
+
+/* Mirrors SamplerInfo (src/sampler/prepared.rs:161-189). */
+typedef struct ticit_hip_program_info {
+  uint64_t qubits, measurement_records, detectors, observables;
+  uint64_t max_active_qubits, symbols, instructions;
+  uint64_t measure_instructions, coin_instructions;
+  uint32_t tier;      /* tier the sampler runs on */
+  uint32_t precision;
+} ticit_hip_program_info;
+
+/* Thread-local message of the last failing call on this thread. */
+const char* ticit_hip_last_error(void);
+
+/* Front end: parse and plan .ticit text into a plan blob
+ * (include/ticit_plan.h). Replaces plan_circuit (src/circuit/mod.rs:274-327)
+ * for hosts that do not serialize their own FactoredInstructionProgram. The
+ * blob is freed with ticit_hip_blob_free. */
+ticit_hip_status ticit_hip_plan_text(const char* text, size_t len, uint64_t** blob, size_t* nwords);
+void ticit_hip_blob_free(uint64_t* blob);
+
+/* Sampler::new / from_input (src/sampler/prepared.rs:492-626): validates the
+ * blob, uploads its tables to the current HIP device and picks a tier. */
+ticit_hip_status ticit_hip_program_create(const uint64_t* blob, size_t nwords, const ticit_hip_options* options,
+                                          ticit_hip_program** out);
+
+/* Circuit::compile in one call: ticit_hip_plan_text + ticit_hip_program_create. */
+ticit_hip_status ticit_hip_compile_text(const char* text, size_t len, const ticit_hip_options* options,
+                                        ticit_hip_program** out);
+
+void ticit_hip_program_destroy(ticit_hip_program* program);
+
+ticit_hip_status ticit_hip_program_info_get(const ticit_hip_program* program, ticit_hip_program_info* out);
+
+/* Sampler::sample_with_seed (src/sampler/prepared.rs:663-670) /
+ * sample_counts_with_seed (keep_records = 0). Shot s draws its noise from
+ * block_seed(0x7eed0000, seed, s) and its branches from
+ * block_seed(0x5eed1234, seed, s): the reference sampler's streams at
+ * sample_chunk_shots = 1, batch_size = 1. */
+ticit_hip_status ticit_hip_sample(ticit_hip_program* program, uint64_t shots, uint64_t seed, int bit_packed,
+                                  int keep_records, ticit_hip_result* out);
+
+/* Same sampling into caller-owned device buffers, enqueued on `stream`
+ * (a hipStream_t, NULL = the default stream) and not synchronized:
+ * meas/det/obs are [shots][row] bytes, observable_ones is [nobservables]
+ * uint64 accumulated (not cleared) by the call. first_shot offsets the
+ * per-shot streams so a large job can be split into launches. */
+ticit_hip_status ticit_hip_sample_device(ticit_hip_program* program, uint64_t shots, uint64_t seed,
+                                         uint64_t first_shot, int bit_packed, int keep_records, uint8_t* meas,
+                                         uint8_t* det, uint8_t* obs, uint64_t* observable_ones, void* stream);
+
+/* Injected randomness (host buffers): exo = symbol-major bit-planes
+ * [nsymbols][ceil(shots/64)] (bit s of word w = shot 64w+s; only exogenous
+ * symbols are read), uniforms = [measure_instructions][shots] branch draws
+ * (branch = u < P(branch=1)), coins = [coin_instructions][ceil(shots/64)].
+ * probs, if non-NULL, receives P(branch = 1) per [measure][shot]. */
+ticit_hip_status ticit_hip_sample_injected(ticit_hip_program* program, uint64_t shots, const uint64_t* exo,
+                                           const double* uniforms, const uint64_t* coins, int bit_packed,
+                                           ticit_hip_result* out, double* probs);
+
+/* Batched sampling of many programs in one submission (configuration C2). */
+ticit_hip_status ticit_hip_sample_batch(ticit_hip_program* const* programs, const uint64_t* shots,
+                                        const uint64_t* seeds, size_t count, int bit_packed, int keep_records,
+                                        ticit_hip_result* results);
+
+void ticit_hip_result_free(ticit_hip_result* result);
+
+/* Device properties the tier planner sizes against; returns 0 when no
+ * device is visible. props = {lds_per_block, compute_units, clock_khz,
+ * warp_size, memory_clock_khz, bus_width_bits}. */
+int ticit_hip_device_query(int64_t* props);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TICIT_HIP_H */

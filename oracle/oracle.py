@@ -1,0 +1,198 @@
+"""ctypes binding of the CPU oracle (test infrastructure only).
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg use
+this module, always as the checker. It loads ``oracle/_build/liboracle.so``
+(parity build) or ``liboracle_fast.so`` (timing build) produced by
+``oracle/Makefile``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD = os.path.join(HERE, "_build")
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def _load(name: str) -> ctypes.CDLL:
+    path = os.path.join(BUILD, name)
+    if not os.path.exists(path):
+        build()
+    lib = ctypes.CDLL(path)
+    lib.oracle_last_error.restype = ctypes.c_char_p
+    lib.oracle_info.argtypes = [_u64p, ctypes.c_size_t, _i64p]
+    lib.oracle_sample.argtypes = [_u64p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_uint64,
+                                  ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, _u8p, _u8p, _u8p, _u64p, _u64p, _f64p]
+    lib.oracle_sample_injected.argtypes = [_u64p, ctypes.c_size_t, ctypes.c_uint64, _u64p, _f64p,
+                                           _u64p, ctypes.c_int, ctypes.c_int, _u8p, _u8p, _u8p,
+                                           _u64p, _u64p, _f64p, _f64p, _f64p,
+                                           ctypes.POINTER(ctypes.c_int)]
+    lib.oracle_presample.argtypes = [_u64p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_uint64, _u64p]
+    lib.oracle_splitmix_next.argtypes = [_u64p]
+    lib.oracle_splitmix_next.restype = ctypes.c_uint64
+    lib.oracle_rand_float.argtypes = [_u64p]
+    lib.oracle_rand_float.restype = ctypes.c_double
+    lib.oracle_block_seed.argtypes = [ctypes.c_uint64] * 3
+    lib.oracle_block_seed.restype = ctypes.c_uint64
+    lib.oracle_categorical_row.argtypes = [_u64p, _f64p, ctypes.c_size_t]
+    lib.oracle_categorical_row.restype = ctypes.c_uint64
+    lib.oracle_bernoulli.argtypes = [_u64p, ctypes.c_double, ctypes.POINTER(ctypes.c_int)]
+    lib.oracle_insert_zero_bit.argtypes = [ctypes.c_size_t, ctypes.c_int]
+    lib.oracle_insert_zero_bit.restype = ctypes.c_size_t
+    lib.oracle_default_batch.argtypes = [ctypes.c_int]
+    lib.oracle_default_batch.restype = ctypes.c_int64
+    return lib
+
+
+_LIBS: dict[str, ctypes.CDLL] = {}
+
+
+def lib(fast: bool = False) -> ctypes.CDLL:
+    name = "liboracle_fast.so" if fast else "liboracle.so"
+    if name not in _LIBS:
+        _LIBS[name] = _load(name)
+    return _LIBS[name]
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def _check(rc: int, fast: bool = False) -> None:
+    if rc != 0:
+        raise OracleError(lib(fast).oracle_last_error().decode())
+
+
+def info(blob: np.ndarray) -> dict:
+    out = np.zeros(10, dtype=np.int64)
+    _check(lib().oracle_info(_ptr(blob, _u64p), blob.size, _ptr(out, _i64p)))
+    keys = ["n", "initial_k", "max_k", "nsymbols", "nrecords", "ndetectors", "nobservables",
+            "ninstr", "nmeasure", "ncoin"]
+    return dict(zip(keys, (int(v) for v in out)))
+
+
+def _cols(n: int, packed: bool) -> int:
+    return (n + 7) // 8 if packed else n
+
+
+def sample(blob: np.ndarray, shots: int, seed: int, *, chunk_shots: int = 0, batch_size: int = 0,
+           threads: int = 1, bit_packed: bool = True, observable: int = 0, fast: bool = False) -> dict:
+    """Reference-stream sampling; chunk/batch 0 pick the reference defaults."""
+    inf = info(blob)
+    meas = np.zeros((shots, _cols(inf["nrecords"], bit_packed)), dtype=np.uint8)
+    det = np.zeros((shots, _cols(inf["ndetectors"], bit_packed)), dtype=np.uint8)
+    obs = np.zeros((shots, _cols(inf["nobservables"], bit_packed)), dtype=np.uint8)
+    counts = np.zeros(4, dtype=np.uint64)
+    ones = np.zeros(max(1, inf["nobservables"]), dtype=np.uint64)
+    timing = np.zeros(1, dtype=np.float64)
+    _check(lib(fast).oracle_sample(_ptr(blob, _u64p), blob.size, shots, seed, chunk_shots, batch_size,
+                                   threads, int(bit_packed), observable, _ptr(meas, _u8p), _ptr(det, _u8p),
+                                   _ptr(obs, _u8p), _ptr(counts, _u64p), _ptr(ones, _u64p),
+                                   _ptr(timing, _f64p)), fast)
+    return dict(measurements=meas, detectors=det, observables=obs, counts=counts,
+                observable_ones=ones[:inf["nobservables"]], sample_s=float(timing[0]))
+
+
+def sample_per_shot(blob: np.ndarray, shots: int, seed: int, **kw) -> dict:
+    """The per-shot stream contract: reference sampling with one-shot chunks
+    and one-shot blocks, so shot s draws from block_seed(.., seed, s)."""
+    return sample(blob, shots, seed, chunk_shots=1, batch_size=1, **kw)
+
+
+def sample_injected(blob: np.ndarray, shots: int, exo: np.ndarray | None, uniforms: np.ndarray | None,
+                    coins: np.ndarray | None, *, bit_packed: bool = True, observable: int = 0,
+                    want_probs: bool = False, want_final_state: bool = False) -> dict:
+    inf = info(blob)
+    words = (shots + 63) // 64
+    exo = np.ascontiguousarray(exo if exo is not None else np.zeros((inf["nsymbols"], words), np.uint64),
+                               dtype=np.uint64)
+    uniforms = np.ascontiguousarray(uniforms if uniforms is not None else np.zeros((inf["nmeasure"], shots)),
+                                    dtype=np.float64)
+    coins = np.ascontiguousarray(coins if coins is not None else np.zeros((inf["ncoin"], words), np.uint64),
+                                 dtype=np.uint64)
+    meas = np.zeros((shots, _cols(inf["nrecords"], bit_packed)), dtype=np.uint8)
+    det = np.zeros((shots, _cols(inf["ndetectors"], bit_packed)), dtype=np.uint8)
+    obs = np.zeros((shots, _cols(inf["nobservables"], bit_packed)), dtype=np.uint8)
+    counts = np.zeros(4, dtype=np.uint64)
+    ones = np.zeros(max(1, inf["nobservables"]), dtype=np.uint64)
+    probs = np.zeros((inf["nmeasure"], shots)) if want_probs else None
+    dim = 1 << inf["max_k"]
+    fre = np.zeros(dim) if want_final_state else None
+    fim = np.zeros(dim) if want_final_state else None
+    fk = ctypes.c_int(0)
+    _check(lib().oracle_sample_injected(_ptr(blob, _u64p), blob.size, shots, _ptr(exo, _u64p),
+                                        _ptr(uniforms, _f64p), _ptr(coins, _u64p), int(bit_packed), observable,
+                                        _ptr(meas, _u8p), _ptr(det, _u8p), _ptr(obs, _u8p), _ptr(counts, _u64p),
+                                        _ptr(ones, _u64p), _ptr(probs, _f64p), _ptr(fre, _f64p), _ptr(fim, _f64p),
+                                        ctypes.byref(fk)))
+    out = dict(measurements=meas, detectors=det, observables=obs, counts=counts,
+               observable_ones=ones[:inf["nobservables"]])
+    if want_probs:
+        out["probs"] = probs
+    if want_final_state:
+        d = 1 << fk.value
+        out["final_state"] = fre[:d] + 1j * fim[:d]
+    return out
+
+
+def presample(blob: np.ndarray, shots: int, stream_seed: int) -> np.ndarray:
+    inf = info(blob)
+    words = (shots + 63) // 64
+    planes = np.zeros((max(inf["nsymbols"], 1), max(words, 1)), dtype=np.uint64)
+    _check(lib().oracle_presample(_ptr(blob, _u64p), blob.size, shots, stream_seed, _ptr(planes, _u64p)))
+    return planes[:inf["nsymbols"], :words]
+
+
+def splitmix_next(state: int) -> tuple[int, int]:
+    s = ctypes.c_uint64(state)
+    v = lib().oracle_splitmix_next(ctypes.byref(s))
+    return int(v), int(s.value)
+
+
+def rand_float(state: int) -> tuple[float, int]:
+    s = ctypes.c_uint64(state)
+    v = lib().oracle_rand_float(ctypes.byref(s))
+    return float(v), int(s.value)
+
+
+def block_seed(base: int, seed: int, block: int) -> int:
+    return int(lib().oracle_block_seed(base, seed, block))
+
+
+def categorical_row(state: int, probs) -> tuple[int, int]:
+    s = ctypes.c_uint64(state)
+    p = np.ascontiguousarray(probs, dtype=np.float64)
+    row = lib().oracle_categorical_row(ctypes.byref(s), _ptr(p, _f64p), p.size)
+    return int(row), int(s.value)
+
+
+def bernoulli(state: int, p: float) -> tuple[bool, int]:
+    s = ctypes.c_uint64(state)
+    out = ctypes.c_int(0)
+    _check(lib().oracle_bernoulli(ctypes.byref(s), p, ctypes.byref(out)))
+    return bool(out.value), int(s.value)
+
+
+def insert_zero_bit(packed: int, bit: int) -> int:
+    return int(lib().oracle_insert_zero_bit(packed, bit))
+
+
+def default_batch(max_k: int) -> int:
+    return int(lib().oracle_default_batch(max_k))

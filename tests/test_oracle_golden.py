@@ -1,0 +1,145 @@
+"""Pins the CPU oracle against the reference's own known-answer tests.
+
+Each vector below is the value a ticit unit test asserts (file:line cited);
+the oracle is a restatement, so these are what make it trustworthy.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from ticit_amd import plan_text
+
+GOLDEN = 0x9E3779B97F4A7C15
+MASK = (1 << 64) - 1
+
+
+def test_splitmix64_published_vectors():
+    # src/random.rs:89-103
+    v, s = O.splitmix_next(0)
+    assert v == 0xE220A8397B1DCDAF
+    v, s = O.splitmix_next(s)
+    assert v == 0x6E789E6AA1B965F4
+    v, s = O.splitmix_next(s)
+    assert v == 0x06C45D188009454F
+    assert s == (GOLDEN * 3) & MASK
+    assert O.splitmix_next(1)[0] == 0x910A2DEC89025CC1
+    assert O.splitmix_next(42)[0] == 0xBDD732262FEB6E95
+
+
+def test_rand_float_pinned():
+    # src/random.rs:105-116
+    u, _ = O.rand_float(0)
+    assert u == (0xE220A8397B1DCDAF >> 11) * 2.0 ** -53
+    assert u == 0.88331080821364261
+
+
+def test_block_seed_definition():
+    # src/random.rs:151-162
+    assert O.block_seed(2, 3, 4) == 2 ^ ((GOLDEN * 5) & MASK) ^ ((0xBF58476D1CE4E5B9 * 4) & MASK)
+    a = O.block_seed(0x7EED0000, 0, 0)
+    assert a != O.block_seed(0x7EED0000, 1, 0)
+    assert a != O.block_seed(0x7EED0000, 0, 1)
+    assert a != O.block_seed(0x5EED1234, 0, 0)
+
+
+def test_categorical_walk_is_inclusive_and_falls_through():
+    # src/random.rs:139-148
+    assert O.categorical_row(0, [0.0, 0.0])[0] == 1
+    assert O.categorical_row(0, [1.0, 0.0])[0] == 0
+
+
+def test_certain_bernoulli_consumes_no_randomness():
+    # src/random.rs:118-129
+    b, s = O.bernoulli(7, 0.0)
+    assert (b, s) == (False, 7)
+    b, s = O.bernoulli(7, 1.0)
+    assert (b, s) == (True, 7)
+    _, s = O.bernoulli(7, 0.5)
+    assert s != 7
+    with pytest.raises(O.OracleError):
+        O.bernoulli(1, -0.1)
+    with pytest.raises(O.OracleError):
+        O.bernoulli(1, float("nan"))
+
+
+def test_insert_zero_bit():
+    # src/active.rs:395-400
+    assert O.insert_zero_bit(0b1011, 0) == 0b10110
+    assert O.insert_zero_bit(0b1011, 2) == 0b10011
+    assert O.insert_zero_bit(0b1011, 4) == 0b01011
+
+
+def test_default_batch_count():
+    # src/sampler/batch/mod.rs:712-717
+    assert O.default_batch(10) == 32
+    assert O.default_batch(0) == 2048
+    assert O.default_batch(15) == 1
+
+
+def test_exact_rows_of_deterministic_circuit():
+    # src/sampler/prepared.rs:1207-1245 (the trailing EXP_VAL probe is
+    # non-destructive and left out: this backend plans no expectation probes)
+    blob = plan_text("M !0\nM 1\nDETECTOR rec[-2] rec[-1]\n"
+                     "OBSERVABLE_INCLUDE(0) rec[-2]\nOBSERVABLE_INCLUDE(2) rec[-1]\n")
+    r = O.sample(blob, 3, 7, bit_packed=False)
+    assert r["measurements"].reshape(-1).tolist() == [1, 0, 1, 0, 1, 0]
+    assert r["detectors"].reshape(-1).tolist() == [1, 1, 1]
+    assert r["observables"].reshape(-1).tolist() == [1, 0, 0, 1, 0, 0, 1, 0, 0]
+    assert r["observable_ones"].tolist() == [3, 0, 0]
+    p = O.sample(blob, 3, 7, bit_packed=True)
+    assert p["measurements"].reshape(-1).tolist() == [1, 1, 1]
+    assert p["detectors"].reshape(-1).tolist() == [1, 1, 1]
+    assert p["observables"].reshape(-1).tolist() == [1, 1, 1]
+
+
+def test_packed_rows_little_bit_order():
+    # src/sampler/prepared.rs:1199-1205; ticit_py/tests/test_api.py:73-88
+    blob = plan_text("X 0 1 2 3 4 5 6 7 10\nM 0 1 2 3 4 5 6 7 8 9 10\n")
+    r = O.sample(blob, 2, 1, bit_packed=True)
+    assert r["measurements"].tolist() == [[0xFF, 0x04], [0xFF, 0x04]]
+
+
+def test_t_gate_window():
+    # src/sampler/batch/mod.rs:719-731: H T M, 200 shots, seed 23, batch 32
+    blob = plan_text("H 0\nT 0\nM 0\n")
+    r = O.sample(blob, 200, 23, batch_size=32, bit_packed=False)
+    ones = int(r["measurements"].sum())
+    assert 50 < ones < 150
+
+
+def test_seed_determinism_and_sensitivity():
+    # src/sampler/batch/mod.rs:733-743
+    blob = plan_text("X_ERROR(0.25) 0\nH 1\nT 1\nH 1\nM 0 1\n")
+    a = O.sample(blob, 300, 99)
+    b = O.sample(blob, 300, 99)
+    c = O.sample(blob, 300, 100)
+    assert np.array_equal(a["measurements"], b["measurements"])
+    assert not np.array_equal(a["measurements"], c["measurements"])
+
+
+def test_thread_count_invariance_of_counts():
+    # src/sampler/prepared.rs:1340-1375: 1 vs 3 workers give equal counts
+    from ticit_amd import workloads as w
+    blob = plan_text(w.surface_code_t(3, 3, 1e-2, 1, 0))
+    a = O.sample(blob, 10000, 5, threads=1)
+    b = O.sample(blob, 10000, 5, threads=3)
+    assert a["observable_ones"].tolist() == b["observable_ones"].tolist()
+    assert sorted(map(bytes, a["measurements"])) == sorted(map(bytes, b["measurements"]))
+
+
+def test_per_shot_contract_is_reference_at_unit_chunks():
+    """Per-shot streams == reference sampling with one-shot chunks/blocks,
+    and a shot's row does not depend on which other shots run with it."""
+    from ticit_amd import workloads as w
+    blob = plan_text(w.surface_code_t(3, 2, 1e-2, 1, 0))
+    full = O.sample_per_shot(blob, 300, 9)
+    part = O.sample(blob, 300, 9, chunk_shots=1, batch_size=1, threads=1)
+    assert np.array_equal(full["measurements"], part["measurements"])
+
+
+def test_presample_sparse_hit_rate():
+    # gpu/plan.rs:1568-1577 window: p = 1/8 Bernoulli group over 20000 draws
+    blob = plan_text("X_ERROR(0.125) 0\nM 0\n")
+    planes = O.presample(blob, 20000, 1234)
+    hits = int(np.unpackbits(planes.view(np.uint8), bitorder="little").sum())
+    assert 2300 <= hits <= 2700

@@ -1,0 +1,235 @@
+"""ticit_amd: the MI355X ``hip`` sampling backend for ticit's compiled
+Clifford+T sampler.
+
+The Python surface mirrors ticit's own binding (ticit_py/src/lib.rs:715-830,
+tests in ticit_py/tests/test_api.py): ``Circuit(text).compile(...)`` returns a
+compiled sampler whose ``sample(shots, seed, bit_packed)`` returns row-major
+measurement/detector/observable records plus counts, with the same field
+names. Everything below calls the C ABI of ``libticit_hip.so``
+(include/ticit_hip.h); there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _capi
+from ._capi import TicitError
+
+__all__ = ["Circuit", "CompiledSampler", "SampleResult", "TicitError", "plan_text"]
+
+TIERS = {"auto": 0, "lane": 1, "group": 2, "wide": 3}
+PRECISIONS = {"fp64": 0, "fp32": 1}
+PLANNERS = {"heuristic": 0, "learned": 1}
+
+
+def plan_text(text: str) -> np.ndarray:
+    """Parse and plan ``.ticit`` text into a plan blob (include/ticit_plan.h)."""
+    lib = _capi.load()
+    data = text.encode()
+    ptr = _capi._u64p()
+    n = ctypes.c_size_t(0)
+    _capi.check(lib.ticit_hip_plan_text(data, len(data), ctypes.byref(ptr), ctypes.byref(n)))
+    try:
+        return np.ctypeslib.as_array(ptr, shape=(n.value,)).copy()
+    finally:
+        lib.ticit_hip_blob_free(ptr)
+
+
+@dataclass
+class SampleResult:
+    """Mirrors ticit's SampleResult (src/sampler/prepared.rs:197-219)."""
+
+    shots: int
+    discarded: int
+    accepted: int
+    logical_errors: int
+    bit_packed: bool
+    measurements: np.ndarray
+    detectors: np.ndarray
+    observables: np.ndarray
+    observable_ones: np.ndarray
+    sample_s: float
+    kernel_s: float
+    probabilities: np.ndarray | None = None
+
+    # ticit_py aliases
+    @property
+    def total_shots(self) -> int:
+        return self.shots
+
+    @property
+    def passed_shots(self) -> int:
+        return self.accepted
+
+    @property
+    def discards(self) -> int:
+        return self.discarded
+
+    @property
+    def logical_error_rate(self) -> float:
+        return self.logical_errors / self.accepted if self.accepted else float("nan")
+
+    def __iter__(self):
+        return iter((self.measurements, self.detectors, self.observables))
+
+
+def _rows(ptr, rows: int, cols: int) -> np.ndarray:
+    if rows == 0 or cols == 0 or not ptr:
+        return np.zeros((rows, cols), dtype=np.uint8)
+    return np.ctypeslib.as_array(ptr, shape=(rows * cols,)).copy().reshape(rows, cols)
+
+
+def _convert(res: _capi.Result, keep_records: bool) -> SampleResult:
+    packed = bool(res.bit_packed)
+    cols = (lambda n: (n + 7) // 8) if packed else (lambda n: n)
+    rows = int(res.record_rows) if keep_records else 0
+    nobs = int(res.nobservables)
+    ones = (np.ctypeslib.as_array(res.observable_ones, shape=(nobs,)).copy()
+            if nobs and res.observable_ones else np.zeros(0, dtype=np.uint64))
+    return SampleResult(
+        shots=int(res.shots), discarded=int(res.discarded), accepted=int(res.accepted),
+        logical_errors=int(res.logical_errors), bit_packed=packed,
+        measurements=_rows(res.measurements, rows, cols(int(res.nmeasurements))),
+        detectors=_rows(res.detectors, rows, cols(int(res.ndetectors))),
+        observables=_rows(res.observables, rows, cols(nobs)),
+        observable_ones=ones.astype(np.uint64), sample_s=float(res.sample_s), kernel_s=float(res.kernel_s))
+
+
+class CompiledSampler:
+    """A compiled circuit bound to the current HIP device (Sampler,
+    src/sampler/prepared.rs:467-478)."""
+
+    backend = "hip"
+
+    def __init__(self, blob: np.ndarray, observable: int = 0, precision: str = "fp64",
+                 tier: str = "auto", planner: str = "heuristic"):
+        lib = _capi.load()
+        self._lib = lib
+        self.blob = np.ascontiguousarray(blob, dtype=np.uint64)
+        opt = _capi.Options()
+        opt.observable = observable
+        opt.precision = PRECISIONS[precision]
+        opt.tier = TIERS[tier]
+        opt.planner_mode = PLANNERS[planner]
+        handle = ctypes.c_void_p()
+        _capi.check(lib.ticit_hip_program_create(self.blob.ctypes.data_as(_capi._u64p), self.blob.size,
+                                                 ctypes.byref(opt), ctypes.byref(handle)))
+        self._handle = handle
+        info = _capi.ProgramInfo()
+        _capi.check(lib.ticit_hip_program_info_get(handle, ctypes.byref(info)))
+        self.num_qubits = int(info.qubits)
+        self.num_measurements = int(info.measurement_records)
+        self.num_detectors = int(info.detectors)
+        self.num_observables = int(info.observables)
+        self.max_active_qubits = int(info.max_active_qubits)
+        self.num_symbols = int(info.symbols)
+        self.num_instructions = int(info.instructions)
+        self.num_measure_instructions = int(info.measure_instructions)
+        self.num_coin_instructions = int(info.coin_instructions)
+        self.tier = {1: "lane", 2: "group", 3: "wide"}.get(int(info.tier), "auto")
+        self.has_postselection = False
+
+    def close(self) -> None:
+        if getattr(self, "_handle", None):
+            self._lib.ticit_hip_program_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        return self._handle
+
+    def sample(self, shots: int, seed: int | None = None, bit_packed: bool = False,
+               keep_records: bool = True) -> SampleResult:
+        """Sampler::sample_with_seed (src/sampler/prepared.rs:663-670)."""
+        if seed is None:
+            seed = int(np.random.SeedSequence().generate_state(1, np.uint64)[0])
+        res = _capi.Result()
+        _capi.check(self._lib.ticit_hip_sample(self._handle, shots, seed, int(bit_packed), int(keep_records),
+                                               ctypes.byref(res)))
+        try:
+            return _convert(res, keep_records)
+        finally:
+            self._lib.ticit_hip_result_free(ctypes.byref(res))
+
+    sample_with_seed = sample
+
+    def sample_injected(self, shots: int, exo: np.ndarray | None = None, uniforms: np.ndarray | None = None,
+                        coins: np.ndarray | None = None, bit_packed: bool = True,
+                        want_probs: bool = False) -> SampleResult:
+        """Sampling with caller-supplied randomness: exogenous symbol
+        bit-planes [nsymbols][ceil(shots/64)], branch uniforms
+        [measure_instructions][shots] and coin bits
+        [coin_instructions][ceil(shots/64)]."""
+        words = (shots + 63) // 64
+
+        def arr(a, shape, dt):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=dt)
+            if a.size != int(np.prod(shape)):
+                raise ValueError(f"injected buffer has {a.size} entries, expected {shape}")
+            return a
+
+        exo = arr(exo, (self.num_symbols, words), np.uint64)
+        uniforms = arr(uniforms, (self.num_measure_instructions, shots), np.float64)
+        coins = arr(coins, (self.num_coin_instructions, words), np.uint64)
+        probs = np.zeros((self.num_measure_instructions, shots)) if want_probs else None
+        res = _capi.Result()
+        ptr = lambda a, t: a.ctypes.data_as(t) if a is not None else None
+        _capi.check(self._lib.ticit_hip_sample_injected(self._handle, shots, ptr(exo, _capi._u64p),
+                                                        ptr(uniforms, _capi._f64p), ptr(coins, _capi._u64p),
+                                                        int(bit_packed), ctypes.byref(res),
+                                                        ptr(probs, _capi._f64p)))
+        try:
+            out = _convert(res, True)
+        finally:
+            self._lib.ticit_hip_result_free(ctypes.byref(res))
+        out.probabilities = probs
+        return out
+
+
+class Circuit:
+    """A parsed ``.ticit`` circuit (ticit's Circuit, src/circuit/mod.rs)."""
+
+    def __init__(self, text: str):
+        self.text = text
+        self.blob = plan_text(text)
+        header = self.blob[:10]
+        self.num_qubits = int(header[2])
+        self.max_active_qubits = int(header[4])
+        self.num_symbols = int(header[5])
+        self.num_measurements = int(header[6])
+        self.num_detectors = int(header[7])
+
+    @classmethod
+    def from_file(cls, path: str) -> "Circuit":
+        with open(path) as f:
+            return cls(f.read())
+
+    def compile(self, postselection_mask=None, *, observable: int = 0, backend: str = "hip",
+                precision: str = "fp64", tier: str = "auto", planner: str = "heuristic",
+                threads: int | None = None) -> CompiledSampler:
+        """Circuit::compile with ticit_py's ``backend=`` selector; only the
+        ``"hip"`` backend lives in this package."""
+        if backend != "hip":
+            raise ValueError(f"backend {backend!r} is not provided by ticit_amd; use backend='hip'")
+        if postselection_mask is not None and any(int(v) for v in postselection_mask):
+            raise TicitError(3, "detector postselection is not supported by the hip backend")
+        return CompiledSampler(self.blob, observable=observable, precision=precision, tier=tier,
+                               planner=planner)
+
+
+def device_properties() -> dict:
+    props = (ctypes.c_int64 * 6)()
+    n = _capi.load().ticit_hip_device_query(props)
+    keys = ["lds_per_block", "compute_units", "clock_khz", "warp_size", "memory_clock_khz", "bus_width_bits"]
+    return dict(devices=int(n), **{k: int(v) for k, v in zip(keys, props)})

@@ -1,0 +1,117 @@
+"""ctypes binding of ``libticit_hip.so`` (include/ticit_hip.h).
+
+The library is built in-tree by ``ticit_amd/build.py``; loading fails loudly
+when it is missing, so no Python path ever stands in for the HIP backend.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+from . import build as _build
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("observable", ctypes.c_uint32), ("precision", ctypes.c_uint32),
+                ("planner_mode", ctypes.c_uint32), ("tier", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 4)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("shots", ctypes.c_uint64), ("discarded", ctypes.c_uint64),
+                ("accepted", ctypes.c_uint64), ("logical_errors", ctypes.c_uint64),
+                ("record_rows", ctypes.c_uint64), ("bit_packed", ctypes.c_int32),
+                ("reserved0", ctypes.c_int32), ("nmeasurements", ctypes.c_uint64),
+                ("ndetectors", ctypes.c_uint64), ("nobservables", ctypes.c_uint64),
+                ("measurements", _u8p), ("detectors", _u8p), ("observables", _u8p),
+                ("observable_ones", _u64p), ("sample_s", ctypes.c_double),
+                ("kernel_s", ctypes.c_double)]
+
+
+class ProgramInfo(ctypes.Structure):
+    _fields_ = [("qubits", ctypes.c_uint64), ("measurement_records", ctypes.c_uint64),
+                ("detectors", ctypes.c_uint64), ("observables", ctypes.c_uint64),
+                ("max_active_qubits", ctypes.c_uint64), ("symbols", ctypes.c_uint64),
+                ("instructions", ctypes.c_uint64), ("measure_instructions", ctypes.c_uint64),
+                ("coin_instructions", ctypes.c_uint64), ("tier", ctypes.c_uint32),
+                ("precision", ctypes.c_uint32)]
+
+
+# ticit_hip_status -> exception kinds of TicitError (src/errors.rs:9-57)
+STATUS_NAMES = {1: "InvalidInput", 2: "Parse", 3: "Unsupported", 4: "Internal",
+                5: "OutOfDeviceMemory", 6: "DeviceError"}
+
+# every symbol include/ticit_hip.h declares
+EXPORTS = [
+    "ticit_hip_last_error", "ticit_hip_plan_text", "ticit_hip_blob_free",
+    "ticit_hip_program_create", "ticit_hip_compile_text", "ticit_hip_program_destroy",
+    "ticit_hip_program_info_get", "ticit_hip_sample", "ticit_hip_sample_device",
+    "ticit_hip_sample_injected", "ticit_hip_sample_batch", "ticit_hip_result_free",
+    "ticit_hip_device_query",
+]
+
+_LIB = None
+
+
+def library_path() -> str:
+    return _build.LIB
+
+
+def load() -> ctypes.CDLL:
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = library_path()
+    if not os.path.exists(path):
+        raise RuntimeError(f"HIP backend library missing: {path} (run ticit_amd/build.py)")
+    lib = ctypes.CDLL(path)
+    lib.ticit_hip_last_error.restype = ctypes.c_char_p
+    lib.ticit_hip_plan_text.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(_u64p),
+                                        ctypes.POINTER(ctypes.c_size_t)]
+    lib.ticit_hip_blob_free.argtypes = [_u64p]
+    lib.ticit_hip_program_create.argtypes = [_u64p, ctypes.c_size_t, ctypes.POINTER(Options),
+                                             ctypes.POINTER(ctypes.c_void_p)]
+    lib.ticit_hip_compile_text.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(Options),
+                                           ctypes.POINTER(ctypes.c_void_p)]
+    lib.ticit_hip_program_destroy.argtypes = [ctypes.c_void_p]
+    lib.ticit_hip_program_info_get.argtypes = [ctypes.c_void_p, ctypes.POINTER(ProgramInfo)]
+    lib.ticit_hip_sample.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
+                                     ctypes.c_int, ctypes.POINTER(Result)]
+    lib.ticit_hip_sample_device.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64,
+                                            ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_void_p]
+    lib.ticit_hip_sample_injected.argtypes = [ctypes.c_void_p, ctypes.c_uint64, _u64p, _f64p, _u64p,
+                                              ctypes.c_int, ctypes.POINTER(Result), _f64p]
+    lib.ticit_hip_sample_batch.argtypes = [ctypes.POINTER(ctypes.c_void_p), _u64p, _u64p, ctypes.c_size_t,
+                                           ctypes.c_int, ctypes.c_int, ctypes.POINTER(Result)]
+    lib.ticit_hip_result_free.argtypes = [ctypes.POINTER(Result)]
+    lib.ticit_hip_device_query.argtypes = [ctypes.POINTER(ctypes.c_int64)]
+    lib.ticit_hip_device_query.restype = ctypes.c_int
+    for name in ("ticit_hip_plan_text", "ticit_hip_program_create", "ticit_hip_compile_text",
+                 "ticit_hip_program_info_get", "ticit_hip_sample", "ticit_hip_sample_device",
+                 "ticit_hip_sample_injected", "ticit_hip_sample_batch"):
+        getattr(lib, name).restype = ctypes.c_int
+    lib.ticit_hip_blob_free.restype = None
+    lib.ticit_hip_program_destroy.restype = None
+    lib.ticit_hip_result_free.restype = None
+    _LIB = lib
+    return lib
+
+
+class TicitError(RuntimeError):
+    """Raised for a failing status; ``kind`` names the TicitError variant."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(message)
+        self.status = status
+        self.kind = STATUS_NAMES.get(status, "Internal")
+
+
+def check(status: int) -> None:
+    if status != 0:
+        raise TicitError(status, load().ticit_hip_last_error().decode(errors="replace"))

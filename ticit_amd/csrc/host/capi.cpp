@@ -1,0 +1,460 @@
+// C ABI of the hip sampling backend (include/ticit_hip.h): program upload,
+// tier selection and the sampling entry points.
+#include "../../../include/ticit_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "device_program.hpp"
+#include "frontend.hpp"
+
+namespace ticit {
+
+size_t lane_tier_lds_bytes(int kmax, const LaneLayout& y);
+size_t group_tier_lds_bytes(int max_k, const GroupLayout& y);
+hipError_t launch_lane_tier(int kmax, const DevProgram& P, const DevLaunch& L, const LaneLayout& Y,
+                            hipStream_t stream);
+hipError_t launch_group_tier(const DevProgram& P, const DevLaunch& L, const GroupLayout& Y, hipStream_t stream);
+
+}  // namespace ticit
+
+using namespace ticit;
+
+namespace {
+
+thread_local std::string g_error;
+
+ticit_hip_status fail(ticit_hip_status s, const std::string& msg) {
+  g_error = msg;
+  return s;
+}
+
+ticit_hip_status from_hip(hipError_t e, const char* what) {
+  if (e == hipErrorOutOfMemory) return fail(TICIT_HIP_OUT_OF_DEVICE_MEMORY, std::string(what) + ": out of device memory");
+  return fail(TICIT_HIP_DEVICE_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+constexpr uint32_t kTierLane = 1, kTierGroup = 2, kTierWide = 3;
+
+}  // namespace
+
+struct ticit_hip_program {
+  Program prog;
+  HostTables tab;
+  DevProgram dev{};
+  std::vector<void*> allocations;
+  ticit_hip_options opt{};
+  uint32_t tier = 0;
+  int device = 0;
+  int* d_error = nullptr;
+  LaneLayout lane{};
+  GroupLayout group{};
+  uint32_t meas_row_packed = 0, det_row_packed = 0, obs_row_packed = 0;
+
+  ~ticit_hip_program() {
+    for (void* p : allocations) (void)hipFree(p);
+  }
+
+  template <class T>
+  hipError_t upload(const std::vector<T>& v, const T*& out) {
+    void* d = nullptr;
+    size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(T);
+    hipError_t e = hipMalloc(&d, bytes);
+    if (e != hipSuccess) return e;
+    allocations.push_back(d);
+    if (!v.empty()) {
+      e = hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+      if (e != hipSuccess) return e;
+    }
+    out = static_cast<const T*>(d);
+    return hipSuccess;
+  }
+
+  uint32_t row_bytes(uint64_t cols, bool packed) const { return uint32_t(packed ? (cols + 7) / 8 : cols); }
+};
+
+namespace {
+
+ticit_hip_status create_from_program(Program prog, const ticit_hip_options* options, ticit_hip_program** out) {
+  auto* h = new ticit_hip_program();
+  h->prog = std::move(prog);
+  if (options) h->opt = *options;
+  try {
+    h->tab = lower_to_device(h->prog);
+  } catch (const TicitError& e) {
+    delete h;
+    return fail(ticit_hip_status(e.kind), e.what());
+  }
+  const HostTables& t = h->tab;
+  int lds_limit = 0;
+  hipError_t e = hipGetDevice(&h->device);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device);
+  if (e != hipSuccess) {
+    delete h;
+    return from_hip(e, "no HIP device");
+  }
+  DevProgram& d = h->dev;
+#define UP(field, vec)                                     \
+  do {                                                     \
+    hipError_t ue = h->upload(vec, d.field);               \
+    if (ue != hipSuccess) {                                \
+      delete h;                                            \
+      return from_hip(ue, "uploading program tables");     \
+    }                                                      \
+  } while (0)
+  UP(instrs, t.instrs);
+  UP(resid, t.resid);
+  UP(det_recs, t.det_recs);
+  UP(obs_off, t.obs_off);
+  UP(obs_recs, t.obs_recs);
+  UP(sym_off, t.sym_off);
+  UP(sym_part, t.sym_part);
+  UP(exo_syms, t.exo_syms);
+  UP(cats, t.cats);
+  UP(rares, t.rares);
+  UP(berns, t.berns);
+  UP(lowps, t.lowps);
+  UP(f64pool, t.f64pool);
+  UP(u64pool, t.u64pool);
+  UP(i32pool, t.i32pool);
+#undef UP
+  d.ninstr = int32_t(h->prog.instrs.size());
+  d.n = t.n;
+  d.initial_k = t.initial_k;
+  d.max_k = t.max_k;
+  d.nrec = t.nrec;
+  d.ndet = t.ndet;
+  d.nobs = t.nobs;
+  d.nslots = t.nslots;
+  d.npartials = t.npartials;
+  d.nsym = t.nsym;
+  d.nexo = int32_t(h->prog.nsymbols ? t.exo_syms.size() : 0);
+  if (t.exo_syms.size() == 1 && t.exo_syms[0] == 0) d.nexo = 0;
+  d.nmeasure = t.nmeasure;
+  d.ncoin = t.ncoin;
+  d.ncat = int32_t(t.cats.size());
+  d.nrare = int32_t(t.rares.size());
+  d.nbern = int32_t(t.berns.size());
+  d.nlowp = int32_t(t.lowps.size());
+  e = hipMalloc(reinterpret_cast<void**>(&h->d_error), sizeof(int));
+  if (e != hipSuccess) {
+    delete h;
+    return from_hip(e, "allocating the error flag");
+  }
+  h->allocations.push_back(h->d_error);
+
+  // Tier choice: per-lane resident state up to k = 5, one shot per
+  // workgroup while the FP64 state fits the LDS, otherwise unsupported.
+  auto words = [](int bits) { return uint32_t((bits + 63) / 64); };
+  h->lane = LaneLayout{words(t.npartials), words(t.nslots), words(t.nrec), words(t.ndet), 0};
+  h->group = GroupLayout{h->lane.pw, h->lane.sw, h->lane.rw, h->lane.dw};
+  uint32_t max_row = std::max<uint32_t>({uint32_t(t.nrec), uint32_t(t.ndet), uint32_t(t.nobs), 1});
+  h->lane.stage_bytes = 64 * ((max_row + 15) / 16 * 16);
+  uint32_t want = h->opt.tier;
+  if (want == 0) {
+    if (t.max_k <= 5 && lane_tier_lds_bytes(t.max_k, h->lane) <= size_t(lds_limit)) want = kTierLane;
+    else if (t.max_k <= 12 && group_tier_lds_bytes(t.max_k, h->group) <= size_t(lds_limit)) want = kTierGroup;
+    else want = kTierWide;
+  }
+  if (want == kTierLane && (t.max_k > 5 || lane_tier_lds_bytes(t.max_k, h->lane) > size_t(lds_limit))) {
+    delete h;
+    return fail(TICIT_HIP_UNSUPPORTED, "lane tier needs max_k <= 5 and its tables in LDS");
+  }
+  if (want == kTierGroup && (t.max_k > 12 || group_tier_lds_bytes(t.max_k, h->group) > size_t(lds_limit))) {
+    delete h;
+    return fail(TICIT_HIP_UNSUPPORTED, "group tier state exceeds the LDS of one workgroup");
+  }
+  if (want == kTierWide) {
+    delete h;
+    return fail(TICIT_HIP_UNSUPPORTED, "active width " + std::to_string(t.max_k) +
+                                           " exceeds the LDS-resident tiers of the hip backend");
+  }
+  h->tier = want;
+  *out = h;
+  return TICIT_HIP_OK;
+}
+
+hipError_t launch(ticit_hip_program* h, const DevLaunch& L, hipStream_t stream) {
+  if (h->tier == kTierLane) return launch_lane_tier(h->tab.max_k, h->dev, L, h->lane, stream);
+  return launch_group_tier(h->dev, L, h->group, stream);
+}
+
+struct DeviceBuffers {
+  std::vector<void*> ptrs;
+  ~DeviceBuffers() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  template <class T>
+  hipError_t alloc(T** out, size_t count) {
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
+    if (e == hipSuccess) ptrs.push_back(p);
+    *out = static_cast<T*>(p);
+    return e;
+  }
+};
+
+ticit_hip_status run_host(ticit_hip_program* h, uint64_t shots, uint64_t seed, int bit_packed, int keep_records,
+                          bool injected, const uint64_t* exo, const double* uniforms, const uint64_t* coins,
+                          double* probs, ticit_hip_result* out) {
+  auto t0 = std::chrono::steady_clock::now();
+  std::memset(out, 0, sizeof(*out));
+  const HostTables& t = h->tab;
+  DevLaunch L{};
+  L.shots = shots;
+  L.seed = seed;
+  L.first_shot = 0;
+  L.bit_packed = bit_packed != 0;
+  L.keep_records = keep_records != 0;
+  L.injected = injected;
+  L.want_probs = probs != nullptr;
+  L.meas_row = h->row_bytes(uint64_t(t.nrec), bit_packed);
+  L.det_row = h->row_bytes(uint64_t(t.ndet), bit_packed);
+  L.obs_row = h->row_bytes(uint64_t(t.nobs), bit_packed);
+  if (shots == 0) {
+    out->bit_packed = bit_packed != 0;
+    out->nmeasurements = uint64_t(t.nrec);
+    out->ndetectors = uint64_t(t.ndet);
+    out->nobservables = uint64_t(t.nobs);
+    out->observable_ones = static_cast<uint64_t*>(calloc(std::max(1, t.nobs), 8));
+    return TICIT_HIP_OK;
+  }
+  if (h->tier == kTierGroup && shots > 0x7fffffffull) return fail(TICIT_HIP_INVALID_INPUT, "too many shots for one call");
+  DeviceBuffers buf;
+  hipError_t e = hipSuccess;
+  const size_t words = (shots + 63) / 64;
+  uint64_t* d_exo = nullptr;
+  double* d_uni = nullptr;
+  uint64_t* d_coins = nullptr;
+  double* d_probs = nullptr;
+#define CK(expr, what)                        \
+  do {                                        \
+    e = (expr);                               \
+    if (e != hipSuccess) return from_hip(e, what); \
+  } while (0)
+  if (keep_records) {
+    CK(buf.alloc(&L.meas, size_t(shots) * L.meas_row), "allocating measurement rows");
+    CK(buf.alloc(&L.det, size_t(shots) * L.det_row), "allocating detector rows");
+    CK(buf.alloc(&L.obs, size_t(shots) * L.obs_row), "allocating observable rows");
+  }
+  CK(buf.alloc(&L.obs_ones, size_t(std::max(1, t.nobs))), "allocating observable counts");
+  CK(hipMemset(L.obs_ones, 0, size_t(std::max(1, t.nobs)) * 8), "clearing observable counts");
+  CK(hipMemset(h->d_error, 0, sizeof(int)), "clearing the error flag");
+  L.error = h->d_error;
+  if (injected) {
+    CK(buf.alloc(&d_exo, size_t(std::max(1, t.nsym)) * words), "allocating exogenous planes");
+    CK(buf.alloc(&d_uni, size_t(std::max(1, t.nmeasure)) * shots), "allocating uniforms");
+    CK(buf.alloc(&d_coins, size_t(std::max(1, t.ncoin)) * words), "allocating coins");
+    if (exo && t.nsym) CK(hipMemcpy(d_exo, exo, size_t(t.nsym) * words * 8, hipMemcpyHostToDevice), "copying exo");
+    else CK(hipMemset(d_exo, 0, size_t(std::max(1, t.nsym)) * words * 8), "clearing exo");
+    if (uniforms && t.nmeasure)
+      CK(hipMemcpy(d_uni, uniforms, size_t(t.nmeasure) * shots * 8, hipMemcpyHostToDevice), "copying uniforms");
+    else CK(hipMemset(d_uni, 0, size_t(std::max(1, t.nmeasure)) * shots * 8), "clearing uniforms");
+    if (coins && t.ncoin) CK(hipMemcpy(d_coins, coins, size_t(t.ncoin) * words * 8, hipMemcpyHostToDevice), "copying coins");
+    else CK(hipMemset(d_coins, 0, size_t(std::max(1, t.ncoin)) * words * 8), "clearing coins");
+    L.exo = d_exo;
+    L.uniforms = d_uni;
+    L.coins = d_coins;
+  }
+  if (probs) {
+    CK(buf.alloc(&d_probs, size_t(std::max(1, t.nmeasure)) * shots), "allocating probabilities");
+    L.probs = d_probs;
+  }
+  hipEvent_t ev0, ev1;
+  CK(hipEventCreate(&ev0), "creating events");
+  CK(hipEventCreate(&ev1), "creating events");
+  CK(hipEventRecord(ev0, nullptr), "recording event");
+  e = launch(h, L, nullptr);
+  if (e != hipSuccess) {
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    return from_hip(e, "launching the sampling kernel");
+  }
+  CK(hipEventRecord(ev1, nullptr), "recording event");
+  CK(hipEventSynchronize(ev1), "running the sampling kernel");
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, ev0, ev1);
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  int derr = 0;
+  CK(hipMemcpy(&derr, h->d_error, sizeof(int), hipMemcpyDeviceToHost), "reading the error flag");
+  if (derr & kDevErrBadProbability) return fail(TICIT_HIP_INVALID_INPUT, "probability must be between 0 and 1");
+  if (derr & kDevErrImpossibleBranch)
+    return fail(TICIT_HIP_INVALID_INPUT, "sampled an impossible active measurement branch");
+
+  out->shots = shots;
+  out->accepted = shots;
+  out->bit_packed = bit_packed != 0;
+  out->nmeasurements = uint64_t(t.nrec);
+  out->ndetectors = uint64_t(t.ndet);
+  out->nobservables = uint64_t(t.nobs);
+  out->observable_ones = static_cast<uint64_t*>(calloc(std::max(1, t.nobs), 8));
+  CK(hipMemcpy(out->observable_ones, L.obs_ones, size_t(t.nobs) * 8, hipMemcpyDeviceToHost), "reading counts");
+  if (t.nobs > 0 && h->opt.observable < uint32_t(t.nobs)) out->logical_errors = out->observable_ones[h->opt.observable];
+  if (keep_records) {
+    out->record_rows = shots;
+    out->measurements = static_cast<uint8_t*>(malloc(std::max<size_t>(1, size_t(shots) * L.meas_row)));
+    out->detectors = static_cast<uint8_t*>(malloc(std::max<size_t>(1, size_t(shots) * L.det_row)));
+    out->observables = static_cast<uint8_t*>(malloc(std::max<size_t>(1, size_t(shots) * L.obs_row)));
+    if (L.meas_row) CK(hipMemcpy(out->measurements, L.meas, size_t(shots) * L.meas_row, hipMemcpyDeviceToHost), "reading rows");
+    if (L.det_row) CK(hipMemcpy(out->detectors, L.det, size_t(shots) * L.det_row, hipMemcpyDeviceToHost), "reading rows");
+    if (L.obs_row) CK(hipMemcpy(out->observables, L.obs, size_t(shots) * L.obs_row, hipMemcpyDeviceToHost), "reading rows");
+  }
+  if (probs && t.nmeasure)
+    CK(hipMemcpy(probs, d_probs, size_t(t.nmeasure) * shots * 8, hipMemcpyDeviceToHost), "reading probabilities");
+#undef CK
+  out->kernel_s = double(ms) * 1e-3;
+  out->sample_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return TICIT_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ticit_hip_last_error(void) { return g_error.c_str(); }
+
+ticit_hip_status ticit_hip_plan_text(const char* text, size_t len, uint64_t** blob, size_t* nwords) {
+  if (!text || !blob || !nwords) return fail(TICIT_HIP_INVALID_INPUT, "null argument");
+  try {
+    Program p = compile_circuit_text(std::string(text, len));
+    std::vector<uint64_t> w = encode_program(p);
+    *blob = static_cast<uint64_t*>(malloc(w.size() * 8));
+    std::memcpy(*blob, w.data(), w.size() * 8);
+    *nwords = w.size();
+    return TICIT_HIP_OK;
+  } catch (const TicitError& e) {
+    return fail(ticit_hip_status(e.kind), e.what());
+  } catch (const std::exception& e) {
+    return fail(TICIT_HIP_INTERNAL, e.what());
+  }
+}
+
+void ticit_hip_blob_free(uint64_t* blob) { free(blob); }
+
+ticit_hip_status ticit_hip_program_create(const uint64_t* blob, size_t nwords, const ticit_hip_options* options,
+                                          ticit_hip_program** out) {
+  if (!blob || !out) return fail(TICIT_HIP_INVALID_INPUT, "null argument");
+  Program p;
+  std::string err;
+  if (!decode_program(blob, nwords, p, err)) return fail(TICIT_HIP_INVALID_INPUT, err);
+  try {
+    return create_from_program(std::move(p), options, out);
+  } catch (const std::exception& e) {
+    return fail(TICIT_HIP_INTERNAL, e.what());
+  }
+}
+
+ticit_hip_status ticit_hip_compile_text(const char* text, size_t len, const ticit_hip_options* options,
+                                        ticit_hip_program** out) {
+  if (!text || !out) return fail(TICIT_HIP_INVALID_INPUT, "null argument");
+  try {
+    Program p = compile_circuit_text(std::string(text, len));
+    return create_from_program(std::move(p), options, out);
+  } catch (const TicitError& e) {
+    return fail(ticit_hip_status(e.kind), e.what());
+  } catch (const std::exception& e) {
+    return fail(TICIT_HIP_INTERNAL, e.what());
+  }
+}
+
+void ticit_hip_program_destroy(ticit_hip_program* program) { delete program; }
+
+ticit_hip_status ticit_hip_program_info_get(const ticit_hip_program* h, ticit_hip_program_info* out) {
+  if (!h || !out) return fail(TICIT_HIP_INVALID_INPUT, "null argument");
+  std::memset(out, 0, sizeof(*out));
+  out->qubits = uint64_t(h->prog.n);
+  out->measurement_records = uint64_t(h->prog.nrecords);
+  out->detectors = uint64_t(h->prog.ndetectors);
+  out->observables = uint64_t(h->prog.observables.size());
+  out->max_active_qubits = uint64_t(h->prog.max_k);
+  out->symbols = uint64_t(h->prog.nsymbols);
+  out->instructions = uint64_t(h->prog.instrs.size());
+  out->measure_instructions = uint64_t(h->tab.nmeasure);
+  out->coin_instructions = uint64_t(h->tab.ncoin);
+  out->tier = h->tier;
+  out->precision = h->opt.precision;
+  return TICIT_HIP_OK;
+}
+
+ticit_hip_status ticit_hip_sample(ticit_hip_program* h, uint64_t shots, uint64_t seed, int bit_packed,
+                                  int keep_records, ticit_hip_result* out) {
+  if (!h || !out) return fail(TICIT_HIP_INVALID_INPUT, "null argument");
+  return run_host(h, shots, seed, bit_packed, keep_records, false, nullptr, nullptr, nullptr, nullptr, out);
+}
+
+ticit_hip_status ticit_hip_sample_injected(ticit_hip_program* h, uint64_t shots, const uint64_t* exo,
+                                           const double* uniforms, const uint64_t* coins, int bit_packed,
+                                           ticit_hip_result* out, double* probs) {
+  if (!h || !out) return fail(TICIT_HIP_INVALID_INPUT, "null argument");
+  return run_host(h, shots, 0, bit_packed, 1, true, exo, uniforms, coins, probs, out);
+}
+
+ticit_hip_status ticit_hip_sample_device(ticit_hip_program* h, uint64_t shots, uint64_t seed, uint64_t first_shot,
+                                         int bit_packed, int keep_records, uint8_t* meas, uint8_t* det,
+                                         uint8_t* obs, uint64_t* observable_ones, void* stream) {
+  if (!h || !observable_ones) return fail(TICIT_HIP_INVALID_INPUT, "null argument");
+  if (keep_records && (!meas || !det || !obs)) return fail(TICIT_HIP_INVALID_INPUT, "null row buffer");
+  if (shots == 0) return TICIT_HIP_OK;
+  const HostTables& t = h->tab;
+  DevLaunch L{};
+  L.shots = shots;
+  L.seed = seed;
+  L.first_shot = first_shot;
+  L.bit_packed = bit_packed != 0;
+  L.keep_records = keep_records != 0;
+  L.meas_row = h->row_bytes(uint64_t(t.nrec), bit_packed);
+  L.det_row = h->row_bytes(uint64_t(t.ndet), bit_packed);
+  L.obs_row = h->row_bytes(uint64_t(t.nobs), bit_packed);
+  L.meas = meas;
+  L.det = det;
+  L.obs = obs;
+  L.obs_ones = reinterpret_cast<unsigned long long*>(observable_ones);
+  L.error = h->d_error;
+  hipError_t e = launch(h, L, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return from_hip(e, "launching the sampling kernel");
+  return TICIT_HIP_OK;
+}
+
+ticit_hip_status ticit_hip_sample_batch(ticit_hip_program* const* programs, const uint64_t* shots,
+                                        const uint64_t* seeds, size_t count, int bit_packed, int keep_records,
+                                        ticit_hip_result* results) {
+  if (!programs || !shots || !seeds || !results) return fail(TICIT_HIP_INVALID_INPUT, "null argument");
+  for (size_t i = 0; i < count; ++i) {
+    ticit_hip_status s = ticit_hip_sample(programs[i], shots[i], seeds[i], bit_packed, keep_records, &results[i]);
+    if (s != TICIT_HIP_OK) return s;
+  }
+  return TICIT_HIP_OK;
+}
+
+void ticit_hip_result_free(ticit_hip_result* r) {
+  if (!r) return;
+  free(r->measurements);
+  free(r->detectors);
+  free(r->observables);
+  free(r->observable_ones);
+  r->measurements = r->detectors = r->observables = nullptr;
+  r->observable_ones = nullptr;
+}
+
+int ticit_hip_device_query(int64_t* props) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return 0;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  int v[6] = {0, 0, 0, 0, 0, 0};
+  (void)hipDeviceGetAttribute(&v[0], hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  (void)hipDeviceGetAttribute(&v[1], hipDeviceAttributeMultiprocessorCount, dev);
+  (void)hipDeviceGetAttribute(&v[2], hipDeviceAttributeClockRate, dev);
+  (void)hipDeviceGetAttribute(&v[3], hipDeviceAttributeWarpSize, dev);
+  (void)hipDeviceGetAttribute(&v[4], hipDeviceAttributeMemoryClockRate, dev);
+  (void)hipDeviceGetAttribute(&v[5], hipDeviceAttributeMemoryBusWidth, dev);
+  for (int i = 0; i < 6; ++i) props[i] = v[i];
+  return n;
+}
+
+}  // extern "C"

@@ -1,0 +1,314 @@
+// Program -> flat device tables, with the static checks ticit's batch
+// executor makes at run time (src/sampler/batch/symbols.rs:14-128,
+// src/sampler/batch/active.rs:272-276, 339-343, 739-748) moved to upload time.
+#include "device_program.hpp"
+
+#include <cmath>
+#include <map>
+#include <set>
+
+#include "frontend.hpp"
+
+namespace ticit {
+
+namespace {
+
+const double kInvSqrt2 = 0.70710678118654752440;
+
+struct Cx {
+  double re, im;
+};
+
+Cx phase_factor(int ph) {
+  switch (ph & 3) {
+    case 0: return Cx{1.0, 0.0};
+    case 1: return Cx{0.0, 1.0};
+    case 2: return Cx{-1.0, 0.0};
+    default: return Cx{0.0, -1.0};
+  }
+}
+
+double gap_denominator(double p) {
+  if (!(p > 0.0 && p < 1.0)) throw TicitError(1, "geometric gap probability must be in (0, 1)");
+  return std::log1p(-p);
+}
+
+void check_probability(double p) {
+  if (!(p >= 0.0 && p <= 1.0)) throw TicitError(1, "probability must be between 0 and 1");
+}
+
+}  // namespace
+
+HostTables lower_to_device(const Program& p) {
+  HostTables t;
+  t.n = p.n;
+  t.initial_k = p.initial_k;
+  t.max_k = p.max_k;
+  t.nrec = p.nrecords;
+  t.ndet = p.ndetectors;
+  t.nobs = int(p.observables.size());
+  t.nsym = p.nsymbols;
+  if (p.max_k >= 62) throw TicitError(3, "active qubit count is too large for machine basis indices");
+
+  // Exogenous symbol set (src/sampler/exogenous.rs:283-298).
+  std::vector<char> exo(size_t(p.nsymbols) + 1, 0);
+  auto mark = [&](int32_t c) {
+    if (c <= 0 || c > p.nsymbols) throw TicitError(1, "exogenous condition exceeds program symbol table");
+    exo[c] = 1;
+  };
+  for (const auto& d : p.categorical) for (int32_t c : d.conds) mark(c);
+  for (const auto& g : p.rare) for (const auto& s : g.sets) for (int32_t c : s) mark(c);
+  for (int32_t c : p.bern_conds) mark(c);
+  for (const auto& g : p.lowp) for (int32_t c : g.conds) mark(c);
+  for (int32_t c = 1; c <= p.nsymbols; ++c)
+    if (exo[c]) t.exo_syms.push_back(c);
+
+  // Noise plan.
+  for (const auto& d : p.categorical) {
+    DevCat dc{};
+    dc.nbits = uint32_t(d.nbits);
+    dc.nrows = uint32_t(d.probs.size());
+    dc.prob_off = uint32_t(t.f64pool.size());
+    dc.assign_off = uint32_t(t.u64pool.size());
+    dc.cond_off = uint32_t(t.i32pool.size());
+    if (d.probs.empty() || d.conds.size() != size_t(d.nbits))
+      throw TicitError(1, "invalid categorical symbolic distribution");
+    for (size_t r = 0; r < d.probs.size(); ++r) {
+      t.f64pool.push_back(d.probs[r]);
+      t.u64pool.push_back(d.assignments[r]);
+    }
+    for (int32_t c : d.conds) t.i32pool.push_back(c);
+    t.cats.push_back(dc);
+  }
+  for (const auto& g : p.rare) {
+    DevRare dr{};
+    dr.enabled = g.event_p > 0.0 && !g.sets.empty();
+    dr.denom = dr.enabled ? gap_denominator(g.event_p) : 0.0;
+    dr.nsets = uint32_t(g.sets.size());
+    dr.nbits = uint32_t(g.nbits);
+    dr.nev = uint32_t(g.event_rows.size());
+    dr.cond_off = uint32_t(t.i32pool.size());
+    for (const auto& s : g.sets) {
+      if (s.size() != size_t(g.nbits)) throw TicitError(1, "rare categorical set has the wrong width");
+      for (int32_t c : s) t.i32pool.push_back(c);
+    }
+    dr.assign_off = uint32_t(t.u64pool.size());
+    for (uint64_t a : g.assignments) t.u64pool.push_back(a);
+    dr.ev_row_off = uint32_t(t.i32pool.size());
+    for (int32_t r : g.event_rows) {
+      if (r < 0 || size_t(r) >= g.assignments.size()) throw TicitError(1, "rare categorical event row out of range");
+      t.i32pool.push_back(r);
+    }
+    dr.ev_prob_off = uint32_t(t.f64pool.size());
+    for (double q : g.event_probs) t.f64pool.push_back(q);
+    if (dr.enabled && dr.nev == 0) throw TicitError(1, "rare categorical group has no event rows");
+    t.rares.push_back(dr);
+  }
+  for (size_t i = 0; i < p.bern_conds.size(); ++i) {
+    DevBern b{};
+    b.cond = p.bern_conds[i];
+    double pr = p.bern_probs[i];
+    check_probability(pr);
+    if (pr <= 0.0) {
+      b.mode = kBernNone;
+    } else if (pr >= 1.0) {
+      b.mode = kBernAll;
+    } else {
+      b.invert = pr > 0.5;
+      double q = b.invert ? 1.0 - pr : pr;
+      if (q <= 0.0) {
+        b.mode = b.invert ? kBernAll : kBernNone;
+        b.invert = 0;
+      } else if (q == 0.5) {
+        b.mode = kBernHalf;
+      } else if (q < 0.02) {
+        b.mode = kBernLow;
+        b.denom = gap_denominator(q);
+      } else {
+        b.mode = kBernCoins;
+        uint64_t raw = uint64_t(q * 256.0);
+        uint64_t top = raw < 128 ? raw : 127;
+        double pt = double(top) / 256.0;
+        b.top_bits = uint32_t(top);
+        double left = q - pt;
+        b.left_denom = left > 0.0 ? gap_denominator(left / (1.0 - pt)) : 0.0;
+      }
+    }
+    t.berns.push_back(b);
+  }
+  for (const auto& g : p.lowp) {
+    DevLowp l{};
+    l.enabled = g.p > 0.0 && !g.conds.empty();
+    l.denom = l.enabled ? gap_denominator(g.p) : 0.0;
+    l.nconds = uint32_t(g.conds.size());
+    l.cond_off = uint32_t(t.i32pool.size());
+    for (int32_t c : g.conds) t.i32pool.push_back(c);
+    t.lowps.push_back(l);
+  }
+
+  // Instructions, exogenous partials and runtime slots.
+  std::map<std::vector<int32_t>, int32_t> partial_index;
+  std::vector<std::vector<int32_t>> partials;
+  std::vector<int32_t> slot_of(size_t(p.nsymbols) + 1, -1);
+  std::vector<char> assigned(size_t(p.nsymbols) + 1, 0);
+  auto slot_for = [&](int32_t c) {
+    if (c <= 0 || c > p.nsymbols) throw TicitError(1, "symbolic condition exceeds batch executor symbol table");
+    if (exo[c]) throw TicitError(4, "an exogenous symbol cannot be assigned at run time");
+    if (slot_of[c] < 0) slot_of[c] = t.nslots++;
+    return slot_of[c];
+  };
+  auto assign = [&](int32_t c) {
+    int32_t s = slot_for(c);
+    if (assigned[c]) throw TicitError(4, "a runtime symbol is assigned twice");
+    assigned[c] = 1;
+    return s;
+  };
+  int k = p.initial_k;
+  for (const auto& in : p.instrs) {
+    DevInstr d{};
+    d.op = uint32_t(in.op);
+    d.nq = in.nq;
+    d.partial = -1;
+    d.record = -1;
+    d.rc_slot = -1;
+    d.branch_slot = -1;
+    d.detector = -1;
+    d.ordinal = -1;
+    if (in.record < 0 || in.record > p.nrecords) throw TicitError(1, "measurement record id out of range");
+    if (in.record > 0) d.record = in.record - 1;
+    if (in.expr.constant) d.op |= kFlagConstant;
+    switch (in.op) {
+      case Op::Rotate: {
+        if (in.nq != k) throw TicitError(4, "rotation kernel dimension does not match batch active state");
+        d.c = std::cos(in.angle);
+        d.s = std::sin(in.angle);
+        Cx e = phase_factor(in.phase);
+        double a = 0.0, b = -d.s;  // (0 - i sin) * i^phase, in num-complex's order
+        d.me_re = a * e.re - b * e.im;
+        d.me_im = a * e.im + b * e.re;
+        d.xmask = in.xmask;
+        d.zmask = in.zmask;
+        if (in.xmask == 0) {
+          d.op |= kFlagDiagonal;
+        } else {
+          if (in.zmask == 0) d.op |= kFlagUniformImag;
+          d.pivot = 63 - __builtin_clzll(in.xmask);
+        }
+        if (__builtin_popcountll(in.xmask & in.zmask) & 1) d.op |= kFlagXzOdd;
+        t.quantum_ops++;
+        t.dense_work += std::ldexp(1.0, k);
+        break;
+      }
+      case Op::Promote:
+        if (k >= p.n) throw TicitError(4, "cannot promote a dormant qubit when none remain");
+        if (in.nq != k) throw TicitError(4, "promotion width does not match batch active state");
+        d.c = std::cos(in.angle);
+        d.s = std::sin(in.angle);
+        t.quantum_ops++;
+        t.dense_work += std::ldexp(3.0, k);
+        ++k;
+        break;
+      case Op::Record:
+        break;
+      case Op::Detector:
+        if (in.detector <= 0 || in.detector > p.ndetectors) throw TicitError(1, "detector id out of range");
+        d.detector = in.detector - 1;
+        if (!in.det_records.empty()) {
+          d.op |= kFlagDetRecords;
+          d.det_off = uint32_t(t.det_recs.size());
+          d.det_len = uint32_t(in.det_records.size());
+          for (int32_t r : in.det_records) {
+            if (r <= 0 || r > p.nrecords)
+              throw TicitError(1, "detector references an out-of-range measurement record");
+            t.det_recs.push_back(uint32_t(r - 1));
+          }
+        }
+        break;
+      case Op::Measure: {
+        if (k == 0) throw TicitError(4, "cannot measure an active Pauli when k == 0");
+        if (in.nq != k) throw TicitError(4, "measurement kernel dimension does not match batch active state");
+        d.xmask = in.xmask;
+        d.zmask = in.zmask;
+        d.pivot = in.pivot;
+        if (in.diagonal) {
+          d.op |= kFlagDiagonal;
+          if (in.diag_phase_bit) d.op |= kFlagDiagPhase;
+          d.zwop = in.zmask & ~(1ull << in.pivot);
+        }
+        Cx e = phase_factor(in.phase);
+        d.c1_re = e.re * kInvSqrt2;
+        d.c1_im = -e.im * kInvSqrt2;
+        d.ordinal = t.nmeasure++;
+        d.branch_slot = assign(in.branch);
+        t.quantum_ops++;
+        t.dense_work += std::ldexp(2.5, k);
+        --k;
+        break;
+      }
+      case Op::Coin:
+        d.ordinal = t.ncoin++;
+        d.branch_slot = assign(in.branch);
+        break;
+    }
+    if (in.exp_val >= 0) throw TicitError(3, "expectation probes are not supported by the hip backend");
+    bool evaluates = !(in.op == Op::Detector && !in.det_records.empty());
+    if (evaluates) {
+      std::vector<int32_t> ex;
+      d.resid_off = uint32_t(t.resid.size());
+      for (int32_t c : in.expr.conds) {
+        if (c <= 0 || c > p.nsymbols) throw TicitError(1, "symbolic condition exceeds batch executor symbol table");
+        if (exo[c]) {
+          ex.push_back(c);
+        } else {
+          if (!assigned[c]) throw TicitError(1, "symbolic condition expression has no concrete batch value");
+          t.resid.push_back(uint32_t(slot_of[c]));
+        }
+      }
+      d.resid_len = uint32_t(t.resid.size()) - d.resid_off;
+      t.expression_terms += in.expr.conds.size();
+      if (!ex.empty()) {
+        auto it = partial_index.find(ex);
+        if (it == partial_index.end()) {
+          it = partial_index.emplace(ex, int32_t(partials.size())).first;
+          partials.push_back(ex);
+        }
+        d.partial = it->second;
+      }
+    }
+    if (in.record_condition > 0) d.rc_slot = assign(in.record_condition);
+    t.instrs.push_back(d);
+  }
+  t.npartials = int(partials.size());
+
+  // symbol -> partial CSR
+  std::vector<std::vector<uint32_t>> by_sym(size_t(p.nsymbols));
+  for (size_t e = 0; e < partials.size(); ++e)
+    for (int32_t c : partials[e]) by_sym[size_t(c - 1)].push_back(uint32_t(e));
+  t.sym_off.push_back(0);
+  for (const auto& v : by_sym) {
+    t.sym_part.insert(t.sym_part.end(), v.begin(), v.end());
+    t.sym_off.push_back(uint32_t(t.sym_part.size()));
+  }
+
+  t.obs_off.push_back(0);
+  for (const auto& ob : p.observables) {
+    for (int32_t r : ob) {
+      if (r <= 0 || r > p.nrecords) throw TicitError(1, "observable references an out-of-range measurement record");
+      t.obs_recs.push_back(uint32_t(r - 1));
+    }
+    t.obs_off.push_back(uint32_t(t.obs_recs.size()));
+  }
+  // keep every table non-empty so device pointers are always valid
+  if (t.resid.empty()) t.resid.push_back(0);
+  if (t.det_recs.empty()) t.det_recs.push_back(0);
+  if (t.obs_recs.empty()) t.obs_recs.push_back(0);
+  if (t.sym_part.empty()) t.sym_part.push_back(0);
+  if (t.exo_syms.empty()) t.exo_syms.push_back(0);
+  if (t.f64pool.empty()) t.f64pool.push_back(0);
+  if (t.u64pool.empty()) t.u64pool.push_back(0);
+  if (t.i32pool.empty()) t.i32pool.push_back(0);
+  if (t.instrs.empty()) t.instrs.push_back(DevInstr{});
+  return t;
+}
+
+}  // namespace ticit

@@ -1,0 +1,1680 @@
+// Circuit front end: parser, lowering, pending-queue optimizer, planner and
+// program freezing. Each stage cites the ticit function whose behaviour it
+// follows; the code itself is written for this library.
+#include "frontend.hpp"
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <sstream>
+#include <unordered_map>
+
+namespace ticit {
+
+// ============================================================================
+// Symbolic booleans (src/symbolic.rs:24-146, src/bits.rs:120-142)
+// ============================================================================
+
+void normalize_conditions(std::vector<int32_t>& c) {
+  std::sort(c.begin(), c.end());
+  size_t w = 0, r = 0;
+  while (r < c.size()) {
+    size_t e = r + 1;
+    while (e < c.size() && c[e] == c[r]) ++e;
+    if ((e - r) & 1) c[w++] = c[r];
+    r = e;
+  }
+  c.resize(w);
+}
+
+SymBool sym_xor(const SymBool& a, const SymBool& b) {
+  SymBool out;
+  out.constant = a.constant != b.constant;
+  out.conds.reserve(a.conds.size() + b.conds.size());
+  size_t i = 0, j = 0;
+  while (i < a.conds.size() && j < b.conds.size()) {
+    if (a.conds[i] < b.conds[j]) out.conds.push_back(a.conds[i++]);
+    else if (a.conds[i] > b.conds[j]) out.conds.push_back(b.conds[j++]);
+    else { ++i; ++j; }
+  }
+  out.conds.insert(out.conds.end(), a.conds.begin() + i, a.conds.end());
+  out.conds.insert(out.conds.end(), b.conds.begin() + j, b.conds.end());
+  return out;
+}
+
+SymBool sym_xor_const(const SymBool& a, bool v) {
+  SymBool out = a;
+  out.constant = a.constant != v;
+  return out;
+}
+
+SymBool sym_var(int32_t c) {
+  SymBool out;
+  out.conds.push_back(c);
+  return out;
+}
+
+namespace {
+
+inline int word_of(int32_t condition) { return (condition - 1) >> 6; }
+
+// ============================================================================
+// Pauli strings: i^phase * prod_q X_q^x Z_q^z (src/pauli.rs:10-16)
+// ============================================================================
+
+struct Pauli {
+  int n = 0;
+  std::vector<uint64_t> x, z;
+  int phase = 0;
+  Pauli() = default;
+  explicit Pauli(int nq) : n(nq), x((nq + 63) / 64, 0), z((nq + 63) / 64, 0) {}
+  bool xb(int q) const { return (x[q >> 6] >> (q & 63)) & 1; }
+  bool zb(int q) const { return (z[q >> 6] >> (q & 63)) & 1; }
+  void setx(int q, bool v) {
+    uint64_t m = 1ull << (q & 63);
+    if (v) x[q >> 6] |= m; else x[q >> 6] &= ~m;
+  }
+  void setz(int q, bool v) {
+    uint64_t m = 1ull << (q & 63);
+    if (v) z[q >> 6] |= m; else z[q >> 6] &= ~m;
+  }
+  void set_phase(int p) { phase = ((p % 4) + 4) % 4; }
+  int ycount() const {
+    int c = 0;
+    for (size_t i = 0; i < x.size(); ++i) c += __builtin_popcountll(x[i] & z[i]);
+    return c;
+  }
+  bool nonidentity() const {
+    for (size_t i = 0; i < x.size(); ++i)
+      if (x[i] | z[i]) return true;
+    return false;
+  }
+  bool same_body(const Pauli& o) const { return x == o.x && z == o.z; }
+  bool operator==(const Pauli& o) const { return n == o.n && phase == o.phase && x == o.x && z == o.z; }
+};
+
+// out <- out * row, carrying the i^2 from row's X crossing out's Z.
+inline void mul_into(Pauli& out, const Pauli& row) {
+  uint64_t carry = 0;
+  for (size_t i = 0; i < out.x.size(); ++i) {
+    carry += __builtin_popcountll(out.z[i] & row.x[i]);
+    out.x[i] ^= row.x[i];
+    out.z[i] ^= row.z[i];
+  }
+  out.set_phase(out.phase + row.phase + 2 * int(carry & 1));
+}
+
+inline Pauli pmul(const Pauli& a, const Pauli& b) {
+  Pauli out = a;
+  mul_into(out, b);
+  return out;
+}
+
+inline bool anticommutes(const Pauli& a, const Pauli& b) {
+  uint64_t acc = 0;
+  for (size_t i = 0; i < a.x.size(); ++i) acc ^= (a.x[i] & b.z[i]) ^ (a.z[i] & b.x[i]);
+  return __builtin_popcountll(acc) & 1;
+}
+
+inline bool hermitian(const Pauli& p) { return ((p.phase - p.ycount()) & 1) == 0; }
+
+// src/pauli.rs:457-466
+inline bool phase_sign(const Pauli& p) {
+  int v = ((p.phase - p.ycount()) % 4 + 4) % 4;
+  if (v == 0) return false;
+  if (v == 2) return true;
+  throw TicitError(1, "Pauli measurement requires a Hermitian Pauli with real coefficient");
+}
+
+inline Pauli pX(int n, int q) { Pauli p(n); p.setx(q, true); return p; }
+inline Pauli pZ(int n, int q) { Pauli p(n); p.setz(q, true); return p; }
+inline Pauli pY(int n, int q) { Pauli p(n); p.setx(q, true); p.setz(q, true); p.phase = 1; return p; }
+
+inline Pauli positive_hermitian(Pauli p) {
+  if (!hermitian(p)) throw TicitError(4, "Pauli frame row requires a Hermitian Pauli body");
+  p.set_phase(p.ycount());
+  return p;
+}
+
+// ============================================================================
+// Clifford tableau: rows[q] = U^dag X_q U, rows[n+q] = U^dag Z_q U
+// (src/frames.rs:292-572)
+// ============================================================================
+
+struct Clifford {
+  int n = 0;
+  std::vector<Pauli> rows;
+  Clifford() = default;
+  explicit Clifford(int nq) : n(nq), rows(2 * nq, Pauli(nq)) {
+    for (int q = 0; q < nq; ++q) {
+      rows[q].setx(q, true);
+      rows[nq + q].setz(q, true);
+    }
+  }
+  static Clifford from_rows(int nq, std::vector<Pauli> xr, std::vector<Pauli> zr) {
+    Clifford c;
+    c.n = nq;
+    c.rows = std::move(xr);
+    for (auto& r : zr) c.rows.push_back(std::move(r));
+    return c;
+  }
+  // U^dag P U, multiplying rows in ascending-q, X-before-Z order.
+  Pauli preimage(const Pauli& p) const {
+    Pauli out(n);
+    out.phase = p.phase;
+    for (size_t w = 0; w < p.x.size(); ++w) {
+      uint64_t bits = p.x[w] | p.z[w];
+      while (bits) {
+        int b = __builtin_ctzll(bits);
+        int q = int(w) * 64 + b;
+        if ((p.x[w] >> b) & 1) mul_into(out, rows[q]);
+        if ((p.z[w] >> b) & 1) mul_into(out, rows[n + q]);
+        bits &= bits - 1;
+      }
+    }
+    return out;
+  }
+  // Decomposition of p over the rows: preimage(result) == p exactly.
+  Pauli coordinates(const Pauli& p) const {
+    Pauli out(n);
+    for (int q = 0; q < n; ++q) {
+      if (anticommutes(p, rows[q])) out.setz(q, true);
+      if (anticommutes(p, rows[n + q])) out.setx(q, true);
+    }
+    Pauli rec = preimage(out);
+    if (!rec.same_body(p)) throw TicitError(4, "frame rows do not span the Pauli body");
+    out.set_phase(p.phase - rec.phase);
+    return out;
+  }
+  void swap_rows(int a, int b) { std::swap(rows[a], rows[b]); }
+  void add_phase(int r, int d) { rows[r].set_phase(rows[r].phase + d); }
+  void mul_rows(int dst, int lhs, int rhs, int extra) {
+    Pauli o = pmul(rows[lhs], rows[rhs]);
+    o.set_phase(o.phase + extra);
+    rows[dst] = std::move(o);
+  }
+  // Left-multiplication gates U <- G U (src/frames.rs:613-951).
+  void h(int q) { swap_rows(q, n + q); }
+  void s(int q) { mul_rows(q, q, n + q, 3); }
+  void sdg(int q) { mul_rows(q, q, n + q, 1); }
+  void sqrt_x(int q) { swap_rows(q, n + q); mul_rows(q, q, n + q, 3); swap_rows(q, n + q); }
+  void sqrt_x_dag(int q) { swap_rows(q, n + q); mul_rows(q, q, n + q, 1); swap_rows(q, n + q); }
+  void sqrt_y(int q) { swap_rows(q, n + q); add_phase(n + q, 2); }
+  void sqrt_y_dag(int q) { swap_rows(q, n + q); add_phase(q, 2); }
+  void gx(int q) { add_phase(n + q, 2); }
+  void gy(int q) { add_phase(q, 2); add_phase(n + q, 2); }
+  void gz(int q) { add_phase(q, 2); }
+  void cx(int c, int t) { mul_rows(c, c, t, 0); mul_rows(n + t, n + c, n + t, 0); }
+  void cy(int c, int t) {
+    int xc = c, zc = n + c, xt = t, zt = n + t;
+    mul_rows(xc, xc, zc, 3);
+    mul_rows(xc, xc, zt, 0);
+    mul_rows(xt, zc, xt, 0);
+    mul_rows(xc, xc, xt, 0);
+    mul_rows(zt, zc, zt, 0);
+  }
+  void cz(int a, int b) { mul_rows(a, a, n + b, 0); mul_rows(b, n + a, b, 0); }
+  void swap(int a, int b) { swap_rows(a, b); swap_rows(n + a, n + b); }
+};
+
+// ============================================================================
+// Symbol context (src/symbolic.rs:203-331)
+// ============================================================================
+
+struct Context {
+  int32_t next = 1;
+  std::map<int32_t, double> bern;  // ascending condition order is a contract
+  std::vector<CategoricalDist> cats;
+  void bump(int32_t c) { next = std::max(next, c + 1); }
+  int32_t fresh() { return next++; }
+  static double check_p(double p) {
+    if (!(p >= 0.0 && p <= 1.0)) throw TicitError(1, "probability must be between 0 and 1");
+    return p;
+  }
+  int32_t fresh_bern(double p) {
+    check_p(p);
+    int32_t c = fresh();
+    bern[c] = p;
+    return c;
+  }
+  std::vector<int32_t> fresh_cat(int nbits, const std::vector<uint64_t>& assignments,
+                                 const std::vector<double>& probs) {
+    double total = 0.0;
+    for (double p : probs) total += check_p(p);
+    if (std::fabs(total - 1.0) > 1e-12)
+      throw TicitError(1, "categorical symbolic distribution probabilities must sum to 1");
+    CategoricalDist d;
+    d.nbits = nbits;
+    for (int i = 0; i < nbits; ++i) d.conds.push_back(fresh());
+    d.assignments = assignments;
+    d.probs = probs;
+    cats.push_back(d);
+    return d.conds;
+  }
+};
+
+// ============================================================================
+// Symbolic Pauli frame (src/frames.rs:71-232)
+// ============================================================================
+
+struct ActiveFrame {
+  int n = 0;
+  std::vector<int32_t> conds;
+  // transposed occupancy: 64 terms per block, [block * n + q]
+  std::vector<uint64_t> xblk, zblk;
+  explicit ActiveFrame(int nq = 0) : n(nq) {}
+  void add(const Pauli& p, int32_t c) {
+    size_t t = conds.size();
+    if ((t & 63) == 0) {
+      xblk.resize(xblk.size() + n, 0);
+      zblk.resize(zblk.size() + n, 0);
+    }
+    size_t base = (t >> 6) * n;
+    uint64_t m = 1ull << (t & 63);
+    for (int q = 0; q < n; ++q) {
+      if (p.xb(q)) xblk[base + q] |= m;
+      if (p.zb(q)) zblk[base + q] |= m;
+    }
+    conds.push_back(c);
+  }
+  SymBool conjugate(const Pauli& p) const {
+    std::vector<int> xs, zs;
+    for (int q = 0; q < n; ++q) {
+      if (p.xb(q)) xs.push_back(q);
+      if (p.zb(q)) zs.push_back(q);
+    }
+    std::vector<int32_t> hit;
+    size_t blocks = (conds.size() + 63) / 64;
+    for (size_t b = 0; b < blocks; ++b) {
+      uint64_t anti = 0;
+      size_t base = b * n;
+      for (int q : xs) anti ^= zblk[base + q];
+      for (int q : zs) anti ^= xblk[base + q];
+      while (anti) {
+        size_t t = b * 64 + __builtin_ctzll(anti);
+        if (t < conds.size()) hit.push_back(conds[t]);
+        anti &= anti - 1;
+      }
+    }
+    SymBool out;
+    normalize_conditions(hit);
+    out.conds = std::move(hit);
+    return out;
+  }
+};
+
+// ============================================================================
+// Circuit IR and parser (src/circuit/parser.rs, src/circuit/ir.rs)
+// ============================================================================
+
+enum class K {
+  Tick, H, S, SDag, SqrtX, SqrtXDag, SqrtY, SqrtYDag, X, Y, Z, CX, CY, CZ, Swap,
+  T, TDag, Rot, MZ, MX, MY, MRZ, MRX, MRY, RZ, RX, RY, MPP, XErr, YErr, ZErr, Dep1, Dep2,
+  FeedbackX, FeedbackY, FeedbackZ,
+};
+
+struct MTarget {
+  int qubit = 0;
+  bool inverted = false;
+};
+
+struct Product {
+  Pauli pauli;
+  bool inverted = false;
+};
+
+struct CInstr {
+  K kind = K::Tick;
+  int line = 0;
+  std::vector<int> qubits;
+  std::vector<MTarget> mtargets;
+  std::vector<Product> products;
+  double probability = 0.0;
+  double angle = 0.0;
+  int feedback_record = 0;  // classically controlled Pauli: record id
+};
+
+struct CDetector {
+  std::vector<int> records;
+  size_t after_instruction = 0;
+};
+
+struct CObservable {
+  int index = 0;
+  std::vector<int> records;
+};
+
+struct Circuit {
+  int nqubits = 0;
+  int nrecords = 0;
+  std::vector<CInstr> instrs;
+  std::vector<CDetector> detectors;
+  std::vector<CObservable> observables;
+};
+
+struct RawLine {
+  std::string name;
+  std::vector<double> args;
+  std::vector<std::string> targets;
+  int line = 0;
+};
+
+[[noreturn]] void parse_error(int line, const std::string& msg) {
+  throw TicitError(2, "line " + std::to_string(line) + ": " + msg);
+}
+
+std::string upper(std::string s) {
+  for (auto& c : s) c = char(std::toupper(static_cast<unsigned char>(c)));
+  return s;
+}
+
+// Reads lines into a flat list, expanding REPEAT blocks.
+void read_block(const std::vector<std::string>& lines, size_t& pos, int depth,
+                std::vector<RawLine>& out) {
+  while (pos < lines.size()) {
+    int lineno = int(pos) + 1;
+    std::string s = lines[pos++];
+    size_t hash = s.find('#');
+    if (hash != std::string::npos) s = s.substr(0, hash);
+    size_t a = 0;
+    while (a < s.size() && std::isspace(static_cast<unsigned char>(s[a]))) ++a;
+    size_t b = s.size();
+    while (b > a && std::isspace(static_cast<unsigned char>(s[b - 1]))) --b;
+    s = s.substr(a, b - a);
+    if (s.empty()) continue;
+    if (s == "}") {
+      if (depth == 0) parse_error(lineno, "unmatched '}'");
+      return;
+    }
+    // mnemonic
+    size_t i = 0;
+    while (i < s.size() && (std::isalnum(static_cast<unsigned char>(s[i])) || s[i] == '_')) ++i;
+    RawLine rl;
+    rl.line = lineno;
+    rl.name = upper(s.substr(0, i));
+    if (rl.name.empty()) parse_error(lineno, "expected an instruction name");
+    if (i < s.size() && s[i] == '[') {  // tag
+      size_t close = s.find(']', i);
+      if (close == std::string::npos) parse_error(lineno, "unterminated instruction tag");
+      i = close + 1;
+    }
+    if (i < s.size() && s[i] == '(') {
+      size_t close = s.find(')', i);
+      if (close == std::string::npos) parse_error(lineno, "unterminated argument list");
+      std::string inner = s.substr(i + 1, close - i - 1);
+      std::stringstream ss(inner);
+      std::string tok;
+      while (std::getline(ss, tok, ',')) {
+        char* end = nullptr;
+        const char* cs = tok.c_str();
+        while (*cs && std::isspace(static_cast<unsigned char>(*cs))) ++cs;
+        if (!*cs) continue;
+        double v = std::strtod(cs, &end);
+        if (end == cs) parse_error(lineno, "malformed numeric argument");
+        rl.args.push_back(v);
+      }
+      i = close + 1;
+    }
+    std::string rest = s.substr(i);
+    if (rl.name == "REPEAT") {
+      std::stringstream ss(rest);
+      long count = -1;
+      std::string brace;
+      ss >> count >> brace;
+      if (count < 0 || brace != "{") parse_error(lineno, "malformed REPEAT");
+      size_t body_start = pos;
+      std::vector<RawLine> body;
+      read_block(lines, pos, depth + 1, body);
+      (void)body_start;
+      for (long r = 0; r < count; ++r) out.insert(out.end(), body.begin(), body.end());
+      continue;
+    }
+    // MPP-style products may be written "X0 * Z1": glue '*' to neighbours.
+    std::string glued;
+    for (size_t c = 0; c < rest.size(); ++c) {
+      if (rest[c] == '*') {
+        while (!glued.empty() && glued.back() == ' ') glued.pop_back();
+        glued.push_back('*');
+        ++c;
+        while (c < rest.size() && std::isspace(static_cast<unsigned char>(rest[c]))) ++c;
+        --c;
+      } else {
+        glued.push_back(std::isspace(static_cast<unsigned char>(rest[c])) ? ' ' : rest[c]);
+      }
+    }
+    std::stringstream ts(glued);
+    std::string tok;
+    while (ts >> tok) rl.targets.push_back(tok);
+    out.push_back(std::move(rl));
+  }
+  if (depth != 0) parse_error(int(lines.size()), "unterminated REPEAT block");
+}
+
+bool parse_int(const std::string& s, long& v) {
+  if (s.empty()) return false;
+  char* end = nullptr;
+  v = std::strtol(s.c_str(), &end, 10);
+  return end && *end == '\0';
+}
+
+int qubit_target(const RawLine& rl, const std::string& t) {
+  long v;
+  if (!parse_int(t, v) || v < 0) parse_error(rl.line, "malformed qubit target '" + t + "'");
+  return int(v);
+}
+
+Circuit parse_circuit(const std::string& text) {
+  std::vector<std::string> lines;
+  {
+    std::stringstream ss(text);
+    std::string l;
+    while (std::getline(ss, l)) lines.push_back(l);
+  }
+  std::vector<RawLine> raw;
+  size_t pos = 0;
+  read_block(lines, pos, 0, raw);
+
+  // Qubit count: one past the largest qubit index named anywhere.
+  int maxq = -1;
+  for (const auto& rl : raw) {
+    if (rl.name == "DETECTOR" || rl.name == "OBSERVABLE_INCLUDE" || rl.name == "DISCARD" ||
+        rl.name == "SHIFT_COORDS" || rl.name == "TICK")
+      continue;
+    for (const auto& t : rl.targets) {
+      std::string body = t;
+      std::stringstream fs(body);
+      std::string f;
+      while (std::getline(fs, f, '*')) {
+        if (!f.empty() && f[0] == '!') f = f.substr(1);
+        if (!f.empty() && (f[0] == 'X' || f[0] == 'Y' || f[0] == 'Z' || f[0] == 'x' ||
+                           f[0] == 'y' || f[0] == 'z'))
+          f = f.substr(1);
+        long v;
+        if (parse_int(f, v) && v >= 0) maxq = std::max(maxq, int(v));
+      }
+    }
+  }
+  Circuit c;
+  c.nqubits = maxq + 1;
+  const int n = c.nqubits;
+
+  auto need_no_args = [](const RawLine& rl) {
+    if (!rl.args.empty()) parse_error(rl.line, rl.name + " takes no arguments");
+  };
+  auto one_prob = [](const RawLine& rl) {
+    if (rl.args.size() != 1) parse_error(rl.line, rl.name + " expects one probability");
+    double p = rl.args[0];
+    if (!(p >= 0.0 && p <= 1.0)) parse_error(rl.line, "probability must be between 0 and 1");
+    return p;
+  };
+  auto opt_prob = [](const RawLine& rl) {
+    if (rl.args.empty()) return 0.0;
+    if (rl.args.size() != 1) parse_error(rl.line, rl.name + " expects at most one probability");
+    double p = rl.args[0];
+    if (!(p >= 0.0 && p <= 1.0)) parse_error(rl.line, "probability must be between 0 and 1");
+    return p;
+  };
+  auto qubits_of = [&](const RawLine& rl) {
+    std::vector<int> qs;
+    for (const auto& t : rl.targets) qs.push_back(qubit_target(rl, t));
+    return qs;
+  };
+  auto records_of = [&](const RawLine& rl) {
+    std::vector<int> recs;
+    for (const auto& t : rl.targets) {
+      if (t.size() < 6 || t.compare(0, 4, "rec[") != 0 || t.back() != ']')
+        parse_error(rl.line, "malformed record target '" + t + "'");
+      long v;
+      if (!parse_int(t.substr(4, t.size() - 5), v) || v >= 0)
+        parse_error(rl.line, "record targets must be negative lookbacks");
+      long r = long(c.nrecords) + 1 + v;
+      if (r <= 0) parse_error(rl.line, "record lookback exceeds the measurement record");
+      recs.push_back(int(r));
+    }
+    return recs;
+  };
+  auto products_of = [&](const RawLine& rl) {
+    std::vector<Product> prods;
+    for (const auto& t : rl.targets) {
+      Product pr;
+      pr.pauli = Pauli(n);
+      std::stringstream fs(t);
+      std::string f;
+      while (std::getline(fs, f, '*')) {
+        if (f.empty()) parse_error(rl.line, "malformed Pauli product");
+        if (f[0] == '!') { pr.inverted = !pr.inverted; f = f.substr(1); }
+        if (f.empty()) parse_error(rl.line, "malformed Pauli product");
+        char axis = char(std::toupper(static_cast<unsigned char>(f[0])));
+        long q;
+        if (!parse_int(f.substr(1), q) || q < 0) parse_error(rl.line, "malformed Pauli target");
+        Pauli fac = axis == 'X' ? pX(n, int(q)) : axis == 'Y' ? pY(n, int(q)) : axis == 'Z' ? pZ(n, int(q)) : Pauli();
+        if (fac.n == 0) parse_error(rl.line, "unknown Pauli axis");
+        mul_into(pr.pauli, fac);
+      }
+      prods.push_back(pr);
+    }
+    return prods;
+  };
+
+  static const std::map<std::string, K> k1 = {
+      {"H", K::H}, {"H_XZ", K::H}, {"S", K::S}, {"SQRT_Z", K::S}, {"S_DAG", K::SDag},
+      {"SQRT_Z_DAG", K::SDag}, {"SQRT_X", K::SqrtX}, {"SQRT_X_DAG", K::SqrtXDag},
+      {"SQRT_Y", K::SqrtY}, {"SQRT_Y_DAG", K::SqrtYDag}, {"X", K::X}, {"Y", K::Y}, {"Z", K::Z},
+  };
+  static const std::map<std::string, K> k2 = {
+      {"CX", K::CX}, {"CNOT", K::CX}, {"ZCX", K::CX}, {"CY", K::CY}, {"ZCY", K::CY},
+      {"CZ", K::CZ}, {"ZCZ", K::CZ}, {"SWAP", K::Swap},
+  };
+
+  for (const auto& rl : raw) {
+    const std::string& op = rl.name;
+    CInstr ci;
+    ci.line = rl.line;
+    if (op == "TICK" || op == "QUBIT_COORDS" || op == "SHIFT_COORDS" || op == "I") {
+      continue;
+    } else if (k1.count(op)) {
+      need_no_args(rl);
+      ci.kind = k1.at(op);
+      ci.qubits = qubits_of(rl);
+    } else if (k2.count(op) && std::any_of(rl.targets.begin(), rl.targets.end(), [](const std::string& t) {
+                 return t.compare(0, 4, "rec[") == 0;
+               })) {
+      // Classically controlled Paulis (src/circuit/parser.rs:1229-1305): one
+      // instruction per (record, qubit) pair.
+      need_no_args(rl);
+      if (rl.targets.size() % 2) parse_error(rl.line, "controlled two-qubit gate requires paired targets");
+      K kind = k2.at(op);
+      for (size_t i = 0; i < rl.targets.size(); i += 2) {
+        std::string a = rl.targets[i], b = rl.targets[i + 1];
+        bool ra = a.compare(0, 4, "rec[") == 0, rb = b.compare(0, 4, "rec[") == 0;
+        CInstr fi;
+        fi.line = rl.line;
+        if (kind == K::CZ && (ra || rb)) {
+          if (!ra) std::swap(a, b);
+          fi.kind = K::FeedbackZ;
+        } else if ((kind == K::CX || kind == K::CY) && ra) {
+          fi.kind = kind == K::CY ? K::FeedbackY : K::FeedbackX;
+        } else if (ra || rb) {
+          parse_error(rl.line, "unsupported classically controlled gate");
+        } else {
+          fi.kind = kind;
+          fi.qubits = {qubit_target(rl, a), qubit_target(rl, b)};
+          c.instrs.push_back(std::move(fi));
+          continue;
+        }
+        RawLine one = rl;
+        one.targets = {a};
+        fi.feedback_record = records_of(one)[0];
+        fi.qubits = {qubit_target(rl, b)};
+        c.instrs.push_back(std::move(fi));
+      }
+      continue;
+    } else if (k2.count(op)) {
+      need_no_args(rl);
+      ci.kind = k2.at(op);
+      ci.qubits = qubits_of(rl);
+      if (ci.qubits.size() % 2) parse_error(rl.line, "two-qubit Clifford requires paired targets");
+      for (size_t i = 0; i < ci.qubits.size(); i += 2)
+        if (ci.qubits[i] == ci.qubits[i + 1]) parse_error(rl.line, "two-qubit gate requires distinct qubits");
+    } else if (op == "T" || op == "T_DAG") {
+      need_no_args(rl);
+      ci.kind = op == "T" ? K::T : K::TDag;
+      ci.qubits = qubits_of(rl);
+    } else if (op == "R_X" || op == "R_Y" || op == "R_Z") {
+      if (rl.args.size() != 1) parse_error(rl.line, op + " expects one angle argument");
+      ci.kind = K::Rot;
+      ci.angle = rl.args[0] * M_PI / 2.0;  // half turns -> kernel angle
+      for (int q : qubits_of(rl)) {
+        Product pr;
+        pr.pauli = op == "R_X" ? pX(n, q) : op == "R_Y" ? pY(n, q) : pZ(n, q);
+        ci.products.push_back(pr);
+      }
+    } else if (op == "M" || op == "MZ" || op == "MX" || op == "MY" || op == "MR" || op == "MRZ" ||
+               op == "MRX" || op == "MRY") {
+      ci.probability = opt_prob(rl);
+      ci.kind = (op == "MX") ? K::MX : (op == "MY") ? K::MY : (op == "M" || op == "MZ") ? K::MZ
+              : (op == "MRX") ? K::MRX : (op == "MRY") ? K::MRY : K::MRZ;
+      for (const auto& t : rl.targets) {
+        MTarget mt;
+        std::string s = t;
+        if (!s.empty() && s[0] == '!') { mt.inverted = true; s = s.substr(1); }
+        mt.qubit = qubit_target(rl, s);
+        ci.mtargets.push_back(mt);
+      }
+      c.nrecords += int(ci.mtargets.size());
+    } else if (op == "R" || op == "RZ" || op == "RX" || op == "RY") {
+      need_no_args(rl);
+      ci.kind = op == "RX" ? K::RX : op == "RY" ? K::RY : K::RZ;
+      ci.qubits = qubits_of(rl);
+    } else if (op == "MPP") {
+      ci.kind = K::MPP;
+      ci.probability = opt_prob(rl);
+      ci.products = products_of(rl);
+      c.nrecords += int(ci.products.size());
+    } else if (op == "X_ERROR" || op == "Y_ERROR" || op == "Z_ERROR") {
+      ci.kind = op == "X_ERROR" ? K::XErr : op == "Y_ERROR" ? K::YErr : K::ZErr;
+      ci.probability = one_prob(rl);
+      ci.qubits = qubits_of(rl);
+    } else if (op == "DEPOLARIZE1") {
+      ci.kind = K::Dep1;
+      ci.probability = one_prob(rl);
+      ci.qubits = qubits_of(rl);
+    } else if (op == "DEPOLARIZE2") {
+      ci.kind = K::Dep2;
+      ci.probability = one_prob(rl);
+      ci.qubits = qubits_of(rl);
+      if (ci.qubits.size() % 2) parse_error(rl.line, "DEPOLARIZE2 requires paired targets");
+    } else if (op == "DETECTOR") {
+      CDetector d;
+      d.records = records_of(rl);
+      d.after_instruction = c.instrs.size();
+      c.detectors.push_back(d);
+      continue;
+    } else if (op == "OBSERVABLE_INCLUDE") {
+      if (rl.args.size() != 1 || rl.args[0] < 0 || rl.args[0] != std::floor(rl.args[0]))
+        parse_error(rl.line, "OBSERVABLE_INCLUDE expects one nonnegative integer index");
+      CObservable o;
+      o.index = int(rl.args[0]);
+      o.records = records_of(rl);
+      c.observables.push_back(o);
+      continue;
+    } else {
+      throw TicitError(3, "line " + std::to_string(rl.line) + ": unsupported instruction '" + op +
+                              "' for the hip backend");
+    }
+    c.instrs.push_back(std::move(ci));
+  }
+  return c;
+}
+
+// ============================================================================
+// Lowering into a frame-factored state (src/circuit/lowering.rs)
+// ============================================================================
+
+struct PendingOp {
+  int kind = 0;  // 0 rotation, 1 measurement, 2 classical record
+  Pauli pauli;
+  SymBool sign;
+  double angle = 0.0;
+  int32_t record = 0;
+  int32_t record_condition = 0;
+  SymBool outcome;
+  int32_t max_condition() const {
+    if (kind == 0) return sign.max_condition();
+    if (kind == 1) return std::max(sign.max_condition(), record_condition);
+    return std::max(outcome.max_condition(), record_condition);
+  }
+};
+
+struct Lowering {
+  int n;
+  Clifford cliff;
+  ActiveFrame frame;
+  Context ctx;
+  std::vector<PendingOp> pending;
+  std::vector<SymBool> records;  // record symbol per 1-based record
+
+  explicit Lowering(int nq) : n(nq), cliff(nq), frame(nq) {}
+
+  // src/factored.rs:507-514
+  std::pair<Pauli, SymBool> prepare(const Pauli& p) {
+    Pauli pre = cliff.preimage(p);
+    SymBool s = frame.conjugate(pre);
+    return {pre, s};
+  }
+  // src/factored.rs:517-536
+  void apply_pauli(const Pauli& p, int32_t cond) {
+    ctx.bump(cond);
+    Pauli pre = cliff.preimage(p);
+    if (pre.nonidentity()) {
+      frame.add(pre, cond);
+      ctx.bump(cond);
+    }
+  }
+  // src/factored.rs:540-555
+  void apply_pauli_symbolic(const Pauli& p, const SymBool& c) {
+    if (c.constant) apply_pauli(p, ctx.fresh_bern(1.0));
+    for (int32_t id : c.conds) apply_pauli(p, id);
+  }
+  void rotation(const Pauli& p, double angle) {
+    PendingOp op;
+    op.kind = 0;
+    op.angle = angle;
+    auto pr = prepare(p);
+    op.pauli = pr.first;
+    op.sign = pr.second;
+    pending.push_back(std::move(op));
+  }
+  void measurement(const Pauli& p, const SymBool& sign, int32_t record, int32_t rc) {
+    auto pr = prepare(p);
+    PendingOp op;
+    op.kind = 1;
+    op.pauli = pr.first;
+    op.sign = sym_xor(pr.second, sign);
+    op.record = record;
+    op.record_condition = rc;
+    pending.push_back(std::move(op));
+  }
+  SymBool readout_sign(bool inverted, double p) {
+    SymBool s;
+    s.constant = inverted;
+    if (p != 0.0) s = sym_xor(s, sym_var(ctx.fresh_bern(p)));
+    return s;
+  }
+  std::pair<int32_t, int32_t> reserve_record() {
+    int32_t c = ctx.fresh();
+    records.push_back(sym_var(c));
+    return {int32_t(records.size()), c};
+  }
+
+  void apply(const CInstr& ci) {
+    switch (ci.kind) {
+      case K::Tick: return;
+      case K::H: for (int q : ci.qubits) cliff.h(q); return;
+      case K::S: for (int q : ci.qubits) cliff.s(q); return;
+      case K::SDag: for (int q : ci.qubits) cliff.sdg(q); return;
+      case K::SqrtX: for (int q : ci.qubits) cliff.sqrt_x(q); return;
+      case K::SqrtXDag: for (int q : ci.qubits) cliff.sqrt_x_dag(q); return;
+      case K::SqrtY: for (int q : ci.qubits) cliff.sqrt_y(q); return;
+      case K::SqrtYDag: for (int q : ci.qubits) cliff.sqrt_y_dag(q); return;
+      case K::X: for (int q : ci.qubits) cliff.gx(q); return;
+      case K::Y: for (int q : ci.qubits) cliff.gy(q); return;
+      case K::Z: for (int q : ci.qubits) cliff.gz(q); return;
+      case K::CX: for (size_t i = 0; i < ci.qubits.size(); i += 2) cliff.cx(ci.qubits[i], ci.qubits[i + 1]); return;
+      case K::CY: for (size_t i = 0; i < ci.qubits.size(); i += 2) cliff.cy(ci.qubits[i], ci.qubits[i + 1]); return;
+      case K::CZ: for (size_t i = 0; i < ci.qubits.size(); i += 2) cliff.cz(ci.qubits[i], ci.qubits[i + 1]); return;
+      case K::Swap: for (size_t i = 0; i < ci.qubits.size(); i += 2) cliff.swap(ci.qubits[i], ci.qubits[i + 1]); return;
+      case K::T:
+      case K::TDag: {
+        // T = exp(-i pi/8 Z) up to global phase (src/circuit/lowering.rs:573-584)
+        double a = ci.kind == K::T ? M_PI / 8.0 : -M_PI / 8.0;
+        for (int q : ci.qubits) rotation(pZ(n, q), a);
+        return;
+      }
+      case K::Rot:
+        for (const auto& pr : ci.products) {
+          Pauli p = pr.pauli;
+          if (pr.inverted) p.set_phase(p.phase + 2);
+          rotation(p, ci.angle);
+        }
+        return;
+      case K::MZ:
+      case K::MX:
+      case K::MY:
+        for (const auto& t : ci.mtargets) {
+          SymBool sign = readout_sign(t.inverted, ci.probability);
+          auto rr = reserve_record();
+          Pauli p = ci.kind == K::MX ? pX(n, t.qubit) : ci.kind == K::MY ? pY(n, t.qubit) : pZ(n, t.qubit);
+          measurement(p, sign, rr.first, rr.second);
+        }
+        return;
+      case K::MRZ:
+      case K::MRX:
+      case K::MRY:
+        // src/circuit/lowering.rs:400-422
+        for (const auto& t : ci.mtargets) {
+          Pauli m = ci.kind == K::MRX ? pX(n, t.qubit) : ci.kind == K::MRY ? pY(n, t.qubit) : pZ(n, t.qubit);
+          Pauli corr = ci.kind == K::MRX ? pZ(n, t.qubit) : pX(n, t.qubit);
+          SymBool sign = readout_sign(t.inverted, ci.probability);
+          auto rr = reserve_record();
+          measurement(m, sign, rr.first, rr.second);
+          apply_pauli_symbolic(corr, sym_xor(sym_var(rr.second), sign));
+        }
+        return;
+      case K::RZ:
+      case K::RX:
+      case K::RY:
+        // src/circuit/lowering.rs:384-398
+        for (int q : ci.qubits) {
+          Pauli m = ci.kind == K::RX ? pX(n, q) : ci.kind == K::RY ? pY(n, q) : pZ(n, q);
+          Pauli corr = ci.kind == K::RX ? pZ(n, q) : pX(n, q);
+          int32_t cond = ctx.fresh();
+          measurement(m, SymBool{}, 0, cond);
+          apply_pauli(corr, cond);
+        }
+        return;
+      case K::MPP:
+        for (const auto& pr : ci.products) {
+          SymBool sign = readout_sign(pr.inverted, ci.probability);
+          auto rr = reserve_record();
+          measurement(pr.pauli, sign, rr.first, rr.second);
+        }
+        return;
+      case K::XErr:
+      case K::YErr:
+      case K::ZErr:
+        for (int q : ci.qubits) {
+          int32_t flip = ctx.fresh_bern(Context::check_p(ci.probability));
+          Pauli p = ci.kind == K::XErr ? pX(n, q) : ci.kind == K::YErr ? pY(n, q) : pZ(n, q);
+          apply_pauli_symbolic(p, sym_var(flip));
+        }
+        return;
+      case K::FeedbackX:
+      case K::FeedbackY:
+      case K::FeedbackZ: {
+        // src/circuit/lowering.rs:370-382
+        int q = ci.qubits[0];
+        Pauli p = ci.kind == K::FeedbackX ? pX(n, q) : ci.kind == K::FeedbackY ? pY(n, q) : pZ(n, q);
+        if (ci.feedback_record <= 0 || ci.feedback_record > int(records.size()))
+          throw TicitError(1, "measurement record target out of range");
+        apply_pauli_symbolic(p, records[ci.feedback_record - 1]);
+        return;
+      }
+      case K::Dep1:
+        // src/circuit/lowering.rs:143-159: bits (x, z), rows I, Z, X, Y
+        for (int q : ci.qubits) {
+          double p = Context::check_p(ci.probability);
+          double third = p / 3.0;
+          auto bits = ctx.fresh_cat(2, {0, 2, 1, 3}, {1.0 - p, third, third, third});
+          apply_pauli_symbolic(pX(n, q), sym_var(bits[0]));
+          apply_pauli_symbolic(pZ(n, q), sym_var(bits[1]));
+        }
+        return;
+      case K::Dep2:
+        // src/circuit/lowering.rs:161-193: bits (x1, z1, x2, z2)
+        for (size_t i = 0; i < ci.qubits.size(); i += 2) {
+          double p = Context::check_p(ci.probability);
+          std::vector<uint64_t> assign{0};
+          for (int x1 = 0; x1 < 2; ++x1)
+            for (int z1 = 0; z1 < 2; ++z1)
+              for (int x2 = 0; x2 < 2; ++x2)
+                for (int z2 = 0; z2 < 2; ++z2)
+                  if (x1 || z1 || x2 || z2)
+                    assign.push_back(uint64_t(x1) | uint64_t(z1) << 1 | uint64_t(x2) << 2 | uint64_t(z2) << 3);
+          std::vector<double> probs(16, p / 15.0);
+          probs[0] = 1.0 - p;
+          auto bits = ctx.fresh_cat(4, assign, probs);
+          int q1 = ci.qubits[i], q2 = ci.qubits[i + 1];
+          apply_pauli_symbolic(pX(n, q1), sym_var(bits[0]));
+          apply_pauli_symbolic(pZ(n, q1), sym_var(bits[1]));
+          apply_pauli_symbolic(pX(n, q2), sym_var(bits[2]));
+          apply_pauli_symbolic(pZ(n, q2), sym_var(bits[3]));
+        }
+        return;
+    }
+  }
+};
+
+// ============================================================================
+// Pending-queue peephole optimizer (src/pending_optimizer.rs)
+// ============================================================================
+
+struct Canon {
+  Pauli body;
+  SymBool sign;
+};
+
+Canon canonical_rotation(const PendingOp& r) {
+  if (!hermitian(r.pauli)) throw TicitError(1, "pending Pauli rotation must have a Hermitian generator");
+  Canon c;
+  c.body = r.pauli;
+  c.sign = sym_xor_const(r.sign, phase_sign(r.pauli));
+  c.body.set_phase(c.body.ycount());
+  return c;
+}
+
+bool rotation_can_cross(const Canon& rot, const PendingOp& op) {
+  if (op.kind == 2) return true;
+  return !anticommutes(rot.body, op.pauli);
+}
+
+void fuse_commuting_rotations(std::vector<PendingOp>& ops, CompileStats& st) {
+  std::vector<char> deleted(ops.size(), 0);
+  for (size_t i = 0; i < ops.size(); ++i) {
+    if (deleted[i] || ops[i].kind != 0) continue;
+    PendingOp earlier = ops[i];
+    Canon a = canonical_rotation(earlier);
+    for (size_t j = i + 1; j < ops.size(); ++j) {
+      if (deleted[j]) continue;
+      if (!rotation_can_cross(a, ops[j])) break;
+      if (ops[j].kind != 0) continue;
+      Canon b = canonical_rotation(ops[j]);
+      if (!a.body.same_body(b.body) || a.sign.conds != b.sign.conds) continue;
+      double dir = a.sign.constant == b.sign.constant ? 1.0 : -1.0;
+      double angle = ops[j].angle + dir * earlier.angle;
+      deleted[i] = 1;
+      st.fused_rotations++;
+      if (angle == 0.0) {
+        deleted[j] = 1;
+        st.cancelled_rotations++;
+      } else {
+        PendingOp fused;
+        fused.kind = 0;
+        fused.angle = angle;
+        fused.pauli = b.body;
+        fused.sign = b.sign;
+        ops[j] = fused;
+      }
+      break;
+    }
+  }
+  std::vector<PendingOp> kept;
+  kept.reserve(ops.size());
+  for (size_t i = 0; i < ops.size(); ++i)
+    if (!deleted[i]) kept.push_back(std::move(ops[i]));
+  ops = std::move(kept);
+}
+
+void move_measurements_earlier(std::vector<PendingOp>& ops, CompileStats& st, bool allow_all) {
+  for (size_t i = 1; i < ops.size(); ++i) {
+    if (ops[i].kind != 1) continue;
+    Pauli body = ops[i].pauli;
+    size_t target = i;
+    for (size_t cursor = i; cursor >= 1; --cursor) {
+      const PendingOp& prev = ops[cursor - 1];
+      if (prev.kind != 0) break;
+      if (anticommutes(body, prev.pauli)) break;
+      if (allow_all) {
+        target = cursor - 1;
+        continue;
+      }
+      if (body.same_body(prev.pauli)) {
+        target = cursor - 1;
+        break;
+      }
+    }
+    size_t cur = i;
+    while (cur > target) {
+      std::swap(ops[cur - 1], ops[cur]);
+      --cur;
+      st.measurement_left_swaps++;
+    }
+  }
+}
+
+// Returns prefix remap: new position of each preserved prefix.
+std::vector<int> optimize_pending(std::vector<PendingOp>& pending, const std::vector<size_t>& preserved,
+                                  CompileStats& st) {
+  size_t count = pending.size();
+  std::vector<int> remap(count + 1, -1);
+  remap[0] = 0;
+  std::vector<size_t> ends;
+  for (size_t p : preserved) {
+    if (p > count) throw TicitError(4, "preserved pending-operation prefix is out of range");
+    if (p > 0) ends.push_back(p);
+  }
+  ends.push_back(count);
+  std::sort(ends.begin(), ends.end());
+  ends.erase(std::unique(ends.begin(), ends.end()), ends.end());
+  std::vector<PendingOp> out;
+  out.reserve(count);
+  size_t start = 0;
+  for (size_t e : ends) {
+    std::vector<PendingOp> seg(pending.begin() + start, pending.begin() + e);
+    int before = st.fused_rotations;
+    fuse_commuting_rotations(seg, st);
+    move_measurements_earlier(seg, st, st.fused_rotations != before);
+    for (auto& op : seg) out.push_back(std::move(op));
+    remap[e] = int(out.size());
+    start = e;
+  }
+  pending = std::move(out);
+  return remap;
+}
+
+// ============================================================================
+// Planner (src/planner.rs)
+// ============================================================================
+
+size_t word_cost(const SymBool& e) {
+  size_t count = 0;
+  int prev = 0;
+  for (int32_t c : e.conds) {
+    int w = word_of(c);
+    if (count == 0 || w != prev) {
+      ++count;
+      prev = w;
+    }
+  }
+  return count;
+}
+
+bool lower_cost(const SymBool& cand, const SymBool& cur) {
+  size_t a = word_cost(cand), b = word_cost(cur);
+  if (a != b) return a < b;
+  return cand.conds.size() < cur.conds.size();
+}
+
+// src/planner.rs:185-230
+SymBool reduce_once(const SymBool& expr, const SymBool& rel) {
+  if (rel.conds.empty() && !rel.constant) return expr;
+  size_t l = 0, r = 0, size = 0, words = 0;
+  int prev = 0;
+  bool have = false, overlap = false;
+  auto add = [&](int32_t c) {
+    ++size;
+    int w = word_of(c);
+    if (!have || w != prev) {
+      ++words;
+      prev = w;
+      have = true;
+    }
+  };
+  while (l < expr.conds.size() || r < rel.conds.size()) {
+    if (r == rel.conds.size() || (l < expr.conds.size() && expr.conds[l] < rel.conds[r])) add(expr.conds[l++]);
+    else if (l == expr.conds.size() || rel.conds[r] < expr.conds[l]) add(rel.conds[r++]);
+    else { overlap = true; ++l; ++r; }
+  }
+  if (!overlap) return expr;
+  size_t cur = word_cost(expr);
+  if (words > cur || (words == cur && size >= expr.conds.size())) return expr;
+  return sym_xor(expr, rel);
+}
+
+struct RelationReducer {
+  std::vector<SymBool> rels;
+  std::unordered_map<int32_t, std::vector<size_t>> index;
+  std::unordered_map<int32_t, bool> fixed;
+  void add(const SymBool& rel) {
+    if (rel.conds.empty()) return;
+    if (rel.conds.size() == 1) {
+      fixed[rel.conds[0]] = rel.constant;
+      return;
+    }
+    size_t i = rels.size();
+    for (int32_t c : rel.conds) index[c].push_back(i);
+    rels.push_back(rel);
+  }
+  SymBool reduce(SymBool e) const {
+    for (;;) {
+      bool changed = false;
+      if (!fixed.empty()) {
+        std::vector<int32_t> rem;
+        rem.reserve(e.conds.size());
+        for (int32_t c : e.conds) {
+          auto it = fixed.find(c);
+          if (it != fixed.end()) {
+            e.constant ^= it->second;
+            changed = true;
+          } else {
+            rem.push_back(c);
+          }
+        }
+        e.conds = std::move(rem);
+      }
+      if (!e.conds.empty() && !rels.empty()) {
+        std::vector<size_t> cand;
+        for (int32_t c : e.conds) {
+          auto it = index.find(c);
+          if (it != index.end()) cand.insert(cand.end(), it->second.begin(), it->second.end());
+        }
+        std::sort(cand.begin(), cand.end());
+        cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+        for (size_t r : cand) {
+          SymBool red = reduce_once(e, rels[r]);
+          if (red != e) {
+            e = red;
+            changed = true;
+          }
+        }
+      }
+      if (!changed) return e;
+    }
+  }
+};
+
+int32_t instr_max_condition(const Instr& in) {
+  int32_t m = std::max(in.branch, in.expr.max_condition());
+  return std::max(m, in.record_condition);
+}
+
+bool instr_has_sign(const Instr& in) { return in.op == Op::Rotate || in.op == Op::Promote; }
+
+// src/planner.rs:1271-1289
+void reduce_program_expressions(std::vector<Instr>& instrs) {
+  RelationReducer red;
+  for (auto& in : instrs) {
+    in.expr = red.reduce(in.expr);
+    if (!instr_has_sign(in) && in.record_condition != 0 && in.op != Op::Detector) {
+      SymBool rel = sym_xor(sym_var(in.record_condition), in.expr);
+      if (!rel.conds.empty() || rel.constant) red.add(rel);
+    }
+  }
+}
+
+struct Planner {
+  int n;
+  int k = 0;
+  int max_k = 0;
+  Context ctx;
+  std::vector<PendingOp> queue;  // operations still to plan
+  size_t head = 0;
+  std::vector<Instr> instrs;
+  std::vector<int> prefix_idx;
+  std::unordered_map<int32_t, SymBool> subs;
+  std::vector<SymBool> rels;
+  std::vector<std::vector<int>> rel_words;
+  std::unordered_map<int32_t, std::vector<size_t>> rel_index;
+  int32_t next_record = 1;
+
+  void set_k(int nk) {
+    if (nk > n) throw TicitError(4, "active qubit count exceeds total qubit count");
+    k = nk;
+    max_k = std::max(max_k, nk);
+  }
+
+  void push(Instr in) {
+    ctx.bump(instr_max_condition(in));
+    instrs.push_back(std::move(in));
+  }
+
+  int32_t take_record(int32_t record, int32_t rc) {
+    if (record == 0 && rc != 0) return 0;
+    if (record == 0) return next_record++;
+    next_record = std::max(next_record, record + 1);
+    return record;
+  }
+
+  Pauli project_active(const Pauli& p) const {
+    Pauli out(k);
+    int y = 0;
+    for (int q = 0; q < k; ++q) {
+      bool xb = p.xb(q), zb = p.zb(q);
+      out.setx(q, xb);
+      out.setz(q, zb);
+      if (xb && zb) ++y;
+    }
+    out.set_phase(y);
+    return out;
+  }
+
+  Pauli embed_active(const Pauli& body) const {
+    Pauli out(n);
+    for (int q = 0; q < body.n; ++q) {
+      out.setx(q, body.xb(q));
+      out.setz(q, body.zb(q));
+    }
+    out.set_phase(body.phase);
+    return out;
+  }
+
+  int highest_dormant_x(const Pauli& p) const {
+    for (int q = n - 1; q >= k; --q)
+      if (p.xb(q)) return q - k;
+    return -1;
+  }
+
+  void substitute(SymBool& e) const {
+    if (e.conds.empty() || subs.empty()) return;
+    std::vector<int32_t> expanded = e.conds;
+    for (;;) {
+      bool changed = false;
+      std::vector<int32_t> next;
+      next.reserve(expanded.size());
+      for (int32_t c : expanded) {
+        auto it = subs.find(c);
+        if (it != subs.end()) {
+          changed = true;
+          e.constant ^= it->second.constant;
+          next.insert(next.end(), it->second.conds.begin(), it->second.conds.end());
+        } else {
+          next.push_back(c);
+        }
+      }
+      normalize_conditions(next);
+      expanded = std::move(next);
+      if (!changed) break;
+    }
+    e.conds = std::move(expanded);
+  }
+
+  // src/planner.rs:517-573
+  void reduce_pending(SymBool& e) const {
+    substitute(e);
+    if (e.conds.empty() || rels.empty()) return;
+    std::vector<size_t> cand;
+    for (int32_t c : e.conds) {
+      auto it = rel_index.find(c);
+      if (it != rel_index.end()) cand.insert(cand.end(), it->second.begin(), it->second.end());
+    }
+    std::sort(cand.begin(), cand.end());
+    std::vector<int> ewords;
+    for (int32_t c : e.conds) {
+      int w = word_of(c);
+      if (ewords.empty() || ewords.back() != w) ewords.push_back(w);
+    }
+    size_t s = 0;
+    while (s < cand.size()) {
+      size_t t = s + 1;
+      while (t < cand.size() && cand[t] == cand[s]) ++t;
+      size_t ri = cand[s];
+      size_t overlap = t - s;
+      const SymBool& rel = rels[ri];
+      bool can = 2 * overlap > rel.conds.size();
+      if (!can) {
+        size_t ew = 0, shared = 0, fresh = 0;
+        for (int w : rel_words[ri]) {
+          while (ew < ewords.size() && ewords[ew] < w) ++ew;
+          if (ew < ewords.size() && ewords[ew] == w) ++shared;
+          else ++fresh;
+        }
+        can = shared > fresh;
+      }
+      if (can) e = reduce_once(e, rel);
+      s = t;
+    }
+  }
+
+  // src/planner.rs:600-637
+  void note_measurement_relation(int32_t rc, const SymBool& outcome) {
+    if (rc == 0) return;
+    SymBool red = outcome;
+    substitute(red);
+    SymBool rel = sym_xor(sym_var(rc), red);
+    ctx.bump(rel.max_condition());
+    bool self_ref = std::binary_search(red.conds.begin(), red.conds.end(), rc);
+    if (!self_ref && (red.conds.size() <= 1 || lower_cost(red, sym_var(rc)))) {
+      subs[rc] = red;
+      return;
+    }
+    size_t ri = rels.size();
+    std::vector<int> words;
+    for (int32_t c : rel.conds) {
+      rel_index[c].push_back(ri);
+      int w = word_of(c);
+      if (words.empty() || words.back() != w) words.push_back(w);
+    }
+    rels.push_back(rel);
+    rel_words.push_back(words);
+  }
+
+  void push_through(const Pauli& p, const SymBool& sign) {
+    ctx.bump(sign.max_condition());
+    for (size_t i = head; i < queue.size(); ++i) {
+      PendingOp& op = queue[i];
+      if (op.kind == 2) continue;
+      if (anticommutes(p, op.pauli)) op.sign = sym_xor(op.sign, sign);
+    }
+  }
+
+  void transform_queue(const Clifford& f) {
+    for (size_t i = head; i < queue.size(); ++i) {
+      PendingOp& op = queue[i];
+      if (op.kind == 2) continue;
+      op.pauli = f.coordinates(op.pauli);
+    }
+  }
+
+  static Pauli fix_row(const Pauli& row, const Pauli& installed, const Pauli& stab) {
+    if (anticommutes(row, installed)) {
+      Pauli prod = pmul(row, stab);
+      prod.set_phase(prod.ycount());
+      return prod;
+    }
+    return row;
+  }
+
+  // src/planner.rs:697-751
+  Clifford promotion_frame(const Pauli& rot, int picked) const {
+    int old_k = k, picked_q = old_k + picked;
+    Pauli stab = pZ(n, picked_q);
+    Pauli px = positive_hermitian(rot);
+    if (!anticommutes(px, stab))
+      throw TicitError(4, "dormant rotation promotion requires an anti-commuting fixed stabilizer");
+    std::vector<Pauli> xr, zr;
+    for (int q = 0; q < old_k; ++q) {
+      zr.push_back(fix_row(pZ(n, q), px, stab));
+      xr.push_back(fix_row(pX(n, q), px, stab));
+    }
+    zr.push_back(stab);
+    xr.push_back(px);
+    for (int q = old_k; q < n; ++q) {
+      if (q == picked_q) continue;
+      zr.push_back(fix_row(pZ(n, q), px, stab));
+      xr.push_back(fix_row(pX(n, q), px, stab));
+    }
+    return Clifford::from_rows(n, std::move(xr), std::move(zr));
+  }
+
+  // src/planner.rs:758-792
+  Clifford replacement_frame(const Pauli& measured, int picked) const {
+    int picked_q = k + picked;
+    Pauli old_stab = pZ(n, picked_q);
+    Pauli new_stab = positive_hermitian(measured);
+    if (!anticommutes(new_stab, old_stab))
+      throw TicitError(4, "dormant measurement replacement requires an anti-commuting fixed stabilizer");
+    std::vector<Pauli> xr, zr;
+    for (int q = 0; q < n; ++q) {
+      if (q == picked_q) {
+        xr.push_back(old_stab);
+        zr.push_back(new_stab);
+      } else {
+        zr.push_back(fix_row(pZ(n, q), new_stab, old_stab));
+        xr.push_back(fix_row(pX(n, q), new_stab, old_stab));
+      }
+    }
+    return Clifford::from_rows(n, std::move(xr), std::move(zr));
+  }
+
+  // src/planner.rs:799-847
+  Clifford measurement_frame(const Pauli& body, const Instr& kern) const {
+    int pivot = kern.pivot;
+    Pauli measured = embed_active(body);
+    Pauli fixed_x = kern.diagonal ? pX(n, pivot) : pZ(n, pivot);
+    std::vector<Pauli> xr, zr;
+    for (int q = 0; q < k; ++q) {
+      if (q == pivot) continue;
+      Pauli zrow = pZ(n, q), xrow = pX(n, q);
+      if (kern.diagonal) {
+        if (body.zb(q)) xrow = pmul(xrow, pX(n, pivot));
+      } else {
+        if (body.xb(q)) zrow = pmul(zrow, pZ(n, pivot));
+        if (body.zb(q)) xrow = pmul(xrow, pZ(n, pivot));
+      }
+      xr.push_back(xrow);
+      zr.push_back(zrow);
+    }
+    xr.push_back(fixed_x);
+    zr.push_back(measured);
+    for (int q = k; q < n; ++q) {
+      xr.push_back(pX(n, q));
+      zr.push_back(pZ(n, q));
+    }
+    return Clifford::from_rows(n, std::move(xr), std::move(zr));
+  }
+
+  static void fill_action(Instr& in, const Pauli& body) {
+    if (body.n >= 63) throw TicitError(3, "Pauli string has too many qubits for active-state basis indexing");
+    if (!hermitian(body)) throw TicitError(4, "Pauli rotation requires P^2 == I");
+    in.nq = body.n;
+    in.xmask = body.x.empty() ? 0 : body.x[0];
+    in.zmask = body.z.empty() ? 0 : body.z[0];
+    in.phase = body.phase;
+  }
+
+  // src/active.rs:279-345
+  static void fill_measure_kernel(Instr& in, const Pauli& body) {
+    fill_action(in, body);
+    if (in.nq == 0) throw TicitError(4, "cannot build an active measurement kernel for k == 0");
+    if (in.xmask) {
+      in.pivot = 63 - __builtin_clzll(in.xmask);
+      in.diagonal = false;
+    } else if (in.zmask) {
+      in.pivot = 63 - __builtin_clzll(in.zmask);
+      in.diagonal = true;
+      if (in.phase == 2) in.diag_phase_bit = 1;
+      else if (in.phase == 0) in.diag_phase_bit = 0;
+      else throw TicitError(4, "diagonal active measurement Pauli must have real eigenvalues");
+    } else {
+      throw TicitError(4, "cannot build an active measurement kernel for identity Pauli");
+    }
+  }
+
+  void process_rotation(PendingOp op) {
+    ctx.bump(op.sign.max_condition());
+    int picked = highest_dormant_x(op.pauli);
+    if (picked < 0) {
+      Pauli body = project_active(op.pauli);
+      SymBool sign = sym_xor_const(op.sign, phase_sign(op.pauli));
+      if (!body.nonidentity()) return;  // a global phase
+      if (!hermitian(body)) throw TicitError(4, "active rotation Pauli must square to identity");
+      Instr in;
+      in.op = Op::Rotate;
+      fill_action(in, body);
+      in.angle = op.angle;
+      in.expr = sign;
+      push(std::move(in));
+      return;
+    }
+    int old_k = k;
+    Clifford f = promotion_frame(op.pauli, picked);
+    op.pauli = f.coordinates(op.pauli);
+    transform_queue(f);
+    if (!op.pauli.same_body(pX(n, old_k)))
+      throw TicitError(4, "dormant rotation tableau reduction did not expose promoted X");
+    Instr in;
+    in.op = Op::Promote;
+    in.nq = old_k;
+    in.angle = op.angle;
+    in.expr = sym_xor_const(op.sign, phase_sign(op.pauli));
+    push(std::move(in));
+    set_k(old_k + 1);
+  }
+
+  void process_measurement(PendingOp op) {
+    ctx.bump(std::max(op.sign.max_condition(), op.record_condition));
+    Pauli body = project_active(op.pauli);
+    int picked = highest_dormant_x(op.pauli);
+    if (picked >= 0) {
+      int picked_q = k + picked;
+      Clifford f = replacement_frame(op.pauli, picked);
+      op.pauli = f.coordinates(op.pauli);
+      transform_queue(f);
+      if (!op.pauli.same_body(pZ(n, picked_q)))
+        throw TicitError(4, "dormant measurement tableau reduction did not expose fixed Z");
+      SymBool base = sym_xor_const(op.sign, phase_sign(op.pauli));
+      int32_t branch = ctx.fresh();
+      push_through(pX(n, picked_q), sym_var(branch));
+      SymBool outcome = sym_xor(base, sym_var(branch));
+      Instr in;
+      in.op = Op::Coin;
+      in.branch = branch;
+      in.expr = outcome;
+      in.record = take_record(op.record, op.record_condition);
+      in.record_condition = op.record_condition;
+      push(std::move(in));
+      note_measurement_relation(op.record_condition, outcome);
+      return;
+    }
+    SymBool base = sym_xor_const(op.sign, phase_sign(op.pauli));
+    if (!body.nonidentity()) {
+      Instr in;
+      in.op = Op::Record;
+      in.expr = base;
+      in.record = take_record(op.record, op.record_condition);
+      in.record_condition = op.record_condition;
+      push(std::move(in));
+      note_measurement_relation(op.record_condition, base);
+      return;
+    }
+    if (!hermitian(body)) throw TicitError(4, "active measurement Pauli must square to identity");
+    Instr in;
+    in.op = Op::Measure;
+    fill_measure_kernel(in, body);
+    Clifford f = measurement_frame(body, in);
+    transform_queue(f);
+    int32_t branch = ctx.fresh();
+    push_through(pX(n, k - 1), sym_var(branch));
+    SymBool outcome = sym_xor(base, sym_var(branch));
+    in.branch = branch;
+    in.expr = outcome;
+    in.record = take_record(op.record, op.record_condition);
+    in.record_condition = op.record_condition;
+    push(std::move(in));
+    note_measurement_relation(op.record_condition, outcome);
+    set_k(k - 1);
+  }
+
+  void process_classical(PendingOp op) {
+    ctx.bump(std::max(op.outcome.max_condition(), op.record_condition));
+    Instr in;
+    in.op = Op::Record;
+    in.expr = op.outcome;
+    in.record = take_record(op.record, op.record_condition);
+    in.record_condition = op.record_condition;
+    push(std::move(in));
+  }
+
+  void run() {
+    while (head < queue.size()) {
+      if (prefix_idx.empty()) prefix_idx.push_back(int(instrs.size()));
+      PendingOp op = queue[head];
+      ++head;  // the planned operation leaves the queue before its effects land
+      if (op.kind == 2) reduce_pending(op.outcome);
+      else reduce_pending(op.sign);
+      if (op.kind == 0) process_rotation(std::move(op));
+      else if (op.kind == 1) process_measurement(std::move(op));
+      else process_classical(std::move(op));
+      prefix_idx.push_back(int(instrs.size()));
+    }
+  }
+};
+
+// src/planner.rs:1308-1336
+bool rare_info(const CategoricalDist& d, RareGroup& g) {
+  size_t false_row = d.assignments.size();
+  uint64_t mask = d.nbits >= 64 ? ~0ull : ((1ull << d.nbits) - 1);
+  for (size_t r = 0; r < d.assignments.size(); ++r)
+    if ((d.assignments[r] & mask) == 0) {
+      false_row = r;
+      break;
+    }
+  if (false_row == d.assignments.size()) return false;
+  double ev = 1.0 - d.probs[false_row];
+  if (!(ev < 0.02)) return false;
+  g.event_p = ev;
+  g.nbits = d.nbits;
+  g.assignments = d.assignments;
+  g.probs = d.probs;
+  if (ev > 0.0) {
+    double inv = 1.0 / ev;
+    for (size_t r = 0; r < d.probs.size(); ++r) {
+      if (r == false_row || d.probs[r] <= 0.0) continue;
+      g.event_rows.push_back(int32_t(r));
+      g.event_probs.push_back(d.probs[r] * inv);
+    }
+  }
+  return true;
+}
+
+// src/planner.rs:1409-1487 (component planning is not run: the dense path is
+// always selected).
+Program freeze(int n, int initial_k, int max_k, std::vector<Instr> instrs, Context& ctx) {
+  if (initial_k > n || max_k > n || initial_k > max_k)
+    throw TicitError(4, "invalid factored instruction program dimensions");
+  reduce_program_expressions(instrs);
+  Program p;
+  p.n = n;
+  p.initial_k = initial_k;
+  p.max_k = max_k;
+  int32_t rec = 0, det = 0, ev = 0;
+  for (auto& in : instrs) {
+    ctx.bump(instr_max_condition(in));
+    rec = std::max(rec, in.record);
+    det = std::max(det, in.detector);
+    if (in.exp_val >= 0) ev = std::max(ev, in.exp_val + 1);
+  }
+  p.instrs = std::move(instrs);
+  p.nsymbols = std::max(0, ctx.next - 1);
+  p.nrecords = rec;
+  p.ndetectors = det;
+  p.nexpvals = ev;
+  const double kLow = 0.02;
+  for (const auto& d : ctx.cats) {
+    RareGroup g;
+    if (rare_info(d, g)) {
+      bool merged = false;
+      for (auto& e : p.rare) {
+        if (e.event_p == g.event_p && e.nbits == g.nbits && e.assignments == g.assignments &&
+            e.probs == g.probs && e.event_rows == g.event_rows && e.event_probs == g.event_probs) {
+          e.sets.push_back(d.conds);
+          merged = true;
+          break;
+        }
+      }
+      if (!merged) {
+        g.sets.push_back(d.conds);
+        p.rare.push_back(std::move(g));
+      }
+    } else {
+      p.categorical.push_back(d);
+    }
+  }
+  for (const auto& kv : ctx.bern) {
+    if (kv.second < kLow) {
+      bool merged = false;
+      for (auto& g : p.lowp)
+        if (g.p == kv.second) {
+          g.conds.push_back(kv.first);
+          merged = true;
+          break;
+        }
+      if (!merged) p.lowp.push_back(BernGroup{kv.second, {kv.first}});
+    } else {
+      p.bern_conds.push_back(kv.first);
+      p.bern_probs.push_back(kv.second);
+    }
+  }
+  return p;
+}
+
+}  // namespace
+
+Program compile_circuit_text(const std::string& text, CompileStats* stats_out) {
+  CompileStats st;
+  Circuit c = parse_circuit(text);
+  st.ncircuit_instructions = int(c.instrs.size());
+  Lowering low(c.nqubits);
+  std::vector<size_t> counts{0};
+  for (const auto& ci : c.instrs) {
+    low.apply(ci);
+    counts.push_back(low.pending.size());
+  }
+  if (int(low.records.size()) != c.nrecords) throw TicitError(4, "circuit measurement record count mismatch");
+  st.npending = int(low.pending.size());
+
+  // Detector anchors are pending-operation prefixes (src/circuit/mod.rs:145-164).
+  std::vector<size_t> det_prefix;
+  for (const auto& d : c.detectors) {
+    if (d.after_instruction >= counts.size()) throw TicitError(4, "detector source position is out of range");
+    det_prefix.push_back(counts[d.after_instruction]);
+  }
+  for (const auto& op : low.pending) low.ctx.bump(op.max_condition());
+  std::vector<int> remap = optimize_pending(low.pending, det_prefix, st);
+  for (auto& pfx : det_prefix) {
+    int r = remap[pfx];
+    if (r < 0) throw TicitError(4, "detector pending-operation prefix was not preserved by optimization");
+    pfx = size_t(r);
+  }
+
+  Planner pl;
+  pl.n = c.nqubits;
+  pl.ctx = low.ctx;
+  pl.queue = std::move(low.pending);
+  pl.run();
+  Program planned = freeze(pl.n, 0, pl.max_k, std::move(pl.instrs), pl.ctx);
+
+  // Splice one detector per declaration at its planned checkpoint
+  // (src/circuit/mod.rs:224-271), then refreeze.
+  std::vector<Instr> instrs;
+  if (!c.detectors.empty()) {
+    std::vector<std::vector<Instr>> events(planned.instrs.size() + 1);
+    for (size_t i = 0; i < c.detectors.size(); ++i) {
+      Instr d;
+      d.op = Op::Detector;
+      d.detector = int32_t(i + 1);
+      for (int r : c.detectors[i].records) {
+        if (r <= 0 || r > int(low.records.size()))
+          throw TicitError(1, "detector references an out-of-range measurement record");
+        d.expr = sym_xor(d.expr, low.records[r - 1]);
+        d.det_records.push_back(r);
+      }
+      size_t pfx = det_prefix[i];
+      size_t checkpoint = 0;
+      if (!(pfx == 0 && pl.prefix_idx.empty())) {
+        if (pfx >= pl.prefix_idx.size()) throw TicitError(4, "detector pending-operation position is out of range");
+        checkpoint = size_t(pl.prefix_idx[pfx]);
+      }
+      events[checkpoint].push_back(std::move(d));
+    }
+    for (auto& e : events[0]) instrs.push_back(std::move(e));
+    for (size_t i = 0; i < planned.instrs.size(); ++i) {
+      instrs.push_back(std::move(planned.instrs[i]));
+      for (auto& e : events[i + 1]) instrs.push_back(std::move(e));
+    }
+    planned = freeze(pl.n, 0, planned.max_k, std::move(instrs), pl.ctx);
+  }
+
+  int nobs = 0;
+  for (const auto& o : c.observables) nobs = std::max(nobs, o.index + 1);
+  planned.observables.assign(nobs, {});
+  for (const auto& o : c.observables)
+    for (int r : o.records) planned.observables[o.index].push_back(r);
+  if (stats_out) *stats_out = st;
+  return planned;
+}
+
+}  // namespace ticit

@@ -1,0 +1,165 @@
+// Flat device tables for one planned program. Shared by the host lowering
+// (host/device_program.cpp) and the HIP kernels; plain structs only.
+//
+// Instruction semantics follow ticit's six opcodes (src/factored.rs:284-292);
+// the precomputed constants are those of PrecomputedActivePauliRotationKernel
+// (src/active.rs:162-230) and PrecomputedActivePauliMeasurementKernel
+// (src/active.rs:242-377), computed once on the host so device and CPU use
+// bit-identical cos/sin values.
+#pragma once
+
+#include <cstdint>
+
+namespace ticit {
+
+enum : uint32_t {
+  kOpRotate = 0,
+  kOpPromote = 1,
+  kOpRecord = 2,
+  kOpDetector = 3,
+  kOpMeasure = 4,
+  kOpCoin = 5,
+};
+
+enum : uint32_t {
+  kFlagDiagonal = 1u << 8,      // rotation with xmask == 0, or diagonal measurement
+  kFlagUniformImag = 1u << 9,   // rotation pairs with zmask == 0
+  kFlagXzOdd = 1u << 10,        // popcount(xmask & zmask) odd
+  kFlagConstant = 1u << 11,     // expression constant
+  kFlagDiagPhase = 1u << 12,    // diagonal measurement of a -1 coefficient body
+  kFlagDetRecords = 1u << 13,   // detector reads record bits, not its expression
+};
+
+struct alignas(16) DevInstr {
+  uint32_t op;          // opcode | flags
+  int32_t nq;           // active width the instruction acts on
+  int32_t partial;      // exogenous partial index, -1 = none
+  uint32_t resid_off;   // runtime-slot list of the expression residual
+  uint32_t resid_len;
+  int32_t record;       // 0-based record written, -1 = none
+  int32_t rc_slot;      // runtime slot assigned the outcome, -1 = none
+  int32_t branch_slot;  // runtime slot of the minted branch symbol, -1 = none
+  int32_t detector;     // 0-based detector written, -1 = none
+  uint32_t det_off;     // detector record list (0-based)
+  uint32_t det_len;
+  int32_t pivot;        // measurement pivot / rotation pair bit
+  int32_t ordinal;      // Measure: measurement ordinal; Coin: coin ordinal
+  int32_t pad0;
+  uint64_t xmask;
+  uint64_t zmask;
+  uint64_t zwop;        // z mask without the pivot bit (diagonal measurement)
+  double c;             // cos(angle)
+  double s;             // sin(angle)
+  double me_re;         // rotation coefficient -i sin(phi) i^phase
+  double me_im;
+  double c1_re;         // measurement partner weight conj(i^phase)/sqrt(2)
+  double c1_im;
+};
+
+// Exogenous noise plan (src/factored.rs:444-462, src/planner.rs:1455-1480).
+struct DevCat {       // categorical distribution drawn once per shot
+  uint32_t nbits, nrows;
+  uint32_t prob_off;  // f64 pool
+  uint32_t assign_off;  // u64 pool
+  uint32_t cond_off;  // i32 pool
+  uint32_t pad;
+};
+
+struct DevRare {      // rare categorical group walked by geometric gaps
+  double denom;       // ln(1 - event_p)
+  uint32_t enabled;   // event_p > 0
+  uint32_t nsets, nbits, nev;
+  uint32_t cond_off;  // i32 pool: nsets * nbits conditions
+  uint32_t assign_off;  // u64 pool: one assignment per row
+  uint32_t ev_row_off;  // i32 pool
+  uint32_t ev_prob_off;  // f64 pool
+};
+
+enum : uint32_t {
+  kBernNone = 0,   // p <= 0: never set
+  kBernAll = 1,    // p >= 1: always set
+  kBernHalf = 2,   // fair coin word
+  kBernLow = 3,    // geometric skip
+  kBernCoins = 4,  // 8 coin flips plus geometric leftover
+};
+
+struct DevBern {      // src/sampler/exogenous.rs:185-266 at one shot per stream
+  int32_t cond;
+  uint32_t mode;
+  uint32_t invert;
+  uint32_t top_bits;
+  double denom;       // kBernLow: ln(1 - p)
+  double left_denom;  // kBernCoins leftover: ln(1 - p'), 0 when no leftover
+};
+
+struct DevLowp {      // low-probability Bernoulli group
+  double denom;
+  uint32_t enabled;
+  uint32_t nconds;
+  uint32_t cond_off;
+  uint32_t pad;
+};
+
+// Device pointers of one uploaded program.
+struct DevProgram {
+  const DevInstr* instrs;
+  const uint32_t* resid;      // runtime slot ids
+  const uint32_t* det_recs;   // 0-based record ids
+  const uint32_t* obs_off;    // nobs + 1 offsets into obs_recs
+  const uint32_t* obs_recs;
+  const uint32_t* sym_off;    // nsym + 1 offsets: symbol (c - 1) -> partial ids
+  const uint32_t* sym_part;
+  const int32_t* exo_syms;    // every exogenous symbol id (injected mode scan)
+  const DevCat* cats;
+  const DevRare* rares;
+  const DevBern* berns;
+  const DevLowp* lowps;
+  const double* f64pool;
+  const uint64_t* u64pool;
+  const int32_t* i32pool;
+  int32_t ninstr, n, initial_k, max_k;
+  int32_t nrec, ndet, nobs, nslots;
+  int32_t npartials, nsym, nexo, nmeasure;
+  int32_t ncoin, ncat, nrare, nbern;
+  int32_t nlowp, pad0, pad1, pad2;
+};
+
+// Per-launch sampling request.
+struct DevLaunch {
+  uint64_t shots;
+  uint64_t seed;
+  uint64_t first_shot;     // global index of shot 0 of this launch
+  int32_t bit_packed;
+  int32_t keep_records;
+  int32_t injected;
+  int32_t want_probs;
+  // outputs (device)
+  uint8_t* meas;           // [shots][meas_row]
+  uint8_t* det;
+  uint8_t* obs;
+  unsigned long long* obs_ones;  // [nobs]
+  int* error;              // nonzero on a device-side failure
+  double* probs;           // optional [nmeasure][shots]
+  // injected inputs (device)
+  const uint64_t* exo;     // [nsym][ceil(shots/64)]
+  const double* uniforms;  // [nmeasure][shots]
+  const uint64_t* coins;   // [ncoin][ceil(shots/64)]
+  uint32_t meas_row, det_row, obs_row, pad;
+};
+
+// LDS table sizes of the two resident tiers (words of bits per shot).
+struct LaneLayout {
+  uint32_t pw, sw, rw, dw;  // partial, runtime-slot, record, detector words
+  uint32_t stage_bytes;     // per-wave row staging
+};
+
+struct GroupLayout {
+  uint32_t pw, sw, rw, dw;
+};
+
+enum : int {
+  kDevErrImpossibleBranch = 1,
+  kDevErrBadProbability = 2,
+};
+
+}  // namespace ticit

@@ -1,0 +1,164 @@
+// Device helpers shared by the sampling kernels: SplitMix64 streams, the
+// exogenous noise walk of one shot, and instruction-expression evaluation.
+//
+// Randomness contract: shot s of a sampling call with `seed` draws its noise
+// from the stream block_seed(0x7eed0000, seed, s) and its measurement
+// branches and dormant coins from block_seed(0x5eed1234, seed, s). That is
+// exactly what ticit's prepared sampler does when run with one-shot chunks
+// and one-shot blocks (src/sampler/prepared.rs:753-778), so every shot is an
+// independent stream and results do not depend on the launch shape.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device_abi.hpp"
+
+namespace ticit {
+
+constexpr uint64_t kExoSeedBase = 0x7eed0000ull;
+constexpr uint64_t kBranchSeedBase = 0x5eed1234ull;
+constexpr double kInvSqrt2 = 0.70710678118654752440;
+
+// src/random.rs:13-19
+__device__ __forceinline__ uint64_t sm_next(uint64_t& s) {
+  s += 0x9e3779b97f4a7c15ull;
+  uint64_t z = s;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+// src/random.rs:22-24: 53 random bits times 2^-53
+__device__ __forceinline__ double sm_uniform(uint64_t& s) {
+  return double(sm_next(s) >> 11) * 0x1.0p-53;
+}
+
+// src/random.rs:77-80
+__device__ __forceinline__ uint64_t seed_for(uint64_t base, uint64_t seed, uint64_t block) {
+  return base ^ (0x9e3779b97f4a7c15ull * (block + 1)) ^ (0xbf58476d1ce4e5b9ull * (seed + 1));
+}
+
+// src/random.rs:64-73: one draw per gap, clamped to i32::MAX
+__device__ __forceinline__ double sample_gap(uint64_t& s, double denom) {
+  double u = sm_uniform(s);
+  u = u < 2.2250738585072014e-308 ? 2.2250738585072014e-308 : u;
+  double gap = floor(log(u) / denom);
+  if (!isfinite(gap) || gap >= 2147483647.0) return 2147483647.0;
+  return gap;
+}
+
+// src/random.rs:40-50: inclusive cumulative walk, falls through to the last row
+__device__ __forceinline__ uint32_t categorical_row(uint64_t& s, const double* probs, uint32_t n) {
+  double r = sm_uniform(s);
+  double cum = 0.0;
+  for (uint32_t i = 0; i < n; ++i) {
+    cum += probs[i];
+    if (r <= cum) return i;
+  }
+  return n - 1;
+}
+
+// or_low_probability_bits over a single shot (src/sampler/exogenous.rs:150-175):
+// a miss costs one draw, a hit two (the second one ends the walk).
+__device__ __forceinline__ bool low_probability_bit(uint64_t& s, double denom) {
+  double gap = sample_gap(s, denom);
+  if (gap >= 1.0) return false;
+  (void)sample_gap(s, denom);
+  return true;
+}
+
+// Walks one shot's exogenous noise in the reference family order
+// (src/sampler/exogenous.rs:536-625) and calls toggle(c) for every symbol c
+// that comes out true (XOR semantics: a symbol toggled twice is false).
+template <class Toggle>
+__device__ void sample_exogenous(const DevProgram& P, uint64_t rng, Toggle&& toggle) {
+  for (int i = 0; i < P.ncat; ++i) {
+    const DevCat d = P.cats[i];
+    uint32_t row = categorical_row(rng, P.f64pool + d.prob_off, d.nrows);
+    uint64_t a = P.u64pool[d.assign_off + row];
+    for (uint32_t b = 0; b < d.nbits; ++b)
+      if ((a >> b) & 1) toggle(P.i32pool[d.cond_off + b]);
+  }
+  for (int i = 0; i < P.nrare; ++i) {
+    const DevRare g = P.rares[i];
+    if (!g.enabled) continue;
+    const int64_t total = int64_t(g.nsets);
+    int64_t draw = 0;
+    for (;;) {
+      int64_t gap = int64_t(sample_gap(rng, g.denom));
+      if (gap >= total - draw) break;
+      draw += gap;
+      uint32_t set = uint32_t(draw);  // one shot per stream: draw / nsets == 0
+      uint32_t er = categorical_row(rng, P.f64pool + g.ev_prob_off, g.nev);
+      uint32_t row = uint32_t(P.i32pool[g.ev_row_off + er]);
+      uint64_t a = P.u64pool[g.assign_off + row];
+      const int32_t* conds = P.i32pool + g.cond_off + size_t(set) * g.nbits;
+      for (uint32_t b = 0; b < g.nbits; ++b)
+        if ((a >> b) & 1) toggle(conds[b]);
+      draw += 1;
+    }
+  }
+  for (int i = 0; i < P.nbern; ++i) {
+    const DevBern b = P.berns[i];
+    bool bit = false;
+    switch (b.mode) {
+      case kBernNone:
+        continue;
+      case kBernAll:
+        bit = true;
+        break;
+      case kBernHalf:
+        bit = sm_next(rng) & 1;
+        break;
+      case kBernLow:
+        bit = low_probability_bit(rng, b.denom);
+        break;
+      default: {  // kBernCoins: src/sampler/exogenous.rs:225-257
+        uint64_t alive = sm_next(rng);
+        uint64_t result = 0;
+        for (int k = 6; k >= 0; --k) {
+          uint64_t shoot = sm_next(rng);
+          if ((b.top_bits >> k) & 1) result |= shoot & alive;
+          alive &= ~shoot;
+        }
+        bit = result & 1;
+        if (b.left_denom != 0.0) bit = low_probability_bit(rng, b.left_denom) || bit;
+        break;
+      }
+    }
+    if (b.invert) bit = !bit;
+    if (bit) toggle(b.cond);
+  }
+  for (int i = 0; i < P.nlowp; ++i) {
+    const DevLowp g = P.lowps[i];
+    if (!g.enabled) continue;
+    const int64_t total = int64_t(g.nconds);
+    int64_t draw = 0;
+    for (;;) {
+      int64_t gap = int64_t(sample_gap(rng, g.denom));
+      if (gap >= total - draw) break;
+      draw += gap;
+      toggle(P.i32pool[g.cond_off + uint32_t(draw)]);
+      draw += 1;
+    }
+  }
+}
+
+__device__ __forceinline__ size_t insert_zero_bit(size_t packed, int bit) {
+  size_t low = (size_t(1) << bit) - 1;
+  return (packed & low) | ((packed & ~low) << 1);
+}
+
+__device__ __forceinline__ bool parity64(uint64_t v) { return __popcll(v) & 1; }
+
+// PrecomputedActivePauliMeasurementKernel::diagonal_source (src/active.rs:347-356)
+__device__ __forceinline__ size_t diagonal_source(const DevInstr& in, size_t packed, bool branch) {
+  size_t wo = insert_zero_bit(packed, in.pivot);
+  int par = in.zwop == 0 ? 0 : int(parity64(uint64_t(wo) & in.zwop));
+  int pv = ((in.op & kFlagDiagPhase) ? 1 : 0) ^ par ^ int(branch);
+  return wo | (size_t(pv) << in.pivot);
+}
+
+__device__ __forceinline__ double clamp01(double v) { return v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v); }
+
+}  // namespace ticit
