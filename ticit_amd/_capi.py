@@ -68,6 +68,14 @@ def load() -> ctypes.CDLL:
     path = library_path()
     if not os.path.exists(path):
         raise RuntimeError(f"HIP backend library missing: {path} (run ticit_amd/build.py)")
+    # torch ships its own libamdhip64.so.7. Importing it first makes the
+    # dynamic linker resolve this library's libamdhip64.so.7 dependency to
+    # the runtime already mapped, so the process holds one HIP runtime and
+    # torch-allocated device buffers and streams are valid here.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(path)
     lib.ticit_hip_last_error.restype = ctypes.c_char_p
     lib.ticit_hip_plan_text.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(_u64p),
