@@ -45,7 +45,7 @@ typedef struct ticit_hip_options {
   uint32_t precision;    /* 0: FP64 amplitudes; 1: FP32 amplitudes, FP64 reductions */
   uint32_t planner_mode; /* 0: heuristic tier choice; 1: learned planner */
   uint32_t tier;         /* 0: automatic; 1: lane tier; 2: group tier; 3: wide tier */
-  uint32_t reserved[4];
+  uint32_t reserved[4];  /* reserved[0]: cap on wide-tier shots in flight (0 = memory budget) */
 } ticit_hip_options;
 
 /* Mirrors SampleResult (src/sampler/prepared.rs:197-219). Buffers are owned
@@ -73,6 +73,8 @@ typedef struct ticit_hip_program_info {
   uint64_t measure_instructions, coin_instructions;
   uint32_t tier;      /* tier the sampler runs on */
   uint32_t precision;
+  uint32_t wide_batch; /* wide tier: shots in flight per pass */
+  uint32_t reserved0;
 } ticit_hip_program_info;
 
 /* Thread-local message of the last failing call on this thread. */
@@ -123,6 +125,14 @@ ticit_hip_status ticit_hip_sample_device(ticit_hip_program* program, uint64_t sh
 ticit_hip_status ticit_hip_sample_injected(ticit_hip_program* program, uint64_t shots, const uint64_t* exo,
                                            const double* uniforms, const uint64_t* coins, int bit_packed,
                                            ticit_hip_result* out, double* probs);
+
+/* Injected sampling that also returns shot 0's final active state as FP64
+ * (re/im must hold 2^max_active_qubits entries; *k receives its width).
+ * Served by the wide tier; used to report amplitude error against the FP64
+ * oracle. */
+ticit_hip_status ticit_hip_final_state(ticit_hip_program* program, uint64_t shots, const uint64_t* exo,
+                                       const double* uniforms, const uint64_t* coins, double* re, double* im,
+                                       int* k);
 
 /* Batched sampling of many programs in one submission (configuration C2). */
 ticit_hip_status ticit_hip_sample_batch(ticit_hip_program* const* programs, const uint64_t* shots,

@@ -52,6 +52,7 @@ def _load(name: str) -> ctypes.CDLL:
     lib.oracle_bernoulli.argtypes = [_u64p, ctypes.c_double, ctypes.POINTER(ctypes.c_int)]
     lib.oracle_insert_zero_bit.argtypes = [ctypes.c_size_t, ctypes.c_int]
     lib.oracle_insert_zero_bit.restype = ctypes.c_size_t
+    lib.oracle_set_fp32_storage.argtypes = [ctypes.c_int]
     lib.oracle_default_batch.argtypes = [ctypes.c_int]
     lib.oracle_default_batch.restype = ctypes.c_int64
     return lib
@@ -150,6 +151,11 @@ def sample_injected(blob: np.ndarray, shots: int, exo: np.ndarray | None, unifor
         d = 1 << fk.value
         out["final_state"] = fre[:d] + 1j * fim[:d]
     return out
+
+
+def set_fp32_storage(on: bool) -> None:
+    """Emulate FP32 amplitude storage in injected runs (precision studies)."""
+    lib().oracle_set_fp32_storage(int(on))
 
 
 def presample(blob: np.ndarray, shots: int, stream_seed: int) -> np.ndarray:

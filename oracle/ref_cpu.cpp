@@ -35,6 +35,7 @@
 namespace {
 
 thread_local std::string g_err;
+thread_local bool g_fp32_storage = false;
 
 struct OracleError : std::runtime_error {
   using std::runtime_error::runtime_error;
@@ -604,6 +605,7 @@ struct Runner {
   std::vector<MeasK> meas;
   std::vector<char> exogenous;
   std::vector<double> re, im, sre, sim;
+  bool fp32 = false;  // emulate FP32 amplitude storage (precision studies)
   explicit Runner(const Program& prog, bool packed, int sel) : p(prog), bit_packed(packed), selected_observable(sel) {
     exogenous.assign(size_t(p.nsymbols) + 1, 0);
     for (const auto& d : p.cats) for (int32_t c : d.conds) exogenous[c] = 1;
@@ -623,6 +625,14 @@ struct Runner {
       }
       if (in.op == TICIT_OP_COIN) coin_ordinal[i] = c++;
       if (in.op == TICIT_OP_ROTATE) rot[i] = rot_kernel(in);
+    }
+  }
+
+  void round_state(size_t base, size_t n) {
+    if (!fp32) return;
+    for (size_t a = 0; a < n; ++a) {
+      re[base + a] = double(float(re[base + a]));
+      im[base + a] = double(float(im[base + a]));
     }
   }
 
@@ -701,7 +711,15 @@ struct Runner {
           if (in.nq != k) throw OracleError("rotation kernel dimension does not match batch active state");
           eval(in.expr);
           size_t dim = size_t(1) << k;
-          for (size_t s = 0; s < shots; ++s) rotate(&re[s * stride], &im[s * stride], dim, rot[i], bit(bits, s));
+          RotK rk = rot[i];
+          if (fp32) {
+            rk.c = double(float(rk.c));
+            rk.minus_even = Cx{double(float(rk.minus_even.re)), double(float(rk.minus_even.im))};
+          }
+          for (size_t s = 0; s < shots; ++s) {
+            rotate(&re[s * stride], &im[s * stride], dim, rk, bit(bits, s));
+            round_state(s * stride, dim);
+          }
           break;
         }
         case TICIT_OP_PROMOTE: {
@@ -709,7 +727,14 @@ struct Runner {
           eval(in.expr);
           size_t dim = size_t(1) << k;
           double c = std::cos(in.angle), sn = std::sin(in.angle);
-          for (size_t s = 0; s < shots; ++s) promote(&re[s * stride], &im[s * stride], dim, c, bit(bits, s) ? sn : -sn);
+          if (fp32) {
+            c = double(float(c));
+            sn = double(float(sn));
+          }
+          for (size_t s = 0; s < shots; ++s) {
+            promote(&re[s * stride], &im[s * stride], dim, c, bit(bits, s) ? sn : -sn);
+            round_state(s * stride, 2 * dim);
+          }
           ++k;
           --ndormant;
           break;
@@ -753,6 +778,7 @@ struct Runner {
             double ps = b ? pt : 1.0 - pt;
             if (ps <= 0.0) throw OracleError("sampled an impossible active measurement branch");
             project(r, m, sre.data(), sim.data(), mk, b, 1.0 / std::sqrt(ps));
+            round_state(s * stride, mk.out_dim);
           }
           --k;
           ++ndormant;
@@ -854,6 +880,9 @@ void copy_out(const Outputs& o, uint8_t* meas, uint8_t* det, uint8_t* obs, uint6
 extern "C" {
 
 const char* oracle_last_error(void) { return g_err.c_str(); }
+
+// Injected runs on this thread emulate FP32 amplitude storage when set.
+void oracle_set_fp32_storage(int on) { g_fp32_storage = on != 0; }
 
 // info[0..9] = n, initial_k, max_k, nsymbols, nrecords, ndetectors,
 // nobservables, ninstr, nmeasure, ncoin
@@ -967,6 +996,7 @@ int oracle_sample_injected(const uint64_t* blob, size_t nwords, uint64_t shots, 
   try {
     Program p = decode(blob, nwords);
     Runner run(p, bit_packed != 0, observable);
+    run.fp32 = g_fp32_storage;
     Outputs o;
     o.obs_ones.assign(p.observables.size(), 0);
     ExoTable exo;

@@ -105,7 +105,7 @@ class CompiledSampler:
     backend = "hip"
 
     def __init__(self, blob: np.ndarray, observable: int = 0, precision: str = "fp64",
-                 tier: str = "auto", planner: str = "heuristic"):
+                 tier: str = "auto", planner: str = "heuristic", wide_batch: int = 0):
         lib = _capi.load()
         self._lib = lib
         self.blob = np.ascontiguousarray(blob, dtype=np.uint64)
@@ -114,6 +114,7 @@ class CompiledSampler:
         opt.precision = PRECISIONS[precision]
         opt.tier = TIERS[tier]
         opt.planner_mode = PLANNERS[planner]
+        opt.reserved[0] = wide_batch
         handle = ctypes.c_void_p()
         _capi.check(lib.ticit_hip_program_create(self.blob.ctypes.data_as(_capi._u64p), self.blob.size,
                                                  ctypes.byref(opt), ctypes.byref(handle)))
@@ -130,6 +131,8 @@ class CompiledSampler:
         self.num_measure_instructions = int(info.measure_instructions)
         self.num_coin_instructions = int(info.coin_instructions)
         self.tier = {1: "lane", 2: "group", 3: "wide"}.get(int(info.tier), "auto")
+        self.precision = precision
+        self.wide_batch = int(info.wide_batch)
         self.has_postselection = False
 
     def close(self) -> None:
@@ -197,6 +200,28 @@ class CompiledSampler:
         return out
 
 
+    def final_state(self, shots: int = 1, exo=None, uniforms=None, coins=None) -> np.ndarray:
+        """Shot 0's final active state (complex128) of an injected run."""
+        words = (shots + 63) // 64
+        exo = np.ascontiguousarray(exo if exo is not None else np.zeros((max(1, self.num_symbols), words)),
+                                   dtype=np.uint64)
+        uniforms = np.ascontiguousarray(
+            uniforms if uniforms is not None else np.zeros((max(1, self.num_measure_instructions), shots)),
+            dtype=np.float64)
+        coins = np.ascontiguousarray(
+            coins if coins is not None else np.zeros((max(1, self.num_coin_instructions), words)), dtype=np.uint64)
+        dim = 1 << self.max_active_qubits
+        re = np.zeros(dim)
+        im = np.zeros(dim)
+        k = ctypes.c_int(0)
+        _capi.check(self._lib.ticit_hip_final_state(
+            self._handle, shots, exo.ctypes.data_as(_capi._u64p), uniforms.ctypes.data_as(_capi._f64p),
+            coins.ctypes.data_as(_capi._u64p), re.ctypes.data_as(_capi._f64p), im.ctypes.data_as(_capi._f64p),
+            ctypes.byref(k)))
+        d = 1 << k.value
+        return re[:d] + 1j * im[:d]
+
+
 class Circuit:
     """A parsed ``.ticit`` circuit (ticit's Circuit, src/circuit/mod.rs)."""
 
@@ -217,7 +242,7 @@ class Circuit:
 
     def compile(self, postselection_mask=None, *, observable: int = 0, backend: str = "hip",
                 precision: str = "fp64", tier: str = "auto", planner: str = "heuristic",
-                threads: int | None = None) -> CompiledSampler:
+                threads: int | None = None, wide_batch: int = 0) -> CompiledSampler:
         """Circuit::compile with ticit_py's ``backend=`` selector; only the
         ``"hip"`` backend lives in this package."""
         if backend != "hip":
@@ -225,7 +250,7 @@ class Circuit:
         if postselection_mask is not None and any(int(v) for v in postselection_mask):
             raise TicitError(3, "detector postselection is not supported by the hip backend")
         return CompiledSampler(self.blob, observable=observable, precision=precision, tier=tier,
-                               planner=planner)
+                               planner=planner, wide_batch=wide_batch)
 
 
 def device_properties() -> dict:

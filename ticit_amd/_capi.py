@@ -38,7 +38,8 @@ class ProgramInfo(ctypes.Structure):
                 ("max_active_qubits", ctypes.c_uint64), ("symbols", ctypes.c_uint64),
                 ("instructions", ctypes.c_uint64), ("measure_instructions", ctypes.c_uint64),
                 ("coin_instructions", ctypes.c_uint64), ("tier", ctypes.c_uint32),
-                ("precision", ctypes.c_uint32)]
+                ("precision", ctypes.c_uint32), ("wide_batch", ctypes.c_uint32),
+                ("reserved0", ctypes.c_uint32)]
 
 
 # ticit_hip_status -> exception kinds of TicitError (src/errors.rs:9-57)
@@ -50,7 +51,8 @@ EXPORTS = [
     "ticit_hip_last_error", "ticit_hip_plan_text", "ticit_hip_blob_free",
     "ticit_hip_program_create", "ticit_hip_compile_text", "ticit_hip_program_destroy",
     "ticit_hip_program_info_get", "ticit_hip_sample", "ticit_hip_sample_device",
-    "ticit_hip_sample_injected", "ticit_hip_sample_batch", "ticit_hip_result_free",
+    "ticit_hip_sample_injected", "ticit_hip_final_state", "ticit_hip_sample_batch",
+    "ticit_hip_result_free",
     "ticit_hip_device_query",
 ]
 
@@ -95,6 +97,8 @@ def load() -> ctypes.CDLL:
                                             ctypes.c_void_p]
     lib.ticit_hip_sample_injected.argtypes = [ctypes.c_void_p, ctypes.c_uint64, _u64p, _f64p, _u64p,
                                               ctypes.c_int, ctypes.POINTER(Result), _f64p]
+    lib.ticit_hip_final_state.argtypes = [ctypes.c_void_p, ctypes.c_uint64, _u64p, _f64p, _u64p, _f64p, _f64p,
+                                          ctypes.POINTER(ctypes.c_int)]
     lib.ticit_hip_sample_batch.argtypes = [ctypes.POINTER(ctypes.c_void_p), _u64p, _u64p, ctypes.c_size_t,
                                            ctypes.c_int, ctypes.c_int, ctypes.POINTER(Result)]
     lib.ticit_hip_result_free.argtypes = [ctypes.POINTER(Result)]
@@ -102,7 +106,7 @@ def load() -> ctypes.CDLL:
     lib.ticit_hip_device_query.restype = ctypes.c_int
     for name in ("ticit_hip_plan_text", "ticit_hip_program_create", "ticit_hip_compile_text",
                  "ticit_hip_program_info_get", "ticit_hip_sample", "ticit_hip_sample_device",
-                 "ticit_hip_sample_injected", "ticit_hip_sample_batch"):
+                 "ticit_hip_sample_injected", "ticit_hip_sample_batch", "ticit_hip_final_state"):
         getattr(lib, name).restype = ctypes.c_int
     lib.ticit_hip_blob_free.restype = None
     lib.ticit_hip_program_destroy.restype = None

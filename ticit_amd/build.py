@@ -24,6 +24,7 @@ SOURCES = [
     "csrc/host/device_program.cpp",
     "csrc/host/capi.cpp",
     "csrc/kernels/sampler.hip",
+    "csrc/kernels/wide_tier.hip",
 ]
 
 FLAGS = [

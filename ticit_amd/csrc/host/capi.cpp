@@ -19,6 +19,9 @@ size_t group_tier_lds_bytes(int max_k, const GroupLayout& y);
 hipError_t launch_lane_tier(int kmax, const DevProgram& P, const DevLaunch& L, const LaneLayout& Y,
                             hipStream_t stream);
 hipError_t launch_group_tier(const DevProgram& P, const DevLaunch& L, const GroupLayout& Y, hipStream_t stream);
+size_t wide_bytes_per_shot(int max_k, int fp32, const WideLayout& y);
+hipError_t launch_wide_tier(const DevProgram& P, const DevLaunch& L, const WideLayout& Y, uint32_t batch, int fp32,
+                            hipStream_t stream, double* dump_re, double* dump_im, int* dump_k);
 
 }  // namespace ticit
 
@@ -53,6 +56,8 @@ struct ticit_hip_program {
   int* d_error = nullptr;
   LaneLayout lane{};
   GroupLayout group{};
+  WideLayout wide{};
+  uint32_t wide_batch = 0;
   uint32_t meas_row_packed = 0, det_row_packed = 0, obs_row_packed = 0;
 
   ~ticit_hip_program() {
@@ -154,7 +159,11 @@ ticit_hip_status create_from_program(Program prog, const ticit_hip_options* opti
   h->group = GroupLayout{h->lane.pw, h->lane.sw, h->lane.rw, h->lane.dw};
   uint32_t max_row = std::max<uint32_t>({uint32_t(t.nrec), uint32_t(t.ndet), uint32_t(t.nobs), 1});
   h->lane.stage_bytes = 64 * ((max_row + 15) / 16 * 16);
+  h->wide = WideLayout{h->lane.pw, h->lane.sw, h->lane.rw, h->lane.dw,
+                       h->lane.pw + h->lane.sw + h->lane.rw + h->lane.dw,
+                       uint32_t(((size_t(1) << t.max_k) / 2 + 2047) / 2048 + 1)};
   uint32_t want = h->opt.tier;
+  if (want == 0 && h->opt.precision == 1) want = kTierWide;  // the FP32 path lives in the wide tier
   if (want == 0) {
     if (t.max_k <= 5 && lane_tier_lds_bytes(t.max_k, h->lane) <= size_t(lds_limit)) want = kTierLane;
     else if (t.max_k <= 12 && group_tier_lds_bytes(t.max_k, h->group) <= size_t(lds_limit)) want = kTierGroup;
@@ -169,18 +178,32 @@ ticit_hip_status create_from_program(Program prog, const ticit_hip_options* opti
     return fail(TICIT_HIP_UNSUPPORTED, "group tier state exceeds the LDS of one workgroup");
   }
   if (want == kTierWide) {
-    delete h;
-    return fail(TICIT_HIP_UNSUPPORTED, "active width " + std::to_string(t.max_k) +
-                                           " exceeds the LDS-resident tiers of the hip backend");
+    if (t.max_k > 30) {
+      delete h;
+      return fail(TICIT_HIP_UNSUPPORTED, "active width " + std::to_string(t.max_k) + " exceeds the wide tier");
+    }
+    // Shots in flight: the reference's workspace policy, at most 16 GiB or
+    // two thirds of free memory (src/gpu/mod.rs:680-711), capped by grid y.
+    size_t free_b = 0, total_b = 0;
+    (void)hipMemGetInfo(&free_b, &total_b);
+    size_t budget = std::min<size_t>(size_t(16) << 30, free_b / 3 * 2);
+    size_t per = wide_bytes_per_shot(t.max_k, h->opt.precision == 1, h->wide);
+    size_t batch = per ? budget / per : 1;
+    if (h->opt.reserved[0]) batch = std::min<size_t>(batch, h->opt.reserved[0]);
+    batch = std::max<size_t>(1, std::min<size_t>(batch, 65535));
+    h->wide_batch = uint32_t(batch);
   }
   h->tier = want;
   *out = h;
   return TICIT_HIP_OK;
 }
 
-hipError_t launch(ticit_hip_program* h, const DevLaunch& L, hipStream_t stream) {
+hipError_t launch(ticit_hip_program* h, const DevLaunch& L, hipStream_t stream, double* dump_re = nullptr,
+                  double* dump_im = nullptr, int* dump_k = nullptr) {
   if (h->tier == kTierLane) return launch_lane_tier(h->tab.max_k, h->dev, L, h->lane, stream);
-  return launch_group_tier(h->dev, L, h->group, stream);
+  if (h->tier == kTierGroup) return launch_group_tier(h->dev, L, h->group, stream);
+  uint32_t batch = uint32_t(std::min<uint64_t>(h->wide_batch, L.shots));
+  return launch_wide_tier(h->dev, L, h->wide, batch, h->opt.precision == 1, stream, dump_re, dump_im, dump_k);
 }
 
 struct DeviceBuffers {
@@ -200,7 +223,8 @@ struct DeviceBuffers {
 
 ticit_hip_status run_host(ticit_hip_program* h, uint64_t shots, uint64_t seed, int bit_packed, int keep_records,
                           bool injected, const uint64_t* exo, const double* uniforms, const uint64_t* coins,
-                          double* probs, ticit_hip_result* out) {
+                          double* probs, ticit_hip_result* out, double* state_re = nullptr,
+                          double* state_im = nullptr, int* state_k = nullptr) {
   auto t0 = std::chrono::steady_clock::now();
   std::memset(out, 0, sizeof(*out));
   const HostTables& t = h->tab;
@@ -264,11 +288,17 @@ ticit_hip_status run_host(ticit_hip_program* h, uint64_t shots, uint64_t seed, i
     CK(buf.alloc(&d_probs, size_t(std::max(1, t.nmeasure)) * shots), "allocating probabilities");
     L.probs = d_probs;
   }
+  double* d_state = nullptr;
+  if (state_re) {
+    if (h->tier != kTierWide) return fail(TICIT_HIP_UNSUPPORTED, "state dumps are served by the wide tier");
+    CK(buf.alloc(&d_state, (size_t(2) << h->tab.max_k)), "allocating the state dump");
+  }
   hipEvent_t ev0, ev1;
   CK(hipEventCreate(&ev0), "creating events");
   CK(hipEventCreate(&ev1), "creating events");
   CK(hipEventRecord(ev0, nullptr), "recording event");
-  e = launch(h, L, nullptr);
+  int dump_k = 0;
+  e = launch(h, L, nullptr, d_state, d_state ? d_state + (size_t(1) << h->tab.max_k) : nullptr, &dump_k);
   if (e != hipSuccess) {
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
@@ -306,6 +336,12 @@ ticit_hip_status run_host(ticit_hip_program* h, uint64_t shots, uint64_t seed, i
   }
   if (probs && t.nmeasure)
     CK(hipMemcpy(probs, d_probs, size_t(t.nmeasure) * shots * 8, hipMemcpyDeviceToHost), "reading probabilities");
+  if (state_re) {
+    size_t nd = size_t(1) << dump_k;
+    CK(hipMemcpy(state_re, d_state, nd * 8, hipMemcpyDeviceToHost), "reading the state");
+    CK(hipMemcpy(state_im, d_state + (size_t(1) << h->tab.max_k), nd * 8, hipMemcpyDeviceToHost), "reading the state");
+    *state_k = dump_k;
+  }
 #undef CK
   out->kernel_s = double(ms) * 1e-3;
   out->sample_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -378,6 +414,7 @@ ticit_hip_status ticit_hip_program_info_get(const ticit_hip_program* h, ticit_hi
   out->coin_instructions = uint64_t(h->tab.ncoin);
   out->tier = h->tier;
   out->precision = h->opt.precision;
+  out->wide_batch = h->wide_batch;
   return TICIT_HIP_OK;
 }
 
@@ -392,6 +429,16 @@ ticit_hip_status ticit_hip_sample_injected(ticit_hip_program* h, uint64_t shots,
                                            ticit_hip_result* out, double* probs) {
   if (!h || !out) return fail(TICIT_HIP_INVALID_INPUT, "null argument");
   return run_host(h, shots, 0, bit_packed, 1, true, exo, uniforms, coins, probs, out);
+}
+
+ticit_hip_status ticit_hip_final_state(ticit_hip_program* h, uint64_t shots, const uint64_t* exo,
+                                       const double* uniforms, const uint64_t* coins, double* re, double* im,
+                                       int* k) {
+  if (!h || !re || !im || !k) return fail(TICIT_HIP_INVALID_INPUT, "null argument");
+  ticit_hip_result r;
+  ticit_hip_status s = run_host(h, shots, 0, 1, 1, true, exo, uniforms, coins, nullptr, &r, re, im, k);
+  if (s == TICIT_HIP_OK) ticit_hip_result_free(&r);
+  return s;
 }
 
 ticit_hip_status ticit_hip_sample_device(ticit_hip_program* h, uint64_t shots, uint64_t seed, uint64_t first_shot,

@@ -157,6 +157,12 @@ struct GroupLayout {
   uint32_t pw, sw, rw, dw;
 };
 
+struct WideLayout {
+  uint32_t pw, sw, rw, dw;  // words per shot
+  uint32_t words;           // pw + sw + rw + dw
+  uint32_t max_tiles;       // probability partials per shot
+};
+
 enum : int {
   kDevErrImpossibleBranch = 1,
   kDevErrBadProbability = 2,

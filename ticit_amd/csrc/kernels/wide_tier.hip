@@ -1,0 +1,513 @@
+// Wide tier: active states too large for one workgroup's LDS (k > 12).
+//
+// A batch of S shots is in flight; shot j's split-complex state occupies
+// re[j * DIM .. ], im[j * DIM .. ] in HBM (DIM = 2^max_k) with a ping-pong
+// partner for projections. Per shot, a row of bit tables (partials, runtime
+// slots, records, detectors) and a control record (branch stream, last
+// branch, inverse norm) live in global memory. The host walks the program:
+//   * runs of Record/Detector/Coin instructions -> one per-shot bit kernel;
+//   * Rotate / Promote -> one streaming kernel over (tile, shot);
+//   * Measure -> FP64 partial sums per (tile, shot), a per-shot decision
+//     kernel (fixed-order sum, clamp, draw, 1/sqrt) and a projection kernel
+//     into the partner buffer.
+// Amplitudes are T = double (FP64) or T = float (FP32 amplitude updates with
+// FP64 probability sums, renormalisation and collapse). Arithmetic follows
+// ticit's contiguous kernels (src/contiguous.rs:93-177, 1088-1113,
+// 1582-1606, 2013-2044).
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "device_common.hpp"
+
+namespace ticit {
+
+
+struct WideCtl {
+  uint64_t brng;
+  double invnorm;
+  int32_t branch;
+  int32_t err;
+};
+
+constexpr int kWideThreads = 256;
+constexpr int kWidePer = 8;                              // amplitudes per thread
+constexpr int kWideTile = kWideThreads * kWidePer;       // amplitudes per block
+
+__device__ __forceinline__ bool wbit(const uint64_t* t, uint32_t i) { return (t[i >> 6] >> (i & 63)) & 1; }
+__device__ __forceinline__ void wset(uint64_t* t, uint32_t i, bool v) {
+  uint64_t m = 1ull << (i & 63);
+  t[i >> 6] = v ? (t[i >> 6] | m) : (t[i >> 6] & ~m);
+}
+
+struct WideShot {
+  uint64_t* p;
+  uint64_t* s;
+  uint64_t* r;
+  uint64_t* d;
+};
+
+__device__ __forceinline__ WideShot wide_shot(uint64_t* bits, const WideLayout& Y, uint32_t j) {
+  WideShot w;
+  w.p = bits + size_t(j) * Y.words;
+  w.s = w.p + Y.pw;
+  w.r = w.s + Y.sw;
+  w.d = w.r + Y.rw;
+  return w;
+}
+
+__device__ __forceinline__ bool wide_eval(const DevProgram& P, const DevInstr& in, const WideShot& w) {
+  bool b = (in.op & kFlagConstant) != 0;
+  if (in.partial >= 0) b ^= wbit(w.p, uint32_t(in.partial));
+  for (uint32_t j = 0; j < in.resid_len; ++j) b ^= wbit(w.s, P.resid[in.resid_off + j]);
+  return b;
+}
+
+// Per-shot setup: clear tables, walk the noise, seed the branch stream,
+// reset the state to |0>.
+template <class T>
+__global__ __launch_bounds__(kWideThreads) void wide_init_kernel(DevProgram P, DevLaunch L, WideLayout Y,
+                                                                 uint64_t batch_start, uint32_t nshots,
+                                                                 uint64_t* bits, WideCtl* ctl, T* re, T* im,
+                                                                 size_t dim) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nshots) return;
+  const uint64_t local = batch_start + j;
+  const uint64_t shot = L.first_shot + local;
+  WideShot w = wide_shot(bits, Y, j);
+  for (uint32_t k = 0; k < Y.words; ++k) w.p[k] = 0;
+  auto toggle = [&](int32_t c) {
+    for (uint32_t q = P.sym_off[c - 1]; q < P.sym_off[c]; ++q) {
+      uint32_t part = P.sym_part[q];
+      w.p[part >> 6] ^= 1ull << (part & 63);
+    }
+  };
+  if (L.injected) {
+    const uint64_t words = (L.shots + 63) / 64;
+    for (int i = 0; i < P.nexo; ++i) {
+      int32_t c = P.exo_syms[i];
+      if ((L.exo[uint64_t(c - 1) * words + (local >> 6)] >> (local & 63)) & 1) toggle(c);
+    }
+  } else {
+    sample_exogenous(P, seed_for(kExoSeedBase, L.seed, shot), toggle);
+  }
+  ctl[j].brng = seed_for(kBranchSeedBase, L.seed, shot);
+  ctl[j].err = 0;
+  ctl[j].branch = 0;
+  ctl[j].invnorm = 1.0;
+  const size_t d0 = size_t(1) << P.initial_k;
+  for (size_t a = 0; a < d0; ++a) {
+    re[j * dim + a] = a == 0 ? T(1) : T(0);
+    im[j * dim + a] = T(0);
+  }
+}
+
+// A run of Record / Detector / Coin instructions for every shot.
+__global__ __launch_bounds__(kWideThreads) void wide_bits_kernel(DevProgram P, DevLaunch L, WideLayout Y,
+                                                                 uint64_t batch_start, uint32_t nshots,
+                                                                 uint64_t* bits, WideCtl* ctl, int first,
+                                                                 int last) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nshots) return;
+  const uint64_t local = batch_start + j;
+  WideShot w = wide_shot(bits, Y, j);
+  uint64_t brng = ctl[j].brng;
+  for (int i = first; i < last; ++i) {
+    const DevInstr in = P.instrs[i];
+    const uint32_t op = in.op & 0xff;
+    if (op == kOpRecord) {
+      bool b = wide_eval(P, in, w);
+      if (in.record >= 0) wset(w.r, uint32_t(in.record), b);
+      if (in.rc_slot >= 0) wset(w.s, uint32_t(in.rc_slot), b);
+    } else if (op == kOpDetector) {
+      bool b = false;
+      if (in.op & kFlagDetRecords) {
+        for (uint32_t q = 0; q < in.det_len; ++q) b ^= wbit(w.r, P.det_recs[in.det_off + q]);
+      } else {
+        b = wide_eval(P, in, w);
+      }
+      wset(w.d, uint32_t(in.detector), b);
+    } else if (op == kOpCoin) {
+      bool coin;
+      if (L.injected) {
+        const uint64_t words = (L.shots + 63) / 64;
+        coin = (L.coins[uint64_t(in.ordinal) * words + (local >> 6)] >> (local & 63)) & 1;
+      } else {
+        coin = sm_next(brng) & 1;
+      }
+      wset(w.s, uint32_t(in.branch_slot), coin);
+      bool b = wide_eval(P, in, w);
+      if (in.record >= 0) wset(w.r, uint32_t(in.record), b);
+      if (in.rc_slot >= 0) wset(w.s, uint32_t(in.rc_slot), b);
+    }
+  }
+  ctl[j].brng = brng;
+}
+
+template <class T>
+__global__ __launch_bounds__(kWideThreads) void wide_rotate_kernel(DevProgram P, WideLayout Y, int instr,
+                                                                   uint64_t* bits, T* re, T* im, size_t dim,
+                                                                   int k) {
+  const uint32_t j = blockIdx.y;
+  const DevInstr in = P.instrs[instr];
+  const bool sign = wide_eval(P, in, wide_shot(bits, Y, j));
+  T* R = re + size_t(j) * dim;
+  T* I = im + size_t(j) * dim;
+  const size_t n = size_t(1) << k;
+  const T c = T(in.c);
+  if (in.op & kFlagDiagonal) {
+    const T mr = T(in.me_re), mi = T(in.me_im);
+#pragma unroll
+    for (int q = 0; q < kWidePer; ++q) {
+      size_t b = size_t(blockIdx.x) * kWideTile + size_t(q) * kWideThreads + threadIdx.x;
+      if (b >= n) break;
+      bool flip = sign != parity64(uint64_t(b) & in.zmask);
+      T fr = c + (flip ? -mr : mr), fi = flip ? -mi : mi;
+      T r = R[b], m = I[b];
+      R[b] = fr * r - fi * m;
+      I[b] = fr * m + fi * r;
+    }
+    return;
+  }
+  const bool uni = in.op & kFlagUniformImag;
+  const bool xz_odd = in.op & kFlagXzOdd;
+  const T q = T(sign ? -in.me_im : in.me_im);
+  const T mr = T(in.me_re), mi = T(in.me_im);
+#pragma unroll
+  for (int u = 0; u < kWidePer; ++u) {
+    size_t idx = size_t(blockIdx.x) * kWideTile + size_t(u) * kWideThreads + threadIdx.x;
+    if (idx >= (n >> 1)) break;
+    size_t i0 = insert_zero_bit(idx, in.pivot), i1 = i0 ^ size_t(in.xmask);
+    T r0 = R[i0], m0 = I[i0], r1 = R[i1], m1 = I[i1];
+    if (uni) {
+      R[i0] = c * r0 - q * m1;
+      I[i0] = c * m0 + q * r1;
+      R[i1] = c * r1 - q * m0;
+      I[i1] = c * m1 + q * r0;
+    } else {
+      bool lodd = parity64(uint64_t(i0) & in.zmask);
+      T ld = (sign != lodd) ? T(-1) : T(1);
+      T rd = xz_odd ? -ld : ld;
+      T lre = ld * mr, lim = ld * mi, rre = rd * mr, rim = rd * mi;
+      R[i0] = c * r0 + rre * r1 - rim * m1;
+      I[i0] = c * m0 + rre * m1 + rim * r1;
+      R[i1] = c * r1 + lre * r0 - lim * m0;
+      I[i1] = c * m1 + lre * m0 + lim * r0;
+    }
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kWideThreads) void wide_promote_kernel(DevProgram P, WideLayout Y, int instr,
+                                                                    uint64_t* bits, T* re, T* im, size_t dim,
+                                                                    int k) {
+  const uint32_t j = blockIdx.y;
+  const DevInstr in = P.instrs[instr];
+  const bool sign = wide_eval(P, in, wide_shot(bits, Y, j));
+  T* R = re + size_t(j) * dim;
+  T* I = im + size_t(j) * dim;
+  const size_t n = size_t(1) << k;
+  const T c = T(in.c), q = T(sign ? in.s : -in.s);
+#pragma unroll
+  for (int u = 0; u < kWidePer; ++u) {
+    size_t b = size_t(blockIdx.x) * kWideTile + size_t(u) * kWideThreads + threadIdx.x;
+    if (b >= n) break;
+    T r = R[b], m = I[b];
+    R[b] = c * r;
+    I[b] = c * m;
+    R[n + b] = -q * m;
+    I[n + b] = q * r;
+  }
+}
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < kWideThreads / 64; ++w) t += red[w];
+  return t;
+}
+
+// FP64 partial sums of P(branch = 1) per (tile, shot).
+template <class T>
+__global__ __launch_bounds__(kWideThreads) void wide_prob_kernel(DevProgram P, int instr, const T* re,
+                                                                 const T* im, size_t dim, int k, double* part,
+                                                                 uint32_t max_tiles) {
+  __shared__ double red[kWideThreads / 64];
+  const uint32_t j = blockIdx.y;
+  const DevInstr in = P.instrs[instr];
+  const T* R = re + size_t(j) * dim;
+  const T* I = im + size_t(j) * dim;
+  const size_t out_dim = (size_t(1) << k) >> 1;
+  const bool diag = in.op & kFlagDiagonal;
+  double acc = 0.0;
+#pragma unroll
+  for (int u = 0; u < kWidePer; ++u) {
+    size_t idx = size_t(blockIdx.x) * kWideTile + size_t(u) * kWideThreads + threadIdx.x;
+    if (idx >= out_dim) break;
+    if (diag) {
+      size_t s = diagonal_source(in, idx, true);
+      double r = double(R[s]), m = double(I[s]);
+      acc += r * r + m * m;
+    } else {
+      size_t s0 = insert_zero_bit(idx, in.pivot), s1 = s0 ^ size_t(in.xmask);
+      double dir = parity64(uint64_t(s0) & in.zmask) ? 1.0 : -1.0;
+      double c1r = dir * in.c1_re, c1i = dir * in.c1_im;
+      double ar = kInvSqrt2 * double(R[s0]) + c1r * double(R[s1]) - c1i * double(I[s1]);
+      double ai = kInvSqrt2 * double(I[s0]) + c1r * double(I[s1]) + c1i * double(R[s1]);
+      acc += ar * ar + ai * ai;
+    }
+  }
+  double t = block_sum(acc, red);
+  if (threadIdx.x == 0) part[size_t(j) * max_tiles + blockIdx.x] = t;
+}
+
+// Per shot: fixed-order sum of the tile partials, clamp, draw, 1/sqrt,
+// branch symbol, outcome, record.
+__global__ __launch_bounds__(kWideThreads) void wide_decide_kernel(DevProgram P, DevLaunch L, WideLayout Y,
+                                                                   uint64_t batch_start, uint32_t nshots,
+                                                                   int instr, uint64_t* bits, WideCtl* ctl,
+                                                                   const double* part, uint32_t ntiles,
+                                                                   uint32_t max_tiles) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nshots) return;
+  const uint64_t local = batch_start + j;
+  const DevInstr in = P.instrs[instr];
+  double pt = 0.0;
+  for (uint32_t t = 0; t < ntiles; ++t) pt += part[size_t(j) * max_tiles + t];
+  pt = clamp01(pt);
+  if (L.want_probs) L.probs[uint64_t(in.ordinal) * L.shots + local] = pt;
+  WideCtl c = ctl[j];
+  bool branch = false;
+  if (L.injected) {
+    double u = L.uniforms[uint64_t(in.ordinal) * L.shots + local];
+    branch = pt <= 0.0 ? false : (pt >= 1.0 ? true : u < pt);
+  } else if (!(pt >= 0.0 && pt <= 1.0)) {
+    c.err |= kDevErrBadProbability;
+  } else {
+    branch = pt <= 0.0 ? false : (pt >= 1.0 ? true : sm_uniform(c.brng) < pt);
+  }
+  double ps = branch ? pt : 1.0 - pt;
+  if (!(ps > 0.0)) {
+    c.err |= kDevErrImpossibleBranch;
+    ps = 1.0;
+  }
+  c.invnorm = 1.0 / sqrt(ps);
+  c.branch = branch;
+  ctl[j] = c;
+  WideShot w = wide_shot(bits, Y, j);
+  wset(w.s, uint32_t(in.branch_slot), branch);
+  bool b = wide_eval(P, in, w);
+  if (in.record >= 0) wset(w.r, uint32_t(in.record), b);
+  if (in.rc_slot >= 0) wset(w.s, uint32_t(in.rc_slot), b);
+}
+
+// Collapse into the partner buffer (the renormalising multiply in FP64).
+template <class T>
+__global__ __launch_bounds__(kWideThreads) void wide_project_kernel(DevProgram P, int instr, const WideCtl* ctl,
+                                                                    const T* re, const T* im, T* ore, T* oim,
+                                                                    size_t dim, int k) {
+  const uint32_t j = blockIdx.y;
+  const DevInstr in = P.instrs[instr];
+  const bool branch = ctl[j].branch;
+  const double inv = ctl[j].invnorm;
+  const T* R = re + size_t(j) * dim;
+  const T* I = im + size_t(j) * dim;
+  T* OR = ore + size_t(j) * dim;
+  T* OI = oim + size_t(j) * dim;
+  const size_t out_dim = (size_t(1) << k) >> 1;
+  const bool diag = in.op & kFlagDiagonal;
+#pragma unroll
+  for (int u = 0; u < kWidePer; ++u) {
+    size_t idx = size_t(blockIdx.x) * kWideTile + size_t(u) * kWideThreads + threadIdx.x;
+    if (idx >= out_dim) break;
+    if (diag) {
+      size_t s = diagonal_source(in, idx, branch);
+      OR[idx] = T(double(R[s]) * inv);
+      OI[idx] = T(double(I[s]) * inv);
+    } else {
+      size_t s0 = insert_zero_bit(idx, in.pivot), s1 = s0 ^ size_t(in.xmask);
+      bool odd = parity64(uint64_t(s0) & in.zmask);
+      double dir = (branch != odd) ? -1.0 : 1.0;
+      double c1r = dir * in.c1_re, c1i = dir * in.c1_im;
+      double ar = kInvSqrt2 * double(R[s0]) + c1r * double(R[s1]) - c1i * double(I[s1]);
+      double ai = kInvSqrt2 * double(I[s0]) + c1r * double(I[s1]) + c1i * double(R[s1]);
+      OR[idx] = T(ar * inv);
+      OI[idx] = T(ai * inv);
+    }
+  }
+}
+
+// Observables, counts, error flags and packed rows.
+__global__ __launch_bounds__(kWideThreads) void wide_finish_kernel(DevProgram P, DevLaunch L, WideLayout Y,
+                                                                   uint64_t batch_start, uint32_t nshots,
+                                                                   uint64_t* bits, const WideCtl* ctl) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nshots) return;
+  const uint64_t local = batch_start + j;
+  WideShot w = wide_shot(bits, Y, j);
+  if (ctl[j].err) atomicOr(L.error, ctl[j].err);
+  auto obs_bit = [&](uint32_t o) {
+    bool b = false;
+    for (uint32_t q = P.obs_off[o]; q < P.obs_off[o + 1]; ++q) b ^= wbit(w.r, P.obs_recs[q]);
+    return b;
+  };
+  for (int o = 0; o < P.nobs; ++o)
+    if (obs_bit(uint32_t(o))) atomicAdd(&L.obs_ones[o], 1ull);
+  if (!L.keep_records) return;
+  auto emit = [&](uint8_t* out, uint32_t row_bytes, auto&& bit_at, uint32_t ncols) {
+    uint8_t* dst = out + local * row_bytes;
+    for (uint32_t byte = 0; byte < row_bytes; ++byte) {
+      if (L.bit_packed) {
+        uint32_t v = 0;
+        for (uint32_t bb = 0; bb < 8; ++bb) {
+          uint32_t col = byte * 8 + bb;
+          if (col < ncols && bit_at(col)) v |= 1u << bb;
+        }
+        dst[byte] = uint8_t(v);
+      } else {
+        dst[byte] = uint8_t(bit_at(byte));
+      }
+    }
+  };
+  emit(L.meas, L.meas_row, [&](uint32_t c) { return wbit(w.r, c); }, uint32_t(P.nrec));
+  emit(L.det, L.det_row, [&](uint32_t c) { return wbit(w.d, c); }, uint32_t(P.ndet));
+  emit(L.obs, L.obs_row, obs_bit, uint32_t(P.nobs));
+}
+
+// Copies shot 0's state (first 2^k amplitudes) out as FP64, for precision
+// checks against the oracle.
+template <class T>
+__global__ void wide_dump_kernel(const T* re, const T* im, size_t n, double* ore, double* oim) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) {
+    ore[i] = double(re[i]);
+    oim[i] = double(im[i]);
+  }
+}
+
+struct WideWorkspace {
+  void* re[2] = {nullptr, nullptr};
+  void* im[2] = {nullptr, nullptr};
+  uint64_t* bits = nullptr;
+  WideCtl* ctl = nullptr;
+  double* part = nullptr;
+};
+
+size_t wide_bytes_per_shot(int max_k, int fp32, const WideLayout& y) {
+  size_t dim = size_t(1) << max_k;
+  size_t t = fp32 ? 4 : 8;
+  return 4 * dim * t + size_t(y.words) * 8 + sizeof(WideCtl) + size_t(y.max_tiles) * 8;
+}
+
+template <class T>
+static hipError_t run_wide(const DevProgram& P, const DevLaunch& L, const WideLayout& Y, uint32_t batch,
+                           hipStream_t stream, double* dump_re, double* dump_im, int* dump_k) {
+  const size_t dim = size_t(1) << P.max_k;
+  WideWorkspace ws;
+  hipError_t e = hipSuccess;
+  auto cleanup = [&]() {
+    for (int b = 0; b < 2; ++b) {
+      (void)hipFree(ws.re[b]);
+      (void)hipFree(ws.im[b]);
+    }
+    (void)hipFree(ws.bits);
+    (void)hipFree(ws.ctl);
+    (void)hipFree(ws.part);
+  };
+#define WCK(x)          \
+  do {                  \
+    e = (x);            \
+    if (e != hipSuccess) { \
+      cleanup();        \
+      return e;         \
+    }                   \
+  } while (0)
+  for (int b = 0; b < 2; ++b) {
+    WCK(hipMalloc(&ws.re[b], size_t(batch) * dim * sizeof(T)));
+    WCK(hipMalloc(&ws.im[b], size_t(batch) * dim * sizeof(T)));
+  }
+  WCK(hipMalloc(reinterpret_cast<void**>(&ws.bits), size_t(batch) * Y.words * 8 + 8));
+  WCK(hipMalloc(reinterpret_cast<void**>(&ws.ctl), size_t(batch) * sizeof(WideCtl)));
+  WCK(hipMalloc(reinterpret_cast<void**>(&ws.part), size_t(batch) * Y.max_tiles * 8 + 8));
+  std::vector<DevInstr> instrs(size_t(P.ninstr));
+  WCK(hipMemcpy(instrs.data(), P.instrs, instrs.size() * sizeof(DevInstr), hipMemcpyDeviceToHost));
+  const unsigned per_shot_blocks = 256;
+  for (uint64_t start = 0; start < L.shots; start += batch) {
+    uint32_t n = uint32_t(std::min<uint64_t>(batch, L.shots - start));
+    dim3 sg((n + per_shot_blocks - 1) / per_shot_blocks), sb(per_shot_blocks);
+    int cur = 0;
+    hipLaunchKernelGGL(wide_init_kernel<T>, sg, sb, 0, stream, P, L, Y, start, n, ws.bits, ws.ctl,
+                       static_cast<T*>(ws.re[0]), static_cast<T*>(ws.im[0]), dim);
+    WCK(hipGetLastError());
+    int k = P.initial_k;
+    int i = 0;
+    while (i < P.ninstr) {
+      const uint32_t op = instrs[size_t(i)].op & 0xff;
+      if (op == kOpRecord || op == kOpDetector || op == kOpCoin) {
+        int e2 = i;
+        while (e2 < P.ninstr) {
+          uint32_t o2 = instrs[size_t(e2)].op & 0xff;
+          if (o2 != kOpRecord && o2 != kOpDetector && o2 != kOpCoin) break;
+          ++e2;
+        }
+        hipLaunchKernelGGL(wide_bits_kernel, sg, sb, 0, stream, P, L, Y, start, n, ws.bits, ws.ctl, i, e2);
+        WCK(hipGetLastError());
+        i = e2;
+        continue;
+      }
+      T* R = static_cast<T*>(ws.re[cur]);
+      T* I = static_cast<T*>(ws.im[cur]);
+      if (op == kOpRotate) {
+        size_t work = instrs[size_t(i)].op & kFlagDiagonal ? (size_t(1) << k) : (size_t(1) << k) >> 1;
+        dim3 g(unsigned((work + kWideTile - 1) / kWideTile), n);
+        hipLaunchKernelGGL(wide_rotate_kernel<T>, g, dim3(kWideThreads), 0, stream, P, Y, i, ws.bits, R, I, dim, k);
+        WCK(hipGetLastError());
+      } else if (op == kOpPromote) {
+        dim3 g(unsigned(((size_t(1) << k) + kWideTile - 1) / kWideTile), n);
+        hipLaunchKernelGGL(wide_promote_kernel<T>, g, dim3(kWideThreads), 0, stream, P, Y, i, ws.bits, R, I, dim, k);
+        WCK(hipGetLastError());
+        ++k;
+      } else {  // Measure
+        size_t out_dim = (size_t(1) << k) >> 1;
+        unsigned tiles = unsigned((out_dim + kWideTile - 1) / kWideTile);
+        dim3 g(tiles, n);
+        hipLaunchKernelGGL(wide_prob_kernel<T>, g, dim3(kWideThreads), 0, stream, P, i, R, I, dim, k, ws.part,
+                           Y.max_tiles);
+        WCK(hipGetLastError());
+        hipLaunchKernelGGL(wide_decide_kernel, sg, sb, 0, stream, P, L, Y, start, n, i, ws.bits, ws.ctl, ws.part,
+                           tiles, Y.max_tiles);
+        WCK(hipGetLastError());
+        hipLaunchKernelGGL(wide_project_kernel<T>, g, dim3(kWideThreads), 0, stream, P, i, ws.ctl, R, I,
+                           static_cast<T*>(ws.re[cur ^ 1]), static_cast<T*>(ws.im[cur ^ 1]), dim, k);
+        WCK(hipGetLastError());
+        cur ^= 1;
+        --k;
+      }
+      ++i;
+    }
+    hipLaunchKernelGGL(wide_finish_kernel, sg, sb, 0, stream, P, L, Y, start, n, ws.bits, ws.ctl);
+    WCK(hipGetLastError());
+    if (dump_re && start == 0) {
+      size_t nd = size_t(1) << k;
+      hipLaunchKernelGGL(wide_dump_kernel<T>, dim3(unsigned((nd + 255) / 256)), dim3(256), 0, stream,
+                         static_cast<const T*>(ws.re[cur]), static_cast<const T*>(ws.im[cur]), nd, dump_re,
+                         dump_im);
+      WCK(hipGetLastError());
+      *dump_k = k;
+    }
+  }
+  WCK(hipStreamSynchronize(stream));
+#undef WCK
+  cleanup();
+  return hipSuccess;
+}
+
+hipError_t launch_wide_tier(const DevProgram& P, const DevLaunch& L, const WideLayout& Y, uint32_t batch, int fp32,
+                            hipStream_t stream, double* dump_re, double* dump_im, int* dump_k) {
+  if (fp32) return run_wide<float>(P, L, Y, batch, stream, dump_re, dump_im, dump_k);
+  return run_wide<double>(P, L, Y, batch, stream, dump_re, dump_im, dump_k);
+}
+
+}  // namespace ticit
