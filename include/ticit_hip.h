@@ -45,7 +45,11 @@ typedef struct ticit_hip_options {
   uint32_t precision;    /* 0: FP64 amplitudes; 1: FP32 amplitudes, FP64 reductions */
   uint32_t planner_mode; /* 0: heuristic tier choice; 1: learned planner */
   uint32_t tier;         /* 0: automatic; 1: lane tier; 2: group tier; 3: wide tier */
-  uint32_t reserved[4];  /* reserved[0]: cap on wide-tier shots in flight (0 = memory budget) */
+  uint32_t wide_batch;   /* cap on wide-tier shots in flight (0 = memory budget) */
+  uint32_t components;   /* 0: as planned (use_active_components, src/planner.rs:1482-1484);
+                            1: dense active state; 2: tensor components
+                            (src/component_plan.rs, src/sampler/batch/runtime.rs:388-552) */
+  uint32_t reserved[2];
 } ticit_hip_options;
 
 /* Mirrors SampleResult (src/sampler/prepared.rs:197-219). Buffers are owned
@@ -74,7 +78,18 @@ typedef struct ticit_hip_program_info {
   uint32_t tier;      /* tier the sampler runs on */
   uint32_t precision;
   uint32_t wide_batch; /* wide tier: shots in flight per pass */
-  uint32_t reserved0;
+  uint32_t components; /* 1 when the state runs as tensor components */
+  /* Algorithmic bytes per shot: packed output rows plus, when the tier keeps
+   * the state in device memory, S * sum over amplitude instructions of
+   * f * 2^k (S = 16 FP64 / 8 FP32 bytes per amplitude; f = 2 rotation,
+   * 3 promotion, 2.5 measurement). */
+  double output_bytes_per_shot;
+  double state_bytes_per_shot; /* the state term alone, whatever the tier */
+  double algorithmic_bytes_per_shot;
+  uint64_t state_dim;       /* amplitudes per shot: 2^max_k dense, else sum of component capacities */
+  uint64_t component_count; /* 1 dense */
+  int32_t max_local_k;      /* widest amplitude vector an instruction acts on */
+  int32_t reserved1;
 } ticit_hip_program_info;
 
 /* Thread-local message of the last failing call on this thread. */

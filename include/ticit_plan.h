@@ -12,7 +12,7 @@
  * Layout: a stream of little-endian uint64 words.
  *
  *   MAGIC, VERSION, n, initial_k, max_k, nsymbols, nrecords, ndetectors,
- *   nexpvals, ninstr,
+ *   nexpvals, flags (version 2 only), ninstr,
  *   ninstr x instruction:
  *     w0 = op (bits 0-7) | diagonal (8) | diag_phase_bit (9) |
  *          expr.constant (10) | nq (16-31) | pivot (32-47) | phase (48-49)
@@ -30,6 +30,11 @@
  *   nobservables x { nrecords, records... }
  *   MAGIC
  *
+ * flags bit 0: use_active_components (src/factored.rs:885), set by the
+ * planner when its component cost model selects the tensor-component
+ * execution (src/planner.rs:1482-1484). Version 1 blobs have no flags word
+ * and run dense.
+ *
  * op codes: 0 rotate, 1 promote, 2 record, 3 detector, 4 measure, 5 coin.
  * Condition, record and detector ids are 1-based; 0 means "none".
  */
@@ -37,7 +42,8 @@
 #define TICIT_PLAN_H
 
 #define TICIT_PLAN_MAGIC 0x4e4c505449434954ull /* "TICITPLN" */
-#define TICIT_PLAN_VERSION 1ull
+#define TICIT_PLAN_VERSION 2ull
+#define TICIT_PLAN_FLAG_COMPONENTS 1ull
 
 #define TICIT_OP_ROTATE 0
 #define TICIT_OP_PROMOTE 1

@@ -53,6 +53,9 @@ def _load(name: str) -> ctypes.CDLL:
     lib.oracle_insert_zero_bit.argtypes = [ctypes.c_size_t, ctypes.c_int]
     lib.oracle_insert_zero_bit.restype = ctypes.c_size_t
     lib.oracle_set_fp32_storage.argtypes = [ctypes.c_int]
+    lib.oracle_set_component_mode.argtypes = [ctypes.c_int]
+    lib.oracle_component_plan.argtypes = [_u64p, ctypes.c_size_t, _i64p, _f64p, _i64p, _i64p, _i64p,
+                                          ctypes.c_size_t]
     lib.oracle_default_batch.argtypes = [ctypes.c_int]
     lib.oracle_default_batch.restype = ctypes.c_int64
     return lib
@@ -156,6 +159,38 @@ def sample_injected(blob: np.ndarray, shots: int, exo: np.ndarray | None, unifor
 def set_fp32_storage(on: bool) -> None:
     """Emulate FP32 amplitude storage in injected runs (precision studies)."""
     lib().oracle_set_fp32_storage(int(on))
+
+
+def set_component_mode(mode: int) -> None:
+    """-1: run as the blob's use_active_components flag says; 0: dense;
+    1: tensor components."""
+    for fast in (False, True):
+        lib(fast).oracle_set_component_mode(int(mode))
+
+
+STEP_KINDS = {0: "none", 1: "ignored_global_phase", 2: "rotation", 3: "promotion", 4: "measurement"}
+
+
+def component_plan(blob: np.ndarray) -> dict:
+    """The component plan of src/component_plan.rs:301-522 for a blob."""
+    inf = info(blob)
+    cap = max(64, inf["ninstr"] + 1, 4 * (inf["n"] + inf["ninstr"]))
+    summary = np.zeros(9, dtype=np.int64)
+    work = np.zeros(2)
+    cmk = np.zeros(cap, dtype=np.int64)
+    steps = np.zeros(6 * cap, dtype=np.int64)
+    merges = np.zeros(cap, dtype=np.int64)
+    _check(lib().oracle_component_plan(_ptr(blob, _u64p), blob.size, _ptr(summary, _i64p), _ptr(work, _f64p),
+                                       _ptr(cmk, _i64p), _ptr(steps, _i64p), _ptr(merges, _i64p), cap))
+    ncomp, nmerge, planned = int(summary[2]), int(summary[7]), bool(summary[8])
+    rows = steps.reshape(cap, 6)[:inf["ninstr"] if planned else 0]
+    return dict(selected=bool(summary[0]), initial_components=int(summary[1]), component_count=ncomp,
+                dense_peak_dimension=int(summary[3]), component_peak_live_dimension=int(summary[4]),
+                component_allocated_dimension=int(summary[5]), quantum_ops=int(summary[6]),
+                planned=planned, estimated_dense_work=float(work[0]), estimated_component_work=float(work[1]),
+                component_max_k=[int(v) for v in cmk[:ncomp]], merges=[int(v) for v in merges[:nmerge]],
+                steps=[dict(kind=STEP_KINDS[int(r[0])], component=int(r[1]), merge_offset=int(r[2]),
+                            merge_count=int(r[3]), pivot=int(r[4]), deactivate_after=bool(r[5])) for r in rows])
 
 
 def presample(blob: np.ndarray, shots: int, stream_seed: int) -> np.ndarray:

@@ -21,6 +21,7 @@
 // An injected mode replaces every random draw with caller-supplied values:
 // exogenous symbol bit-planes, one uniform per (measure instruction, shot)
 // and one coin bit per (coin instruction, shot).
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -36,6 +37,7 @@ namespace {
 
 thread_local std::string g_err;
 thread_local bool g_fp32_storage = false;
+int g_component_mode = -1;  // read by worker threads: not thread-local
 
 struct OracleError : std::runtime_error {
   using std::runtime_error::runtime_error;
@@ -86,6 +88,7 @@ struct LowP {
 
 struct Program {
   int n = 0, initial_k = 0, max_k = 0, nsymbols = 0, nrecords = 0, ndetectors = 0, nexpvals = 0;
+  bool use_components = false;  // FactoredInstructionProgram::use_active_components
   std::vector<Instr> instrs;
   std::vector<Cat> cats;
   std::vector<Rare> rare;
@@ -118,7 +121,9 @@ struct Rd {
 
 Program decode(const uint64_t* w, size_t n) {
   Rd r{w, n};
-  if (r.next() != TICIT_PLAN_MAGIC || r.next() != TICIT_PLAN_VERSION) throw OracleError("bad plan blob header");
+  if (r.next() != TICIT_PLAN_MAGIC) throw OracleError("bad plan blob header");
+  const uint64_t version = r.next();
+  if (version != 1 && version != TICIT_PLAN_VERSION) throw OracleError("bad plan blob header");
   Program p;
   p.n = r.i32();
   p.initial_k = r.i32();
@@ -127,6 +132,7 @@ Program decode(const uint64_t* w, size_t n) {
   p.nrecords = r.i32();
   p.ndetectors = r.i32();
   p.nexpvals = r.i32();
+  if (version >= 2) p.use_components = (r.next() & TICIT_PLAN_FLAG_COMPONENTS) != 0;
   p.instrs.resize(r.cnt());
   for (auto& in : p.instrs) {
     uint64_t w0 = r.next();
@@ -574,6 +580,243 @@ void project(double* re, double* im, double* sre, double* sim, const MeasK& k, b
   std::memcpy(im, sim, k.out_dim * sizeof(double));
 }
 
+// ------------------------------------------------------ component plan ----
+// src/component_plan.rs:277-522: simulate which active coordinates an
+// instruction links, keep each connected group as its own small vector and
+// merge groups only when an instruction spans them.
+
+enum CompKind { kCompNone = 0, kCompIgnoredPhase = 1, kCompRotation = 2, kCompPromotion = 3, kCompMeasurement = 4 };
+
+struct CompStep {
+  int kind = kCompNone;
+  int comp = 0;
+  size_t merge_off = 0, merge_count = 0;
+  Instr local;  // kernel in the component's local bit order
+  bool deactivate = false;
+};
+
+struct CompPlan {
+  bool selected = false;
+  int initial_components = 0;
+  int component_count = 0;
+  std::vector<int> comp_max_k;
+  std::vector<CompStep> steps;  // one per instruction, empty when not planned
+  std::vector<int> merges;
+  double dense_work = 0.0, comp_work = 0.0;
+  size_t dense_peak = 0, peak_live = 0, allocated = 0;
+  int quantum_ops = 0;
+};
+
+// src/component_plan.rs:36-42
+constexpr double kCompDispatchWork = 32.0;
+constexpr double kCompMinSaved = 8192.0;
+constexpr double kCompWorkRatio = 1.8;
+constexpr double kCompAllocRatio = 1.25;
+
+CompPlan build_component_plan(const Program& p) {
+  struct PComp {
+    bool active;
+    std::vector<int> coords;
+  };
+  CompPlan plan;
+  plan.initial_components = p.initial_k;
+  plan.dense_peak = size_t(1) << p.max_k;
+  if (p.max_k < 8 || p.nexpvals != 0) return plan;
+  plan.steps.assign(p.instrs.size(), CompStep{});
+  std::vector<PComp> comps;
+  std::vector<int> order;       // active position -> coordinate
+  std::vector<int> coord_comp;  // coordinate -> component, -1 = none
+  std::vector<int> coord_pos;   // coordinate -> active position, -1 = none
+  for (int c = 0; c < p.initial_k; ++c) {
+    comps.push_back(PComp{true, {c}});
+    order.push_back(c);
+    coord_comp.push_back(int(comps.size()) - 1);
+    coord_pos.push_back(c);
+    plan.comp_max_k.push_back(1);
+  }
+  auto live_dim = [&]() {
+    size_t t = 0;
+    for (const auto& c : comps)
+      if (c.active && !c.coords.empty()) t += size_t(1) << c.coords.size();
+    return t;
+  };
+  auto update_peak = [&]() { plan.peak_live = std::max(plan.peak_live, live_dim()); };
+  auto refresh = [&]() {
+    std::fill(coord_pos.begin(), coord_pos.end(), -1);
+    for (size_t i = 0; i < order.size(); ++i) coord_pos[size_t(order[i])] = int(i);
+  };
+  auto touched = [&](const Instr& in) {
+    std::vector<int> t;
+    uint64_t support = in.xmask | in.zmask;
+    for (size_t pos = 0; pos < order.size(); ++pos) {
+      if (!((support >> pos) & 1)) continue;
+      int c = coord_comp[size_t(order[pos])];
+      if (c < 0) throw OracleError("component planner found an active coordinate without a component");
+      if (std::find(t.begin(), t.end(), c) == t.end()) t.push_back(c);
+    }
+    // an identity action is pinned to the first live component
+    if (t.empty() && !order.empty()) t.push_back(coord_comp[size_t(order[0])]);
+    return t;
+  };
+  auto target_of = [&](const std::vector<int>& t) {
+    int best = t[0];
+    for (size_t i = 1; i < t.size(); ++i) {
+      int cand = t[i];
+      size_t bs = comps[size_t(best)].coords.size(), cs = comps[size_t(cand)].coords.size();
+      bool take = bs != cs ? bs < cs : best > cand;
+      if (take) best = cand;
+    }
+    return best;
+  };
+  auto ordered = [&](int target, std::vector<int> t) {
+    std::sort(t.begin(), t.end());
+    t.erase(std::unique(t.begin(), t.end()), t.end());
+    t.erase(std::remove(t.begin(), t.end(), target), t.end());
+    t.insert(t.begin(), target);
+    return t;
+  };
+  auto merge = [&](const std::vector<int>& src) {
+    double work = 0.0;
+    int target = src[0];
+    size_t tk = comps[size_t(target)].coords.size();
+    for (size_t i = 1; i < src.size(); ++i) {
+      PComp& s = comps[size_t(src[i])];
+      if (!s.active) throw OracleError("component merge source is inactive");
+      tk += s.coords.size();
+      work += 2.0 * std::ldexp(1.0, int(tk));
+      for (int c : s.coords) coord_comp[size_t(c)] = target;
+      auto moved = std::move(s.coords);
+      s.coords.clear();
+      comps[size_t(target)].coords.insert(comps[size_t(target)].coords.end(), moved.begin(), moved.end());
+      s.active = false;
+    }
+    plan.comp_max_k[size_t(target)] = std::max(plan.comp_max_k[size_t(target)], int(tk));
+    return work;
+  };
+  auto remap = [&](const Instr& in, int comp) {
+    Instr l = in;
+    const auto& cc = comps[size_t(comp)].coords;
+    l.nq = int(cc.size());
+    l.xmask = l.zmask = 0;
+    for (size_t lp = 0; lp < cc.size(); ++lp) {
+      int gp = coord_pos[size_t(cc[lp])];
+      if (gp < 0) throw OracleError("component planner tried to remap an inactive coordinate");
+      if ((in.xmask >> gp) & 1) l.xmask |= 1ull << lp;
+      if ((in.zmask >> gp) & 1) l.zmask |= 1ull << lp;
+    }
+    return l;
+  };
+  auto push_merges = [&](CompStep& st, const std::vector<int>& src) {
+    st.merge_off = plan.merges.size();
+    st.merge_count = src.size();
+    plan.merges.insert(plan.merges.end(), src.begin(), src.end());
+  };
+
+  for (size_t i = 0; i < p.instrs.size(); ++i) {
+    const Instr& in = p.instrs[i];
+    const int gk = int(order.size());
+    refresh();
+    const double dense_dim = std::ldexp(1.0, gk);
+    CompStep& st = plan.steps[i];
+    if (in.op == TICIT_OP_ROTATE) {
+      if (in.nq != gk) throw OracleError("component planner saw a rotation with the wrong active width");
+      plan.quantum_ops++;
+      plan.dense_work += dense_dim;
+      auto t = touched(in);
+      if (t.empty()) {
+        st.kind = kCompIgnoredPhase;
+        continue;
+      }
+      int target = target_of(t);
+      auto src = ordered(target, t);
+      plan.comp_work += merge(src);
+      push_merges(st, src);
+      st.kind = kCompRotation;
+      st.comp = target;
+      st.local = remap(in, target);
+      plan.comp_work += std::ldexp(1.0, int(comps[size_t(target)].coords.size())) + kCompDispatchWork;
+      update_peak();
+    } else if (in.op == TICIT_OP_PROMOTE) {
+      plan.quantum_ops++;
+      plan.dense_work += 2.0 * dense_dim;
+      int coord = int(coord_comp.size());
+      int comp = int(comps.size());
+      comps.push_back(PComp{true, {coord}});
+      order.push_back(coord);
+      coord_comp.push_back(comp);
+      coord_pos.push_back(int(order.size()) - 1);
+      plan.comp_max_k.push_back(1);
+      st.kind = kCompPromotion;
+      st.comp = comp;
+      plan.comp_work += 2.0 + kCompDispatchWork;
+      update_peak();
+    } else if (in.op == TICIT_OP_MEASURE) {
+      if (in.exp_val >= 0) continue;  // expectation probes stay dense
+      if (in.nq != gk) throw OracleError("component planner saw a measurement with the wrong active width");
+      plan.quantum_ops++;
+      plan.dense_work += 2.0 * dense_dim;
+      auto t = touched(in);
+      if (t.empty()) throw OracleError("active measurement has no component support");
+      int target = target_of(t);
+      auto src = ordered(target, t);
+      plan.comp_work += merge(src);
+      update_peak();
+      push_merges(st, src);
+      refresh();
+      Instr l = remap(in, target);
+      if (in.pivot >= gk) throw OracleError("component planner saw an invalid measurement pivot");
+      int pcoord = order[size_t(in.pivot)];
+      auto& cc = comps[size_t(target)].coords;
+      auto it = std::find(cc.begin(), cc.end(), pcoord);
+      if (it == cc.end()) throw OracleError("component measurement pivot is not in its target component");
+      int lpivot = int(it - cc.begin());
+      // src/active.rs:290-339 with the explicit local pivot
+      l.pivot = lpivot;
+      if (l.xmask) {
+        if (!((l.xmask >> lpivot) & 1)) throw OracleError("nondiagonal active measurement pivot must have an X component");
+        l.diagonal = false;
+      } else {
+        if (!((l.zmask >> lpivot) & 1)) throw OracleError("diagonal active measurement pivot must have a Z component");
+        l.diagonal = true;
+      }
+      plan.comp_work += 2.0 * std::ldexp(1.0, int(cc.size())) + kCompDispatchWork;
+      cc.erase(it);
+      order.erase(order.begin() + in.pivot);
+      coord_comp[size_t(pcoord)] = -1;
+      st.kind = kCompMeasurement;
+      st.comp = target;
+      st.local = l;
+      st.deactivate = cc.empty();
+      if (st.deactivate) comps[size_t(target)].active = false;
+      update_peak();
+    }
+  }
+  plan.component_count = int(comps.size());
+  for (int mk : plan.comp_max_k) plan.allocated += size_t(1) << mk;
+  // src/component_plan.rs:282-298 (negated comparisons reject NaN)
+  bool sel = p.max_k >= 8 && plan.quantum_ops >= 4;
+  if (sel && !(plan.dense_work >= kCompWorkRatio * plan.comp_work)) sel = false;
+  if (sel && !(plan.dense_work - plan.comp_work >= kCompMinSaved)) sel = false;
+  if (sel && !(double(plan.allocated) <= kCompAllocRatio * double(plan.dense_peak))) sel = false;
+  plan.selected = sel;
+  return plan;
+}
+
+// src/sampler/batch/mod.rs:597-680: target (x) source in place, walking
+// the source basis downwards so no unread target entry is overwritten.
+void merge_into(double* tre, double* tim, int tk, const double* sre, const double* sim, int sk) {
+  size_t td = size_t(1) << tk, sd = size_t(1) << sk;
+  for (size_t sb = sd; sb-- > 0;) {
+    double sr = sre[sb], si = sim[sb];
+    size_t base = sb << tk;
+    for (size_t tb = 0; tb < td; ++tb) {
+      double tr = tre[tb], ti = tim[tb];
+      tre[base + tb] = tr * sr - ti * si;
+      tim[base + tb] = tr * si + ti * sr;
+    }
+  }
+}
+
 // --------------------------------------------------------------- executor ----
 
 // Where the random decisions of one block come from.
@@ -606,7 +849,31 @@ struct Runner {
   std::vector<char> exogenous;
   std::vector<double> re, im, sre, sim;
   bool fp32 = false;  // emulate FP32 amplitude storage (precision studies)
-  explicit Runner(const Program& prog, bool packed, int sel) : p(prog), bit_packed(packed), selected_observable(sel) {
+  // tensor-component execution (src/sampler/batch/runtime.rs:388-552)
+  CompPlan cplan;
+  bool comp = false;
+  std::vector<size_t> comp_off;
+  size_t comp_stride = 0;
+  std::vector<RotK> crot;
+  std::vector<MeasK> cmeas;
+  explicit Runner(const Program& prog, bool packed, int sel, int component_mode = -1)
+      : p(prog), bit_packed(packed), selected_observable(sel) {
+    cplan = build_component_plan(p);
+    comp = component_mode < 0 ? p.use_components : component_mode != 0;
+    if (comp && cplan.steps.empty()) throw OracleError("program does not contain an executable active component plan");
+    if (comp) {
+      for (int mk : cplan.comp_max_k) {
+        comp_off.push_back(comp_stride);
+        comp_stride += size_t(1) << mk;
+      }
+      crot.resize(p.instrs.size());
+      cmeas.resize(p.instrs.size());
+      for (size_t i = 0; i < p.instrs.size(); ++i) {
+        const CompStep& st = cplan.steps[i];
+        if (st.kind == kCompRotation) crot[i] = rot_kernel(st.local);
+        if (st.kind == kCompMeasurement) cmeas[i] = meas_kernel(st.local);
+      }
+    }
     exogenous.assign(size_t(p.nsymbols) + 1, 0);
     for (const auto& d : p.cats) for (int32_t c : d.conds) exogenous[c] = 1;
     for (const auto& g : p.rare) for (const auto& st : g.sets) for (int32_t c : st) exogenous[c] = 1;
@@ -640,21 +907,56 @@ struct Runner {
   void run_block(const ExoTable& exo, size_t off, size_t shots, Source& src, Outputs& out, double* probs_out,
                  bool want_final) {
     const size_t words = (shots + 63) / 64;
-    const size_t stride = size_t(1) << p.max_k;
+    const size_t stride = comp ? comp_stride : size_t(1) << p.max_k;
     if (re.size() < stride * shots) {
       re.resize(stride * shots);
       im.resize(stride * shots);
       sre.resize(stride);
       sim.resize(stride);
     }
-    // src/sampler/batch/mod.rs:424-436: only the first 2^initial_k entries
-    // are cleared; promotion fully writes every newly exposed half.
-    for (size_t s = 0; s < shots; ++s) {
-      size_t d0 = size_t(1) << p.initial_k;
-      std::fill(re.begin() + s * stride, re.begin() + s * stride + d0, 0.0);
-      std::fill(im.begin() + s * stride, im.begin() + s * stride + d0, 0.0);
-      re[s * stride] = 1.0;
+    std::vector<int> ck;  // component widths (identical for every shot)
+    if (comp) {
+      // src/sampler/batch/mod.rs:366-402: initial components hold |0>
+      ck.assign(cplan.comp_max_k.size(), 0);
+      for (int c = 0; c < cplan.initial_components; ++c) {
+        ck[size_t(c)] = 1;
+        for (size_t s = 0; s < shots; ++s) {
+          size_t b = s * stride + comp_off[size_t(c)];
+          re[b] = 1.0;
+          im[b] = 0.0;
+          re[b + 1] = 0.0;
+          im[b + 1] = 0.0;
+        }
+      }
+    } else {
+      // src/sampler/batch/mod.rs:424-436: only the first 2^initial_k entries
+      // are cleared; promotion fully writes every newly exposed half.
+      for (size_t s = 0; s < shots; ++s) {
+        size_t d0 = size_t(1) << p.initial_k;
+        std::fill(re.begin() + s * stride, re.begin() + s * stride + d0, 0.0);
+        std::fill(im.begin() + s * stride, im.begin() + s * stride + d0, 0.0);
+        re[s * stride] = 1.0;
+      }
     }
+    // src/sampler/batch/mod.rs:597-680 for every shot of the block
+    auto do_merges = [&](const CompStep& st) {
+      if (st.merge_count == 0) throw OracleError("active component instruction has an invalid merge range");
+      int t = cplan.merges[st.merge_off];
+      if (t != st.comp || ck[size_t(t)] == 0) throw OracleError("active component merge has an invalid target");
+      for (size_t m = 1; m < st.merge_count; ++m) {
+        int src = cplan.merges[st.merge_off + m];
+        if (src == t || ck[size_t(src)] == 0) throw OracleError("active component merge has an invalid source");
+        if (ck[size_t(t)] + ck[size_t(src)] > cplan.comp_max_k[size_t(t)])
+          throw OracleError("active component merge target has insufficient storage");
+        for (size_t s = 0; s < shots; ++s) {
+          size_t bt = s * stride + comp_off[size_t(t)], bs = s * stride + comp_off[size_t(src)];
+          merge_into(&re[bt], &im[bt], ck[size_t(t)], &re[bs], &im[bs], ck[size_t(src)]);
+          round_state(bt, size_t(1) << (ck[size_t(t)] + ck[size_t(src)]));
+        }
+        ck[size_t(t)] += ck[size_t(src)];
+        ck[size_t(src)] = 0;
+      }
+    };
     int k = p.initial_k;
     int ndormant = p.n - p.initial_k;
     std::vector<uint64_t> sym(size_t(p.nsymbols) * words, 0);
@@ -709,6 +1011,24 @@ struct Runner {
       switch (in.op) {
         case TICIT_OP_ROTATE: {
           if (in.nq != k) throw OracleError("rotation kernel dimension does not match batch active state");
+          if (comp) {
+            const CompStep& st = cplan.steps[i];
+            if (st.kind == kCompIgnoredPhase) break;
+            eval(in.expr);
+            do_merges(st);
+            RotK rk = crot[i];
+            if (fp32) {
+              rk.c = double(float(rk.c));
+              rk.minus_even = Cx{double(float(rk.minus_even.re)), double(float(rk.minus_even.im))};
+            }
+            size_t dim = size_t(1) << ck[size_t(st.comp)];
+            for (size_t s = 0; s < shots; ++s) {
+              size_t b = s * stride + comp_off[size_t(st.comp)];
+              rotate(&re[b], &im[b], dim, rk, bit(bits, s));
+              round_state(b, dim);
+            }
+            break;
+          }
           eval(in.expr);
           size_t dim = size_t(1) << k;
           RotK rk = rot[i];
@@ -725,6 +1045,28 @@ struct Runner {
         case TICIT_OP_PROMOTE: {
           if (ndormant == 0) throw OracleError("cannot promote a dormant qubit when none remain");
           eval(in.expr);
+          if (comp) {
+            // src/sampler/batch/runtime.rs:511-550
+            const CompStep& st = cplan.steps[i];
+            if (ck[size_t(st.comp)] != 0 || cplan.comp_max_k[size_t(st.comp)] < 1)
+              throw OracleError("active component promotion target is unavailable");
+            double c = std::cos(in.angle), sn = std::sin(in.angle);
+            if (fp32) {
+              c = double(float(c));
+              sn = double(float(sn));
+            }
+            ck[size_t(st.comp)] = 1;
+            for (size_t s = 0; s < shots; ++s) {
+              size_t b = s * stride + comp_off[size_t(st.comp)];
+              re[b] = c;
+              im[b] = 0.0;
+              re[b + 1] = 0.0;
+              im[b + 1] = bit(bits, s) ? sn : -sn;
+            }
+            ++k;
+            --ndormant;
+            break;
+          }
           size_t dim = size_t(1) << k;
           double c = std::cos(in.angle), sn = std::sin(in.angle);
           if (fp32) {
@@ -760,11 +1102,18 @@ struct Runner {
         case TICIT_OP_MEASURE: {
           if (k == 0) throw OracleError("cannot measure an active Pauli when k == 0");
           if (in.nq != k) throw OracleError("measurement kernel dimension does not match batch active state");
-          const MeasK& mk = meas[i];
+          size_t coff = 0;
+          if (comp) {
+            const CompStep& st = cplan.steps[i];
+            if (st.kind != kCompMeasurement) throw OracleError("component measurement step does not match its instruction");
+            do_merges(st);
+            coff = comp_off[size_t(st.comp)];
+          }
+          const MeasK& mk = comp ? cmeas[i] : meas[i];
           std::vector<uint64_t> br(words, 0);
           for (size_t s = 0; s < shots; ++s) {
-            double* r = &re[s * stride];
-            double* m = &im[s * stride];
+            double* r = &re[s * stride + coff];
+            double* m = &im[s * stride + coff];
             double pt = clamp01(prob_true(r, m, mk));
             if (probs_out) probs_out[measure_ordinal[i] * src.shots_total + src.first_shot + s] = pt;
             bool b;
@@ -778,7 +1127,13 @@ struct Runner {
             double ps = b ? pt : 1.0 - pt;
             if (ps <= 0.0) throw OracleError("sampled an impossible active measurement branch");
             project(r, m, sre.data(), sim.data(), mk, b, 1.0 / std::sqrt(ps));
-            round_state(s * stride, mk.out_dim);
+            round_state(s * stride + coff, mk.out_dim);
+          }
+          if (comp) {
+            const CompStep& st = cplan.steps[i];
+            ck[size_t(st.comp)] -= 1;
+            if (st.deactivate && ck[size_t(st.comp)] != 0)
+              throw OracleError("active component measurement did not remove its last coordinate");
           }
           --k;
           ++ndormant;
@@ -808,6 +1163,7 @@ struct Runner {
       }
     }
     if (want_final) {
+      if (comp) throw OracleError("the final state of a component-mode run is not assembled");
       out.final_k = k;
       size_t dim = size_t(1) << k;
       out.final_re.assign(re.begin(), re.begin() + dim);
@@ -884,6 +1240,44 @@ const char* oracle_last_error(void) { return g_err.c_str(); }
 // Injected runs on this thread emulate FP32 amplitude storage when set.
 void oracle_set_fp32_storage(int on) { g_fp32_storage = on != 0; }
 
+// -1: execute as the blob's use_active_components flag says; 0: dense;
+// 1: tensor components (the program must have a component plan).
+void oracle_set_component_mode(int mode) { g_component_mode = mode < 0 ? -1 : (mode ? 1 : 0); }
+
+// The component plan of src/component_plan.rs:301-522.
+// summary[0..8] = selected, initial_components, component_count,
+//   dense_peak_dimension, component_peak_live_dimension,
+//   component_allocated_dimension, quantum ops, merges, planned (steps exist)
+// work[0..1] = estimated dense work, estimated component work
+// comp_max_k[component_count]; steps[ninstr][6] = kind, component,
+//   merge offset, merge count, local pivot, deactivate_after;
+// merges[nmerges]. Arrays are filled up to `cap` entries each.
+int oracle_component_plan(const uint64_t* blob, size_t nwords, int64_t* summary, double* work, int64_t* comp_max_k,
+                          int64_t* steps, int64_t* merges, size_t cap) {
+  try {
+    Program p = decode(blob, nwords);
+    CompPlan c = build_component_plan(p);
+    int64_t v[9] = {c.selected, c.initial_components, c.component_count, int64_t(c.dense_peak),
+                    int64_t(c.peak_live), int64_t(c.allocated), c.quantum_ops, int64_t(c.merges.size()),
+                    !c.steps.empty()};
+    std::memcpy(summary, v, sizeof v);
+    work[0] = c.dense_work;
+    work[1] = c.comp_work;
+    for (size_t i = 0; i < c.comp_max_k.size() && i < cap; ++i) comp_max_k[i] = c.comp_max_k[i];
+    for (size_t i = 0; i < c.steps.size() && i < cap; ++i) {
+      const CompStep& st = c.steps[i];
+      int64_t row[6] = {st.kind, st.comp, int64_t(st.merge_off), int64_t(st.merge_count), st.local.pivot,
+                        st.deactivate};
+      std::memcpy(steps + 6 * i, row, sizeof row);
+    }
+    for (size_t i = 0; i < c.merges.size() && i < cap; ++i) merges[i] = c.merges[i];
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
 // info[0..9] = n, initial_k, max_k, nsymbols, nrecords, ndetectors,
 // nobservables, ninstr, nmeasure, ncoin
 int oracle_info(const uint64_t* blob, size_t nwords, int64_t* info) {
@@ -932,7 +1326,7 @@ int oracle_sample(const uint64_t* blob, size_t nwords, uint64_t shots, uint64_t 
     auto t0 = std::chrono::steady_clock::now();
     auto worker = [&](size_t wid) {
       try {
-        Runner run(p, bit_packed != 0, observable);
+        Runner run(p, bit_packed != 0, observable, g_component_mode);
         Outputs& o = outs[wid];
         o.obs_ones.assign(p.observables.size(), 0);
         ExoTable exo;
@@ -995,7 +1389,7 @@ int oracle_sample_injected(const uint64_t* blob, size_t nwords, uint64_t shots, 
                            double* probs, double* final_re, double* final_im, int* final_k) {
   try {
     Program p = decode(blob, nwords);
-    Runner run(p, bit_packed != 0, observable);
+    Runner run(p, bit_packed != 0, observable, g_component_mode);
     run.fp32 = g_fp32_storage;
     Outputs o;
     o.obs_ones.assign(p.observables.size(), 0);

@@ -110,8 +110,13 @@ def test_wide_fp32_tracks_fp32_storage_emulation(hip):
 def test_wide_final_state_amplitudes(hip, precision, tol):
     text = random_circuit(16, 14, 14, 7, final_measure=False)
     c = hip.Circuit(text)
-    s = c.compile(tier="wide", precision=precision)
-    o = O.sample_injected(c.blob, 1, None, None, None, want_final_state=True)
+    # the dense layout, whichever execution the planner chose
+    s = c.compile(tier="wide", precision=precision, components="dense")
+    O.set_component_mode(0)
+    try:
+        o = O.sample_injected(c.blob, 1, None, None, None, want_final_state=True)
+    finally:
+        O.set_component_mode(-1)
     st = s.final_state(1)
     ref = o["final_state"]
     assert st.shape == ref.shape and ref.size > 1

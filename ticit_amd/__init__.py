@@ -23,6 +23,7 @@ __all__ = ["Circuit", "CompiledSampler", "SampleResult", "TicitError", "plan_tex
 TIERS = {"auto": 0, "lane": 1, "group": 2, "wide": 3}
 PRECISIONS = {"fp64": 0, "fp32": 1}
 PLANNERS = {"heuristic": 0, "learned": 1}
+COMPONENTS = {"planned": 0, "dense": 1, "components": 2}
 
 
 def plan_text(text: str) -> np.ndarray:
@@ -105,7 +106,8 @@ class CompiledSampler:
     backend = "hip"
 
     def __init__(self, blob: np.ndarray, observable: int = 0, precision: str = "fp64",
-                 tier: str = "auto", planner: str = "heuristic", wide_batch: int = 0):
+                 tier: str = "auto", planner: str = "heuristic", wide_batch: int = 0,
+                 components: str = "planned"):
         lib = _capi.load()
         self._lib = lib
         self.blob = np.ascontiguousarray(blob, dtype=np.uint64)
@@ -114,7 +116,8 @@ class CompiledSampler:
         opt.precision = PRECISIONS[precision]
         opt.tier = TIERS[tier]
         opt.planner_mode = PLANNERS[planner]
-        opt.reserved[0] = wide_batch
+        opt.wide_batch = wide_batch
+        opt.components = COMPONENTS[components]
         handle = ctypes.c_void_p()
         _capi.check(lib.ticit_hip_program_create(self.blob.ctypes.data_as(_capi._u64p), self.blob.size,
                                                  ctypes.byref(opt), ctypes.byref(handle)))
@@ -133,6 +136,13 @@ class CompiledSampler:
         self.tier = {1: "lane", 2: "group", 3: "wide"}.get(int(info.tier), "auto")
         self.precision = precision
         self.wide_batch = int(info.wide_batch)
+        self.output_bytes_per_shot = float(info.output_bytes_per_shot)
+        self.state_bytes_per_shot = float(info.state_bytes_per_shot)
+        self.algorithmic_bytes_per_shot = float(info.algorithmic_bytes_per_shot)
+        self.components = bool(info.components)
+        self.state_dim = int(info.state_dim)
+        self.component_count = int(info.component_count)
+        self.max_local_k = int(info.max_local_k)
         self.has_postselection = False
 
     def close(self) -> None:
@@ -242,7 +252,8 @@ class Circuit:
 
     def compile(self, postselection_mask=None, *, observable: int = 0, backend: str = "hip",
                 precision: str = "fp64", tier: str = "auto", planner: str = "heuristic",
-                threads: int | None = None, wide_batch: int = 0) -> CompiledSampler:
+                threads: int | None = None, wide_batch: int = 0,
+                components: str = "planned") -> CompiledSampler:
         """Circuit::compile with ticit_py's ``backend=`` selector; only the
         ``"hip"`` backend lives in this package."""
         if backend != "hip":
@@ -250,7 +261,7 @@ class Circuit:
         if postselection_mask is not None and any(int(v) for v in postselection_mask):
             raise TicitError(3, "detector postselection is not supported by the hip backend")
         return CompiledSampler(self.blob, observable=observable, precision=precision, tier=tier,
-                               planner=planner, wide_batch=wide_batch)
+                               planner=planner, wide_batch=wide_batch, components=components)
 
 
 def device_properties() -> dict:

@@ -18,7 +18,8 @@ _f64p = ctypes.POINTER(ctypes.c_double)
 class Options(ctypes.Structure):
     _fields_ = [("observable", ctypes.c_uint32), ("precision", ctypes.c_uint32),
                 ("planner_mode", ctypes.c_uint32), ("tier", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 4)]
+                ("wide_batch", ctypes.c_uint32), ("components", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 2)]
 
 
 class Result(ctypes.Structure):
@@ -39,7 +40,10 @@ class ProgramInfo(ctypes.Structure):
                 ("instructions", ctypes.c_uint64), ("measure_instructions", ctypes.c_uint64),
                 ("coin_instructions", ctypes.c_uint64), ("tier", ctypes.c_uint32),
                 ("precision", ctypes.c_uint32), ("wide_batch", ctypes.c_uint32),
-                ("reserved0", ctypes.c_uint32)]
+                ("components", ctypes.c_uint32), ("output_bytes_per_shot", ctypes.c_double),
+                ("state_bytes_per_shot", ctypes.c_double), ("algorithmic_bytes_per_shot", ctypes.c_double),
+                ("state_dim", ctypes.c_uint64), ("component_count", ctypes.c_uint64),
+                ("max_local_k", ctypes.c_int32), ("reserved1", ctypes.c_int32)]
 
 
 # ticit_hip_status -> exception kinds of TicitError (src/errors.rs:9-57)

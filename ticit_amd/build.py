@@ -21,6 +21,7 @@ ARCH = "gfx950"
 SOURCES = [
     "csrc/host/frontend.cpp",
     "csrc/host/program.cpp",
+    "csrc/host/component_plan.cpp",
     "csrc/host/device_program.cpp",
     "csrc/host/capi.cpp",
     "csrc/kernels/sampler.hip",

@@ -15,11 +15,11 @@
 namespace ticit {
 
 size_t lane_tier_lds_bytes(int kmax, const LaneLayout& y);
-size_t group_tier_lds_bytes(int max_k, const GroupLayout& y);
+size_t group_tier_lds_bytes(uint32_t state_dim, const GroupLayout& y);
 hipError_t launch_lane_tier(int kmax, const DevProgram& P, const DevLaunch& L, const LaneLayout& Y,
                             hipStream_t stream);
 hipError_t launch_group_tier(const DevProgram& P, const DevLaunch& L, const GroupLayout& Y, hipStream_t stream);
-size_t wide_bytes_per_shot(int max_k, int fp32, const WideLayout& y);
+size_t wide_bytes_per_shot(uint32_t state_dim, int fp32, const WideLayout& y);
 hipError_t launch_wide_tier(const DevProgram& P, const DevLaunch& L, const WideLayout& Y, uint32_t batch, int fp32,
                             hipStream_t stream, double* dump_re, double* dump_im, int* dump_k);
 
@@ -88,8 +88,15 @@ ticit_hip_status create_from_program(Program prog, const ticit_hip_options* opti
   auto* h = new ticit_hip_program();
   h->prog = std::move(prog);
   if (options) h->opt = *options;
+  // Execution layout: the program's own choice (use_active_components,
+  // src/planner.rs:1482-1484) unless the caller forces one.
+  if (h->opt.components > 2) {
+    delete h;
+    return fail(TICIT_HIP_INVALID_INPUT, "components must be 0 (planned), 1 (dense) or 2 (tensor components)");
+  }
+  bool use_components = h->opt.components == 2 || (h->opt.components == 0 && h->prog.use_components);
   try {
-    h->tab = lower_to_device(h->prog);
+    h->tab = lower_to_device(h->prog, use_components);
   } catch (const TicitError& e) {
     delete h;
     return fail(ticit_hip_status(e.kind), e.what());
@@ -127,9 +134,10 @@ ticit_hip_status create_from_program(Program prog, const ticit_hip_options* opti
   UP(u64pool, t.u64pool);
   UP(i32pool, t.i32pool);
 #undef UP
-  d.ninstr = int32_t(h->prog.instrs.size());
+  d.ninstr = t.ninstr;
   d.n = t.n;
-  d.initial_k = t.initial_k;
+  // component runs start from the explicit |0> components of the stream
+  d.initial_k = t.components ? 0 : t.initial_k;
   d.max_k = t.max_k;
   d.nrec = t.nrec;
   d.ndet = t.ndet;
@@ -145,6 +153,9 @@ ticit_hip_status create_from_program(Program prog, const ticit_hip_options* opti
   d.nrare = int32_t(t.rares.size());
   d.nbern = int32_t(t.berns.size());
   d.nlowp = int32_t(t.lowps.size());
+  d.state_dim = t.state_dim;
+  d.ncomp = int32_t(t.comp_off.size());
+  d.max_local_k = t.max_local_k;
   e = hipMalloc(reinterpret_cast<void**>(&h->d_error), sizeof(int));
   if (e != hipSuccess) {
     delete h;
@@ -161,35 +172,33 @@ ticit_hip_status create_from_program(Program prog, const ticit_hip_options* opti
   h->lane.stage_bytes = 64 * ((max_row + 15) / 16 * 16);
   h->wide = WideLayout{h->lane.pw, h->lane.sw, h->lane.rw, h->lane.dw,
                        h->lane.pw + h->lane.sw + h->lane.rw + h->lane.dw,
-                       uint32_t(((size_t(1) << t.max_k) / 2 + 2047) / 2048 + 1)};
+                       uint32_t(((size_t(1) << t.max_local_k) / 2 + 2047) / 2048 + 1)};
+  const bool lane_ok = !t.components && t.max_k <= 5 && lane_tier_lds_bytes(t.max_k, h->lane) <= size_t(lds_limit);
+  const bool group_ok = t.max_local_k <= 12 && group_tier_lds_bytes(t.state_dim, h->group) <= size_t(lds_limit);
   uint32_t want = h->opt.tier;
   if (want == 0 && h->opt.precision == 1) want = kTierWide;  // the FP32 path lives in the wide tier
-  if (want == 0) {
-    if (t.max_k <= 5 && lane_tier_lds_bytes(t.max_k, h->lane) <= size_t(lds_limit)) want = kTierLane;
-    else if (t.max_k <= 12 && group_tier_lds_bytes(t.max_k, h->group) <= size_t(lds_limit)) want = kTierGroup;
-    else want = kTierWide;
-  }
-  if (want == kTierLane && (t.max_k > 5 || lane_tier_lds_bytes(t.max_k, h->lane) > size_t(lds_limit))) {
+  if (want == 0) want = lane_ok ? kTierLane : (group_ok ? kTierGroup : kTierWide);
+  if (want == kTierLane && !lane_ok) {
     delete h;
-    return fail(TICIT_HIP_UNSUPPORTED, "lane tier needs max_k <= 5 and its tables in LDS");
+    return fail(TICIT_HIP_UNSUPPORTED, "lane tier needs a dense max_k <= 5 and its tables in LDS");
   }
-  if (want == kTierGroup && (t.max_k > 12 || group_tier_lds_bytes(t.max_k, h->group) > size_t(lds_limit))) {
+  if (want == kTierGroup && !group_ok) {
     delete h;
     return fail(TICIT_HIP_UNSUPPORTED, "group tier state exceeds the LDS of one workgroup");
   }
   if (want == kTierWide) {
-    if (t.max_k > 30) {
+    if (t.max_local_k > 30 || t.state_dim > (uint32_t(1) << 31)) {
       delete h;
-      return fail(TICIT_HIP_UNSUPPORTED, "active width " + std::to_string(t.max_k) + " exceeds the wide tier");
+      return fail(TICIT_HIP_UNSUPPORTED, "active width " + std::to_string(t.max_local_k) + " exceeds the wide tier");
     }
     // Shots in flight: the reference's workspace policy, at most 16 GiB or
     // two thirds of free memory (src/gpu/mod.rs:680-711), capped by grid y.
     size_t free_b = 0, total_b = 0;
     (void)hipMemGetInfo(&free_b, &total_b);
     size_t budget = std::min<size_t>(size_t(16) << 30, free_b / 3 * 2);
-    size_t per = wide_bytes_per_shot(t.max_k, h->opt.precision == 1, h->wide);
+    size_t per = wide_bytes_per_shot(t.state_dim, h->opt.precision == 1, h->wide);
     size_t batch = per ? budget / per : 1;
-    if (h->opt.reserved[0]) batch = std::min<size_t>(batch, h->opt.reserved[0]);
+    if (h->opt.wide_batch) batch = std::min<size_t>(batch, h->opt.wide_batch);
     batch = std::max<size_t>(1, std::min<size_t>(batch, 65535));
     h->wide_batch = uint32_t(batch);
   }
@@ -291,6 +300,8 @@ ticit_hip_status run_host(ticit_hip_program* h, uint64_t shots, uint64_t seed, i
   double* d_state = nullptr;
   if (state_re) {
     if (h->tier != kTierWide) return fail(TICIT_HIP_UNSUPPORTED, "state dumps are served by the wide tier");
+    if (h->tab.components)
+      return fail(TICIT_HIP_UNSUPPORTED, "state dumps need the dense layout (components = 1)");
     CK(buf.alloc(&d_state, (size_t(2) << h->tab.max_k)), "allocating the state dump");
   }
   hipEvent_t ev0, ev1;
@@ -415,6 +426,15 @@ ticit_hip_status ticit_hip_program_info_get(const ticit_hip_program* h, ticit_hi
   out->tier = h->tier;
   out->precision = h->opt.precision;
   out->wide_batch = h->wide_batch;
+  auto packed = [](int n) { return double((n + 7) / 8); };
+  out->output_bytes_per_shot = packed(h->tab.nrec) + packed(h->tab.ndet) + packed(h->tab.nobs);
+  out->state_bytes_per_shot = h->tab.amp_traffic * (h->opt.precision == 1 ? 8.0 : 16.0);
+  out->components = h->tab.components ? 1 : 0;
+  out->state_dim = h->tab.state_dim;
+  out->component_count = h->tab.comp_off.size();
+  out->max_local_k = h->tab.max_local_k;
+  out->algorithmic_bytes_per_shot =
+      out->output_bytes_per_shot + (h->tier == kTierWide ? out->state_bytes_per_shot : 0.0);
   return TICIT_HIP_OK;
 }
 

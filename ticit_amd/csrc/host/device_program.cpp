@@ -3,10 +3,12 @@
 // src/sampler/batch/active.rs:272-276, 339-343, 739-748) moved to upload time.
 #include "device_program.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <map>
 #include <set>
 
+#include "component_plan.hpp"
 #include "frontend.hpp"
 
 namespace ticit {
@@ -39,7 +41,7 @@ void check_probability(double p) {
 
 }  // namespace
 
-HostTables lower_to_device(const Program& p) {
+HostTables lower_to_device(const Program& p, bool components) {
   HostTables t;
   t.n = p.n;
   t.initial_k = p.initial_k;
@@ -163,40 +165,131 @@ HostTables lower_to_device(const Program& p) {
     assigned[c] = 1;
     return s;
   };
-  int k = p.initial_k;
-  for (const auto& in : p.instrs) {
+  // Execution layout: dense (one component at offset 0) or the planner's
+  // tensor components (src/sampler/batch/runtime.rs:388-552).
+  ComponentPlan plan;
+  std::vector<int32_t> width;  // current width of every component
+  t.components = components;
+  if (components) {
+    plan = plan_components(p);
+    if (!plan.planned) throw TicitError(4, "program does not contain an executable active component plan");
+    uint32_t off = 0;
+    for (int32_t mk : plan.comp_max_k) {
+      t.comp_off.push_back(off);
+      off += uint32_t(1) << mk;
+      t.max_local_k = std::max(t.max_local_k, mk);
+    }
+    t.state_dim = off;
+    width.assign(plan.comp_max_k.size(), 0);
+  } else {
+    t.comp_off.push_back(0);
+    t.state_dim = uint32_t(1) << p.max_k;
+    t.max_local_k = p.max_k;
+  }
+  auto blank = [](uint32_t op) {
     DevInstr d{};
-    d.op = uint32_t(in.op);
-    d.nq = in.nq;
+    d.op = op;
     d.partial = -1;
     d.record = -1;
     d.rc_slot = -1;
     d.branch_slot = -1;
     d.detector = -1;
     d.ordinal = -1;
+    d.src_comp = -1;
+    return d;
+  };
+  // src/active.rs:193-229
+  auto fill_rotation = [&](DevInstr& d, const Instr& in) {
+    d.c = std::cos(in.angle);
+    d.s = std::sin(in.angle);
+    Cx e = phase_factor(in.phase);
+    double a = 0.0, b = -d.s;  // (0 - i sin) * i^phase, in num-complex's order
+    d.me_re = a * e.re - b * e.im;
+    d.me_im = a * e.im + b * e.re;
+    d.xmask = in.xmask;
+    d.zmask = in.zmask;
+    if (in.xmask == 0) {
+      d.op |= kFlagDiagonal;
+    } else {
+      if (in.zmask == 0) d.op |= kFlagUniformImag;
+      d.pivot = 63 - __builtin_clzll(in.xmask);
+    }
+    if (__builtin_popcountll(in.xmask & in.zmask) & 1) d.op |= kFlagXzOdd;
+  };
+  // src/active.rs:290-339
+  auto fill_measure = [&](DevInstr& d, const Instr& in) {
+    d.xmask = in.xmask;
+    d.zmask = in.zmask;
+    d.pivot = in.pivot;
+    if (in.diagonal) {
+      d.op |= kFlagDiagonal;
+      if (in.diag_phase_bit) d.op |= kFlagDiagPhase;
+      d.zwop = in.zmask & ~(1ull << in.pivot);
+    }
+    Cx e = phase_factor(in.phase);
+    d.c1_re = e.re * kInvSqrt2;
+    d.c1_im = -e.im * kInvSqrt2;
+  };
+  // src/sampler/batch/mod.rs:597-680: every source of a step folded into its
+  // component, each fold its own device instruction
+  auto emit_merges = [&](const ComponentStep& st) {
+    if (width[size_t(st.comp)] == 0) throw TicitError(4, "active component merge target is inactive");
+    for (int32_t src : st.sources) {
+      if (width[size_t(src)] == 0) throw TicitError(4, "active component merge source is inactive");
+      DevInstr m = blank(kOpMerge);
+      m.comp = st.comp;
+      m.comp_off = t.comp_off[size_t(st.comp)];
+      m.nq = width[size_t(st.comp)];
+      m.src_comp = src;
+      m.src_off = t.comp_off[size_t(src)];
+      m.src_k = width[size_t(src)];
+      if (m.nq + m.src_k > plan.comp_max_k[size_t(st.comp)])
+        throw TicitError(4, "active component merge target has insufficient storage");
+      t.amp_traffic += std::ldexp(1.0, m.nq) + std::ldexp(1.0, m.src_k) + std::ldexp(1.0, m.nq + m.src_k);
+      width[size_t(st.comp)] += m.src_k;
+      width[size_t(src)] = 0;
+      t.instrs.push_back(m);
+    }
+  };
+  if (components) {
+    // src/sampler/batch/mod.rs:366-402: initial components hold |0>
+    for (int32_t c = 0; c < plan.initial_components; ++c) {
+      DevInstr z = blank(kOpPromote | kFlagFresh | kFlagZeroState);
+      z.comp = c;
+      z.comp_off = t.comp_off[size_t(c)];
+      z.op |= kFlagConstant;  // never evaluated: a fixed state
+      width[size_t(c)] = 1;
+      t.instrs.push_back(z);
+    }
+  }
+
+  int k = p.initial_k;
+  for (size_t ii = 0; ii < p.instrs.size(); ++ii) {
+    const Instr& in = p.instrs[ii];
+    const ComponentStep* st = components ? &plan.steps[ii] : nullptr;
+    DevInstr d = blank(uint32_t(in.op));
+    d.nq = in.nq;
     if (in.record < 0 || in.record > p.nrecords) throw TicitError(1, "measurement record id out of range");
     if (in.record > 0) d.record = in.record - 1;
     if (in.expr.constant) d.op |= kFlagConstant;
     switch (in.op) {
       case Op::Rotate: {
         if (in.nq != k) throw TicitError(4, "rotation kernel dimension does not match batch active state");
-        d.c = std::cos(in.angle);
-        d.s = std::sin(in.angle);
-        Cx e = phase_factor(in.phase);
-        double a = 0.0, b = -d.s;  // (0 - i sin) * i^phase, in num-complex's order
-        d.me_re = a * e.re - b * e.im;
-        d.me_im = a * e.im + b * e.re;
-        d.xmask = in.xmask;
-        d.zmask = in.zmask;
-        if (in.xmask == 0) {
-          d.op |= kFlagDiagonal;
-        } else {
-          if (in.zmask == 0) d.op |= kFlagUniformImag;
-          d.pivot = 63 - __builtin_clzll(in.xmask);
-        }
-        if (__builtin_popcountll(in.xmask & in.zmask) & 1) d.op |= kFlagXzOdd;
         t.quantum_ops++;
         t.dense_work += std::ldexp(1.0, k);
+        if (st) {
+          if (st->kind == StepKind::IgnoredPhase) continue;  // a phase on the k == 0 scalar
+          emit_merges(*st);
+          if (st->local.nq != width[size_t(st->comp)])
+            throw TicitError(4, "component rotation width does not match its component");
+          fill_rotation(d, st->local);
+          d.nq = st->local.nq;
+          d.comp = st->comp;
+          d.comp_off = t.comp_off[size_t(st->comp)];
+        } else {
+          fill_rotation(d, in);
+        }
+        t.amp_traffic += std::ldexp(2.0, d.nq);
         break;
       }
       case Op::Promote:
@@ -206,6 +299,17 @@ HostTables lower_to_device(const Program& p) {
         d.s = std::sin(in.angle);
         t.quantum_ops++;
         t.dense_work += std::ldexp(3.0, k);
+        if (st) {
+          if (width[size_t(st->comp)] != 0) throw TicitError(4, "active component promotion target is unavailable");
+          d.op |= kFlagFresh;
+          d.nq = 0;
+          d.comp = st->comp;
+          d.comp_off = t.comp_off[size_t(st->comp)];
+          width[size_t(st->comp)] = 1;
+          t.amp_traffic += 2.0;
+        } else {
+          t.amp_traffic += std::ldexp(3.0, k);
+        }
         ++k;
         break;
       case Op::Record:
@@ -227,21 +331,26 @@ HostTables lower_to_device(const Program& p) {
       case Op::Measure: {
         if (k == 0) throw TicitError(4, "cannot measure an active Pauli when k == 0");
         if (in.nq != k) throw TicitError(4, "measurement kernel dimension does not match batch active state");
-        d.xmask = in.xmask;
-        d.zmask = in.zmask;
-        d.pivot = in.pivot;
-        if (in.diagonal) {
-          d.op |= kFlagDiagonal;
-          if (in.diag_phase_bit) d.op |= kFlagDiagPhase;
-          d.zwop = in.zmask & ~(1ull << in.pivot);
-        }
-        Cx e = phase_factor(in.phase);
-        d.c1_re = e.re * kInvSqrt2;
-        d.c1_im = -e.im * kInvSqrt2;
-        d.ordinal = t.nmeasure++;
-        d.branch_slot = assign(in.branch);
         t.quantum_ops++;
         t.dense_work += std::ldexp(2.5, k);
+        if (st) {
+          if (st->kind != StepKind::Measurement) throw TicitError(4, "component measurement step does not match its instruction");
+          emit_merges(*st);
+          if (st->local.nq != width[size_t(st->comp)])
+            throw TicitError(4, "component measurement width does not match its component");
+          fill_measure(d, st->local);
+          d.nq = st->local.nq;
+          d.comp = st->comp;
+          d.comp_off = t.comp_off[size_t(st->comp)];
+          width[size_t(st->comp)] -= 1;
+          if (st->deactivate && width[size_t(st->comp)] != 0)
+            throw TicitError(4, "active component measurement did not remove its last coordinate");
+        } else {
+          fill_measure(d, in);
+        }
+        t.amp_traffic += std::ldexp(2.5, d.nq);
+        d.ordinal = t.nmeasure++;
+        d.branch_slot = assign(in.branch);
         --k;
         break;
       }
@@ -298,6 +407,7 @@ HostTables lower_to_device(const Program& p) {
     }
     t.obs_off.push_back(uint32_t(t.obs_recs.size()));
   }
+  t.ninstr = int32_t(t.instrs.size());
   // keep every table non-empty so device pointers are always valid
   if (t.resid.empty()) t.resid.push_back(0);
   if (t.det_recs.empty()) t.det_recs.push_back(0);

@@ -26,11 +26,24 @@ struct HostTables {
   size_t expression_terms = 0;
   size_t quantum_ops = 0;
   double dense_work = 0.0;  // sum over amplitude instructions of 2^k
+  // amplitudes moved per shot when the state streams from memory: a rotation
+  // reads and writes 2^k, a promotion reads 2^k and writes 2^(k+1), a
+  // measurement reads 2^k twice (probability, projection) and writes 2^(k-1)
+  double amp_traffic = 0.0;
+  // execution layout: one component spanning the state (dense) or the
+  // planner's tensor components
+  bool components = false;
+  uint32_t state_dim = 0;
+  std::vector<uint32_t> comp_off;  // per component
+  int32_t max_local_k = 0;
+  int32_t ninstr = 0;  // device instructions (merges included, ignored phases dropped)
 };
 
 // Validates the program (every residual symbol assigned before use, active
 // widths consistent with the opcode sequence, ids in range) and builds the
 // tables. Throws TicitError on failure.
-HostTables lower_to_device(const Program& p);
+// With components = true the instruction stream is lowered through the
+// tensor-component plan (throws if the program has none).
+HostTables lower_to_device(const Program& p, bool components = false);
 
 }  // namespace ticit

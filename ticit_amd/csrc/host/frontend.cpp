@@ -3,6 +3,8 @@
 // follows; the code itself is written for this library.
 #include "frontend.hpp"
 
+#include "component_plan.hpp"
+
 #include <algorithm>
 #include <cctype>
 #include <cmath>
@@ -1598,6 +1600,8 @@ Program freeze(int n, int initial_k, int max_k, std::vector<Instr> instrs, Conte
       p.bern_probs.push_back(kv.second);
     }
   }
+  // src/planner.rs:1482-1484: the component cost model decides the execution
+  p.use_components = plan_components(p).selected;
   return p;
 }
 

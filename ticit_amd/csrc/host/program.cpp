@@ -60,6 +60,7 @@ std::vector<uint64_t> encode_program(const Program& p) {
   o.push_back(uint64_t(p.nrecords));
   o.push_back(uint64_t(p.ndetectors));
   o.push_back(uint64_t(p.nexpvals));
+  o.push_back(p.use_components ? TICIT_PLAN_FLAG_COMPONENTS : 0);
   o.push_back(uint64_t(p.instrs.size()));
   for (const auto& in : p.instrs) {
     uint64_t w0 = uint64_t(uint8_t(in.op)) | (uint64_t(in.diagonal) << 8) |
@@ -135,7 +136,8 @@ bool decode_program(const uint64_t* words, size_t nwords, Program& p, std::strin
     err = "plan blob has a bad magic word";
     return false;
   }
-  if (r.next() != TICIT_PLAN_VERSION) {
+  const uint64_t version = r.next();
+  if (version != 1 && version != TICIT_PLAN_VERSION) {
     err = "plan blob has an unsupported version";
     return false;
   }
@@ -147,6 +149,7 @@ bool decode_program(const uint64_t* words, size_t nwords, Program& p, std::strin
   p.nrecords = r.i32();
   p.ndetectors = r.i32();
   p.nexpvals = r.i32();
+  if (version >= 2) p.use_components = (r.next() & TICIT_PLAN_FLAG_COMPONENTS) != 0;
   size_t ninstr = r.count(8);
   p.instrs.resize(ninstr);
   for (size_t i = 0; i < ninstr && r.ok; ++i) {

@@ -90,6 +90,9 @@ struct Program {
   int32_t nrecords = 0;
   int32_t ndetectors = 0;
   int32_t nexpvals = 0;
+  // FactoredInstructionProgram::use_active_components (src/factored.rs:885):
+  // the planner selected the tensor-component execution of the active state
+  bool use_components = false;
   std::vector<Instr> instrs;
   std::vector<CategoricalDist> categorical;
   std::vector<RareGroup> rare;

@@ -19,6 +19,7 @@ enum : uint32_t {
   kOpDetector = 3,
   kOpMeasure = 4,
   kOpCoin = 5,
+  kOpMerge = 6,  // device only: component src folded into component comp
 };
 
 enum : uint32_t {
@@ -28,6 +29,8 @@ enum : uint32_t {
   kFlagConstant = 1u << 11,     // expression constant
   kFlagDiagPhase = 1u << 12,    // diagonal measurement of a -1 coefficient body
   kFlagDetRecords = 1u << 13,   // detector reads record bits, not its expression
+  kFlagFresh = 1u << 14,        // promotion that starts a new one-qubit component
+  kFlagZeroState = 1u << 15,    // fresh component in |0> (initial components)
 };
 
 struct alignas(16) DevInstr {
@@ -44,7 +47,11 @@ struct alignas(16) DevInstr {
   uint32_t det_len;
   int32_t pivot;        // measurement pivot / rotation pair bit
   int32_t ordinal;      // Measure: measurement ordinal; Coin: coin ordinal
-  int32_t pad0;
+  uint32_t comp_off;    // amplitude offset of the component acted on (0 dense)
+  int32_t comp;         // component id (0 dense)
+  int32_t src_comp;     // Merge: component folded in
+  uint32_t src_off;     // Merge: its amplitude offset
+  int32_t src_k;        // Merge: its width
   uint64_t xmask;
   uint64_t zmask;
   uint64_t zwop;        // z mask without the pivot bit (diagonal measurement)
@@ -121,7 +128,10 @@ struct DevProgram {
   int32_t nrec, ndet, nobs, nslots;
   int32_t npartials, nsym, nexo, nmeasure;
   int32_t ncoin, ncat, nrare, nbern;
-  int32_t nlowp, pad0, pad1, pad2;
+  int32_t nlowp;
+  uint32_t state_dim;   // amplitudes per shot: 2^max_k dense, sum of component capacities otherwise
+  int32_t ncomp;        // components (1 dense)
+  int32_t max_local_k;  // widest component (max_k dense)
 };
 
 // Per-launch sampling request.

@@ -293,7 +293,7 @@ __global__ __launch_bounds__(kGroupThreads) void group_tier_kernel(DevProgram P,
   const int tid = threadIdx.x;
   const uint64_t local = blockIdx.x;
   const uint64_t shot = L.first_shot + local;
-  const int DIM = 1 << P.max_k;
+  const int DIM = int(P.state_dim);
   double* re = reinterpret_cast<double*>(smem);
   double* im = re + DIM;
   double* red = im + DIM;  // [kGroupThreads / 64 + 1]
@@ -340,15 +340,20 @@ __global__ __launch_bounds__(kGroupThreads) void group_tier_kernel(DevProgram P,
     return b;
   };
 
-  int k = P.initial_k;
   uint64_t brng = seed_for(kBranchSeedBase, L.seed, shot);  // identical in every thread
   int err = 0;
+  double* const re0 = re;
+  double* const im0 = im;
   for (int i = 0; i < P.ninstr; ++i) {
     const DevInstr in = P.instrs[i];
     const uint32_t op = in.op & 0xff;
+    // every amplitude instruction acts on one component: 2^nq amplitudes at
+    // comp_off (the whole state, offset 0, when the state is dense)
+    re = re0 + in.comp_off;
+    im = im0 + in.comp_off;
     if (op == kOpRotate) {
       const bool sign = eval(in);
-      const int dim = 1 << k;
+      const int dim = 1 << in.nq;
       const double c = in.c;
       if (in.op & kFlagDiagonal) {
         for (int b = tid; b < dim; b += kGroupThreads) {
@@ -385,16 +390,47 @@ __global__ __launch_bounds__(kGroupThreads) void group_tier_kernel(DevProgram P,
       __syncthreads();
     } else if (op == kOpPromote) {
       const bool sign = eval(in);
-      const int dim = 1 << k;
       const double c = in.c, q = sign ? in.s : -in.s;
-      for (int b = tid; b < dim; b += kGroupThreads) {
-        double r = re[b], m = im[b];
-        re[b] = c * r;
-        im[b] = c * m;
-        re[dim + b] = -q * m;
-        im[dim + b] = q * r;
+      if (in.op & kFlagFresh) {
+        // src/sampler/batch/runtime.rs:511-550: a new one-qubit component
+        if (tid == 0) {
+          const bool zero = in.op & kFlagZeroState;
+          re[0] = zero ? 1.0 : c;
+          im[0] = 0.0;
+          re[1] = 0.0;
+          im[1] = zero ? 0.0 : q;
+        }
+      } else {
+        const int dim = 1 << in.nq;
+        for (int b = tid; b < dim; b += kGroupThreads) {
+          double r = re[b], m = im[b];
+          re[b] = c * r;
+          im[b] = c * m;
+          re[dim + b] = -q * m;
+          im[dim + b] = q * r;
+        }
       }
-      ++k;
+      __syncthreads();
+    } else if (op == kOpMerge) {
+      // src/sampler/batch/mod.rs:597-680: target (x) source into the target's
+      // storage. Outputs with a non-zero source index lie above the target's
+      // live 2^nq entries and go first; the in-place zero-index slice after
+      // a barrier.
+      const int tk = in.nq, td = 1 << in.nq, total = 1 << (in.nq + in.src_k);
+      const double* sr = re0 + in.src_off;
+      const double* si = im0 + in.src_off;
+      for (int o = td + tid; o < total; o += kGroupThreads) {
+        const int tb = o & (td - 1), sb = o >> tk;
+        const double a = re[tb], b = im[tb], cr = sr[sb], ci = si[sb];
+        re[o] = a * cr - b * ci;
+        im[o] = a * ci + b * cr;
+      }
+      __syncthreads();
+      for (int o = tid; o < td; o += kGroupThreads) {
+        const double a = re[o], b = im[o], cr = sr[0], ci = si[0];
+        re[o] = a * cr - b * ci;
+        im[o] = a * ci + b * cr;
+      }
       __syncthreads();
     } else if (op == kOpRecord || op == kOpDetector) {
       bool b = false;
@@ -418,7 +454,7 @@ __global__ __launch_bounds__(kGroupThreads) void group_tier_kernel(DevProgram P,
     } else if (op == kOpMeasure || op == kOpCoin) {
       bool branch = false;
       if (op == kOpMeasure) {
-        const int out_dim = (1 << k) >> 1;
+        const int out_dim = (1 << in.nq) >> 1;
         const bool diag = in.op & kFlagDiagonal;
         double part = 0.0;
         for (int idx = tid; idx < out_dim; idx += kGroupThreads) {
@@ -487,7 +523,6 @@ __global__ __launch_bounds__(kGroupThreads) void group_tier_kernel(DevProgram P,
             im[idx] = ni[j];
           }
         }
-        --k;
       } else {
         if (L.injected) {
           const uint64_t words = (L.shots + 63) / 64;
@@ -551,8 +586,8 @@ size_t lane_tier_lds_bytes(int kmax, const LaneLayout& y) {
   return (dim + sdim) * 64 * 16 + size_t(y.pw + y.sw + y.rw + y.dw) * 64 * 8 + y.stage_bytes;
 }
 
-size_t group_tier_lds_bytes(int max_k, const GroupLayout& y) {
-  size_t dim = size_t(1) << max_k;
+size_t group_tier_lds_bytes(uint32_t state_dim, const GroupLayout& y) {
+  size_t dim = state_dim;
   return dim * 16 + (kGroupThreads / 64 + 2) * 8 + size_t(y.pw + y.sw + y.rw + y.dw) * 8;
 }
 
@@ -581,8 +616,8 @@ hipError_t launch_lane_tier(int kmax, const DevProgram& P, const DevLaunch& L, c
 }
 
 hipError_t launch_group_tier(const DevProgram& P, const DevLaunch& L, const GroupLayout& Y, hipStream_t stream) {
-  if (P.max_k > 12) return hipErrorInvalidValue;
-  size_t lds = group_tier_lds_bytes(P.max_k, Y);
+  if (P.max_local_k > 12) return hipErrorInvalidValue;  // projections hold 2^11 outputs in registers
+  size_t lds = group_tier_lds_bytes(P.state_dim, Y);
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&group_tier_kernel),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
   if (e != hipSuccess) return e;

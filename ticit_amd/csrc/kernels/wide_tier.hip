@@ -151,8 +151,8 @@ __global__ __launch_bounds__(kWideThreads) void wide_rotate_kernel(DevProgram P,
   const uint32_t j = blockIdx.y;
   const DevInstr in = P.instrs[instr];
   const bool sign = wide_eval(P, in, wide_shot(bits, Y, j));
-  T* R = re + size_t(j) * dim;
-  T* I = im + size_t(j) * dim;
+  T* R = re + size_t(j) * dim + in.comp_off;
+  T* I = im + size_t(j) * dim + in.comp_off;
   const size_t n = size_t(1) << k;
   const T c = T(in.c);
   if (in.op & kFlagDiagonal) {
@@ -204,8 +204,8 @@ __global__ __launch_bounds__(kWideThreads) void wide_promote_kernel(DevProgram P
   const uint32_t j = blockIdx.y;
   const DevInstr in = P.instrs[instr];
   const bool sign = wide_eval(P, in, wide_shot(bits, Y, j));
-  T* R = re + size_t(j) * dim;
-  T* I = im + size_t(j) * dim;
+  T* R = re + size_t(j) * dim + in.comp_off;
+  T* I = im + size_t(j) * dim + in.comp_off;
   const size_t n = size_t(1) << k;
   const T c = T(in.c), q = T(sign ? in.s : -in.s);
 #pragma unroll
@@ -217,6 +217,51 @@ __global__ __launch_bounds__(kWideThreads) void wide_promote_kernel(DevProgram P
     I[b] = c * m;
     R[n + b] = -q * m;
     I[n + b] = q * r;
+  }
+}
+
+// A promotion that starts a new one-qubit component, or an initial |0>
+// component (src/sampler/batch/runtime.rs:511-550, batch/mod.rs:376-402).
+template <class T>
+__global__ __launch_bounds__(kWideThreads) void wide_fresh_kernel(DevProgram P, WideLayout Y, int instr,
+                                                                  uint32_t nshots, uint64_t* bits, T* re, T* im,
+                                                                  size_t dim) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nshots) return;
+  const DevInstr in = P.instrs[instr];
+  const bool zero = in.op & kFlagZeroState;
+  const bool sign = zero ? false : wide_eval(P, in, wide_shot(bits, Y, j));
+  T* R = re + size_t(j) * dim + in.comp_off;
+  T* I = im + size_t(j) * dim + in.comp_off;
+  R[0] = zero ? T(1) : T(in.c);
+  I[0] = T(0);
+  R[1] = T(0);
+  I[1] = zero ? T(0) : T(sign ? in.s : -in.s);
+}
+
+// target (x) source of src/sampler/batch/mod.rs:597-680, written into the
+// target's partner buffer: out[(sb << tk) + tb] = t[tb] * s[sb].
+template <class T>
+__global__ __launch_bounds__(kWideThreads) void wide_merge_kernel(DevProgram P, int instr, const T* tre,
+                                                                  const T* tim, const T* sre, const T* sim,
+                                                                  T* ore, T* oim, size_t dim) {
+  const uint32_t j = blockIdx.y;
+  const DevInstr in = P.instrs[instr];
+  const size_t tk = size_t(in.nq), td = size_t(1) << tk, total = size_t(1) << (tk + size_t(in.src_k));
+  const T* TR = tre + size_t(j) * dim + in.comp_off;
+  const T* TI = tim + size_t(j) * dim + in.comp_off;
+  const T* SR = sre + size_t(j) * dim + in.src_off;
+  const T* SI = sim + size_t(j) * dim + in.src_off;
+  T* OR = ore + size_t(j) * dim + in.comp_off;
+  T* OI = oim + size_t(j) * dim + in.comp_off;
+#pragma unroll
+  for (int u = 0; u < kWidePer; ++u) {
+    size_t o = size_t(blockIdx.x) * kWideTile + size_t(u) * kWideThreads + threadIdx.x;
+    if (o >= total) break;
+    size_t tb = o & (td - 1), sb = o >> tk;
+    T a = TR[tb], b = TI[tb], cr = SR[sb], ci = SI[sb];
+    OR[o] = a * cr - b * ci;
+    OI[o] = a * ci + b * cr;
   }
 }
 
@@ -237,8 +282,8 @@ __global__ __launch_bounds__(kWideThreads) void wide_prob_kernel(DevProgram P, i
   __shared__ double red[kWideThreads / 64];
   const uint32_t j = blockIdx.y;
   const DevInstr in = P.instrs[instr];
-  const T* R = re + size_t(j) * dim;
-  const T* I = im + size_t(j) * dim;
+  const T* R = re + size_t(j) * dim + in.comp_off;
+  const T* I = im + size_t(j) * dim + in.comp_off;
   const size_t out_dim = (size_t(1) << k) >> 1;
   const bool diag = in.op & kFlagDiagonal;
   double acc = 0.0;
@@ -312,10 +357,10 @@ __global__ __launch_bounds__(kWideThreads) void wide_project_kernel(DevProgram P
   const DevInstr in = P.instrs[instr];
   const bool branch = ctl[j].branch;
   const double inv = ctl[j].invnorm;
-  const T* R = re + size_t(j) * dim;
-  const T* I = im + size_t(j) * dim;
-  T* OR = ore + size_t(j) * dim;
-  T* OI = oim + size_t(j) * dim;
+  const T* R = re + size_t(j) * dim + in.comp_off;
+  const T* I = im + size_t(j) * dim + in.comp_off;
+  T* OR = ore + size_t(j) * dim + in.comp_off;
+  T* OI = oim + size_t(j) * dim + in.comp_off;
   const size_t out_dim = (size_t(1) << k) >> 1;
   const bool diag = in.op & kFlagDiagonal;
 #pragma unroll
@@ -395,8 +440,8 @@ struct WideWorkspace {
   double* part = nullptr;
 };
 
-size_t wide_bytes_per_shot(int max_k, int fp32, const WideLayout& y) {
-  size_t dim = size_t(1) << max_k;
+size_t wide_bytes_per_shot(uint32_t state_dim, int fp32, const WideLayout& y) {
+  size_t dim = state_dim;
   size_t t = fp32 ? 4 : 8;
   return 4 * dim * t + size_t(y.words) * 8 + sizeof(WideCtl) + size_t(y.max_tiles) * 8;
 }
@@ -404,7 +449,7 @@ size_t wide_bytes_per_shot(int max_k, int fp32, const WideLayout& y) {
 template <class T>
 static hipError_t run_wide(const DevProgram& P, const DevLaunch& L, const WideLayout& Y, uint32_t batch,
                            hipStream_t stream, double* dump_re, double* dump_im, int* dump_k) {
-  const size_t dim = size_t(1) << P.max_k;
+  const size_t dim = P.state_dim;
   WideWorkspace ws;
   hipError_t e = hipSuccess;
   auto cleanup = [&]() {
@@ -437,14 +482,18 @@ static hipError_t run_wide(const DevProgram& P, const DevLaunch& L, const WideLa
   for (uint64_t start = 0; start < L.shots; start += batch) {
     uint32_t n = uint32_t(std::min<uint64_t>(batch, L.shots - start));
     dim3 sg((n + per_shot_blocks - 1) / per_shot_blocks), sb(per_shot_blocks);
-    int cur = 0;
-    hipLaunchKernelGGL(wide_init_kernel<T>, sg, sb, 0, stream, P, L, Y, start, n, ws.bits, ws.ctl,
-                       static_cast<T*>(ws.re[0]), static_cast<T*>(ws.im[0]), dim);
+    // which of the two buffers holds each component's live amplitudes
+    std::vector<int> cur(size_t(std::max(P.ncomp, 1)), 0);
+    auto RE = [&](int b) { return static_cast<T*>(ws.re[b]); };
+    auto IM = [&](int b) { return static_cast<T*>(ws.im[b]); };
+    hipLaunchKernelGGL(wide_init_kernel<T>, sg, sb, 0, stream, P, L, Y, start, n, ws.bits, ws.ctl, RE(0), IM(0),
+                       dim);
     WCK(hipGetLastError());
-    int k = P.initial_k;
+    int k = P.initial_k;  // dense width, for the final-state dump
     int i = 0;
     while (i < P.ninstr) {
-      const uint32_t op = instrs[size_t(i)].op & 0xff;
+      const DevInstr& in = instrs[size_t(i)];
+      const uint32_t op = in.op & 0xff;
       if (op == kOpRecord || op == kOpDetector || op == kOpCoin) {
         int e2 = i;
         while (e2 < P.ninstr) {
@@ -457,32 +506,47 @@ static hipError_t run_wide(const DevProgram& P, const DevLaunch& L, const WideLa
         i = e2;
         continue;
       }
-      T* R = static_cast<T*>(ws.re[cur]);
-      T* I = static_cast<T*>(ws.im[cur]);
+      const int cb = cur[size_t(in.comp)];
+      T* R = RE(cb);
+      T* I = IM(cb);
+      const int w = in.nq;  // width of the component acted on
       if (op == kOpRotate) {
-        size_t work = instrs[size_t(i)].op & kFlagDiagonal ? (size_t(1) << k) : (size_t(1) << k) >> 1;
+        size_t work = in.op & kFlagDiagonal ? (size_t(1) << w) : (size_t(1) << w) >> 1;
         dim3 g(unsigned((work + kWideTile - 1) / kWideTile), n);
-        hipLaunchKernelGGL(wide_rotate_kernel<T>, g, dim3(kWideThreads), 0, stream, P, Y, i, ws.bits, R, I, dim, k);
+        hipLaunchKernelGGL(wide_rotate_kernel<T>, g, dim3(kWideThreads), 0, stream, P, Y, i, ws.bits, R, I, dim, w);
         WCK(hipGetLastError());
       } else if (op == kOpPromote) {
-        dim3 g(unsigned(((size_t(1) << k) + kWideTile - 1) / kWideTile), n);
-        hipLaunchKernelGGL(wide_promote_kernel<T>, g, dim3(kWideThreads), 0, stream, P, Y, i, ws.bits, R, I, dim, k);
+        if (in.op & kFlagFresh) {
+          hipLaunchKernelGGL(wide_fresh_kernel<T>, sg, sb, 0, stream, P, Y, i, n, ws.bits, R, I, dim);
+        } else {
+          dim3 g(unsigned(((size_t(1) << w) + kWideTile - 1) / kWideTile), n);
+          hipLaunchKernelGGL(wide_promote_kernel<T>, g, dim3(kWideThreads), 0, stream, P, Y, i, ws.bits, R, I, dim,
+                             w);
+          ++k;
+        }
         WCK(hipGetLastError());
-        ++k;
+      } else if (op == kOpMerge) {
+        const int sbuf = cur[size_t(in.src_comp)];
+        size_t total = size_t(1) << (w + in.src_k);
+        dim3 g(unsigned((total + kWideTile - 1) / kWideTile), n);
+        hipLaunchKernelGGL(wide_merge_kernel<T>, g, dim3(kWideThreads), 0, stream, P, i, R, I, RE(sbuf), IM(sbuf),
+                           RE(cb ^ 1), IM(cb ^ 1), dim);
+        WCK(hipGetLastError());
+        cur[size_t(in.comp)] ^= 1;
       } else {  // Measure
-        size_t out_dim = (size_t(1) << k) >> 1;
+        size_t out_dim = (size_t(1) << w) >> 1;
         unsigned tiles = unsigned((out_dim + kWideTile - 1) / kWideTile);
         dim3 g(tiles, n);
-        hipLaunchKernelGGL(wide_prob_kernel<T>, g, dim3(kWideThreads), 0, stream, P, i, R, I, dim, k, ws.part,
+        hipLaunchKernelGGL(wide_prob_kernel<T>, g, dim3(kWideThreads), 0, stream, P, i, R, I, dim, w, ws.part,
                            Y.max_tiles);
         WCK(hipGetLastError());
         hipLaunchKernelGGL(wide_decide_kernel, sg, sb, 0, stream, P, L, Y, start, n, i, ws.bits, ws.ctl, ws.part,
                            tiles, Y.max_tiles);
         WCK(hipGetLastError());
-        hipLaunchKernelGGL(wide_project_kernel<T>, g, dim3(kWideThreads), 0, stream, P, i, ws.ctl, R, I,
-                           static_cast<T*>(ws.re[cur ^ 1]), static_cast<T*>(ws.im[cur ^ 1]), dim, k);
+        hipLaunchKernelGGL(wide_project_kernel<T>, g, dim3(kWideThreads), 0, stream, P, i, ws.ctl, R, I, RE(cb ^ 1),
+                           IM(cb ^ 1), dim, w);
         WCK(hipGetLastError());
-        cur ^= 1;
+        cur[size_t(in.comp)] ^= 1;
         --k;
       }
       ++i;
@@ -490,9 +554,13 @@ static hipError_t run_wide(const DevProgram& P, const DevLaunch& L, const WideLa
     hipLaunchKernelGGL(wide_finish_kernel, sg, sb, 0, stream, P, L, Y, start, n, ws.bits, ws.ctl);
     WCK(hipGetLastError());
     if (dump_re && start == 0) {
+      if (P.ncomp != 1) {
+        cleanup();
+        return hipErrorInvalidValue;  // component states are not assembled
+      }
       size_t nd = size_t(1) << k;
       hipLaunchKernelGGL(wide_dump_kernel<T>, dim3(unsigned((nd + 255) / 256)), dim3(256), 0, stream,
-                         static_cast<const T*>(ws.re[cur]), static_cast<const T*>(ws.im[cur]), nd, dump_re,
+                         static_cast<const T*>(RE(cur[0])), static_cast<const T*>(IM(cur[0])), nd, dump_re,
                          dump_im);
       WCK(hipGetLastError());
       *dump_k = k;
