@@ -49,7 +49,8 @@ typedef struct ticit_hip_options {
   uint32_t components;   /* 0: as planned (use_active_components, src/planner.rs:1482-1484);
                             1: dense active state; 2: tensor components
                             (src/component_plan.rs, src/sampler/batch/runtime.rs:388-552) */
-  uint32_t reserved[2];
+  uint32_t group_threads; /* group tier threads per shot: 64, 128 or 256 (0 = 256) */
+  uint32_t reserved;
 } ticit_hip_options;
 
 /* Mirrors SampleResult (src/sampler/prepared.rs:197-219). Buffers are owned
@@ -155,6 +156,39 @@ ticit_hip_status ticit_hip_sample_batch(ticit_hip_program* const* programs, cons
                                         ticit_hip_result* results);
 
 void ticit_hip_result_free(ticit_hip_result* result);
+
+/* Planning features of a plan blob, computed on the host without a device:
+ * the quantities of ticit's component cost model (ActiveComponentPlan,
+ * src/component_plan.rs:87-104) plus the program's shape. out receives
+ * min(nout, TICIT_HIP_NFEATURES) values indexed by TICIT_HIP_FEATURE_*. */
+enum {
+  TICIT_HIP_FEATURE_QUBITS = 0,
+  TICIT_HIP_FEATURE_INITIAL_K,
+  TICIT_HIP_FEATURE_MAX_K,
+  TICIT_HIP_FEATURE_SYMBOLS,
+  TICIT_HIP_FEATURE_RECORDS,
+  TICIT_HIP_FEATURE_DETECTORS,
+  TICIT_HIP_FEATURE_INSTRUCTIONS,
+  TICIT_HIP_FEATURE_MEASUREMENTS,     /* active measurements */
+  TICIT_HIP_FEATURE_ROTATIONS,        /* active rotations */
+  TICIT_HIP_FEATURE_PROMOTIONS,       /* T-branch splits */
+  TICIT_HIP_FEATURE_QUANTUM_OPS,
+  TICIT_HIP_FEATURE_EXPRESSION_TERMS,
+  TICIT_HIP_FEATURE_SELECTED,         /* use_active_components of the blob */
+  TICIT_HIP_FEATURE_PLANNED,          /* a component plan exists (max_k >= 8) */
+  TICIT_HIP_FEATURE_DENSE_WORK,       /* estimated_dense_work */
+  TICIT_HIP_FEATURE_COMPONENT_WORK,   /* estimated_component_work */
+  TICIT_HIP_FEATURE_DENSE_PEAK,       /* dense_peak_dimension */
+  TICIT_HIP_FEATURE_PEAK_LIVE,        /* component_peak_live_dimension */
+  TICIT_HIP_FEATURE_ALLOCATED,        /* component_allocated_dimension */
+  TICIT_HIP_FEATURE_MAX_COMPONENT_K,
+  TICIT_HIP_FEATURE_MERGES,
+  TICIT_HIP_FEATURE_DENSE_AMP_TRAFFIC,     /* amplitudes streamed per shot, dense layout */
+  TICIT_HIP_FEATURE_COMPONENT_AMP_TRAFFIC, /* the same for the component layout (0 unplanned) */
+  TICIT_HIP_FEATURE_OUTPUT_BYTES,          /* packed row bytes per shot */
+  TICIT_HIP_NFEATURES
+};
+ticit_hip_status ticit_hip_plan_features(const uint64_t* blob, size_t nwords, double* out, size_t nout);
 
 /* Device properties the tier planner sizes against; returns 0 when no
  * device is visible. props = {lds_per_block, compute_units, clock_khz,

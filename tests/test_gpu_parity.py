@@ -160,3 +160,17 @@ def test_device_entry_point_split_launches(hip):
     assert np.array_equal(meas.cpu().numpy(), ref.measurements)
     assert np.array_equal(det.cpu().numpy(), ref.detectors)
     assert int(ones.item()) == int(ref.observable_ones[0])
+
+
+@pytest.mark.parametrize("tps", [64, 128, 256])
+def test_group_threads_per_shot_give_identical_rows(hip, tps):
+    """Packing 4, 2 or 1 shots per workgroup changes the launch shape and
+    the reduction tree of a shot's probability sums (last-bit differences),
+    not the rows: they equal the oracle's."""
+    c = hip.Circuit(w.config2_batch()[94][0])  # component layout, local k 6
+    s = c.compile(tier="group", group_threads=tps)
+    shots = 333  # not a multiple of any shots-per-workgroup
+    r = s.sample(shots, seed=5, bit_packed=True)
+    o = O.sample_per_shot(c.blob, shots, 5)
+    assert np.array_equal(r.measurements, o["measurements"])
+    assert r.observable_ones.tolist() == o["observable_ones"].tolist()

@@ -107,7 +107,7 @@ class CompiledSampler:
 
     def __init__(self, blob: np.ndarray, observable: int = 0, precision: str = "fp64",
                  tier: str = "auto", planner: str = "heuristic", wide_batch: int = 0,
-                 components: str = "planned"):
+                 components: str = "planned", group_threads: int = 0):
         lib = _capi.load()
         self._lib = lib
         self.blob = np.ascontiguousarray(blob, dtype=np.uint64)
@@ -118,6 +118,7 @@ class CompiledSampler:
         opt.planner_mode = PLANNERS[planner]
         opt.wide_batch = wide_batch
         opt.components = COMPONENTS[components]
+        opt.group_threads = group_threads
         handle = ctypes.c_void_p()
         _capi.check(lib.ticit_hip_program_create(self.blob.ctypes.data_as(_capi._u64p), self.blob.size,
                                                  ctypes.byref(opt), ctypes.byref(handle)))
@@ -253,7 +254,7 @@ class Circuit:
     def compile(self, postselection_mask=None, *, observable: int = 0, backend: str = "hip",
                 precision: str = "fp64", tier: str = "auto", planner: str = "heuristic",
                 threads: int | None = None, wide_batch: int = 0,
-                components: str = "planned") -> CompiledSampler:
+                components: str = "planned", group_threads: int = 0) -> CompiledSampler:
         """Circuit::compile with ticit_py's ``backend=`` selector; only the
         ``"hip"`` backend lives in this package."""
         if backend != "hip":
@@ -261,7 +262,28 @@ class Circuit:
         if postselection_mask is not None and any(int(v) for v in postselection_mask):
             raise TicitError(3, "detector postselection is not supported by the hip backend")
         return CompiledSampler(self.blob, observable=observable, precision=precision, tier=tier,
-                               planner=planner, wide_batch=wide_batch, components=components)
+                               planner=planner, wide_batch=wide_batch, components=components,
+                               group_threads=group_threads)
+
+
+FEATURE_NAMES = [
+    "qubits", "initial_k", "max_k", "symbols", "records", "detectors", "instructions", "measurements",
+    "rotations", "promotions", "quantum_ops", "expression_terms", "selected", "planned", "dense_work",
+    "component_work", "dense_peak", "peak_live", "allocated", "max_component_k", "merges",
+    "dense_amp_traffic", "component_amp_traffic", "output_bytes",
+]
+
+
+def plan_features(blob: np.ndarray) -> dict:
+    """Host-side planning features of a plan blob (no device needed): the
+    component cost model's quantities (src/component_plan.rs:87-104) and the
+    program's shape."""
+    lib = _capi.load()
+    blob = np.ascontiguousarray(blob, dtype=np.uint64)
+    out = np.zeros(len(FEATURE_NAMES))
+    _capi.check(lib.ticit_hip_plan_features(blob.ctypes.data_as(_capi._u64p), blob.size,
+                                            out.ctypes.data_as(_capi._f64p), out.size))
+    return dict(zip(FEATURE_NAMES, out.tolist()))
 
 
 def device_properties() -> dict:

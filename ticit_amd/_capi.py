@@ -19,7 +19,7 @@ class Options(ctypes.Structure):
     _fields_ = [("observable", ctypes.c_uint32), ("precision", ctypes.c_uint32),
                 ("planner_mode", ctypes.c_uint32), ("tier", ctypes.c_uint32),
                 ("wide_batch", ctypes.c_uint32), ("components", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 2)]
+                ("group_threads", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 class Result(ctypes.Structure):
@@ -56,7 +56,7 @@ EXPORTS = [
     "ticit_hip_program_create", "ticit_hip_compile_text", "ticit_hip_program_destroy",
     "ticit_hip_program_info_get", "ticit_hip_sample", "ticit_hip_sample_device",
     "ticit_hip_sample_injected", "ticit_hip_final_state", "ticit_hip_sample_batch",
-    "ticit_hip_result_free",
+    "ticit_hip_result_free", "ticit_hip_plan_features",
     "ticit_hip_device_query",
 ]
 
@@ -106,11 +106,13 @@ def load() -> ctypes.CDLL:
     lib.ticit_hip_sample_batch.argtypes = [ctypes.POINTER(ctypes.c_void_p), _u64p, _u64p, ctypes.c_size_t,
                                            ctypes.c_int, ctypes.c_int, ctypes.POINTER(Result)]
     lib.ticit_hip_result_free.argtypes = [ctypes.POINTER(Result)]
+    lib.ticit_hip_plan_features.argtypes = [_u64p, ctypes.c_size_t, _f64p, ctypes.c_size_t]
     lib.ticit_hip_device_query.argtypes = [ctypes.POINTER(ctypes.c_int64)]
     lib.ticit_hip_device_query.restype = ctypes.c_int
     for name in ("ticit_hip_plan_text", "ticit_hip_program_create", "ticit_hip_compile_text",
                  "ticit_hip_program_info_get", "ticit_hip_sample", "ticit_hip_sample_device",
-                 "ticit_hip_sample_injected", "ticit_hip_sample_batch", "ticit_hip_final_state"):
+                 "ticit_hip_sample_injected", "ticit_hip_sample_batch", "ticit_hip_final_state",
+                 "ticit_hip_plan_features"):
         getattr(lib, name).restype = ctypes.c_int
     lib.ticit_hip_blob_free.restype = None
     lib.ticit_hip_program_destroy.restype = None

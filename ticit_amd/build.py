@@ -25,6 +25,7 @@ SOURCES = [
     "csrc/host/device_program.cpp",
     "csrc/host/capi.cpp",
     "csrc/kernels/sampler.hip",
+    "csrc/kernels/group_tier.hip",
     "csrc/kernels/wide_tier.hip",
 ]
 

@@ -5,23 +5,32 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <map>
+#include <mutex>
 #include <cstring>
 #include <string>
 #include <vector>
 
+#include "component_plan.hpp"
 #include "device_program.hpp"
 #include "frontend.hpp"
 
 namespace ticit {
 
 size_t lane_tier_lds_bytes(int kmax, const LaneLayout& y);
-size_t group_tier_lds_bytes(uint32_t state_dim, const GroupLayout& y);
+size_t group_tier_lds_bytes(uint32_t state_dim, const GroupLayout& y, int tps);
+int group_max_local_k(int tps);
 hipError_t launch_lane_tier(int kmax, const DevProgram& P, const DevLaunch& L, const LaneLayout& Y,
                             hipStream_t stream);
-hipError_t launch_group_tier(const DevProgram& P, const DevLaunch& L, const GroupLayout& Y, hipStream_t stream);
+hipError_t launch_group_tier(const DevProgram& P, const DevLaunch& L, const GroupLayout& Y, hipStream_t stream,
+                             int tps);
 size_t wide_bytes_per_shot(uint32_t state_dim, int fp32, const WideLayout& y);
-hipError_t launch_wide_tier(const DevProgram& P, const DevLaunch& L, const WideLayout& Y, uint32_t batch, int fp32,
-                            hipStream_t stream, double* dump_re, double* dump_im, int* dump_k);
+struct WideWorkspace;
+WideWorkspace* wide_workspace_new();
+void wide_workspace_free(WideWorkspace* ws);
+hipError_t launch_wide_tier(const DevProgram& P, const DevInstr* host_instrs, const DevLaunch& L,
+                            const WideLayout& Y, uint32_t batch, int fp32, WideWorkspace* ws, hipStream_t stream,
+                            double* dump_re, double* dump_im, int* dump_k);
 
 }  // namespace ticit
 
@@ -174,7 +183,13 @@ ticit_hip_status create_from_program(Program prog, const ticit_hip_options* opti
                        h->lane.pw + h->lane.sw + h->lane.rw + h->lane.dw,
                        uint32_t(((size_t(1) << t.max_local_k) / 2 + 2047) / 2048 + 1)};
   const bool lane_ok = !t.components && t.max_k <= 5 && lane_tier_lds_bytes(t.max_k, h->lane) <= size_t(lds_limit);
-  const bool group_ok = t.max_local_k <= 12 && group_tier_lds_bytes(t.state_dim, h->group) <= size_t(lds_limit);
+  const int tps = h->opt.group_threads ? int(h->opt.group_threads) : 256;
+  if (tps != 64 && tps != 128 && tps != 256) {
+    delete h;
+    return fail(TICIT_HIP_INVALID_INPUT, "group_threads must be 64, 128 or 256");
+  }
+  const bool group_ok =
+      t.max_local_k <= group_max_local_k(tps) && group_tier_lds_bytes(t.state_dim, h->group, tps) <= size_t(lds_limit);
   uint32_t want = h->opt.tier;
   if (want == 0 && h->opt.precision == 1) want = kTierWide;  // the FP32 path lives in the wide tier
   if (want == 0) want = lane_ok ? kTierLane : (group_ok ? kTierGroup : kTierWide);
@@ -210,9 +225,19 @@ ticit_hip_status create_from_program(Program prog, const ticit_hip_options* opti
 hipError_t launch(ticit_hip_program* h, const DevLaunch& L, hipStream_t stream, double* dump_re = nullptr,
                   double* dump_im = nullptr, int* dump_k = nullptr) {
   if (h->tier == kTierLane) return launch_lane_tier(h->tab.max_k, h->dev, L, h->lane, stream);
-  if (h->tier == kTierGroup) return launch_group_tier(h->dev, L, h->group, stream);
+  if (h->tier == kTierGroup)
+    return launch_group_tier(h->dev, L, h->group, stream, h->opt.group_threads ? int(h->opt.group_threads) : 256);
   uint32_t batch = uint32_t(std::min<uint64_t>(h->wide_batch, L.shots));
-  return launch_wide_tier(h->dev, L, h->wide, batch, h->opt.precision == 1, stream, dump_re, dump_im, dump_k);
+  // One wide-tier workspace per device, shared by every handle on it and
+  // grown to the largest batch asked for; calls are serialized on it (host
+  // side by the mutex, device side by its completion event).
+  static std::mutex mu;
+  static std::map<int, WideWorkspace*> pool;
+  std::lock_guard<std::mutex> lock(mu);
+  WideWorkspace*& ws = pool[h->device];
+  if (!ws) ws = wide_workspace_new();
+  return launch_wide_tier(h->dev, h->tab.instrs.data(), L, h->wide, batch, h->opt.precision == 1, ws, stream,
+                          dump_re, dump_im, dump_k);
 }
 
 struct DeviceBuffers {
@@ -382,6 +407,54 @@ ticit_hip_status ticit_hip_plan_text(const char* text, size_t len, uint64_t** bl
 }
 
 void ticit_hip_blob_free(uint64_t* blob) { free(blob); }
+
+ticit_hip_status ticit_hip_plan_features(const uint64_t* blob, size_t nwords, double* out, size_t nout) {
+  if (!blob || (!out && nout)) return fail(TICIT_HIP_INVALID_INPUT, "null argument");
+  Program p;
+  std::string err;
+  if (!decode_program(blob, nwords, p, err)) return fail(TICIT_HIP_INVALID_INPUT, err);
+  try {
+    double f[TICIT_HIP_NFEATURES] = {};
+    HostTables dense = lower_to_device(p, false);
+    ComponentPlan plan = plan_components(p);
+    int nrot = 0, nprom = 0;
+    for (const Instr& in : p.instrs) {
+      nrot += in.op == Op::Rotate;
+      nprom += in.op == Op::Promote;
+    }
+    auto packed = [](int n) { return double((n + 7) / 8); };
+    f[TICIT_HIP_FEATURE_QUBITS] = p.n;
+    f[TICIT_HIP_FEATURE_INITIAL_K] = p.initial_k;
+    f[TICIT_HIP_FEATURE_MAX_K] = p.max_k;
+    f[TICIT_HIP_FEATURE_SYMBOLS] = p.nsymbols;
+    f[TICIT_HIP_FEATURE_RECORDS] = p.nrecords;
+    f[TICIT_HIP_FEATURE_DETECTORS] = p.ndetectors;
+    f[TICIT_HIP_FEATURE_INSTRUCTIONS] = double(p.instrs.size());
+    f[TICIT_HIP_FEATURE_MEASUREMENTS] = dense.nmeasure;
+    f[TICIT_HIP_FEATURE_ROTATIONS] = nrot;
+    f[TICIT_HIP_FEATURE_PROMOTIONS] = nprom;
+    f[TICIT_HIP_FEATURE_QUANTUM_OPS] = double(dense.quantum_ops);
+    f[TICIT_HIP_FEATURE_EXPRESSION_TERMS] = double(dense.expression_terms);
+    f[TICIT_HIP_FEATURE_SELECTED] = p.use_components ? 1.0 : 0.0;
+    f[TICIT_HIP_FEATURE_PLANNED] = plan.planned ? 1.0 : 0.0;
+    f[TICIT_HIP_FEATURE_DENSE_WORK] = plan.dense_work;
+    f[TICIT_HIP_FEATURE_COMPONENT_WORK] = plan.component_work;
+    f[TICIT_HIP_FEATURE_DENSE_PEAK] = double(plan.dense_peak);
+    f[TICIT_HIP_FEATURE_PEAK_LIVE] = double(plan.peak_live);
+    f[TICIT_HIP_FEATURE_ALLOCATED] = double(plan.allocated);
+    f[TICIT_HIP_FEATURE_MAX_COMPONENT_K] = plan.max_component_k;
+    f[TICIT_HIP_FEATURE_MERGES] = plan.merges;
+    f[TICIT_HIP_FEATURE_DENSE_AMP_TRAFFIC] = dense.amp_traffic;
+    if (plan.planned) f[TICIT_HIP_FEATURE_COMPONENT_AMP_TRAFFIC] = lower_to_device(p, true).amp_traffic;
+    f[TICIT_HIP_FEATURE_OUTPUT_BYTES] = packed(p.nrecords) + packed(p.ndetectors) + packed(int(p.observables.size()));
+    std::memcpy(out, f, std::min<size_t>(nout, TICIT_HIP_NFEATURES) * sizeof(double));
+    return TICIT_HIP_OK;
+  } catch (const TicitError& e) {
+    return fail(ticit_hip_status(e.kind), e.what());
+  } catch (const std::exception& e) {
+    return fail(TICIT_HIP_INTERNAL, e.what());
+  }
+}
 
 ticit_hip_status ticit_hip_program_create(const uint64_t* blob, size_t nwords, const ticit_hip_options* options,
                                           ticit_hip_program** out) {

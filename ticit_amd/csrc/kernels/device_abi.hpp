@@ -155,6 +155,9 @@ struct DevLaunch {
   const double* uniforms;  // [nmeasure][shots]
   const uint64_t* coins;   // [ncoin][ceil(shots/64)]
   uint32_t meas_row, det_row, obs_row, pad;
+  // optional device word added to `seed` at run time, so a captured graph
+  // can be replayed with fresh seeds
+  const uint64_t* seed_offset;
 };
 
 // LDS table sizes of the two resident tiers (words of bits per shot).

@@ -34,6 +34,11 @@ __device__ __forceinline__ double sm_uniform(uint64_t& s) {
 }
 
 // src/random.rs:77-80
+// The sampling seed of a launch (plus its run-time offset, if any).
+__device__ __forceinline__ uint64_t launch_seed(const DevLaunch& L) {
+  return L.seed_offset ? L.seed + *L.seed_offset : L.seed;
+}
+
 __device__ __forceinline__ uint64_t seed_for(uint64_t base, uint64_t seed, uint64_t block) {
   return base ^ (0x9e3779b97f4a7c15ull * (block + 1)) ^ (0xbf58476d1ce4e5b9ull * (seed + 1));
 }

@@ -1,0 +1,470 @@
+// Group tier: one shot per group of TPS threads, 256 / TPS shots per
+// workgroup, every shot's active state (split-complex FP64) and bit tables
+// resident in LDS.
+//
+// All shots of a workgroup run the same instruction stream in lockstep, so
+// the workgroup barriers are uniform; each shot's amplitude work is spread
+// over its own TPS threads. TPS = 256 is one shot per workgroup (the default
+// launch); 64 and 128 pack 4 or 2 shots per workgroup for small states.
+// Projections go through registers, kProjPer outputs per thread, so a shot
+// group of TPS threads handles local widths up to log2(2 * kProjPer * TPS).
+//
+// Two entry points share the body: a plain grid over one program's shots,
+// and a persistent kernel that pulls (program, shot block) work units of a
+// batch of programs from an atomic counter.
+//
+// Arithmetic follows ticit's contiguous kernels (src/contiguous.rs:93-177,
+// 1088-1113, 1582-1606, 2013-2044); probabilities are reduced with a fixed
+// tree (thread partials in index order, then waves in order).
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+
+namespace ticit {
+
+constexpr int kGroupBlock = 256;
+constexpr int kProjPer = 8;
+
+__device__ __forceinline__ double group_wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Bytes of LDS one shot needs.
+__host__ __device__ inline size_t group_shot_bytes(uint32_t state_dim, const GroupLayout& y) {
+  return size_t(state_dim) * 16 + size_t(y.pw + y.sw + y.rw + y.dw) * 8;
+}
+
+// Runs shots [first_local, first_local + 256 / TPS) of launch L.
+template <int TPS>
+__device__ void group_shots(const DevProgram& P, const DevLaunch& L, const GroupLayout& Y, uint64_t first_local,
+                            unsigned char* smem, size_t region_bytes) {
+  constexpr int SPB = kGroupBlock / TPS;
+  constexpr int WPG = TPS / 64;  // waves per shot group
+  const int tid = threadIdx.x;
+  const int g = tid / TPS;       // shot group
+  const int t = tid % TPS;       // thread within the group
+  const uint64_t local = first_local + uint64_t(g);
+  const bool valid = local < L.shots;
+  const uint64_t shot = L.first_shot + local;
+  const uint32_t DIM = P.state_dim;
+  double* red = reinterpret_cast<double*>(smem);  // [SPB * WPG]
+  unsigned char* region = smem + 64 + size_t(g) * region_bytes;
+  double* const re0 = reinterpret_cast<double*>(region);
+  double* const im0 = re0 + DIM;
+  uint64_t* pbits = reinterpret_cast<uint64_t*>(im0 + DIM);
+  uint64_t* sbits = pbits + Y.pw;
+  uint64_t* rbits = sbits + Y.sw;
+  uint64_t* dbits = rbits + Y.rw;
+
+  for (uint32_t w = t; w < Y.pw + Y.sw + Y.rw + Y.dw; w += TPS) pbits[w] = 0;
+  const int dim0 = 1 << P.initial_k;
+  for (int a = t; a < dim0; a += TPS) {
+    re0[a] = a == 0 ? 1.0 : 0.0;
+    im0[a] = 0.0;
+  }
+  __syncthreads();
+  if (t == 0 && valid) {
+    auto toggle = [&](int32_t c) {
+      for (uint32_t j = P.sym_off[c - 1]; j < P.sym_off[c]; ++j) {
+        uint32_t part = P.sym_part[j];
+        pbits[part >> 6] ^= 1ull << (part & 63);
+      }
+    };
+    if (L.injected) {
+      const uint64_t words = (L.shots + 63) / 64;
+      for (int i = 0; i < P.nexo; ++i) {
+        int32_t c = P.exo_syms[i];
+        if ((L.exo[uint64_t(c - 1) * words + (local >> 6)] >> (local & 63)) & 1) toggle(c);
+      }
+    } else {
+      sample_exogenous(P, seed_for(kExoSeedBase, launch_seed(L), shot), toggle);
+    }
+  }
+  __syncthreads();
+
+  auto bit_of = [&](const uint64_t* tab, uint32_t idx) -> bool { return (tab[idx >> 6] >> (idx & 63)) & 1; };
+  auto set_bit = [&](uint64_t* tab, uint32_t idx, bool v) {
+    uint64_t m = 1ull << (idx & 63);
+    tab[idx >> 6] = v ? (tab[idx >> 6] | m) : (tab[idx >> 6] & ~m);
+  };
+  auto eval = [&](const DevInstr& in) -> bool {
+    bool b = (in.op & kFlagConstant) != 0;
+    if (in.partial >= 0) b ^= bit_of(pbits, uint32_t(in.partial));
+    for (uint32_t j = 0; j < in.resid_len; ++j) b ^= bit_of(sbits, P.resid[in.resid_off + j]);
+    return b;
+  };
+
+  uint64_t brng = seed_for(kBranchSeedBase, launch_seed(L), shot);  // identical in the group
+  int err = 0;
+  for (int i = 0; i < P.ninstr; ++i) {
+    const DevInstr in = P.instrs[i];
+    const uint32_t op = in.op & 0xff;
+    // every amplitude instruction acts on one component: 2^nq amplitudes at
+    // comp_off (the whole state, offset 0, when the state is dense)
+    double* re = re0 + in.comp_off;
+    double* im = im0 + in.comp_off;
+    if (op == kOpRotate) {
+      const bool sign = eval(in);
+      const int dim = 1 << in.nq;
+      const double c = in.c;
+      if (in.op & kFlagDiagonal) {
+        for (int b = t; b < dim; b += TPS) {
+          bool flip = sign != parity64(uint64_t(b) & in.zmask);
+          double fr = c + (flip ? -in.me_re : in.me_re), fi = flip ? -in.me_im : in.me_im;
+          double r = re[b], m = im[b];
+          re[b] = fr * r - fi * m;
+          im[b] = fr * m + fi * r;
+        }
+      } else {
+        const bool uni = in.op & kFlagUniformImag;
+        const bool xz_odd = in.op & kFlagXzOdd;
+        const double q = sign ? -in.me_im : in.me_im;
+        for (int idx = t; idx < (dim >> 1); idx += TPS) {
+          int i0 = int(insert_zero_bit(size_t(idx), in.pivot)), i1 = i0 ^ int(in.xmask);
+          double r0 = re[i0], m0 = im[i0], r1 = re[i1], m1 = im[i1];
+          if (uni) {
+            re[i0] = c * r0 - q * m1;
+            im[i0] = c * m0 + q * r1;
+            re[i1] = c * r1 - q * m0;
+            im[i1] = c * m1 + q * r0;
+          } else {
+            bool lodd = parity64(uint64_t(i0) & in.zmask);
+            double ld = (sign != lodd) ? -1.0 : 1.0;
+            double rd = xz_odd ? -ld : ld;
+            double lre = ld * in.me_re, lim = ld * in.me_im, rre = rd * in.me_re, rim = rd * in.me_im;
+            re[i0] = c * r0 + rre * r1 - rim * m1;
+            im[i0] = c * m0 + rre * m1 + rim * r1;
+            re[i1] = c * r1 + lre * r0 - lim * m0;
+            im[i1] = c * m1 + lre * m0 + lim * r0;
+          }
+        }
+      }
+      __syncthreads();
+    } else if (op == kOpPromote) {
+      const bool sign = eval(in);
+      const double c = in.c, q = sign ? in.s : -in.s;
+      if (in.op & kFlagFresh) {
+        // src/sampler/batch/runtime.rs:511-550: a new one-qubit component
+        if (t == 0) {
+          const bool zero = in.op & kFlagZeroState;
+          re[0] = zero ? 1.0 : c;
+          im[0] = 0.0;
+          re[1] = 0.0;
+          im[1] = zero ? 0.0 : q;
+        }
+      } else {
+        const int dim = 1 << in.nq;
+        for (int b = t; b < dim; b += TPS) {
+          double r = re[b], m = im[b];
+          re[b] = c * r;
+          im[b] = c * m;
+          re[dim + b] = -q * m;
+          im[dim + b] = q * r;
+        }
+      }
+      __syncthreads();
+    } else if (op == kOpMerge) {
+      // src/sampler/batch/mod.rs:597-680: target (x) source into the target's
+      // storage. Outputs with a non-zero source index lie above the target's
+      // live 2^nq entries and go first; the in-place zero-index slice after
+      // a barrier.
+      const int tk = in.nq, td = 1 << in.nq, total = 1 << (in.nq + in.src_k);
+      const double* sr = re0 + in.src_off;
+      const double* si = im0 + in.src_off;
+      for (int o = td + t; o < total; o += TPS) {
+        const int tb = o & (td - 1), sb = o >> tk;
+        const double a = re[tb], b = im[tb], cr = sr[sb], ci = si[sb];
+        re[o] = a * cr - b * ci;
+        im[o] = a * ci + b * cr;
+      }
+      __syncthreads();
+      for (int o = t; o < td; o += TPS) {
+        const double a = re[o], b = im[o], cr = sr[0], ci = si[0];
+        re[o] = a * cr - b * ci;
+        im[o] = a * ci + b * cr;
+      }
+      __syncthreads();
+    } else if (op == kOpRecord || op == kOpDetector) {
+      bool b = false;
+      if (op == kOpDetector && (in.op & kFlagDetRecords)) {
+        for (uint32_t j = 0; j < in.det_len; ++j) b ^= bit_of(rbits, P.det_recs[in.det_off + j]);
+      } else {
+        b = eval(in);
+      }
+      // One thread per shot writes; no reader of this instruction reads the
+      // bit being written, and the barrier orders it before later readers.
+      if (t == 0) {
+        if (op == kOpDetector) {
+          set_bit(dbits, uint32_t(in.detector), b);
+        } else {
+          if (in.record >= 0) set_bit(rbits, uint32_t(in.record), b);
+          if (in.rc_slot >= 0) set_bit(sbits, uint32_t(in.rc_slot), b);
+        }
+      }
+      __syncthreads();
+    } else if (op == kOpMeasure || op == kOpCoin) {
+      bool branch = false;
+      if (op == kOpMeasure) {
+        const int out_dim = (1 << in.nq) >> 1;
+        const bool diag = in.op & kFlagDiagonal;
+        double part = 0.0;
+        for (int idx = t; idx < out_dim; idx += TPS) {
+          if (diag) {
+            int s = int(diagonal_source(in, size_t(idx), true));
+            part += re[s] * re[s] + im[s] * im[s];
+          } else {
+            int s0 = int(insert_zero_bit(size_t(idx), in.pivot)), s1 = s0 ^ int(in.xmask);
+            double dir = parity64(uint64_t(s0) & in.zmask) ? 1.0 : -1.0;
+            double c1r = dir * in.c1_re, c1i = dir * in.c1_im;
+            double ar = kInvSqrt2 * re[s0] + c1r * re[s1] - c1i * im[s1];
+            double ai = kInvSqrt2 * im[s0] + c1r * im[s1] + c1i * re[s1];
+            part += ar * ar + ai * ai;
+          }
+        }
+        part = group_wave_sum(part);
+        if ((tid & 63) == 0) red[tid >> 6] = part;
+        __syncthreads();
+        double pt = 0.0;
+        for (int w = 0; w < WPG; ++w) pt += red[g * WPG + w];
+        pt = clamp01(pt);
+        if (valid) {
+          if (L.want_probs && t == 0) L.probs[uint64_t(in.ordinal) * L.shots + local] = pt;
+          if (L.injected) {
+            double u = L.uniforms[uint64_t(in.ordinal) * L.shots + local];
+            branch = pt <= 0.0 ? false : (pt >= 1.0 ? true : u < pt);
+          } else if (!(pt >= 0.0 && pt <= 1.0)) {
+            err |= kDevErrBadProbability;
+          } else {
+            branch = pt <= 0.0 ? false : (pt >= 1.0 ? true : sm_uniform(brng) < pt);
+          }
+        }
+        double ps = branch ? pt : 1.0 - pt;
+        if (!(ps > 0.0)) {
+          if (valid) err |= kDevErrImpossibleBranch;
+          ps = 1.0;
+        }
+        const double inv = 1.0 / sqrt(ps);
+        double nr[kProjPer], ni[kProjPer];
+#pragma unroll
+        for (int j = 0; j < kProjPer; ++j) {
+          int idx = t + j * TPS;
+          nr[j] = 0.0;
+          ni[j] = 0.0;
+          if (idx < out_dim) {
+            if (diag) {
+              int s = int(diagonal_source(in, size_t(idx), branch));
+              nr[j] = re[s] * inv;
+              ni[j] = im[s] * inv;
+            } else {
+              int s0 = int(insert_zero_bit(size_t(idx), in.pivot)), s1 = s0 ^ int(in.xmask);
+              bool odd = parity64(uint64_t(s0) & in.zmask);
+              double dir = (branch != odd) ? -1.0 : 1.0;
+              double c1r = dir * in.c1_re, c1i = dir * in.c1_im;
+              double ar = kInvSqrt2 * re[s0] + c1r * re[s1] - c1i * im[s1];
+              double ai = kInvSqrt2 * im[s0] + c1r * im[s1] + c1i * re[s1];
+              nr[j] = ar * inv;
+              ni[j] = ai * inv;
+            }
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kProjPer; ++j) {
+          int idx = t + j * TPS;
+          if (idx < out_dim) {
+            re[idx] = nr[j];
+            im[idx] = ni[j];
+          }
+        }
+      } else if (valid) {
+        if (L.injected) {
+          const uint64_t words = (L.shots + 63) / 64;
+          branch = (L.coins[uint64_t(in.ordinal) * words + (local >> 6)] >> (local & 63)) & 1;
+        } else {
+          branch = sm_next(brng) & 1;
+        }
+      }
+      if (t == 0) set_bit(sbits, uint32_t(in.branch_slot), branch);
+      __syncthreads();
+      bool b = eval(in);
+      if (t == 0) {
+        if (in.record >= 0) set_bit(rbits, uint32_t(in.record), b);
+        if (in.rc_slot >= 0) set_bit(sbits, uint32_t(in.rc_slot), b);
+      }
+      __syncthreads();
+    }
+  }
+  if (!valid) return;
+  if (err && t == 0) atomicOr(L.error, err);
+
+  for (int o = t; o < P.nobs; o += TPS) {
+    bool b = false;
+    for (uint32_t j = P.obs_off[o]; j < P.obs_off[o + 1]; ++j) b ^= bit_of(rbits, P.obs_recs[j]);
+    if (b) atomicAdd(&L.obs_ones[o], 1ull);
+  }
+  if (!L.keep_records) return;
+  auto emit = [&](uint8_t* out, uint32_t row_bytes, auto&& bit_at, uint32_t ncols) {
+    uint8_t* dst = out + local * row_bytes;
+    for (uint32_t byte = t; byte < row_bytes; byte += TPS) {
+      if (L.bit_packed) {
+        uint32_t v = 0;
+        for (uint32_t bb = 0; bb < 8; ++bb) {
+          uint32_t col = byte * 8 + bb;
+          if (col < ncols && bit_at(col)) v |= 1u << bb;
+        }
+        dst[byte] = uint8_t(v);
+      } else {
+        dst[byte] = uint8_t(bit_at(byte));
+      }
+    }
+  };
+  emit(L.meas, L.meas_row, [&](uint32_t c) { return bit_of(rbits, c); }, uint32_t(P.nrec));
+  emit(L.det, L.det_row, [&](uint32_t c) { return bit_of(dbits, c); }, uint32_t(P.ndet));
+  emit(
+      L.obs, L.obs_row,
+      [&](uint32_t o) {
+        bool b = false;
+        for (uint32_t j = P.obs_off[o]; j < P.obs_off[o + 1]; ++j) b ^= bit_of(rbits, P.obs_recs[j]);
+        return b;
+      },
+      uint32_t(P.nobs));
+}
+
+template <int TPS>
+__global__ __launch_bounds__(kGroupBlock) void group_tier_kernel(DevProgram P, DevLaunch L, GroupLayout Y,
+                                                                 uint32_t region_bytes) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  group_shots<TPS>(P, L, Y, uint64_t(blockIdx.x) * (kGroupBlock / TPS), smem, region_bytes);
+}
+
+// Persistent kernel over a batch: unit u of item i covers shots
+// [(u - unit_off[i]) * SPB, ...) of that item; workgroups take units from
+// *counter until all `total_units` are done.
+template <int TPS>
+__global__ __launch_bounds__(kGroupBlock) void group_batch_kernel(const DevProgram* progs, const DevLaunch* launches,
+                                                                  const GroupLayout* layouts,
+                                                                  const uint64_t* unit_off, uint32_t nitems,
+                                                                  uint64_t total_units, unsigned long long* counter,
+                                                                  uint32_t region_bytes) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ unsigned long long unit_sh;
+  __shared__ uint32_t item_sh;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      unsigned long long u = atomicAdd(counter, 1ull);
+      uint32_t lo = 0, hi = nitems;  // last item with unit_off <= u
+      while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) / 2;
+        if (unit_off[mid] <= u) lo = mid;
+        else hi = mid;
+      }
+      unit_sh = u;
+      item_sh = lo;
+    }
+    __syncthreads();
+    const unsigned long long u = unit_sh;
+    const uint32_t item = item_sh;
+    __syncthreads();
+    if (u >= total_units) return;  // uniform: every thread read the same unit
+    const DevProgram P = progs[item];
+    const DevLaunch L = launches[item];
+    const GroupLayout Y = layouts[item];
+    group_shots<TPS>(P, L, Y, (u - unit_off[item]) * (kGroupBlock / TPS), smem, region_bytes);
+    __syncthreads();
+  }
+}
+
+__global__ void reset_counter_kernel(unsigned long long* counter) { *counter = 0ull; }
+
+// --------------------------------------------------------------------------
+// Host side
+// --------------------------------------------------------------------------
+
+size_t group_region_bytes(uint32_t state_dim, const GroupLayout& y) {
+  return (group_shot_bytes(state_dim, y) + 15) / 16 * 16;
+}
+
+size_t group_tier_lds_bytes(uint32_t state_dim, const GroupLayout& y, int tps) {
+  return 64 + size_t(kGroupBlock / tps) * group_region_bytes(state_dim, y);
+}
+
+// Largest local width a shot group of `tps` threads projects through registers.
+int group_max_local_k(int tps) {
+  int k = 0;
+  while ((size_t(1) << (k + 1)) <= size_t(2) * kProjPer * size_t(tps)) ++k;
+  return k;
+}
+
+template <int TPS>
+static hipError_t launch_group_t(const DevProgram& P, const DevLaunch& L, const GroupLayout& Y, hipStream_t stream) {
+  size_t region = group_region_bytes(P.state_dim, Y);
+  size_t lds = 64 + size_t(kGroupBlock / TPS) * region;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&group_tier_kernel<TPS>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+  if (e != hipSuccess) return e;
+  constexpr int SPB = kGroupBlock / TPS;
+  dim3 grid(unsigned((L.shots + SPB - 1) / SPB));
+  hipLaunchKernelGGL(group_tier_kernel<TPS>, grid, dim3(kGroupBlock), lds, stream, P, L, Y, uint32_t(region));
+  return hipGetLastError();
+}
+
+hipError_t launch_group_tier(const DevProgram& P, const DevLaunch& L, const GroupLayout& Y, hipStream_t stream,
+                             int tps) {
+  if (P.max_local_k > group_max_local_k(tps)) return hipErrorInvalidValue;
+  if (L.shots == 0) return hipSuccess;
+  switch (tps) {
+    case 64: return launch_group_t<64>(P, L, Y, stream);
+    case 128: return launch_group_t<128>(P, L, Y, stream);
+    case 256: return launch_group_t<256>(P, L, Y, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int TPS>
+static hipError_t launch_group_batch_t(const DevProgram* progs, const DevLaunch* launches, const GroupLayout* layouts,
+                                       const uint64_t* unit_off, uint32_t nitems, uint64_t total_units,
+                                       unsigned long long* counter, uint32_t region_bytes, int grid,
+                                       hipStream_t stream) {
+  size_t lds = 64 + size_t(kGroupBlock / TPS) * region_bytes;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&group_batch_kernel<TPS>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, counter);
+  hipLaunchKernelGGL(group_batch_kernel<TPS>, dim3(unsigned(grid)), dim3(kGroupBlock), lds, stream, progs, launches,
+                     layouts, unit_off, nitems, total_units, counter, region_bytes);
+  return hipGetLastError();
+}
+
+hipError_t launch_group_batch(const DevProgram* progs, const DevLaunch* launches, const GroupLayout* layouts,
+                              const uint64_t* unit_off, uint32_t nitems, uint64_t total_units,
+                              unsigned long long* counter, uint32_t region_bytes, int tps, int grid,
+                              hipStream_t stream) {
+  switch (tps) {
+    case 64:
+      return launch_group_batch_t<64>(progs, launches, layouts, unit_off, nitems, total_units, counter, region_bytes,
+                                      grid, stream);
+    case 128:
+      return launch_group_batch_t<128>(progs, launches, layouts, unit_off, nitems, total_units, counter,
+                                       region_bytes, grid, stream);
+    case 256:
+      return launch_group_batch_t<256>(progs, launches, layouts, unit_off, nitems, total_units, counter,
+                                       region_bytes, grid, stream);
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+// Workgroups of the persistent kernel that fit on the device at once.
+int group_batch_occupancy(int tps, uint32_t region_bytes) {
+  size_t lds = 64 + size_t(kGroupBlock / tps) * region_bytes;
+  int blocks = 0;
+  const void* fn = tps == 64 ? reinterpret_cast<const void*>(&group_batch_kernel<64>)
+                   : tps == 128 ? reinterpret_cast<const void*>(&group_batch_kernel<128>)
+                                : reinterpret_cast<const void*>(&group_batch_kernel<256>);
+  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, kGroupBlock, lds) != hipSuccess) return 0;
+  return blocks;
+}
+
+}  // namespace ticit

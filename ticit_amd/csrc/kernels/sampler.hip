@@ -7,9 +7,8 @@
 //     lane's shot runs the scalar evaluation order of ticit's contiguous
 //     kernels (src/contiguous.rs:93-177, 1088-1113, 1582-1606, 2013-2044),
 //     so amplitudes and probabilities are bit-identical to the CPU oracle.
-//   * group tier (5 < max_k <= LDS limit): one shot per 256-thread
-//     workgroup, amplitudes split-complex FP64 in LDS, butterflies spread
-//     over the workgroup, probabilities reduced with a fixed FP64 tree.
+//   * group tier (group_tier.hip): states up to the LDS of a workgroup,
+//     64-256 threads per shot.
 // Both fuse the exogenous-noise walk (frame), expression evaluation,
 // amplitude kernels, per-shot branch sampling, record/detector/observable
 // XOR and row packing into a single launch.
@@ -62,7 +61,7 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
       if ((wv >> lane) & 1) toggle(c);
     }
   } else {
-    sample_exogenous(P, seed_for(kExoSeedBase, L.seed, shot), toggle);
+    sample_exogenous(P, seed_for(kExoSeedBase, launch_seed(L), shot), toggle);
   }
 
   for (int a = 0; a < DIM; ++a) amp[a * 64 + lane] = make_double2(a == 0 ? 1.0 : 0.0, 0.0);
@@ -87,7 +86,7 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
   };
 
   int k = P.initial_k;
-  uint64_t brng = seed_for(kBranchSeedBase, L.seed, shot);
+  uint64_t brng = seed_for(kBranchSeedBase, launch_seed(L), shot);
   int err = 0;
   for (int i = 0; i < P.ninstr; ++i) {
     const DevInstr in = P.instrs[i];
@@ -277,306 +276,6 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
 }
 
 // --------------------------------------------------------------------------
-// Group tier
-// --------------------------------------------------------------------------
-
-constexpr int kGroupThreads = 256;
-constexpr int kGroupPer = 8;  // projected outputs per thread: 2^(12-1) / 256
-
-__device__ __forceinline__ double wave_sum(double v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
-__global__ __launch_bounds__(kGroupThreads) void group_tier_kernel(DevProgram P, DevLaunch L, GroupLayout Y) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int tid = threadIdx.x;
-  const uint64_t local = blockIdx.x;
-  const uint64_t shot = L.first_shot + local;
-  const int DIM = int(P.state_dim);
-  double* re = reinterpret_cast<double*>(smem);
-  double* im = re + DIM;
-  double* red = im + DIM;  // [kGroupThreads / 64 + 1]
-  uint64_t* pbits = reinterpret_cast<uint64_t*>(red + kGroupThreads / 64 + 2);
-  uint64_t* sbits = pbits + Y.pw;
-  uint64_t* rbits = sbits + Y.sw;
-  uint64_t* dbits = rbits + Y.rw;
-
-  for (uint32_t w = tid; w < Y.pw + Y.sw + Y.rw + Y.dw; w += kGroupThreads) pbits[w] = 0;
-  const int dim0 = 1 << P.initial_k;
-  for (int a = tid; a < dim0; a += kGroupThreads) {
-    re[a] = a == 0 ? 1.0 : 0.0;
-    im[a] = 0.0;
-  }
-  __syncthreads();
-  if (tid == 0) {
-    auto toggle = [&](int32_t c) {
-      for (uint32_t j = P.sym_off[c - 1]; j < P.sym_off[c]; ++j) {
-        uint32_t part = P.sym_part[j];
-        pbits[part >> 6] ^= 1ull << (part & 63);
-      }
-    };
-    if (L.injected) {
-      const uint64_t words = (L.shots + 63) / 64;
-      for (int i = 0; i < P.nexo; ++i) {
-        int32_t c = P.exo_syms[i];
-        if ((L.exo[uint64_t(c - 1) * words + (local >> 6)] >> (local & 63)) & 1) toggle(c);
-      }
-    } else {
-      sample_exogenous(P, seed_for(kExoSeedBase, L.seed, shot), toggle);
-    }
-  }
-  __syncthreads();
-
-  auto bit_of = [&](const uint64_t* tab, uint32_t idx) -> bool { return (tab[idx >> 6] >> (idx & 63)) & 1; };
-  auto set_bit = [&](uint64_t* tab, uint32_t idx, bool v) {
-    uint64_t m = 1ull << (idx & 63);
-    tab[idx >> 6] = v ? (tab[idx >> 6] | m) : (tab[idx >> 6] & ~m);
-  };
-  auto eval = [&](const DevInstr& in) -> bool {
-    bool b = (in.op & kFlagConstant) != 0;
-    if (in.partial >= 0) b ^= bit_of(pbits, uint32_t(in.partial));
-    for (uint32_t j = 0; j < in.resid_len; ++j) b ^= bit_of(sbits, P.resid[in.resid_off + j]);
-    return b;
-  };
-
-  uint64_t brng = seed_for(kBranchSeedBase, L.seed, shot);  // identical in every thread
-  int err = 0;
-  double* const re0 = re;
-  double* const im0 = im;
-  for (int i = 0; i < P.ninstr; ++i) {
-    const DevInstr in = P.instrs[i];
-    const uint32_t op = in.op & 0xff;
-    // every amplitude instruction acts on one component: 2^nq amplitudes at
-    // comp_off (the whole state, offset 0, when the state is dense)
-    re = re0 + in.comp_off;
-    im = im0 + in.comp_off;
-    if (op == kOpRotate) {
-      const bool sign = eval(in);
-      const int dim = 1 << in.nq;
-      const double c = in.c;
-      if (in.op & kFlagDiagonal) {
-        for (int b = tid; b < dim; b += kGroupThreads) {
-          bool flip = sign != parity64(uint64_t(b) & in.zmask);
-          double fr = c + (flip ? -in.me_re : in.me_re), fi = flip ? -in.me_im : in.me_im;
-          double r = re[b], m = im[b];
-          re[b] = fr * r - fi * m;
-          im[b] = fr * m + fi * r;
-        }
-      } else {
-        const bool uni = in.op & kFlagUniformImag;
-        const bool xz_odd = in.op & kFlagXzOdd;
-        const double q = sign ? -in.me_im : in.me_im;
-        for (int idx = tid; idx < (dim >> 1); idx += kGroupThreads) {
-          int i0 = int(insert_zero_bit(size_t(idx), in.pivot)), i1 = i0 ^ int(in.xmask);
-          double r0 = re[i0], m0 = im[i0], r1 = re[i1], m1 = im[i1];
-          if (uni) {
-            re[i0] = c * r0 - q * m1;
-            im[i0] = c * m0 + q * r1;
-            re[i1] = c * r1 - q * m0;
-            im[i1] = c * m1 + q * r0;
-          } else {
-            bool lodd = parity64(uint64_t(i0) & in.zmask);
-            double ld = (sign != lodd) ? -1.0 : 1.0;
-            double rd = xz_odd ? -ld : ld;
-            double lre = ld * in.me_re, lim = ld * in.me_im, rre = rd * in.me_re, rim = rd * in.me_im;
-            re[i0] = c * r0 + rre * r1 - rim * m1;
-            im[i0] = c * m0 + rre * m1 + rim * r1;
-            re[i1] = c * r1 + lre * r0 - lim * m0;
-            im[i1] = c * m1 + lre * m0 + lim * r0;
-          }
-        }
-      }
-      __syncthreads();
-    } else if (op == kOpPromote) {
-      const bool sign = eval(in);
-      const double c = in.c, q = sign ? in.s : -in.s;
-      if (in.op & kFlagFresh) {
-        // src/sampler/batch/runtime.rs:511-550: a new one-qubit component
-        if (tid == 0) {
-          const bool zero = in.op & kFlagZeroState;
-          re[0] = zero ? 1.0 : c;
-          im[0] = 0.0;
-          re[1] = 0.0;
-          im[1] = zero ? 0.0 : q;
-        }
-      } else {
-        const int dim = 1 << in.nq;
-        for (int b = tid; b < dim; b += kGroupThreads) {
-          double r = re[b], m = im[b];
-          re[b] = c * r;
-          im[b] = c * m;
-          re[dim + b] = -q * m;
-          im[dim + b] = q * r;
-        }
-      }
-      __syncthreads();
-    } else if (op == kOpMerge) {
-      // src/sampler/batch/mod.rs:597-680: target (x) source into the target's
-      // storage. Outputs with a non-zero source index lie above the target's
-      // live 2^nq entries and go first; the in-place zero-index slice after
-      // a barrier.
-      const int tk = in.nq, td = 1 << in.nq, total = 1 << (in.nq + in.src_k);
-      const double* sr = re0 + in.src_off;
-      const double* si = im0 + in.src_off;
-      for (int o = td + tid; o < total; o += kGroupThreads) {
-        const int tb = o & (td - 1), sb = o >> tk;
-        const double a = re[tb], b = im[tb], cr = sr[sb], ci = si[sb];
-        re[o] = a * cr - b * ci;
-        im[o] = a * ci + b * cr;
-      }
-      __syncthreads();
-      for (int o = tid; o < td; o += kGroupThreads) {
-        const double a = re[o], b = im[o], cr = sr[0], ci = si[0];
-        re[o] = a * cr - b * ci;
-        im[o] = a * ci + b * cr;
-      }
-      __syncthreads();
-    } else if (op == kOpRecord || op == kOpDetector) {
-      bool b = false;
-      if (op == kOpDetector && (in.op & kFlagDetRecords)) {
-        for (uint32_t j = 0; j < in.det_len; ++j) b ^= bit_of(rbits, P.det_recs[in.det_off + j]);
-      } else {
-        b = eval(in);
-      }
-      // Only thread 0 writes; concurrent readers never read the bit being
-      // written (an instruction never reads its own outcome), so one barrier
-      // after the write orders it before every later reader.
-      if (tid == 0) {
-        if (op == kOpDetector) {
-          set_bit(dbits, uint32_t(in.detector), b);
-        } else {
-          if (in.record >= 0) set_bit(rbits, uint32_t(in.record), b);
-          if (in.rc_slot >= 0) set_bit(sbits, uint32_t(in.rc_slot), b);
-        }
-      }
-      __syncthreads();
-    } else if (op == kOpMeasure || op == kOpCoin) {
-      bool branch = false;
-      if (op == kOpMeasure) {
-        const int out_dim = (1 << in.nq) >> 1;
-        const bool diag = in.op & kFlagDiagonal;
-        double part = 0.0;
-        for (int idx = tid; idx < out_dim; idx += kGroupThreads) {
-          if (diag) {
-            int s = int(diagonal_source(in, size_t(idx), true));
-            part += re[s] * re[s] + im[s] * im[s];
-          } else {
-            int s0 = int(insert_zero_bit(size_t(idx), in.pivot)), s1 = s0 ^ int(in.xmask);
-            double dir = parity64(uint64_t(s0) & in.zmask) ? 1.0 : -1.0;
-            double c1r = dir * in.c1_re, c1i = dir * in.c1_im;
-            double ar = kInvSqrt2 * re[s0] + c1r * re[s1] - c1i * im[s1];
-            double ai = kInvSqrt2 * im[s0] + c1r * im[s1] + c1i * re[s1];
-            part += ar * ar + ai * ai;
-          }
-        }
-        part = wave_sum(part);
-        if ((tid & 63) == 0) red[tid >> 6] = part;
-        __syncthreads();
-        double pt = 0.0;
-        for (int w = 0; w < kGroupThreads / 64; ++w) pt += red[w];
-        pt = clamp01(pt);
-        if (L.want_probs && tid == 0) L.probs[uint64_t(in.ordinal) * L.shots + local] = pt;
-        if (L.injected) {
-          double u = L.uniforms[uint64_t(in.ordinal) * L.shots + local];
-          branch = pt <= 0.0 ? false : (pt >= 1.0 ? true : u < pt);
-        } else if (!(pt >= 0.0 && pt <= 1.0)) {
-          err |= kDevErrBadProbability;
-        } else {
-          branch = pt <= 0.0 ? false : (pt >= 1.0 ? true : sm_uniform(brng) < pt);
-        }
-        double ps = branch ? pt : 1.0 - pt;
-        if (!(ps > 0.0)) {
-          err |= kDevErrImpossibleBranch;
-          ps = 1.0;
-        }
-        const double inv = 1.0 / sqrt(ps);
-        double nr[kGroupPer], ni[kGroupPer];
-#pragma unroll
-        for (int j = 0; j < kGroupPer; ++j) {
-          int idx = tid + j * kGroupThreads;
-          nr[j] = 0.0;
-          ni[j] = 0.0;
-          if (idx < out_dim) {
-            if (diag) {
-              int s = int(diagonal_source(in, size_t(idx), branch));
-              nr[j] = re[s] * inv;
-              ni[j] = im[s] * inv;
-            } else {
-              int s0 = int(insert_zero_bit(size_t(idx), in.pivot)), s1 = s0 ^ int(in.xmask);
-              bool odd = parity64(uint64_t(s0) & in.zmask);
-              double dir = (branch != odd) ? -1.0 : 1.0;
-              double c1r = dir * in.c1_re, c1i = dir * in.c1_im;
-              double ar = kInvSqrt2 * re[s0] + c1r * re[s1] - c1i * im[s1];
-              double ai = kInvSqrt2 * im[s0] + c1r * im[s1] + c1i * re[s1];
-              nr[j] = ar * inv;
-              ni[j] = ai * inv;
-            }
-          }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < kGroupPer; ++j) {
-          int idx = tid + j * kGroupThreads;
-          if (idx < out_dim) {
-            re[idx] = nr[j];
-            im[idx] = ni[j];
-          }
-        }
-      } else {
-        if (L.injected) {
-          const uint64_t words = (L.shots + 63) / 64;
-          branch = (L.coins[uint64_t(in.ordinal) * words + (local >> 6)] >> (local & 63)) & 1;
-        } else {
-          branch = sm_next(brng) & 1;
-        }
-      }
-      if (tid == 0) set_bit(sbits, uint32_t(in.branch_slot), branch);
-      __syncthreads();
-      bool b = eval(in);
-      if (tid == 0) {
-        if (in.record >= 0) set_bit(rbits, uint32_t(in.record), b);
-        if (in.rc_slot >= 0) set_bit(sbits, uint32_t(in.rc_slot), b);
-      }
-      __syncthreads();
-    }
-  }
-  if (err && tid == 0) atomicOr(L.error, err);
-
-  for (int o = tid; o < P.nobs; o += kGroupThreads) {
-    bool b = false;
-    for (uint32_t j = P.obs_off[o]; j < P.obs_off[o + 1]; ++j) b ^= bit_of(rbits, P.obs_recs[j]);
-    if (b) atomicAdd(&L.obs_ones[o], 1ull);
-  }
-  if (!L.keep_records) return;
-  auto emit = [&](uint8_t* out, uint32_t row_bytes, auto&& bit_at, uint32_t ncols) {
-    uint8_t* dst = out + local * row_bytes;
-    for (uint32_t byte = tid; byte < row_bytes; byte += kGroupThreads) {
-      if (L.bit_packed) {
-        uint32_t v = 0;
-        for (uint32_t bb = 0; bb < 8; ++bb) {
-          uint32_t col = byte * 8 + bb;
-          if (col < ncols && bit_at(col)) v |= 1u << bb;
-        }
-        dst[byte] = uint8_t(v);
-      } else {
-        dst[byte] = uint8_t(bit_at(byte));
-      }
-    }
-  };
-  emit(L.meas, L.meas_row, [&](uint32_t c) { return bit_of(rbits, c); }, uint32_t(P.nrec));
-  emit(L.det, L.det_row, [&](uint32_t c) { return bit_of(dbits, c); }, uint32_t(P.ndet));
-  emit(
-      L.obs, L.obs_row,
-      [&](uint32_t o) {
-        bool b = false;
-        for (uint32_t j = P.obs_off[o]; j < P.obs_off[o + 1]; ++j) b ^= bit_of(rbits, P.obs_recs[j]);
-        return b;
-      },
-      uint32_t(P.nobs));
-}
-
-// --------------------------------------------------------------------------
 // Host launchers
 // --------------------------------------------------------------------------
 
@@ -584,11 +283,6 @@ size_t lane_tier_lds_bytes(int kmax, const LaneLayout& y) {
   size_t dim = size_t(1) << kmax;
   size_t sdim = kmax > 0 ? dim / 2 : 0;
   return (dim + sdim) * 64 * 16 + size_t(y.pw + y.sw + y.rw + y.dw) * 64 * 8 + y.stage_bytes;
-}
-
-size_t group_tier_lds_bytes(uint32_t state_dim, const GroupLayout& y) {
-  size_t dim = state_dim;
-  return dim * 16 + (kGroupThreads / 64 + 2) * 8 + size_t(y.pw + y.sw + y.rw + y.dw) * 8;
 }
 
 template <int K>
@@ -613,16 +307,6 @@ hipError_t launch_lane_tier(int kmax, const DevProgram& P, const DevLaunch& L, c
     case 5: return launch_lane_k<5>(P, L, Y, stream);
     default: return hipErrorInvalidValue;
   }
-}
-
-hipError_t launch_group_tier(const DevProgram& P, const DevLaunch& L, const GroupLayout& Y, hipStream_t stream) {
-  if (P.max_local_k > 12) return hipErrorInvalidValue;  // projections hold 2^11 outputs in registers
-  size_t lds = group_tier_lds_bytes(P.state_dim, Y);
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&group_tier_kernel),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(group_tier_kernel, dim3(unsigned(L.shots)), dim3(kGroupThreads), lds, stream, P, L, Y);
-  return hipGetLastError();
 }
 
 }  // namespace ticit

@@ -89,9 +89,9 @@ __global__ __launch_bounds__(kWideThreads) void wide_init_kernel(DevProgram P, D
       if ((L.exo[uint64_t(c - 1) * words + (local >> 6)] >> (local & 63)) & 1) toggle(c);
     }
   } else {
-    sample_exogenous(P, seed_for(kExoSeedBase, L.seed, shot), toggle);
+    sample_exogenous(P, seed_for(kExoSeedBase, launch_seed(L), shot), toggle);
   }
-  ctl[j].brng = seed_for(kBranchSeedBase, L.seed, shot);
+  ctl[j].brng = seed_for(kBranchSeedBase, launch_seed(L), shot);
   ctl[j].err = 0;
   ctl[j].branch = 0;
   ctl[j].invnorm = 1.0;
@@ -432,13 +432,75 @@ __global__ void wide_dump_kernel(const T* re, const T* im, size_t n, double* ore
   }
 }
 
+// Device workspace of one program handle, kept across calls and grown only
+// when a call needs more: re-allocating gigabytes per call costs far more
+// than the sampling (fresh pages are mapped on first touch).
 struct WideWorkspace {
   void* re[2] = {nullptr, nullptr};
   void* im[2] = {nullptr, nullptr};
   uint64_t* bits = nullptr;
   WideCtl* ctl = nullptr;
   double* part = nullptr;
+  size_t amp_bytes = 0;   // bytes of each of re[0..1], im[0..1]
+  size_t bit_bytes = 0, ctl_bytes = 0, part_bytes = 0;
+  hipEvent_t done = nullptr;  // end of the last call's work
 };
+
+WideWorkspace* wide_workspace_new() { return new WideWorkspace(); }
+
+void wide_workspace_free(WideWorkspace* ws) {
+  if (!ws) return;
+  if (ws->done) (void)hipEventSynchronize(ws->done);
+  for (int b = 0; b < 2; ++b) {
+    (void)hipFree(ws->re[b]);
+    (void)hipFree(ws->im[b]);
+  }
+  (void)hipFree(ws->bits);
+  (void)hipFree(ws->ctl);
+  (void)hipFree(ws->part);
+  if (ws->done) (void)hipEventDestroy(ws->done);
+  delete ws;
+}
+
+static hipError_t grow(void** p, size_t& have, size_t need) {
+  if (need <= have) return hipSuccess;
+  (void)hipFree(*p);
+  *p = nullptr;
+  have = 0;
+  hipError_t e = hipMalloc(p, need);
+  if (e == hipSuccess) have = need;
+  return e;
+}
+
+static hipError_t wide_workspace_reserve(WideWorkspace& ws, size_t batch, size_t dim, size_t elem,
+                                         const WideLayout& Y) {
+  if (!ws.done) {
+    hipError_t e = hipEventCreateWithFlags(&ws.done, hipEventDisableTiming);
+    if (e != hipSuccess) return e;
+  }
+  const size_t amp = batch * dim * elem;
+  if (amp > ws.amp_bytes) {
+    // an earlier call may still be reading the buffers
+    hipError_t e = hipEventSynchronize(ws.done);
+    if (e != hipSuccess) return e;
+    for (int b = 0; b < 2; ++b) {
+      size_t h = ws.amp_bytes;
+      if ((e = grow(&ws.re[b], h, amp)) != hipSuccess) return e;
+      h = ws.amp_bytes;
+      if ((e = grow(&ws.im[b], h, amp)) != hipSuccess) return e;
+    }
+    ws.amp_bytes = amp;
+  }
+  hipError_t e = hipSuccess;
+  const size_t bb = batch * Y.words * 8 + 8, cb = batch * sizeof(WideCtl), pb = batch * Y.max_tiles * 8 + 8;
+  if (bb > ws.bit_bytes || cb > ws.ctl_bytes || pb > ws.part_bytes) {
+    if ((e = hipEventSynchronize(ws.done)) != hipSuccess) return e;
+    if ((e = grow(reinterpret_cast<void**>(&ws.bits), ws.bit_bytes, bb)) != hipSuccess) return e;
+    if ((e = grow(reinterpret_cast<void**>(&ws.ctl), ws.ctl_bytes, cb)) != hipSuccess) return e;
+    if ((e = grow(reinterpret_cast<void**>(&ws.part), ws.part_bytes, pb)) != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
 
 size_t wide_bytes_per_shot(uint32_t state_dim, int fp32, const WideLayout& y) {
   size_t dim = state_dim;
@@ -446,38 +508,24 @@ size_t wide_bytes_per_shot(uint32_t state_dim, int fp32, const WideLayout& y) {
   return 4 * dim * t + size_t(y.words) * 8 + sizeof(WideCtl) + size_t(y.max_tiles) * 8;
 }
 
+// Enqueues the whole run on `stream` without synchronizing; `instrs` is the
+// host copy of P.instrs.
 template <class T>
-static hipError_t run_wide(const DevProgram& P, const DevLaunch& L, const WideLayout& Y, uint32_t batch,
-                           hipStream_t stream, double* dump_re, double* dump_im, int* dump_k) {
+static hipError_t run_wide(const DevProgram& P, const DevInstr* instrs, const DevLaunch& L, const WideLayout& Y,
+                           uint32_t batch, WideWorkspace& ws, hipStream_t stream, double* dump_re, double* dump_im,
+                           int* dump_k) {
   const size_t dim = P.state_dim;
-  WideWorkspace ws;
-  hipError_t e = hipSuccess;
-  auto cleanup = [&]() {
-    for (int b = 0; b < 2; ++b) {
-      (void)hipFree(ws.re[b]);
-      (void)hipFree(ws.im[b]);
-    }
-    (void)hipFree(ws.bits);
-    (void)hipFree(ws.ctl);
-    (void)hipFree(ws.part);
-  };
-#define WCK(x)          \
-  do {                  \
-    e = (x);            \
+  hipError_t e = wide_workspace_reserve(ws, batch, dim, sizeof(T), Y);
+  if (e != hipSuccess) return e;
+  // calls on one handle may come on different streams: order them
+  if ((e = hipStreamWaitEvent(stream, ws.done, 0)) != hipSuccess) return e;
+#define WCK(x)             \
+  do {                     \
+    e = (x);               \
     if (e != hipSuccess) { \
-      cleanup();        \
-      return e;         \
-    }                   \
+      return e;            \
+    }                      \
   } while (0)
-  for (int b = 0; b < 2; ++b) {
-    WCK(hipMalloc(&ws.re[b], size_t(batch) * dim * sizeof(T)));
-    WCK(hipMalloc(&ws.im[b], size_t(batch) * dim * sizeof(T)));
-  }
-  WCK(hipMalloc(reinterpret_cast<void**>(&ws.bits), size_t(batch) * Y.words * 8 + 8));
-  WCK(hipMalloc(reinterpret_cast<void**>(&ws.ctl), size_t(batch) * sizeof(WideCtl)));
-  WCK(hipMalloc(reinterpret_cast<void**>(&ws.part), size_t(batch) * Y.max_tiles * 8 + 8));
-  std::vector<DevInstr> instrs(size_t(P.ninstr));
-  WCK(hipMemcpy(instrs.data(), P.instrs, instrs.size() * sizeof(DevInstr), hipMemcpyDeviceToHost));
   const unsigned per_shot_blocks = 256;
   for (uint64_t start = 0; start < L.shots; start += batch) {
     uint32_t n = uint32_t(std::min<uint64_t>(batch, L.shots - start));
@@ -492,12 +540,12 @@ static hipError_t run_wide(const DevProgram& P, const DevLaunch& L, const WideLa
     int k = P.initial_k;  // dense width, for the final-state dump
     int i = 0;
     while (i < P.ninstr) {
-      const DevInstr& in = instrs[size_t(i)];
+      const DevInstr& in = instrs[i];
       const uint32_t op = in.op & 0xff;
       if (op == kOpRecord || op == kOpDetector || op == kOpCoin) {
         int e2 = i;
         while (e2 < P.ninstr) {
-          uint32_t o2 = instrs[size_t(e2)].op & 0xff;
+          uint32_t o2 = instrs[e2].op & 0xff;
           if (o2 != kOpRecord && o2 != kOpDetector && o2 != kOpCoin) break;
           ++e2;
         }
@@ -554,10 +602,7 @@ static hipError_t run_wide(const DevProgram& P, const DevLaunch& L, const WideLa
     hipLaunchKernelGGL(wide_finish_kernel, sg, sb, 0, stream, P, L, Y, start, n, ws.bits, ws.ctl);
     WCK(hipGetLastError());
     if (dump_re && start == 0) {
-      if (P.ncomp != 1) {
-        cleanup();
-        return hipErrorInvalidValue;  // component states are not assembled
-      }
+      if (P.ncomp != 1) return hipErrorInvalidValue;  // component states are not assembled
       size_t nd = size_t(1) << k;
       hipLaunchKernelGGL(wide_dump_kernel<T>, dim3(unsigned((nd + 255) / 256)), dim3(256), 0, stream,
                          static_cast<const T*>(RE(cur[0])), static_cast<const T*>(IM(cur[0])), nd, dump_re,
@@ -566,16 +611,17 @@ static hipError_t run_wide(const DevProgram& P, const DevLaunch& L, const WideLa
       *dump_k = k;
     }
   }
-  WCK(hipStreamSynchronize(stream));
+  WCK(hipEventRecord(ws.done, stream));
 #undef WCK
-  cleanup();
   return hipSuccess;
 }
 
-hipError_t launch_wide_tier(const DevProgram& P, const DevLaunch& L, const WideLayout& Y, uint32_t batch, int fp32,
-                            hipStream_t stream, double* dump_re, double* dump_im, int* dump_k) {
-  if (fp32) return run_wide<float>(P, L, Y, batch, stream, dump_re, dump_im, dump_k);
-  return run_wide<double>(P, L, Y, batch, stream, dump_re, dump_im, dump_k);
+hipError_t launch_wide_tier(const DevProgram& P, const DevInstr* host_instrs, const DevLaunch& L,
+                            const WideLayout& Y, uint32_t batch, int fp32, WideWorkspace* ws, hipStream_t stream,
+                            double* dump_re, double* dump_im, int* dump_k) {
+  if (!ws) return hipErrorInvalidValue;
+  if (fp32) return run_wide<float>(P, host_instrs, L, Y, batch, *ws, stream, dump_re, dump_im, dump_k);
+  return run_wide<double>(P, host_instrs, L, Y, batch, *ws, stream, dump_re, dump_im, dump_k);
 }
 
 }  // namespace ticit
