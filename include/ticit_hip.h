@@ -150,7 +150,59 @@ ticit_hip_status ticit_hip_final_state(ticit_hip_program* program, uint64_t shot
                                        const double* uniforms, const uint64_t* coins, double* re, double* im,
                                        int* k);
 
-/* Batched sampling of many programs in one submission (configuration C2). */
+/* ---- Batched sampling of many programs -------------------------------
+ * No single counterpart in the reference, which samples one circuit per
+ * call; a batch is `count` of those calls (Sampler::sample_with_seed,
+ * src/sampler/prepared.rs:663-670, one per item) run as one hipGraph.
+ * Every item's rows equal what ticit_hip_sample_device gives for the same
+ * program, shots and seed (seed + the launch's seed_offset).
+ *
+ * ticit_hip_batch_create captures the graph (after timing launch shapes
+ * when options->autotune is set); ticit_hip_batch_launch enqueues one replay
+ * on `stream` without synchronizing; ticit_hip_batch_wait synchronizes with
+ * the last replay and reports device-side sampling errors. Output buffers
+ * are caller-owned device memory, as for ticit_hip_sample_device, and must
+ * outlive the batch; observable_ones accumulate across replays. */
+typedef struct ticit_hip_batch ticit_hip_batch;
+
+typedef struct ticit_hip_batch_item {
+  ticit_hip_program* program;
+  uint64_t shots;
+  uint64_t seed;
+  uint8_t* measurements; /* [shots][row] device buffers */
+  uint8_t* detectors;
+  uint8_t* observables;
+  uint64_t* observable_ones; /* [nobservables] device */
+} ticit_hip_batch_item;
+
+typedef struct ticit_hip_batch_options {
+  int32_t bit_packed;
+  int32_t keep_records;
+  uint32_t autotune;       /* 1: time the launch-shape candidates of each group bucket */
+  uint32_t autotune_shots; /* shots per program in a timing run (0 = 8192) */
+  uint32_t group_grid_percent; /* persistent group-tier grid, percent of full occupancy
+                                  (0 = 100); below 100 leaves room for wide-tier kernels */
+  uint32_t reserved[3];
+} ticit_hip_batch_options;
+
+typedef struct ticit_hip_batch_info {
+  uint64_t items, lane_items, wide_items, group_buckets;
+  uint32_t bucket_tps[16];          /* threads per shot chosen per bucket */
+  uint32_t bucket_items[16];
+  uint32_t bucket_region_bytes[16]; /* LDS bytes per shot */
+  uint32_t bucket_grid[16];         /* persistent workgroups */
+  double bucket_tuned_ms[16][3];    /* timing-run ms for 64/128/256 threads per shot (0 = not run) */
+} ticit_hip_batch_info;
+
+ticit_hip_status ticit_hip_batch_create(const ticit_hip_batch_item* items, size_t count,
+                                        const ticit_hip_batch_options* options, ticit_hip_batch** out);
+ticit_hip_status ticit_hip_batch_launch(ticit_hip_batch* batch, uint64_t seed_offset, void* stream);
+ticit_hip_status ticit_hip_batch_wait(ticit_hip_batch* batch);
+ticit_hip_status ticit_hip_batch_info_get(const ticit_hip_batch* batch, ticit_hip_batch_info* out);
+void ticit_hip_batch_destroy(ticit_hip_batch* batch);
+
+/* One-shot convenience over the batch API with host results (one per item,
+ * freed with ticit_hip_result_free). */
 ticit_hip_status ticit_hip_sample_batch(ticit_hip_program* const* programs, const uint64_t* shots,
                                         const uint64_t* seeds, size_t count, int bit_packed, int keep_records,
                                         ticit_hip_result* results);

@@ -1,0 +1,87 @@
+"""Batched sampling (ticit_hip_batch_*, ticit_hip_sample_batch; gpu-marked).
+
+A batch is many single-program sampling calls run as one captured graph, so
+each item's rows must equal ticit_hip_sample_device for the same program,
+shots and seed, whatever launch shape the autotuner picked (threads per shot
+change only the summation tree of a shot's probability sums, which leaves
+the rows unchanged on these inputs).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from ticit_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda:0")
+
+
+def buffers(s, shots, packed=True):
+    row = (lambda n: (n + 7) // 8) if packed else (lambda n: n)
+    mk = lambda n: torch.zeros((shots, max(1, row(n))), dtype=torch.uint8, device=DEV)
+    return (mk(s.num_measurements), mk(s.num_detectors), mk(s.num_observables),
+            torch.zeros(max(1, s.num_observables), dtype=torch.int64, device=DEV))
+
+
+def mixed_set(hip):
+    """C2 circuits spanning every tier and both layouts, shots trimmed."""
+    batch = W.config2_batch()
+    picks = list(range(0, 48)) + [94, 214, 235, 212, 250]
+    samplers, shots, seeds = [], [], []
+    for j, i in enumerate(picks):
+        s = hip.Circuit(batch[i][0]).compile()
+        samplers.append(s)
+        shots.append(400 if s.tier == "wide" else 1000 + 37 * j)
+        seeds.append(1000 + i)
+    return samplers, shots, seeds
+
+
+def test_batch_rows_equal_single_program_runs(hip):
+    samplers, shots, seeds = mixed_set(hip)
+    tiers = {s.tier for s in samplers}
+    assert tiers == {"lane", "group", "wide"}, tiers
+    assert any(s.components for s in samplers) and not all(s.components for s in samplers)
+    outs = [buffers(s, n) for s, n in zip(samplers, shots)]
+    b = hip.Batch(samplers, shots, seeds, outs, autotune=True, autotune_shots=256)
+    b.launch(0)
+    b.wait()
+    info = b.info()
+    print(info, flush=True)
+    assert info["items"] == len(samplers) and info["wide_items"] > 0 and info["buckets"]
+    for s, n, sd, out in zip(samplers, shots, seeds, outs):
+        ref = buffers(s, n)
+        s.sample_device(n, sd, ref)
+        torch.cuda.synchronize()
+        for a, r in zip(out, ref):
+            assert torch.equal(a, r), (s.tier, s.components)
+
+
+def test_batch_replays_take_the_seed_offset(hip):
+    samplers, shots, seeds = mixed_set(hip)
+    samplers, shots, seeds = samplers[40:], shots[40:], seeds[40:]
+    outs = [buffers(s, n) for s, n in zip(samplers, shots)]
+    b = hip.Batch(samplers, shots, seeds, outs, autotune=False)
+    b.launch(0)
+    b.launch(7)
+    b.wait()
+    for s, n, sd, out in zip(samplers, shots, seeds, outs):
+        ref = buffers(s, n)
+        s.sample_device(n, sd, ref)        # the first replay's counts
+        s.sample_device(n, sd + 7, ref)    # rows and counts of the second
+        torch.cuda.synchronize()
+        for a, r in zip(out, ref):
+            assert torch.equal(a, r)
+
+
+def test_sample_batch_host_results_match_the_oracle(hip):
+    batch = W.config2_batch()
+    samplers = [hip.Circuit(batch[i][0]).compile() for i in (3, 94, 235)]
+    shots = [700, 500, 150]
+    seeds = [11, 12, 13]
+    res = hip.sample_batch(samplers, shots, seeds, bit_packed=True)
+    for s, n, sd, r in zip(samplers, shots, seeds, res):
+        o = O.sample_per_shot(s.blob, n, sd)
+        assert np.array_equal(r.measurements, o["measurements"])
+        assert r.shots == n

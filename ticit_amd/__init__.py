@@ -18,7 +18,7 @@ import numpy as np
 from . import _capi
 from ._capi import TicitError
 
-__all__ = ["Circuit", "CompiledSampler", "SampleResult", "TicitError", "plan_text"]
+__all__ = ["Batch", "Circuit", "CompiledSampler", "SampleResult", "TicitError", "plan_features", "plan_text"]
 
 TIERS = {"auto": 0, "lane": 1, "group": 2, "wide": 3}
 PRECISIONS = {"fp64": 0, "fp32": 1}
@@ -176,6 +176,16 @@ class CompiledSampler:
 
     sample_with_seed = sample
 
+    def sample_device(self, shots: int, seed: int, outputs, first_shot: int = 0, bit_packed: bool = True,
+                      keep_records: bool = True, stream: int = 0) -> None:
+        """Sampling into caller device buffers (ticit_hip_sample_device):
+        ``outputs`` = (measurements, detectors, observables, observable_ones)
+        objects with ``data_ptr()``; enqueued on ``stream``, not synchronized."""
+        ptr = lambda t: ctypes.c_void_p(int(t.data_ptr())) if t is not None else None
+        _capi.check(self._lib.ticit_hip_sample_device(self._handle, shots, seed, first_shot, int(bit_packed),
+                                                      int(keep_records), ptr(outputs[0]), ptr(outputs[1]),
+                                                      ptr(outputs[2]), ptr(outputs[3]), ctypes.c_void_p(stream or None)))
+
     def sample_injected(self, shots: int, exo: np.ndarray | None = None, uniforms: np.ndarray | None = None,
                         coins: np.ndarray | None = None, bit_packed: bool = True,
                         want_probs: bool = False) -> SampleResult:
@@ -264,6 +274,88 @@ class Circuit:
         return CompiledSampler(self.blob, observable=observable, precision=precision, tier=tier,
                                planner=planner, wide_batch=wide_batch, components=components,
                                group_threads=group_threads)
+
+
+def sample_batch(samplers, shots, seeds, bit_packed: bool = False, keep_records: bool = True) -> list:
+    """ticit_hip_sample_batch: several programs in one batched submission,
+    host results (one SampleResult per program)."""
+    lib = _capi.load()
+    n = len(samplers)
+    progs = (ctypes.c_void_p * max(1, n))(*[s.handle.value for s in samplers])
+    sh = (ctypes.c_uint64 * max(1, n))(*[int(v) for v in shots])
+    sd = (ctypes.c_uint64 * max(1, n))(*[int(v) for v in seeds])
+    res = (_capi.Result * max(1, n))()
+    _capi.check(lib.ticit_hip_sample_batch(progs, sh, sd, n, int(bit_packed), int(keep_records), res))
+    out = []
+    try:
+        for i in range(n):
+            out.append(_convert(res[i], keep_records))
+    finally:
+        for i in range(n):
+            lib.ticit_hip_result_free(ctypes.byref(res[i]))
+    return out
+
+
+class Batch:
+    """Many compiled samplers run as one captured hipGraph
+    (ticit_hip_batch_*). ``outputs[i]`` holds device buffers (anything with
+    ``data_ptr()``, e.g. torch tensors) for item i: measurements,
+    detectors, observables ([shots][row] uint8) and observable_ones
+    ([nobservables] 64-bit counts, accumulated across launches)."""
+
+    def __init__(self, samplers, shots, seeds, outputs, bit_packed: bool = True, keep_records: bool = True,
+                 autotune: bool = True, autotune_shots: int = 0, group_grid_percent: int = 0):
+        lib = _capi.load()
+        self._lib = lib
+        self._keep = (list(samplers), list(outputs))  # buffers and handles outlive the batch
+        n = len(self._keep[0])
+        items = (_capi.BatchItem * max(1, n))()
+        ptr = lambda t: int(t.data_ptr()) if t is not None else None
+        for i, (s, sh, sd, out) in enumerate(zip(samplers, shots, seeds, outputs)):
+            items[i].program = s.handle.value
+            items[i].shots = int(sh)
+            items[i].seed = int(sd)
+            items[i].measurements = ptr(out[0])
+            items[i].detectors = ptr(out[1])
+            items[i].observables = ptr(out[2])
+            items[i].observable_ones = ptr(out[3])
+        opt = _capi.BatchOptions()
+        opt.bit_packed = int(bit_packed)
+        opt.keep_records = int(keep_records)
+        opt.autotune = int(autotune)
+        opt.autotune_shots = int(autotune_shots)
+        opt.group_grid_percent = int(group_grid_percent)
+        handle = ctypes.c_void_p()
+        _capi.check(lib.ticit_hip_batch_create(items, n, ctypes.byref(opt), ctypes.byref(handle)))
+        self._handle = handle
+
+    def launch(self, seed_offset: int = 0, stream: int = 0) -> None:
+        """Enqueue one replay on ``stream`` (a hipStream_t as an int; 0 =
+        default stream); every item draws from seed + seed_offset."""
+        _capi.check(self._lib.ticit_hip_batch_launch(self._handle, int(seed_offset), ctypes.c_void_p(stream or None)))
+
+    def wait(self) -> None:
+        _capi.check(self._lib.ticit_hip_batch_wait(self._handle))
+
+    def info(self) -> dict:
+        inf = _capi.BatchInfo()
+        _capi.check(self._lib.ticit_hip_batch_info_get(self._handle, ctypes.byref(inf)))
+        nb = min(16, int(inf.group_buckets))
+        return dict(items=int(inf.items), lane_items=int(inf.lane_items), wide_items=int(inf.wide_items),
+                    buckets=[dict(threads_per_shot=int(inf.bucket_tps[j]), items=int(inf.bucket_items[j]),
+                                  region_bytes=int(inf.bucket_region_bytes[j]), grid=int(inf.bucket_grid[j]),
+                                  tuned_ms=[float(v) for v in inf.bucket_tuned_ms[j]]) for j in range(nb)])
+
+    def close(self) -> None:
+        if getattr(self, "_handle", None):
+            self._lib.ticit_hip_batch_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 FEATURE_NAMES = [

@@ -46,6 +46,25 @@ class ProgramInfo(ctypes.Structure):
                 ("max_local_k", ctypes.c_int32), ("reserved1", ctypes.c_int32)]
 
 
+class BatchItem(ctypes.Structure):
+    _fields_ = [("program", ctypes.c_void_p), ("shots", ctypes.c_uint64), ("seed", ctypes.c_uint64),
+                ("measurements", ctypes.c_void_p), ("detectors", ctypes.c_void_p),
+                ("observables", ctypes.c_void_p), ("observable_ones", ctypes.c_void_p)]
+
+
+class BatchOptions(ctypes.Structure):
+    _fields_ = [("bit_packed", ctypes.c_int32), ("keep_records", ctypes.c_int32), ("autotune", ctypes.c_uint32),
+                ("autotune_shots", ctypes.c_uint32), ("group_grid_percent", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 3)]
+
+
+class BatchInfo(ctypes.Structure):
+    _fields_ = [("items", ctypes.c_uint64), ("lane_items", ctypes.c_uint64), ("wide_items", ctypes.c_uint64),
+                ("group_buckets", ctypes.c_uint64), ("bucket_tps", ctypes.c_uint32 * 16),
+                ("bucket_items", ctypes.c_uint32 * 16), ("bucket_region_bytes", ctypes.c_uint32 * 16),
+                ("bucket_grid", ctypes.c_uint32 * 16), ("bucket_tuned_ms", (ctypes.c_double * 3) * 16)]
+
+
 # ticit_hip_status -> exception kinds of TicitError (src/errors.rs:9-57)
 STATUS_NAMES = {1: "InvalidInput", 2: "Parse", 3: "Unsupported", 4: "Internal",
                 5: "OutOfDeviceMemory", 6: "DeviceError"}
@@ -57,6 +76,8 @@ EXPORTS = [
     "ticit_hip_program_info_get", "ticit_hip_sample", "ticit_hip_sample_device",
     "ticit_hip_sample_injected", "ticit_hip_final_state", "ticit_hip_sample_batch",
     "ticit_hip_result_free", "ticit_hip_plan_features",
+    "ticit_hip_batch_create", "ticit_hip_batch_launch", "ticit_hip_batch_wait", "ticit_hip_batch_info_get",
+    "ticit_hip_batch_destroy",
     "ticit_hip_device_query",
 ]
 
@@ -107,12 +128,20 @@ def load() -> ctypes.CDLL:
                                            ctypes.c_int, ctypes.c_int, ctypes.POINTER(Result)]
     lib.ticit_hip_result_free.argtypes = [ctypes.POINTER(Result)]
     lib.ticit_hip_plan_features.argtypes = [_u64p, ctypes.c_size_t, _f64p, ctypes.c_size_t]
+    lib.ticit_hip_batch_create.argtypes = [ctypes.POINTER(BatchItem), ctypes.c_size_t, ctypes.POINTER(BatchOptions),
+                                           ctypes.POINTER(ctypes.c_void_p)]
+    lib.ticit_hip_batch_launch.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]
+    lib.ticit_hip_batch_wait.argtypes = [ctypes.c_void_p]
+    lib.ticit_hip_batch_info_get.argtypes = [ctypes.c_void_p, ctypes.POINTER(BatchInfo)]
+    lib.ticit_hip_batch_destroy.argtypes = [ctypes.c_void_p]
+    lib.ticit_hip_batch_destroy.restype = None
     lib.ticit_hip_device_query.argtypes = [ctypes.POINTER(ctypes.c_int64)]
     lib.ticit_hip_device_query.restype = ctypes.c_int
     for name in ("ticit_hip_plan_text", "ticit_hip_program_create", "ticit_hip_compile_text",
                  "ticit_hip_program_info_get", "ticit_hip_sample", "ticit_hip_sample_device",
                  "ticit_hip_sample_injected", "ticit_hip_sample_batch", "ticit_hip_final_state",
-                 "ticit_hip_plan_features"):
+                 "ticit_hip_plan_features", "ticit_hip_batch_create", "ticit_hip_batch_launch",
+                 "ticit_hip_batch_wait", "ticit_hip_batch_info_get"):
         getattr(lib, name).restype = ctypes.c_int
     lib.ticit_hip_blob_free.restype = None
     lib.ticit_hip_program_destroy.restype = None

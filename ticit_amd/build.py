@@ -24,6 +24,7 @@ SOURCES = [
     "csrc/host/component_plan.cpp",
     "csrc/host/device_program.cpp",
     "csrc/host/capi.cpp",
+    "csrc/host/batch.cpp",
     "csrc/kernels/sampler.hip",
     "csrc/kernels/group_tier.hip",
     "csrc/kernels/wide_tier.hip",

@@ -12,31 +12,11 @@
 #include <vector>
 
 #include "component_plan.hpp"
-#include "device_program.hpp"
-#include "frontend.hpp"
-
-namespace ticit {
-
-size_t lane_tier_lds_bytes(int kmax, const LaneLayout& y);
-size_t group_tier_lds_bytes(uint32_t state_dim, const GroupLayout& y, int tps);
-int group_max_local_k(int tps);
-hipError_t launch_lane_tier(int kmax, const DevProgram& P, const DevLaunch& L, const LaneLayout& Y,
-                            hipStream_t stream);
-hipError_t launch_group_tier(const DevProgram& P, const DevLaunch& L, const GroupLayout& Y, hipStream_t stream,
-                             int tps);
-size_t wide_bytes_per_shot(uint32_t state_dim, int fp32, const WideLayout& y);
-struct WideWorkspace;
-WideWorkspace* wide_workspace_new();
-void wide_workspace_free(WideWorkspace* ws);
-hipError_t launch_wide_tier(const DevProgram& P, const DevInstr* host_instrs, const DevLaunch& L,
-                            const WideLayout& Y, uint32_t batch, int fp32, WideWorkspace* ws, hipStream_t stream,
-                            double* dump_re, double* dump_im, int* dump_k);
-
-}  // namespace ticit
+#include "handle.hpp"
 
 using namespace ticit;
 
-namespace {
+namespace ticit {
 
 thread_local std::string g_error;
 
@@ -50,46 +30,7 @@ ticit_hip_status from_hip(hipError_t e, const char* what) {
   return fail(TICIT_HIP_DEVICE_ERROR, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-constexpr uint32_t kTierLane = 1, kTierGroup = 2, kTierWide = 3;
-
-}  // namespace
-
-struct ticit_hip_program {
-  Program prog;
-  HostTables tab;
-  DevProgram dev{};
-  std::vector<void*> allocations;
-  ticit_hip_options opt{};
-  uint32_t tier = 0;
-  int device = 0;
-  int* d_error = nullptr;
-  LaneLayout lane{};
-  GroupLayout group{};
-  WideLayout wide{};
-  uint32_t wide_batch = 0;
-  uint32_t meas_row_packed = 0, det_row_packed = 0, obs_row_packed = 0;
-
-  ~ticit_hip_program() {
-    for (void* p : allocations) (void)hipFree(p);
-  }
-
-  template <class T>
-  hipError_t upload(const std::vector<T>& v, const T*& out) {
-    void* d = nullptr;
-    size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(T);
-    hipError_t e = hipMalloc(&d, bytes);
-    if (e != hipSuccess) return e;
-    allocations.push_back(d);
-    if (!v.empty()) {
-      e = hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
-      if (e != hipSuccess) return e;
-    }
-    out = static_cast<const T*>(d);
-    return hipSuccess;
-  }
-
-  uint32_t row_bytes(uint64_t cols, bool packed) const { return uint32_t(packed ? (cols + 7) / 8 : cols); }
-};
+}  // namespace ticit
 
 namespace {
 
@@ -557,17 +498,6 @@ ticit_hip_status ticit_hip_sample_device(ticit_hip_program* h, uint64_t shots, u
   L.error = h->d_error;
   hipError_t e = launch(h, L, static_cast<hipStream_t>(stream));
   if (e != hipSuccess) return from_hip(e, "launching the sampling kernel");
-  return TICIT_HIP_OK;
-}
-
-ticit_hip_status ticit_hip_sample_batch(ticit_hip_program* const* programs, const uint64_t* shots,
-                                        const uint64_t* seeds, size_t count, int bit_packed, int keep_records,
-                                        ticit_hip_result* results) {
-  if (!programs || !shots || !seeds || !results) return fail(TICIT_HIP_INVALID_INPUT, "null argument");
-  for (size_t i = 0; i < count; ++i) {
-    ticit_hip_status s = ticit_hip_sample(programs[i], shots[i], seeds[i], bit_packed, keep_records, &results[i]);
-    if (s != TICIT_HIP_OK) return s;
-  }
   return TICIT_HIP_OK;
 }
 

@@ -1,0 +1,85 @@
+// The program handle behind the C ABI and the device entry points shared by
+// the single-program and batched paths.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../../include/ticit_hip.h"
+#include "device_program.hpp"
+#include "frontend.hpp"
+
+namespace ticit {
+
+size_t lane_tier_lds_bytes(int kmax, const LaneLayout& y);
+size_t group_tier_lds_bytes(uint32_t state_dim, const GroupLayout& y, int tps);
+size_t group_region_bytes(uint32_t state_dim, const GroupLayout& y);
+int group_max_local_k(int tps);
+int group_batch_occupancy(int tps, uint32_t region_bytes);
+hipError_t launch_lane_tier(int kmax, const DevProgram& P, const DevLaunch& L, const LaneLayout& Y,
+                            hipStream_t stream);
+hipError_t launch_group_tier(const DevProgram& P, const DevLaunch& L, const GroupLayout& Y, hipStream_t stream,
+                             int tps);
+hipError_t launch_group_batch(const DevProgram* progs, const DevLaunch* launches, const GroupLayout* layouts,
+                              const uint64_t* unit_off, uint32_t nitems, uint64_t total_units,
+                              unsigned long long* counter, uint32_t region_bytes, int tps, int grid,
+                              hipStream_t stream);
+hipError_t launch_batch_prologue(int* error, uint64_t* seed_offset, uint64_t value, hipStream_t stream);
+size_t wide_bytes_per_shot(uint32_t state_dim, int fp32, const WideLayout& y);
+struct WideWorkspace;
+WideWorkspace* wide_workspace_new();
+void wide_workspace_free(WideWorkspace* ws);
+hipError_t wide_workspace_prepare(WideWorkspace* ws, uint32_t batch, uint32_t state_dim, int fp32,
+                                  const WideLayout& Y);
+// `ordered` = order the run after the workspace's previous user with an
+// event (off inside graph capture, where the graph orders the work).
+hipError_t launch_wide_tier(const DevProgram& P, const DevInstr* host_instrs, const DevLaunch& L,
+                            const WideLayout& Y, uint32_t batch, int fp32, WideWorkspace* ws, hipStream_t stream,
+                            double* dump_re, double* dump_im, int* dump_k, bool ordered = true);
+
+constexpr uint32_t kTierLane = 1, kTierGroup = 2, kTierWide = 3;
+
+ticit_hip_status fail(ticit_hip_status s, const std::string& msg);
+ticit_hip_status from_hip(hipError_t e, const char* what);
+
+}  // namespace ticit
+
+struct ticit_hip_program {
+  ticit::Program prog;
+  ticit::HostTables tab;
+  ticit::DevProgram dev{};
+  std::vector<void*> allocations;
+  ticit_hip_options opt{};
+  uint32_t tier = 0;
+  int device = 0;
+  int* d_error = nullptr;
+  ticit::LaneLayout lane{};
+  ticit::GroupLayout group{};
+  ticit::WideLayout wide{};
+  uint32_t wide_batch = 0;
+  uint32_t meas_row_packed = 0, det_row_packed = 0, obs_row_packed = 0;
+
+  ~ticit_hip_program() {
+    for (void* p : allocations) (void)hipFree(p);
+  }
+
+  template <class T>
+  hipError_t upload(const std::vector<T>& v, const T*& out) {
+    void* d = nullptr;
+    size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(T);
+    hipError_t e = hipMalloc(&d, bytes);
+    if (e != hipSuccess) return e;
+    allocations.push_back(d);
+    if (!v.empty()) {
+      e = hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+      if (e != hipSuccess) return e;
+    }
+    out = static_cast<const T*>(d);
+    return hipSuccess;
+  }
+
+  uint32_t row_bytes(uint64_t cols, bool packed) const { return uint32_t(packed ? (cols + 7) / 8 : cols); }
+};

@@ -98,7 +98,7 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
   uint64_t brng = seed_for(kBranchSeedBase, launch_seed(L), shot);  // identical in the group
   int err = 0;
   for (int i = 0; i < P.ninstr; ++i) {
-    const DevInstr in = P.instrs[i];
+    const DevInstr& in = P.instrs[i];  // fields read where used
     const uint32_t op = in.op & 0xff;
     // every amplitude instruction acts on one component: 2^nq amplitudes at
     // comp_off (the whole state, offset 0, when the state is dense)
@@ -363,19 +363,35 @@ __global__ __launch_bounds__(kGroupBlock) void group_batch_kernel(const DevProgr
       item_sh = lo;
     }
     __syncthreads();
-    const unsigned long long u = unit_sh;
-    const uint32_t item = item_sh;
+    // the same unit in every lane: make that visible to the compiler so the
+    // per-item tables load through the scalar unit into SGPRs
+    const uint64_t u = (uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(unit_sh >> 32))) << 32) |
+                       uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(unit_sh)));
+    const uint32_t item = __builtin_amdgcn_readfirstlane(item_sh);
     __syncthreads();
     if (u >= total_units) return;  // uniform: every thread read the same unit
-    const DevProgram P = progs[item];
-    const DevLaunch L = launches[item];
-    const GroupLayout Y = layouts[item];
+    // fields are read where used (scalar loads), not copied into registers
+    const DevProgram& P = progs[item];
+    const DevLaunch& L = launches[item];
+    const GroupLayout& Y = layouts[item];
     group_shots<TPS>(P, L, Y, (u - unit_off[item]) * (kGroupBlock / TPS), smem, region_bytes);
     __syncthreads();
   }
 }
 
 __global__ void reset_counter_kernel(unsigned long long* counter) { *counter = 0ull; }
+
+// Run before every replay of a batch graph: clears the error word and sets
+// the seed offset the graph's launches read.
+__global__ void batch_prologue_kernel(int* error, uint64_t* seed_offset, uint64_t value) {
+  *error = 0;
+  *seed_offset = value;
+}
+
+hipError_t launch_batch_prologue(int* error, uint64_t* seed_offset, uint64_t value, hipStream_t stream) {
+  hipLaunchKernelGGL(batch_prologue_kernel, dim3(1), dim3(1), 0, stream, error, seed_offset, value);
+  return hipGetLastError();
+}
 
 // --------------------------------------------------------------------------
 // Host side

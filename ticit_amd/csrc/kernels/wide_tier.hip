@@ -513,12 +513,12 @@ size_t wide_bytes_per_shot(uint32_t state_dim, int fp32, const WideLayout& y) {
 template <class T>
 static hipError_t run_wide(const DevProgram& P, const DevInstr* instrs, const DevLaunch& L, const WideLayout& Y,
                            uint32_t batch, WideWorkspace& ws, hipStream_t stream, double* dump_re, double* dump_im,
-                           int* dump_k) {
+                           int* dump_k, bool ordered) {
   const size_t dim = P.state_dim;
   hipError_t e = wide_workspace_reserve(ws, batch, dim, sizeof(T), Y);
   if (e != hipSuccess) return e;
-  // calls on one handle may come on different streams: order them
-  if ((e = hipStreamWaitEvent(stream, ws.done, 0)) != hipSuccess) return e;
+  // calls sharing the workspace may come on different streams: order them
+  if (ordered && (e = hipStreamWaitEvent(stream, ws.done, 0)) != hipSuccess) return e;
 #define WCK(x)             \
   do {                     \
     e = (x);               \
@@ -611,17 +611,25 @@ static hipError_t run_wide(const DevProgram& P, const DevInstr* instrs, const De
       *dump_k = k;
     }
   }
-  WCK(hipEventRecord(ws.done, stream));
+  if (ordered) WCK(hipEventRecord(ws.done, stream));
 #undef WCK
   return hipSuccess;
 }
 
 hipError_t launch_wide_tier(const DevProgram& P, const DevInstr* host_instrs, const DevLaunch& L,
                             const WideLayout& Y, uint32_t batch, int fp32, WideWorkspace* ws, hipStream_t stream,
-                            double* dump_re, double* dump_im, int* dump_k) {
+                            double* dump_re, double* dump_im, int* dump_k, bool ordered) {
   if (!ws) return hipErrorInvalidValue;
-  if (fp32) return run_wide<float>(P, host_instrs, L, Y, batch, *ws, stream, dump_re, dump_im, dump_k);
-  return run_wide<double>(P, host_instrs, L, Y, batch, *ws, stream, dump_re, dump_im, dump_k);
+  if (fp32) return run_wide<float>(P, host_instrs, L, Y, batch, *ws, stream, dump_re, dump_im, dump_k, ordered);
+  return run_wide<double>(P, host_instrs, L, Y, batch, *ws, stream, dump_re, dump_im, dump_k, ordered);
+}
+
+// Grows the workspace for `batch` shots of a program ahead of a graph
+// capture (no allocation may happen while capturing).
+hipError_t wide_workspace_prepare(WideWorkspace* ws, uint32_t batch, uint32_t state_dim, int fp32,
+                                  const WideLayout& Y) {
+  if (!ws) return hipErrorInvalidValue;
+  return wide_workspace_reserve(*ws, batch, state_dim, fp32 ? sizeof(float) : sizeof(double), Y);
 }
 
 }  // namespace ticit
