@@ -90,7 +90,9 @@ typedef struct ticit_hip_program_info {
   uint64_t state_dim;       /* amplitudes per shot: 2^max_k dense, else sum of component capacities */
   uint64_t component_count; /* 1 dense */
   int32_t max_local_k;      /* widest amplitude vector an instruction acts on */
-  int32_t reserved1;
+  int32_t planner_source;   /* layout decided by: 0 the plan's heuristic flag, 1 the learned
+                               planner, 2 the caller (options.components) */
+  double planner_p_components; /* learned planner's P(components faster), -1 when not run */
 } ticit_hip_program_info;
 
 /* Thread-local message of the last failing call on this thread. */
@@ -241,6 +243,43 @@ enum {
   TICIT_HIP_NFEATURES
 };
 ticit_hip_status ticit_hip_plan_features(const uint64_t* blob, size_t nwords, double* out, size_t nout);
+
+/* ---- Learned planner -------------------------------------------------
+ * Replaces, when planner_mode = 1, the dense-versus-components decision of
+ * should_select_component_plan (src/component_plan.rs:282-298) with a
+ * 12 -> 64 -> 64 -> 3 FP32 MLP (ReLU): outputs are the logits of "dense"
+ * and "components" and the standardized log10 time per shot of the faster
+ * layout. Features (TICIT_HIP_PLANNER_FEATURES, raw): qubits, T count,
+ * mean log2 active dimension, nonzero fraction (component peak live /
+ * dense peak), Clifford depth (circuit layers), measurement records,
+ * log2(1 + estimated dense work), log2(1 + estimated component work),
+ * log2 dense peak, log2(1 + component peak live), log2(1 + component
+ * allocation), quantum-op count. No counterpart in the reference. */
+#define TICIT_HIP_PLANNER_FEATURES 12
+#define TICIT_HIP_MLP_PARAMS 5187 /* W1[12][64] b1[64] W2[64][64] b2[64] W3[64][3] b3[3] */
+typedef struct ticit_hip_mlp ticit_hip_mlp;
+
+ticit_hip_status ticit_hip_planner_features(const uint64_t* blob, size_t nwords, float* out);
+ticit_hip_status ticit_hip_mlp_create(const float* params, ticit_hip_mlp** out);
+/* Adam training on the device: step s uses samples order[s*batch .. +batch)
+ * (batch <= 64); loss = softmax cross-entropy on y (0 dense, 1 components)
+ * + time_weight * (output 2 - t)^2, averaged per minibatch into losses[s]. */
+ticit_hip_status ticit_hip_mlp_train(ticit_hip_mlp* mlp, const float* x, const int32_t* y, const float* t, size_t n,
+                                     const uint32_t* order, size_t steps, uint32_t batch, float lr,
+                                     float time_weight, float* losses);
+/* Gradients of the mean loss over the n <= 64 samples (one minibatch),
+ * without updating the parameters. */
+ticit_hip_status ticit_hip_mlp_gradients(ticit_hip_mlp* mlp, const float* x, const int32_t* y, const float* t,
+                                         size_t n, float time_weight, float* grads, float* loss);
+/* Inference kernel: out[n][3]. x is standardized by the caller. */
+ticit_hip_status ticit_hip_mlp_predict(const ticit_hip_mlp* mlp, const float* x, size_t n, float* out);
+ticit_hip_status ticit_hip_mlp_params_get(const ticit_hip_mlp* mlp, float* out);
+void ticit_hip_mlp_destroy(ticit_hip_mlp* mlp);
+/* Installs (params != NULL) or removes the process-wide learned planner used
+ * by planner_mode = 1. Features are standardized as (f - mean) / scale; when
+ * |P(components) - 1/2| < margin the plan's heuristic choice stands. */
+ticit_hip_status ticit_hip_planner_set(const float* params, const float* feature_mean, const float* feature_scale,
+                                       float margin);
 
 /* Device properties the tier planner sizes against; returns 0 when no
  * device is visible. props = {lds_per_block, compute_units, clock_khz,

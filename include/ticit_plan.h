@@ -32,8 +32,9 @@
  *
  * flags bit 0: use_active_components (src/factored.rs:885), set by the
  * planner when its component cost model selects the tensor-component
- * execution (src/planner.rs:1482-1484). Version 1 blobs have no flags word
- * and run dense.
+ * execution (src/planner.rs:1482-1484); bits 32-63: TICK-separated layers
+ * of the source circuit (0 = unknown), a planning feature only. Version 1
+ * blobs have no flags word and run dense.
  *
  * op codes: 0 rotate, 1 promote, 2 record, 3 detector, 4 measure, 5 coin.
  * Condition, record and detector ids are 1-based; 0 means "none".

@@ -74,5 +74,5 @@ def set_component_flag(blob: np.ndarray, on: bool) -> np.ndarray:
     out = np.array(blob, dtype=np.uint64, copy=True)
     if int(out[1]) != VERSION:
         raise ValueError("component flag needs a version-2 blob")
-    out[9] = np.uint64(1 if on else 0)
+    out[9] = np.uint64((int(out[9]) & ~1) | (1 if on else 0))
     return out

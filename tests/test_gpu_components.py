@@ -79,7 +79,7 @@ def away_from(probs, uni):
                                        ("c2_235", "wide")])
 def test_component_seeded_rows_match_oracle(hip, name, tier):
     blob = case_blob(hip, name)
-    assert int(blob[9]) == 1, "the planner selected the component execution"
+    assert int(blob[9]) & 1, "the planner selected the component execution"
     s = hip.CompiledSampler(blob, tier=tier, wide_batch=61)
     assert s.components and s.tier == tier
     shots = 400 if name != "c2_235" else 120
@@ -164,4 +164,4 @@ def test_planned_layout_is_the_reference_choice(hip):
     for i in (94, 235, 0, 1):
         c = hip.Circuit(_c2(i))
         s = c.compile()
-        assert s.components == bool(int(c.blob[9])) == O.component_plan(c.blob)["selected"]
+        assert s.components == bool(int(c.blob[9]) & 1) == O.component_plan(c.blob)["selected"]

@@ -17,6 +17,7 @@ import numpy as np
 
 from . import _capi
 from ._capi import TicitError
+from .planner import planner_features
 
 __all__ = ["Batch", "Circuit", "CompiledSampler", "SampleResult", "TicitError", "plan_features", "plan_text"]
 
@@ -111,6 +112,9 @@ class CompiledSampler:
         lib = _capi.load()
         self._lib = lib
         self.blob = np.ascontiguousarray(blob, dtype=np.uint64)
+        if planner == "learned":
+            from .planner import ensure_installed
+            ensure_installed()
         opt = _capi.Options()
         opt.observable = observable
         opt.precision = PRECISIONS[precision]
@@ -144,6 +148,8 @@ class CompiledSampler:
         self.state_dim = int(info.state_dim)
         self.component_count = int(info.component_count)
         self.max_local_k = int(info.max_local_k)
+        self.planner_source = {0: "heuristic", 1: "learned", 2: "caller"}[int(info.planner_source)]
+        self.planner_p_components = float(info.planner_p_components)
         self.has_postselection = False
 
     def close(self) -> None:

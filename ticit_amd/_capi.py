@@ -43,7 +43,8 @@ class ProgramInfo(ctypes.Structure):
                 ("components", ctypes.c_uint32), ("output_bytes_per_shot", ctypes.c_double),
                 ("state_bytes_per_shot", ctypes.c_double), ("algorithmic_bytes_per_shot", ctypes.c_double),
                 ("state_dim", ctypes.c_uint64), ("component_count", ctypes.c_uint64),
-                ("max_local_k", ctypes.c_int32), ("reserved1", ctypes.c_int32)]
+                ("max_local_k", ctypes.c_int32), ("planner_source", ctypes.c_int32),
+                ("planner_p_components", ctypes.c_double)]
 
 
 class BatchItem(ctypes.Structure):
@@ -77,7 +78,9 @@ EXPORTS = [
     "ticit_hip_sample_injected", "ticit_hip_final_state", "ticit_hip_sample_batch",
     "ticit_hip_result_free", "ticit_hip_plan_features",
     "ticit_hip_batch_create", "ticit_hip_batch_launch", "ticit_hip_batch_wait", "ticit_hip_batch_info_get",
-    "ticit_hip_batch_destroy",
+    "ticit_hip_batch_destroy", "ticit_hip_planner_features", "ticit_hip_mlp_create", "ticit_hip_mlp_train",
+    "ticit_hip_mlp_gradients", "ticit_hip_mlp_predict", "ticit_hip_mlp_params_get", "ticit_hip_mlp_destroy",
+    "ticit_hip_planner_set",
     "ticit_hip_device_query",
 ]
 
@@ -135,13 +138,28 @@ def load() -> ctypes.CDLL:
     lib.ticit_hip_batch_info_get.argtypes = [ctypes.c_void_p, ctypes.POINTER(BatchInfo)]
     lib.ticit_hip_batch_destroy.argtypes = [ctypes.c_void_p]
     lib.ticit_hip_batch_destroy.restype = None
+    _f32p = ctypes.POINTER(ctypes.c_float)
+    lib.ticit_hip_planner_features.argtypes = [_u64p, ctypes.c_size_t, _f32p]
+    lib.ticit_hip_mlp_create.argtypes = [_f32p, ctypes.POINTER(ctypes.c_void_p)]
+    lib.ticit_hip_mlp_train.argtypes = [ctypes.c_void_p, _f32p, ctypes.POINTER(ctypes.c_int32), _f32p, ctypes.c_size_t,
+                                        ctypes.POINTER(ctypes.c_uint32), ctypes.c_size_t, ctypes.c_uint32,
+                                        ctypes.c_float, ctypes.c_float, _f32p]
+    lib.ticit_hip_mlp_gradients.argtypes = [ctypes.c_void_p, _f32p, ctypes.POINTER(ctypes.c_int32), _f32p,
+                                            ctypes.c_size_t, ctypes.c_float, _f32p, _f32p]
+    lib.ticit_hip_mlp_predict.argtypes = [ctypes.c_void_p, _f32p, ctypes.c_size_t, _f32p]
+    lib.ticit_hip_mlp_params_get.argtypes = [ctypes.c_void_p, _f32p]
+    lib.ticit_hip_mlp_destroy.argtypes = [ctypes.c_void_p]
+    lib.ticit_hip_mlp_destroy.restype = None
+    lib.ticit_hip_planner_set.argtypes = [_f32p, _f32p, _f32p, ctypes.c_float]
     lib.ticit_hip_device_query.argtypes = [ctypes.POINTER(ctypes.c_int64)]
     lib.ticit_hip_device_query.restype = ctypes.c_int
     for name in ("ticit_hip_plan_text", "ticit_hip_program_create", "ticit_hip_compile_text",
                  "ticit_hip_program_info_get", "ticit_hip_sample", "ticit_hip_sample_device",
                  "ticit_hip_sample_injected", "ticit_hip_sample_batch", "ticit_hip_final_state",
                  "ticit_hip_plan_features", "ticit_hip_batch_create", "ticit_hip_batch_launch",
-                 "ticit_hip_batch_wait", "ticit_hip_batch_info_get"):
+                 "ticit_hip_batch_wait", "ticit_hip_batch_info_get", "ticit_hip_planner_features",
+                 "ticit_hip_mlp_create", "ticit_hip_mlp_train", "ticit_hip_mlp_gradients", "ticit_hip_mlp_predict",
+                 "ticit_hip_mlp_params_get", "ticit_hip_planner_set"):
         getattr(lib, name).restype = ctypes.c_int
     lib.ticit_hip_blob_free.restype = None
     lib.ticit_hip_program_destroy.restype = None

@@ -25,9 +25,11 @@ SOURCES = [
     "csrc/host/device_program.cpp",
     "csrc/host/capi.cpp",
     "csrc/host/batch.cpp",
+    "csrc/host/planner.cpp",
     "csrc/kernels/sampler.hip",
     "csrc/kernels/group_tier.hip",
     "csrc/kernels/wide_tier.hip",
+    "csrc/kernels/planner_mlp.hip",
 ]
 
 FLAGS = [

@@ -45,6 +45,17 @@ ticit_hip_status create_from_program(Program prog, const ticit_hip_options* opti
     return fail(TICIT_HIP_INVALID_INPUT, "components must be 0 (planned), 1 (dense) or 2 (tensor components)");
   }
   bool use_components = h->opt.components == 2 || (h->opt.components == 0 && h->prog.use_components);
+  if (h->opt.components != 0) {
+    h->planner_source = 2;
+  } else if (h->opt.planner_mode == 1) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    bool learned = false;
+    if (learned_layout(h->prog, dev, &learned, &h->planner_p)) {
+      use_components = learned;
+      h->planner_source = 1;
+    }
+  }
   try {
     h->tab = lower_to_device(h->prog, use_components);
   } catch (const TicitError& e) {
@@ -444,6 +455,8 @@ ticit_hip_status ticit_hip_program_info_get(const ticit_hip_program* h, ticit_hi
   out->output_bytes_per_shot = packed(h->tab.nrec) + packed(h->tab.ndet) + packed(h->tab.nobs);
   out->state_bytes_per_shot = h->tab.amp_traffic * (h->opt.precision == 1 ? 8.0 : 16.0);
   out->components = h->tab.components ? 1 : 0;
+  out->planner_source = h->planner_source;
+  out->planner_p_components = h->planner_p;
   out->state_dim = h->tab.state_dim;
   out->component_count = h->tab.comp_off.size();
   out->max_local_k = h->tab.max_local_k;

@@ -352,6 +352,7 @@ struct CObservable {
 struct Circuit {
   int nqubits = 0;
   int nrecords = 0;
+  int ticks = 0;  // TICK lines, REPEAT bodies expanded
   std::vector<CInstr> instrs;
   std::vector<CDetector> detectors;
   std::vector<CObservable> observables;
@@ -575,7 +576,11 @@ Circuit parse_circuit(const std::string& text) {
     const std::string& op = rl.name;
     CInstr ci;
     ci.line = rl.line;
-    if (op == "TICK" || op == "QUBIT_COORDS" || op == "SHIFT_COORDS" || op == "I") {
+    if (op == "TICK") {
+      ++c.ticks;
+      continue;
+    }
+    if (op == "QUBIT_COORDS" || op == "SHIFT_COORDS" || op == "I") {
       continue;
     } else if (k1.count(op)) {
       need_no_args(rl);
@@ -1677,6 +1682,7 @@ Program compile_circuit_text(const std::string& text, CompileStats* stats_out) {
   planned.observables.assign(nobs, {});
   for (const auto& o : c.observables)
     for (int r : o.records) planned.observables[o.index].push_back(r);
+  planned.circuit_layers = c.ticks + 1;
   if (stats_out) *stats_out = st;
   return planned;
 }

@@ -42,6 +42,9 @@ hipError_t launch_wide_tier(const DevProgram& P, const DevInstr* host_instrs, co
 
 constexpr uint32_t kTierLane = 1, kTierGroup = 2, kTierWide = 3;
 
+// Learned layout decision (planner.cpp); false = keep the heuristic's.
+bool learned_layout(const Program& p, int device, bool* use_components, float* p_components);
+
 ticit_hip_status fail(ticit_hip_status s, const std::string& msg);
 ticit_hip_status from_hip(hipError_t e, const char* what);
 
@@ -60,6 +63,8 @@ struct ticit_hip_program {
   ticit::GroupLayout group{};
   ticit::WideLayout wide{};
   uint32_t wide_batch = 0;
+  int32_t planner_source = 0;
+  float planner_p = -1.f;
   uint32_t meas_row_packed = 0, det_row_packed = 0, obs_row_packed = 0;
 
   ~ticit_hip_program() {

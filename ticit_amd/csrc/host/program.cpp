@@ -60,7 +60,7 @@ std::vector<uint64_t> encode_program(const Program& p) {
   o.push_back(uint64_t(p.nrecords));
   o.push_back(uint64_t(p.ndetectors));
   o.push_back(uint64_t(p.nexpvals));
-  o.push_back(p.use_components ? TICIT_PLAN_FLAG_COMPONENTS : 0);
+  o.push_back((p.use_components ? TICIT_PLAN_FLAG_COMPONENTS : 0) | (uint64_t(uint32_t(p.circuit_layers)) << 32));
   o.push_back(uint64_t(p.instrs.size()));
   for (const auto& in : p.instrs) {
     uint64_t w0 = uint64_t(uint8_t(in.op)) | (uint64_t(in.diagonal) << 8) |
@@ -149,7 +149,11 @@ bool decode_program(const uint64_t* words, size_t nwords, Program& p, std::strin
   p.nrecords = r.i32();
   p.ndetectors = r.i32();
   p.nexpvals = r.i32();
-  if (version >= 2) p.use_components = (r.next() & TICIT_PLAN_FLAG_COMPONENTS) != 0;
+  if (version >= 2) {
+    const uint64_t flags = r.next();
+    p.use_components = (flags & TICIT_PLAN_FLAG_COMPONENTS) != 0;
+    p.circuit_layers = int32_t(flags >> 32);
+  }
   size_t ninstr = r.count(8);
   p.instrs.resize(ninstr);
   for (size_t i = 0; i < ninstr && r.ok; ++i) {

@@ -93,6 +93,9 @@ struct Program {
   // FactoredInstructionProgram::use_active_components (src/factored.rs:885):
   // the planner selected the tensor-component execution of the active state
   bool use_components = false;
+  // TICK-separated layers of the source circuit (0 = unknown): the depth
+  // feature of the learned planner, carried in the blob's flags word
+  int32_t circuit_layers = 0;
   std::vector<Instr> instrs;
   std::vector<CategoricalDist> categorical;
   std::vector<RareGroup> rare;

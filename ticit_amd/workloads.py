@@ -254,16 +254,24 @@ def config3_circuit() -> str:
 
 
 def planner_corpus_circuit(index: int) -> str:
-    """C4 corpus: C0/C3-shaped circuits with varied width, depth and T count."""
+    """C4 corpus: circuits of the C0 layer generator with varied width, depth
+    and T count, half of them with C3-style mid-circuit measurement layers.
+    Sized so that most programs reach the active widths (max_k >= 8) where
+    dense and tensor-component execution are both available."""
     rng = SplitMix64(block_seed(CONFIG_SEED_BASE, 4, index))
-    n = 6 + rng.below(19)
-    layers = 8 + rng.below(33)
-    t = 1 + rng.below(min(16, n * layers))
+    n = 10 + rng.below(19)          # 10..28 qubits
+    layers = 10 + rng.below(51)     # 10..60 layers
+    t = 8 + rng.below(17)           # 8..24 T gates
     mids = {}
     if rng.next() & 1:
-        layer = rng.below(layers)
-        qs = sorted({rng.below(n) for _ in range(1 + rng.below(4))})
-        mids[layer] = qs
+        every = 10 + rng.below(11)
+        for layer in range(every - 1, layers - 1, every):
+            m = 1 + rng.below(max(1, n // 4))
+            qs = list(range(n))
+            for i in range(m):
+                j = i + rng.below(n - i)
+                qs[i], qs[j] = qs[j], qs[i]
+            mids[layer] = sorted(qs[:m])
     return random_clifford_t(n, layers, t, rng, mid_measure=mids)
 
 
