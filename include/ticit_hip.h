@@ -93,6 +93,9 @@ typedef struct ticit_hip_program_info {
   int32_t planner_source;   /* layout decided by: 0 the plan's heuristic flag, 1 the learned
                                planner, 2 the caller (options.components) */
   double planner_p_components; /* learned planner's P(components faster), -1 when not run */
+  uint32_t lds_bytes_per_block;  /* lane/group tier launch with bit-packed rows (0 wide) */
+  uint32_t blocks_per_cu;        /* resident workgroups per CU of that launch */
+  uint32_t table_words[4];       /* per-shot bit-table words: partials, slots, records, detectors */
 } ticit_hip_program_info;
 
 /* Thread-local message of the last failing call on this thread. */

@@ -150,6 +150,9 @@ class CompiledSampler:
         self.max_local_k = int(info.max_local_k)
         self.planner_source = {0: "heuristic", 1: "learned", 2: "caller"}[int(info.planner_source)]
         self.planner_p_components = float(info.planner_p_components)
+        self.lds_bytes_per_block = int(info.lds_bytes_per_block)
+        self.blocks_per_cu = int(info.blocks_per_cu)
+        self.table_words = [int(v) for v in info.table_words]
         self.has_postselection = False
 
     def close(self) -> None:

@@ -44,7 +44,8 @@ class ProgramInfo(ctypes.Structure):
                 ("state_bytes_per_shot", ctypes.c_double), ("algorithmic_bytes_per_shot", ctypes.c_double),
                 ("state_dim", ctypes.c_uint64), ("component_count", ctypes.c_uint64),
                 ("max_local_k", ctypes.c_int32), ("planner_source", ctypes.c_int32),
-                ("planner_p_components", ctypes.c_double)]
+                ("planner_p_components", ctypes.c_double), ("lds_bytes_per_block", ctypes.c_uint32),
+                ("blocks_per_cu", ctypes.c_uint32), ("table_words", ctypes.c_uint32 * 4)]
 
 
 class BatchItem(ctypes.Structure):

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <map>
 #include <mutex>
@@ -80,6 +81,7 @@ ticit_hip_status create_from_program(Program prog, const ticit_hip_options* opti
     }                                                      \
   } while (0)
   UP(instrs, t.instrs);
+  UP(steps, t.steps);
   UP(resid, t.resid);
   UP(det_recs, t.det_recs);
   UP(obs_off, t.obs_off);
@@ -127,7 +129,8 @@ ticit_hip_status create_from_program(Program prog, const ticit_hip_options* opti
   // Tier choice: per-lane resident state up to k = 5, one shot per
   // workgroup while the FP64 state fits the LDS, otherwise unsupported.
   auto words = [](int bits) { return uint32_t((bits + 63) / 64); };
-  h->lane = LaneLayout{words(t.npartials), words(t.nslots), words(t.nrec), words(t.ndet), 0};
+  h->lane = LaneLayout{words(t.npartials), words(t.nslots), words(t.nrec), words(t.ndet), 0,
+                       uint32_t(t.npartials), uint32_t(t.nslots), uint32_t(t.nrec), uint32_t(t.ndet)};
   h->group = GroupLayout{h->lane.pw, h->lane.sw, h->lane.rw, h->lane.dw};
   uint32_t max_row = std::max<uint32_t>({uint32_t(t.nrec), uint32_t(t.ndet), uint32_t(t.nobs), 1});
   h->lane.stage_bytes = 64 * ((max_row + 15) / 16 * 16);
@@ -456,6 +459,17 @@ ticit_hip_status ticit_hip_program_info_get(const ticit_hip_program* h, ticit_hi
   out->state_bytes_per_shot = h->tab.amp_traffic * (h->opt.precision == 1 ? 8.0 : 16.0);
   out->components = h->tab.components ? 1 : 0;
   out->planner_source = h->planner_source;
+  out->table_words[0] = h->lane.pw;
+  out->table_words[1] = h->lane.sw;
+  out->table_words[2] = h->lane.rw;
+  out->table_words[3] = h->lane.dw;
+  if (h->tier == kTierLane) {
+    uint32_t row = std::max({h->row_bytes(uint64_t(h->tab.nrec), true), h->row_bytes(uint64_t(h->tab.ndet), true),
+                             h->row_bytes(uint64_t(h->tab.nobs), true)});
+    lane_tier_occupancy(h->tab.max_k, h->lane, row, &out->lds_bytes_per_block, &out->blocks_per_cu);
+  } else if (h->tier == kTierGroup) {
+    out->lds_bytes_per_block = uint32_t(group_tier_lds_bytes(h->tab.state_dim, h->group, 256));
+  }
   out->planner_p_components = h->planner_p;
   out->state_dim = h->tab.state_dim;
   out->component_count = h->tab.comp_off.size();

@@ -408,6 +408,34 @@ HostTables lower_to_device(const Program& p, bool components) {
     t.obs_off.push_back(uint32_t(t.obs_recs.size()));
   }
   t.ninstr = int32_t(t.instrs.size());
+  for (size_t i = 0; i < t.instrs.size(); ++i) {
+    const DevInstr& d = t.instrs[i];
+    DevStep s{};
+    s.op = d.op;
+    s.partial = d.partial;
+    s.resid_off = d.resid_off;
+    s.resid_len = d.resid_len;
+    s.full = int32_t(i);
+    switch (d.op & 0xff) {
+      case kOpDetector:
+        s.a = d.detector;
+        s.b = int32_t(d.det_off);
+        s.c = int32_t(d.det_len);
+        break;
+      case kOpCoin:
+        s.full = d.ordinal;
+        s.a = d.record;
+        s.b = d.rc_slot;
+        s.c = d.branch_slot;
+        break;
+      default:
+        s.a = d.record;
+        s.b = d.rc_slot;
+        s.c = d.branch_slot;
+        break;
+    }
+    t.steps.push_back(s);
+  }
   // keep every table non-empty so device pointers are always valid
   if (t.resid.empty()) t.resid.push_back(0);
   if (t.det_recs.empty()) t.det_recs.push_back(0);
@@ -418,6 +446,7 @@ HostTables lower_to_device(const Program& p, bool components) {
   if (t.u64pool.empty()) t.u64pool.push_back(0);
   if (t.i32pool.empty()) t.i32pool.push_back(0);
   if (t.instrs.empty()) t.instrs.push_back(DevInstr{});
+  if (t.steps.empty()) t.steps.push_back(DevStep{});
   return t;
 }
 

@@ -11,6 +11,7 @@ namespace ticit {
 
 struct HostTables {
   std::vector<DevInstr> instrs;
+  std::vector<DevStep> steps;  // compact view, one per instr
   std::vector<uint32_t> resid, det_recs, obs_off, obs_recs, sym_off, sym_part;
   std::vector<int32_t> exo_syms;
   std::vector<DevCat> cats;

@@ -15,6 +15,7 @@
 namespace ticit {
 
 size_t lane_tier_lds_bytes(int kmax, const LaneLayout& y);
+void lane_tier_occupancy(int kmax, const LaneLayout& Y, uint32_t row_bytes, uint32_t* lds, uint32_t* blocks);
 size_t group_tier_lds_bytes(uint32_t state_dim, const GroupLayout& y, int tps);
 size_t group_region_bytes(uint32_t state_dim, const GroupLayout& y);
 int group_max_local_k(int tps);

@@ -63,6 +63,21 @@ struct alignas(16) DevInstr {
   double c1_im;
 };
 
+// Compact view of one instruction for the resident tiers' interpreter loop:
+// everything a record, detector, coin or branch update reads, in 32 bytes,
+// so the stream the loop walks stays in the scalar data cache; amplitude
+// instructions read their full DevInstr through `full`.
+struct alignas(16) DevStep {
+  uint32_t op;          // opcode | flags (as DevInstr::op)
+  int32_t partial;      // exogenous partial index, -1 = none
+  uint32_t resid_off;   // runtime-slot list of the expression residual
+  uint32_t resid_len;
+  int32_t a;            // Record/Measure/Coin: record (-1 none); Detector: detector
+  int32_t b;            // Record/Measure/Coin: rc_slot (-1 none); Detector: det_off
+  int32_t c;            // Measure/Coin: branch_slot; Detector: det_len
+  int32_t full;         // index of the DevInstr (Coin: its ordinal)
+};
+
 // Exogenous noise plan (src/factored.rs:444-462, src/planner.rs:1455-1480).
 struct DevCat {       // categorical distribution drawn once per shot
   uint32_t nbits, nrows;
@@ -110,6 +125,7 @@ struct DevLowp {      // low-probability Bernoulli group
 // Device pointers of one uploaded program.
 struct DevProgram {
   const DevInstr* instrs;
+  const DevStep* steps;       // [ninstr] compact view of instrs
   const uint32_t* resid;      // runtime slot ids
   const uint32_t* det_recs;   // 0-based record ids
   const uint32_t* obs_off;    // nobs + 1 offsets into obs_recs
@@ -162,8 +178,11 @@ struct DevLaunch {
 
 // LDS table sizes of the two resident tiers (words of bits per shot).
 struct LaneLayout {
-  uint32_t pw, sw, rw, dw;  // partial, runtime-slot, record, detector words
+  uint32_t pw, sw, rw, dw;  // partial, runtime-slot, record, detector words per shot
   uint32_t stage_bytes;     // per-wave row staging
+  // bit-sliced tables of the lane tier: one 64-bit word per item, bit l =
+  // the shot of lane l
+  uint32_t np, ns, nr, nd;  // partials, runtime slots, records, detectors
 };
 
 struct GroupLayout {

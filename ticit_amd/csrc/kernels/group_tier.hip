@@ -88,24 +88,26 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
     uint64_t m = 1ull << (idx & 63);
     tab[idx >> 6] = v ? (tab[idx >> 6] | m) : (tab[idx >> 6] & ~m);
   };
-  auto eval = [&](const DevInstr& in) -> bool {
-    bool b = (in.op & kFlagConstant) != 0;
-    if (in.partial >= 0) b ^= bit_of(pbits, uint32_t(in.partial));
-    for (uint32_t j = 0; j < in.resid_len; ++j) b ^= bit_of(sbits, P.resid[in.resid_off + j]);
+  auto eval = [&](const DevStep& st) -> bool {
+    bool b = (st.op & kFlagConstant) != 0;
+    if (st.partial >= 0) b ^= bit_of(pbits, uint32_t(st.partial));
+    for (uint32_t j = 0; j < st.resid_len; ++j) b ^= bit_of(sbits, P.resid[st.resid_off + j]);
     return b;
   };
 
   uint64_t brng = seed_for(kBranchSeedBase, launch_seed(L), shot);  // identical in the group
   int err = 0;
   for (int i = 0; i < P.ninstr; ++i) {
-    const DevInstr& in = P.instrs[i];  // fields read where used
-    const uint32_t op = in.op & 0xff;
+    // the compact step drives the loop; amplitude work reads the full record
+    const DevStep st = P.steps[i];
+    const uint32_t op = st.op & 0xff;
+    const DevInstr& in = P.instrs[op == kOpCoin ? 0 : st.full];  // fields read where used
     // every amplitude instruction acts on one component: 2^nq amplitudes at
     // comp_off (the whole state, offset 0, when the state is dense)
     double* re = re0 + in.comp_off;
     double* im = im0 + in.comp_off;
     if (op == kOpRotate) {
-      const bool sign = eval(in);
+      const bool sign = eval(st);
       const int dim = 1 << in.nq;
       const double c = in.c;
       if (in.op & kFlagDiagonal) {
@@ -142,7 +144,7 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
       }
       __syncthreads();
     } else if (op == kOpPromote) {
-      const bool sign = eval(in);
+      const bool sign = eval(st);
       const double c = in.c, q = sign ? in.s : -in.s;
       if (in.op & kFlagFresh) {
         // src/sampler/batch/runtime.rs:511-550: a new one-qubit component
@@ -187,19 +189,19 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
       __syncthreads();
     } else if (op == kOpRecord || op == kOpDetector) {
       bool b = false;
-      if (op == kOpDetector && (in.op & kFlagDetRecords)) {
-        for (uint32_t j = 0; j < in.det_len; ++j) b ^= bit_of(rbits, P.det_recs[in.det_off + j]);
+      if (op == kOpDetector && (st.op & kFlagDetRecords)) {
+        for (int32_t j = 0; j < st.c; ++j) b ^= bit_of(rbits, P.det_recs[st.b + j]);
       } else {
-        b = eval(in);
+        b = eval(st);
       }
       // One thread per shot writes; no reader of this instruction reads the
       // bit being written, and the barrier orders it before later readers.
       if (t == 0) {
         if (op == kOpDetector) {
-          set_bit(dbits, uint32_t(in.detector), b);
+          set_bit(dbits, uint32_t(st.a), b);
         } else {
-          if (in.record >= 0) set_bit(rbits, uint32_t(in.record), b);
-          if (in.rc_slot >= 0) set_bit(sbits, uint32_t(in.rc_slot), b);
+          if (st.a >= 0) set_bit(rbits, uint32_t(st.a), b);
+          if (st.b >= 0) set_bit(sbits, uint32_t(st.b), b);
         }
       }
       __syncthreads();
@@ -280,17 +282,17 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
       } else if (valid) {
         if (L.injected) {
           const uint64_t words = (L.shots + 63) / 64;
-          branch = (L.coins[uint64_t(in.ordinal) * words + (local >> 6)] >> (local & 63)) & 1;
+          branch = (L.coins[uint64_t(st.full) * words + (local >> 6)] >> (local & 63)) & 1;
         } else {
           branch = sm_next(brng) & 1;
         }
       }
-      if (t == 0) set_bit(sbits, uint32_t(in.branch_slot), branch);
+      if (t == 0) set_bit(sbits, uint32_t(st.c), branch);
       __syncthreads();
-      bool b = eval(in);
+      bool b = eval(st);
       if (t == 0) {
-        if (in.record >= 0) set_bit(rbits, uint32_t(in.record), b);
-        if (in.rc_slot >= 0) set_bit(sbits, uint32_t(in.rc_slot), b);
+        if (st.a >= 0) set_bit(rbits, uint32_t(st.a), b);
+        if (st.b >= 0) set_bit(sbits, uint32_t(st.b), b);
       }
       __syncthreads();
     }

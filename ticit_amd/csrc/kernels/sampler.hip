@@ -34,65 +34,88 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
 
   double2* amp = reinterpret_cast<double2*>(smem);  // [DIM][64]
   double2* scr = amp + DIM * 64;                    // [SDIM][64]
-  uint64_t* pbits = reinterpret_cast<uint64_t*>(scr + SDIM * 64);
-  uint64_t* sbits = pbits + Y.pw * 64;
-  uint64_t* rbits = sbits + Y.sw * 64;
-  uint64_t* dbits = rbits + Y.rw * 64;
-  uint8_t* stage = reinterpret_cast<uint8_t*>(dbits + Y.dw * 64);
+  // Bit-sliced tables: word i holds item i of all 64 shots (bit l = lane l),
+  // so a record, detector or expression update is one uniform word
+  // operation for the whole wave.
+  uint64_t* Pw = reinterpret_cast<uint64_t*>(scr + SDIM * 64);  // exogenous partials
+  uint64_t* Sw = Pw + Y.np;                                     // runtime slots
+  uint64_t* Rw = Sw + Y.ns;                                     // records
+  uint64_t* Dw = Rw + Y.nr;                                     // detectors
+  uint8_t* stage = reinterpret_cast<uint8_t*>(Dw + Y.nd);
 
-  for (uint32_t w = 0; w < Y.pw; ++w) pbits[w * 64 + lane] = 0;
-  for (uint32_t w = 0; w < Y.sw; ++w) sbits[w * 64 + lane] = 0;
-  for (uint32_t w = 0; w < Y.rw; ++w) rbits[w * 64 + lane] = 0;
-  for (uint32_t w = 0; w < Y.dw; ++w) dbits[w * 64 + lane] = 0;
+  for (uint32_t w = lane; w < Y.np + Y.ns + Y.nr + Y.nd; w += 64) Pw[w] = 0;
+  __syncthreads();
 
   // Frame: noise events toggle the exogenous partial bits they feed.
-  auto toggle = [&](int32_t c) {
-    uint32_t b = P.sym_off[c - 1], e = P.sym_off[c];
-    for (uint32_t j = b; j < e; ++j) {
-      uint32_t part = P.sym_part[j];
-      pbits[(part >> 6) * 64 + lane] ^= 1ull << (part & 63);
-    }
-  };
   if (L.injected) {
     const uint64_t words = (L.shots + 63) / 64;
     for (int i = 0; i < P.nexo; ++i) {
-      int32_t c = P.exo_syms[i];
-      uint64_t wv = L.exo[uint64_t(c - 1) * words + blockIdx.x];
-      if ((wv >> lane) & 1) toggle(c);
+      const int32_t c = P.exo_syms[i];
+      const uint64_t wv = L.exo[uint64_t(c - 1) * words + blockIdx.x];
+      if (wv)
+        for (uint32_t j = P.sym_off[c - 1]; j < P.sym_off[c]; ++j) Pw[P.sym_part[j]] ^= wv;  // same value in every lane
+      __syncthreads();
     }
   } else {
-    sample_exogenous(P, seed_for(kExoSeedBase, launch_seed(L), shot), toggle);
+    const uint64_t me = 1ull << lane;
+    sample_exogenous(P, seed_for(kExoSeedBase, launch_seed(L), shot), [&](int32_t c) {
+      for (uint32_t j = P.sym_off[c - 1]; j < P.sym_off[c]; ++j) atomicXor(&Pw[P.sym_part[j]], me);
+    });
   }
+  __syncthreads();
 
   for (int a = 0; a < DIM; ++a) amp[a * 64 + lane] = make_double2(a == 0 ? 1.0 : 0.0, 0.0);
 
-  auto bit_of = [&](const uint64_t* tab, uint32_t idx) -> bool {
-    return (tab[(idx >> 6) * 64 + lane] >> (idx & 63)) & 1;
+  // expression value of all 64 shots
+  auto eval = [&](const DevStep& st) -> uint64_t {
+    uint64_t v = (st.op & kFlagConstant) ? ~0ull : 0ull;
+    if (st.partial >= 0) v ^= Pw[st.partial];
+    for (uint32_t j = 0; j < st.resid_len; ++j) v ^= Sw[P.resid[st.resid_off + j]];
+    return v;
   };
-  auto set_bit = [&](uint64_t* tab, uint32_t idx, bool v) {
-    uint64_t m = 1ull << (idx & 63);
-    uint64_t& w = tab[(idx >> 6) * 64 + lane];
-    w = v ? (w | m) : (w & ~m);
-  };
-  auto eval = [&](const DevInstr& in) -> bool {
-    bool b = (in.op & kFlagConstant) != 0;
-    if (in.partial >= 0) b ^= bit_of(pbits, uint32_t(in.partial));
-    for (uint32_t j = 0; j < in.resid_len; ++j) b ^= bit_of(sbits, P.resid[in.resid_off + j]);
-    return b;
-  };
-  auto write_outcome = [&](const DevInstr& in, bool b) {
-    if (in.record >= 0) set_bit(rbits, uint32_t(in.record), b);
-    if (in.rc_slot >= 0) set_bit(sbits, uint32_t(in.rc_slot), b);
+  // Every lane stores the same word; the wave's LDS operations complete in
+  // issue order, so later reads by any lane see it.
+  auto write_outcome = [&](const DevStep& st, uint64_t v) {
+    if (st.a >= 0) Rw[st.a] = v;
+    if (st.b >= 0) Sw[st.b] = v;
   };
 
   int k = P.initial_k;
   uint64_t brng = seed_for(kBranchSeedBase, launch_seed(L), shot);
   int err = 0;
   for (int i = 0; i < P.ninstr; ++i) {
-    const DevInstr in = P.instrs[i];
-    const uint32_t op = in.op & 0xff;
+    // the compact step drives the loop; amplitude work reads the full record
+    const DevStep st = P.steps[i];
+    const uint32_t op = st.op & 0xff;
+    if (op == kOpRecord) {
+      write_outcome(st, eval(st));
+      continue;
+    }
+    if (op == kOpDetector) {
+      uint64_t v = 0;
+      if (st.op & kFlagDetRecords) {
+        for (int32_t j = 0; j < st.c; ++j) v ^= Rw[P.det_recs[st.b + j]];
+      } else {
+        v = eval(st);
+      }
+      Dw[st.a] = v;
+      continue;
+    }
+    if (op == kOpCoin) {
+      uint64_t coins;
+      if (L.injected) {
+        const uint64_t words = (L.shots + 63) / 64;
+        coins = L.coins[uint64_t(st.full) * words + blockIdx.x];
+      } else {
+        coins = __ballot(sm_next(brng) & 1);  // one word per shot's stream
+      }
+      Sw[st.c] = coins;
+      write_outcome(st, eval(st));
+      continue;
+    }
+    const DevInstr& in = P.instrs[st.full];
     if (op == kOpRotate) {
-      const bool sign = eval(in);
+      const bool sign = (eval(st) >> lane) & 1;
       const int dim = 1 << k;
       const double c = in.c;
       if (in.op & kFlagDiagonal) {
@@ -133,7 +156,7 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
         }
       }
     } else if (op == kOpPromote) {
-      const bool sign = eval(in);
+      const bool sign = (eval(st) >> lane) & 1;
       const int dim = 1 << k;
       const double c = in.c, q = sign ? in.s : -in.s;
       for (int b = 0; b < dim; ++b) {
@@ -142,16 +165,6 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
         amp[(dim + b) * 64 + lane] = make_double2(-q * v.y, q * v.x);
       }
       ++k;
-    } else if (op == kOpRecord) {
-      write_outcome(in, eval(in));
-    } else if (op == kOpDetector) {
-      bool b = false;
-      if (in.op & kFlagDetRecords) {
-        for (uint32_t j = 0; j < in.det_len; ++j) b ^= bit_of(rbits, P.det_recs[in.det_off + j]);
-      } else {
-        b = eval(in);
-      }
-      set_bit(dbits, uint32_t(in.detector), b);
     } else if (op == kOpMeasure) {
       const int out_dim = (1 << k) >> 1;
       const bool diag = in.op & kFlagDiagonal;
@@ -174,13 +187,14 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
       }
       pt = clamp01(pt);
       if (L.want_probs && live) L.probs[uint64_t(in.ordinal) * L.shots + local] = pt;
-      bool branch;
+      bool branch = false;
       if (L.injected) {
-        double u = L.uniforms[uint64_t(in.ordinal) * L.shots + local];
-        branch = pt <= 0.0 ? false : (pt >= 1.0 ? true : u < pt);
+        if (live) {
+          double u = L.uniforms[uint64_t(in.ordinal) * L.shots + local];
+          branch = pt <= 0.0 ? false : (pt >= 1.0 ? true : u < pt);
+        }
       } else if (!(pt >= 0.0 && pt <= 1.0)) {
         err |= kDevErrBadProbability;
-        branch = false;
       } else {
         branch = pt <= 0.0 ? false : (pt >= 1.0 ? true : sm_uniform(brng) < pt);
       }
@@ -209,49 +223,43 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
         for (int idx = 0; idx < out_dim; ++idx) amp[idx * 64 + lane] = scr[idx * 64 + lane];
       }
       --k;
-      set_bit(sbits, uint32_t(in.branch_slot), branch);
-      write_outcome(in, eval(in));
-    } else {  // kOpCoin
-      bool coin;
-      if (L.injected) {
-        const uint64_t words = (L.shots + 63) / 64;
-        coin = (L.coins[uint64_t(in.ordinal) * words + blockIdx.x] >> lane) & 1;
-      } else {
-        coin = sm_next(brng) & 1;  // one word per 64 shots, one live shot
-      }
-      set_bit(sbits, uint32_t(in.branch_slot), coin);
-      write_outcome(in, eval(in));
+      Sw[st.c] = __ballot(branch);
+      write_outcome(st, eval(st));
     }
   }
   if (err && live) atomicOr(L.error, err);
 
   // Observables (src/sampler/prepared.rs:881-907) and their popcounts.
   const uint32_t nlive = uint32_t(min<uint64_t>(64, L.shots - uint64_t(blockIdx.x) * 64));
-  uint8_t* row = stage + size_t(lane) * (Y.stage_bytes / 64);
+  const uint64_t live_mask = nlive >= 64 ? ~0ull : ((1ull << nlive) - 1);
+  auto obs_word = [&](uint32_t o) {
+    uint64_t v = 0;
+    for (uint32_t j = P.obs_off[o]; j < P.obs_off[o + 1]; ++j) v ^= Rw[P.obs_recs[j]];
+    return v;
+  };
   for (int o = 0; o < P.nobs; ++o) {
-    bool b = false;
-    for (uint32_t j = P.obs_off[o]; j < P.obs_off[o + 1]; ++j) b ^= bit_of(rbits, P.obs_recs[j]);
-    unsigned long long ones = __popcll(__ballot(b && live));
+    unsigned long long ones = __popcll(obs_word(uint32_t(o)) & live_mask);
     if (lane == 0 && ones) atomicAdd(&L.obs_ones[o], ones);
   }
   if (!L.keep_records) return;
 
-  // Row packing (src/sampler/prepared.rs:909-941): a lane's record words
-  // are its packed row in little-endian byte order; rows are staged in LDS
-  // so the wave writes one contiguous, 16-byte-vectorized block per table.
-  auto emit = [&](uint8_t* out, uint32_t row_bytes, auto&& bit_at, uint32_t ncols) {
+  // Row packing (src/sampler/prepared.rs:909-941): each lane builds its
+  // shot's row from the sliced words; rows are staged in LDS so the wave
+  // writes one contiguous block per table.
+  uint8_t* row = stage + size_t(lane) * (Y.stage_bytes / 64);
+  auto emit = [&](uint8_t* out, uint32_t row_bytes, auto&& word_at, uint32_t ncols) {
     if (row_bytes == 0) return;
     if (L.bit_packed) {
       for (uint32_t byte = 0; byte < row_bytes; ++byte) {
         uint32_t v = 0;
         for (uint32_t bb = 0; bb < 8; ++bb) {
           uint32_t col = byte * 8 + bb;
-          if (col < ncols && bit_at(col)) v |= 1u << bb;
+          if (col < ncols && ((word_at(col) >> lane) & 1)) v |= 1u << bb;
         }
         row[byte] = uint8_t(v);
       }
     } else {
-      for (uint32_t col = 0; col < ncols; ++col) row[col] = uint8_t(bit_at(col));
+      for (uint32_t col = 0; col < ncols; ++col) row[col] = uint8_t((word_at(col) >> lane) & 1);
     }
     __syncthreads();
     const size_t total = size_t(nlive) * row_bytes;
@@ -263,16 +271,9 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
     }
     __syncthreads();
   };
-  emit(L.meas, L.meas_row, [&](uint32_t c) { return bit_of(rbits, c); }, uint32_t(P.nrec));
-  emit(L.det, L.det_row, [&](uint32_t c) { return bit_of(dbits, c); }, uint32_t(P.ndet));
-  emit(
-      L.obs, L.obs_row,
-      [&](uint32_t o) {
-        bool b = false;
-        for (uint32_t j = P.obs_off[o]; j < P.obs_off[o + 1]; ++j) b ^= bit_of(rbits, P.obs_recs[j]);
-        return b;
-      },
-      uint32_t(P.nobs));
+  emit(L.meas, L.meas_row, [&](uint32_t c) { return Rw[c]; }, uint32_t(P.nrec));
+  emit(L.det, L.det_row, [&](uint32_t c) { return Dw[c]; }, uint32_t(P.ndet));
+  emit(L.obs, L.obs_row, obs_word, uint32_t(P.nobs));
 }
 
 // --------------------------------------------------------------------------
@@ -282,11 +283,17 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
 size_t lane_tier_lds_bytes(int kmax, const LaneLayout& y) {
   size_t dim = size_t(1) << kmax;
   size_t sdim = kmax > 0 ? dim / 2 : 0;
-  return (dim + sdim) * 64 * 16 + size_t(y.pw + y.sw + y.rw + y.dw) * 64 * 8 + y.stage_bytes;
+  return (dim + sdim) * 64 * 16 + size_t(y.np + y.ns + y.nr + y.nd) * 8 + y.stage_bytes;
 }
 
 template <int K>
-static hipError_t launch_lane_k(const DevProgram& P, const DevLaunch& L, const LaneLayout& Y, hipStream_t stream) {
+static hipError_t launch_lane_k(const DevProgram& P, const DevLaunch& L, const LaneLayout& Yp, hipStream_t stream) {
+  // Row staging sized for this launch's rows (packed rows are 8x smaller
+  // than the program's worst case): less LDS per wave, more waves per CU.
+  LaneLayout Y = Yp;
+  uint32_t row = L.keep_records ? (L.meas_row > L.det_row ? L.meas_row : L.det_row) : 0;
+  row = row > L.obs_row ? row : L.obs_row;
+  Y.stage_bytes = 64 * ((row + 15) / 16 * 16);
   size_t lds = lane_tier_lds_bytes(K, Y);
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&lane_tier_kernel<K>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
@@ -294,6 +301,34 @@ static hipError_t launch_lane_k(const DevProgram& P, const DevLaunch& L, const L
   dim3 grid(unsigned((L.shots + 63) / 64));
   hipLaunchKernelGGL(lane_tier_kernel<K>, grid, dim3(64), lds, stream, P, L, Y);
   return hipGetLastError();
+}
+
+// LDS bytes and resident workgroups per CU of a lane-tier launch with
+// packed rows of `row_bytes`.
+template <int K>
+static void lane_occupancy_k(const LaneLayout& Yp, uint32_t row_bytes, uint32_t* lds, uint32_t* blocks) {
+  LaneLayout Y = Yp;
+  Y.stage_bytes = 64 * ((row_bytes + 15) / 16 * 16);
+  *lds = uint32_t(lane_tier_lds_bytes(K, Y));
+  int b = 0;
+  if (hipFuncSetAttribute(reinterpret_cast<const void*>(&lane_tier_kernel<K>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, int(*lds)) == hipSuccess)
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, reinterpret_cast<const void*>(&lane_tier_kernel<K>), 64,
+                                                       *lds);
+  *blocks = uint32_t(b);
+}
+
+void lane_tier_occupancy(int kmax, const LaneLayout& Y, uint32_t row_bytes, uint32_t* lds, uint32_t* blocks) {
+  *lds = *blocks = 0;
+  switch (kmax) {
+    case 0: return lane_occupancy_k<0>(Y, row_bytes, lds, blocks);
+    case 1: return lane_occupancy_k<1>(Y, row_bytes, lds, blocks);
+    case 2: return lane_occupancy_k<2>(Y, row_bytes, lds, blocks);
+    case 3: return lane_occupancy_k<3>(Y, row_bytes, lds, blocks);
+    case 4: return lane_occupancy_k<4>(Y, row_bytes, lds, blocks);
+    case 5: return lane_occupancy_k<5>(Y, row_bytes, lds, blocks);
+    default: return;
+  }
 }
 
 hipError_t launch_lane_tier(int kmax, const DevProgram& P, const DevLaunch& L, const LaneLayout& Y,
