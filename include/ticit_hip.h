@@ -83,7 +83,8 @@ typedef struct ticit_hip_program_info {
   /* Algorithmic bytes per shot: packed output rows plus, when the tier keeps
    * the state in device memory, S * sum over amplitude instructions of
    * f * 2^k (S = 16 FP64 / 8 FP32 bytes per amplitude; f = 2 rotation,
-   * 3 promotion, 2.5 measurement). */
+   * 3 promotion, 2.5 nondiagonal and 1.5 diagonal measurement; a merge
+   * counts 2^a + 2^b + 2^(a+b)). */
   double output_bytes_per_shot;
   double state_bytes_per_shot; /* the state term alone, whatever the tier */
   double algorithmic_bytes_per_shot;
@@ -187,7 +188,9 @@ typedef struct ticit_hip_batch_options {
   uint32_t autotune_shots; /* shots per program in a timing run (0 = 8192) */
   uint32_t group_grid_percent; /* persistent group-tier grid, percent of full occupancy
                                   (0 = 100); below 100 leaves room for wide-tier kernels */
-  uint32_t reserved[3];
+  uint32_t wide_cu_percent;    /* > 0: split the CUs between wide-tier work (this share) and
+                                  group/lane work on CU-masked streams that run concurrently */
+  uint32_t reserved[2];
 } ticit_hip_batch_options;
 
 typedef struct ticit_hip_batch_info {

@@ -54,6 +54,7 @@ def _load(name: str) -> ctypes.CDLL:
     lib.oracle_insert_zero_bit.restype = ctypes.c_size_t
     lib.oracle_set_fp32_storage.argtypes = [ctypes.c_int]
     lib.oracle_set_component_mode.argtypes = [ctypes.c_int]
+    lib.oracle_set_exact_sums.argtypes = [ctypes.c_int]
     lib.oracle_component_plan.argtypes = [_u64p, ctypes.c_size_t, _i64p, _f64p, _i64p, _i64p, _i64p,
                                           ctypes.c_size_t]
     lib.oracle_default_batch.argtypes = [ctypes.c_int]
@@ -159,6 +160,12 @@ def sample_injected(blob: np.ndarray, shots: int, exo: np.ndarray | None, unifor
 def set_fp32_storage(on: bool) -> None:
     """Emulate FP32 amplitude storage in injected runs (precision studies)."""
     lib().oracle_set_fp32_storage(int(on))
+
+
+def set_exact_sums(on: bool) -> None:
+    """Compensated (Neumaier) probability sums in injected runs: a
+    reference closer to exact than the reference sampler's scalar order."""
+    lib().oracle_set_exact_sums(int(on))
 
 
 def set_component_mode(mode: int) -> None:

@@ -38,6 +38,7 @@ namespace {
 thread_local std::string g_err;
 thread_local bool g_fp32_storage = false;
 int g_component_mode = -1;  // read by worker threads: not thread-local
+bool g_exact_sums = false;  // Neumaier-compensated probability sums (accuracy studies)
 
 struct OracleError : std::runtime_error {
   using std::runtime_error::runtime_error;
@@ -535,7 +536,37 @@ size_t diagonal_source(const MeasK& k, size_t packed, bool branch) {
 double clamp01(double v) { return v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v); }
 
 // src/contiguous.rs:1149-1320 scalar paths, branch = true
+// Neumaier's compensated summation (Neumaier 1974), used only to measure
+// how far a plain sum is from the exact one.
+struct CompSum {
+  double s = 0.0, c = 0.0;
+  void add(double x) {
+    double t = s + x;
+    c += std::fabs(s) >= std::fabs(x) ? (s - t) + x : (x - t) + s;
+    s = t;
+  }
+  double value() const { return s + c; }
+};
+
 double prob_true(const double* re, const double* im, const MeasK& k) {
+  if (g_exact_sums) {
+    CompSum acc;
+    for (size_t idx = 0; idx < k.out_dim; ++idx) {
+      if (k.diagonal) {
+        size_t s = diagonal_source(k, idx, true);
+        acc.add(re[s] * re[s] + im[s] * im[s]);
+      } else {
+        size_t s0 = insert_zero_bit(idx, k.pivot), s1 = s0 ^ size_t(k.xmask);
+        bool odd = parity(uint64_t(s0) & k.zmask);
+        double dir = (true != odd) ? -1.0 : 1.0;
+        double c1r = dir * k.coef1_even.re, c1i = dir * k.coef1_even.im;
+        double ar = kInvSqrt2 * re[s0] + c1r * re[s1] - c1i * im[s1];
+        double ai = kInvSqrt2 * im[s0] + c1r * im[s1] + c1i * re[s1];
+        acc.add(ar * ar + ai * ai);
+      }
+    }
+    return clamp01(acc.value());
+  }
   double p = 0.0;
   if (k.diagonal) {
     for (size_t idx = 0; idx < k.out_dim; ++idx) {
@@ -1243,6 +1274,10 @@ void oracle_set_fp32_storage(int on) { g_fp32_storage = on != 0; }
 // -1: execute as the blob's use_active_components flag says; 0: dense;
 // 1: tensor components (the program must have a component plan).
 void oracle_set_component_mode(int mode) { g_component_mode = mode < 0 ? -1 : (mode ? 1 : 0); }
+
+// 1: probability sums with Neumaier compensation instead of the reference's
+// plain scalar order, to measure the summation error of either side.
+void oracle_set_exact_sums(int on) { g_exact_sums = on != 0; }
 
 // The component plan of src/component_plan.rs:301-522.
 // summary[0..8] = selected, initial_components, component_count,

@@ -143,3 +143,21 @@ def test_component_probabilities_match_dense(which):
     assert np.array_equal(dense["measurements"], comp["measurements"])
     if "probs" in dense and dense["probs"].size:
         assert np.max(np.abs(dense["probs"] - comp["probs"])) <= 1e-12
+
+
+def test_compensated_sums_agree_with_the_scalar_order_on_small_states():
+    # a k <= 12 program: both orders are within a few ulps of each other
+    from ticit_amd import plan_text
+    from ticit_amd import workloads as W
+    blob = plan_text(W.config0_circuit())
+    rng = np.random.default_rng(4)
+    inf = O.info(blob)
+    uni = rng.random((inf["nmeasure"], 16))
+    a = O.sample_injected(blob, 16, None, uni, None, want_probs=True)
+    O.set_exact_sums(True)
+    try:
+        b = O.sample_injected(blob, 16, None, uni, None, want_probs=True)
+    finally:
+        O.set_exact_sums(False)
+    assert np.array_equal(a["measurements"], b["measurements"])
+    assert np.max(np.abs(a["probs"] - b["probs"])) <= 1e-13

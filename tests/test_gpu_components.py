@@ -165,3 +165,26 @@ def test_planned_layout_is_the_reference_choice(hip):
         c = hip.Circuit(_c2(i))
         s = c.compile()
         assert s.components == bool(int(c.blob[9]) & 1) == O.component_plan(c.blob)["selected"]
+
+
+def test_fp64_probabilities_against_compensated_sums(hip):
+    """C3 at width 17: the device's tree-reduced FP64 probabilities against
+    the oracle with Neumaier-compensated sums (closer to exact than the
+    reference's scalar order); also reports the scalar order's own distance."""
+    blob = case_blob(hip, "c3")
+    s = hip.CompiledSampler(blob)
+    shots = 8
+    rng = np.random.default_rng(3)
+    uni = rng.random((s.num_measure_instructions, shots))
+    plain = O.sample_injected(blob, shots, None, uni, None, want_probs=True)
+    O.set_exact_sums(True)
+    try:
+        exact = O.sample_injected(blob, shots, None, uni, None, want_probs=True)
+    finally:
+        O.set_exact_sums(False)
+    r = s.sample_injected(shots, None, uni, None, want_probs=True)
+    assert np.array_equal(r.measurements, exact["measurements"])
+    d_gpu = float(np.max(np.abs(r.probabilities - exact["probs"])))
+    d_scalar = float(np.max(np.abs(plain["probs"] - exact["probs"])))
+    print(f"c3 fp64: device vs compensated {d_gpu:.2e}; scalar order vs compensated {d_scalar:.2e}", flush=True)
+    assert d_gpu <= 1e-12

@@ -313,7 +313,8 @@ class Batch:
     ([nobservables] 64-bit counts, accumulated across launches)."""
 
     def __init__(self, samplers, shots, seeds, outputs, bit_packed: bool = True, keep_records: bool = True,
-                 autotune: bool = True, autotune_shots: int = 0, group_grid_percent: int = 0):
+                 autotune: bool = True, autotune_shots: int = 0, group_grid_percent: int = 0,
+                 wide_cu_percent: int = 0):
         lib = _capi.load()
         self._lib = lib
         self._keep = (list(samplers), list(outputs))  # buffers and handles outlive the batch
@@ -334,6 +335,7 @@ class Batch:
         opt.autotune = int(autotune)
         opt.autotune_shots = int(autotune_shots)
         opt.group_grid_percent = int(group_grid_percent)
+        opt.wide_cu_percent = int(wide_cu_percent)
         handle = ctypes.c_void_p()
         _capi.check(lib.ticit_hip_batch_create(items, n, ctypes.byref(opt), ctypes.byref(handle)))
         self._handle = handle

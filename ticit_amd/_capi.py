@@ -57,7 +57,7 @@ class BatchItem(ctypes.Structure):
 class BatchOptions(ctypes.Structure):
     _fields_ = [("bit_packed", ctypes.c_int32), ("keep_records", ctypes.c_int32), ("autotune", ctypes.c_uint32),
                 ("autotune_shots", ctypes.c_uint32), ("group_grid_percent", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 3)]
+                ("wide_cu_percent", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2)]
 
 
 class BatchInfo(ctypes.Structure):

@@ -45,6 +45,7 @@ struct ticit_hip_batch {
   int device = 0;
   int bit_packed = 1, keep_records = 1;
   int group_grid_percent = 100;  // persistent group grid, percent of full occupancy
+  int wide_cu_percent = 0;        // > 0: CU-masked streams split the device between wide and group work
   std::vector<ticit_hip_program*> progs;
   std::vector<DevLaunch> launches;
   std::vector<uint32_t> lane_items, wide_items;
@@ -55,12 +56,14 @@ struct ticit_hip_batch {
   int* d_error = nullptr;
   WideWorkspace* ws = nullptr;
   hipStream_t main = nullptr, side = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr, join_main = nullptr;
   hipGraph_t graph = nullptr;        // lane and group work
   hipGraphExec_t exec = nullptr;
   hipGraph_t wide_graph = nullptr;   // wide work, replayed on `side` beside `graph`
   hipGraphExec_t wide_exec = nullptr;
   hipStream_t last = nullptr;
+  bool split = false;  // main/side are CU-masked halves of the device
+  int group_cus = 0;   // CUs the group/lane stream may use when split
 
   ~ticit_hip_batch() {
     if (last) (void)hipStreamSynchronize(last);
@@ -71,6 +74,7 @@ struct ticit_hip_batch {
     if (wide_graph) (void)hipGraphDestroy(wide_graph);
     if (fork) (void)hipEventDestroy(fork);
     if (join) (void)hipEventDestroy(join);
+    if (join_main) (void)hipEventDestroy(join_main);
     if (main) (void)hipStreamDestroy(main);
     if (side) (void)hipStreamDestroy(side);
     wide_workspace_free(ws);
@@ -128,6 +132,7 @@ hipError_t run_bucket(ticit_hip_batch& b, ticit_hip_batch::Bucket& k, int tps, c
   int occ = group_batch_occupancy(tps, k.region);
   int cus = 0;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b.device);
+  if (b.split && b.group_cus > 0) cus = b.group_cus;
   int grid = std::max(1, occ) * std::max(1, cus);
   if (b.group_grid_percent > 0 && b.group_grid_percent < 100)
     grid = std::max(cus, grid * b.group_grid_percent / 100);
@@ -220,6 +225,7 @@ ticit_hip_status ticit_hip_batch_create(const ticit_hip_batch_item* items, size_
   b->bit_packed = opt.bit_packed ? 1 : 0;
   b->keep_records = opt.keep_records ? 1 : 0;
   if (opt.group_grid_percent) b->group_grid_percent = int(std::min<uint32_t>(opt.group_grid_percent, 100));
+  if (opt.wide_cu_percent) b->wide_cu_percent = int(std::min<uint32_t>(opt.wide_cu_percent, 95));
   hipError_t e = hipGetDevice(&b->device);
   int lds_limit = 0;
   if (e == hipSuccess) e = hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, b->device);
@@ -285,10 +291,30 @@ ticit_hip_status ticit_hip_batch_create(const ticit_hip_batch_item* items, size_
     }
   }
 
-  if ((e = hipStreamCreateWithFlags(&b->main, hipStreamNonBlocking)) != hipSuccess) return bail_hip(e, "streams");
-  if ((e = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking)) != hipSuccess) return bail_hip(e, "streams");
+  if (b->wide_cu_percent > 0 && !b->wide_items.empty()) {
+    // Spatial split: the wide (HBM-streaming) work gets every CU whose
+    // index falls in the first `wide_cu_percent` of each 20-CU stride, the
+    // group and lane work the rest, so both run at once on their own CUs.
+    int cus = 0;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device);
+    std::vector<uint32_t> wm(size_t((cus + 31) / 32), 0), gm(wm.size(), 0);
+    for (int c = 0; c < cus; ++c) {
+      const bool wide = (c % 20) < (b->wide_cu_percent * 20 + 50) / 100;
+      (wide ? wm : gm)[size_t(c / 32)] |= 1u << (c % 32);
+      b->group_cus += wide ? 0 : 1;
+    }
+    if ((e = hipExtStreamCreateWithCUMask(&b->side, uint32_t(wm.size()), wm.data())) != hipSuccess)
+      return bail_hip(e, "CU-masked streams");
+    if ((e = hipExtStreamCreateWithCUMask(&b->main, uint32_t(gm.size()), gm.data())) != hipSuccess)
+      return bail_hip(e, "CU-masked streams");
+    b->split = true;
+  } else {
+    if ((e = hipStreamCreateWithFlags(&b->main, hipStreamNonBlocking)) != hipSuccess) return bail_hip(e, "streams");
+    if ((e = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking)) != hipSuccess) return bail_hip(e, "streams");
+  }
   if ((e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming)) != hipSuccess) return bail_hip(e, "events");
   if ((e = hipEventCreateWithFlags(&b->join, hipEventDisableTiming)) != hipSuccess) return bail_hip(e, "events");
+  if ((e = hipEventCreateWithFlags(&b->join_main, hipEventDisableTiming)) != hipSuccess) return bail_hip(e, "events");
   if (!b->buckets.empty() && (e = b->alloc(&b->d_counters, b->buckets.size())) != hipSuccess)
     return bail_hip(e, "allocating counters");
 
@@ -387,7 +413,15 @@ ticit_hip_status ticit_hip_batch_launch(ticit_hip_batch* b, uint64_t seed_offset
     if ((e = hipGraphLaunch(b->wide_exec, b->side)) != hipSuccess) return from_hip(e, "launching the batch graph");
     if ((e = hipEventRecord(b->join, b->side)) != hipSuccess) return from_hip(e, "joining the batch");
   }
-  if ((e = hipGraphLaunch(b->exec, s)) != hipSuccess) return from_hip(e, "launching the batch graph");
+  if (b->split) {
+    // the group and lane graph runs on its CU-masked stream beside the wide one
+    if ((e = hipStreamWaitEvent(b->main, b->fork, 0)) != hipSuccess) return from_hip(e, "forking the batch");
+    if ((e = hipGraphLaunch(b->exec, b->main)) != hipSuccess) return from_hip(e, "launching the batch graph");
+    if ((e = hipEventRecord(b->join_main, b->main)) != hipSuccess) return from_hip(e, "joining the batch");
+    if ((e = hipStreamWaitEvent(s, b->join_main, 0)) != hipSuccess) return from_hip(e, "joining the batch");
+  } else if ((e = hipGraphLaunch(b->exec, s)) != hipSuccess) {
+    return from_hip(e, "launching the batch graph");
+  }
   if (b->wide_exec && (e = hipStreamWaitEvent(s, b->join, 0)) != hipSuccess) return from_hip(e, "joining the batch");
   b->last = s;
   return TICIT_HIP_OK;

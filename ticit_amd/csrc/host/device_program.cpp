@@ -232,8 +232,15 @@ HostTables lower_to_device(const Program& p, bool components) {
   };
   // src/sampler/batch/mod.rs:597-680: every source of a step folded into its
   // component, each fold its own device instruction
+  // Traffic of a merge chain as the wide tier runs it: the factors are read
+  // (once per pass that uses them) and the merged state is written only when
+  // the next operation is not a measurement of it.
+  double chain_reads = 0.0;
+  bool chain = false;
   auto emit_merges = [&](const ComponentStep& st) {
     if (width[size_t(st.comp)] == 0) throw TicitError(4, "active component merge target is inactive");
+    chain = !st.sources.empty();
+    chain_reads = chain ? std::ldexp(1.0, width[size_t(st.comp)]) : 0.0;
     for (int32_t src : st.sources) {
       if (width[size_t(src)] == 0) throw TicitError(4, "active component merge source is inactive");
       DevInstr m = blank(kOpMerge);
@@ -245,7 +252,7 @@ HostTables lower_to_device(const Program& p, bool components) {
       m.src_k = width[size_t(src)];
       if (m.nq + m.src_k > plan.comp_max_k[size_t(st.comp)])
         throw TicitError(4, "active component merge target has insufficient storage");
-      t.amp_traffic += std::ldexp(1.0, m.nq) + std::ldexp(1.0, m.src_k) + std::ldexp(1.0, m.nq + m.src_k);
+      chain_reads += std::ldexp(1.0, m.src_k);
       width[size_t(st.comp)] += m.src_k;
       width[size_t(src)] = 0;
       t.instrs.push_back(m);
@@ -286,6 +293,7 @@ HostTables lower_to_device(const Program& p, bool components) {
           d.nq = st->local.nq;
           d.comp = st->comp;
           d.comp_off = t.comp_off[size_t(st->comp)];
+          if (chain) t.amp_traffic += chain_reads + std::ldexp(1.0, d.nq);  // one pass writes the product
         } else {
           fill_rotation(d, in);
         }
@@ -348,7 +356,14 @@ HostTables lower_to_device(const Program& p, bool components) {
         } else {
           fill_measure(d, in);
         }
-        t.amp_traffic += std::ldexp(2.5, d.nq);
+        if (st && chain) {
+          // measured straight from the factors: they are read by the
+          // probability and the projection passes, only the projected half
+          // is written
+          t.amp_traffic += 2.0 * chain_reads + std::ldexp(0.5, d.nq);
+        } else {
+          t.amp_traffic += std::ldexp((d.op & kFlagDiagonal) ? 1.5 : 2.5, d.nq);
+        }
         d.ordinal = t.nmeasure++;
         d.branch_slot = assign(in.branch);
         --k;

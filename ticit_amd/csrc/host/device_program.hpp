@@ -29,7 +29,9 @@ struct HostTables {
   double dense_work = 0.0;  // sum over amplitude instructions of 2^k
   // amplitudes moved per shot when the state streams from memory: a rotation
   // reads and writes 2^k, a promotion reads 2^k and writes 2^(k+1), a
-  // measurement reads 2^k twice (probability, projection) and writes 2^(k-1)
+  // nondiagonal measurement reads 2^k twice (probability, projection) and
+  // writes 2^(k-1); a diagonal one reads only one half twice and writes it
+  // (3 * 2^(k-1)); a merge reads both factors and writes their product
   double amp_traffic = 0.0;
   // execution layout: one component spanning the state (dense) or the
   // planner's tensor components

@@ -33,6 +33,34 @@ struct WideCtl {
 constexpr int kWideThreads = 256;
 constexpr int kWidePer = 8;                              // amplitudes per thread
 constexpr int kWideTile = kWideThreads * kWidePer;       // amplitudes per block
+constexpr int kWidePairs = kWidePer / 2;                 // adjacent pairs per thread
+
+// Two adjacent amplitudes in one access (16 bytes FP64, 8 bytes FP32).
+template <class T>
+struct Pair2;
+template <>
+struct Pair2<double> {
+  using V = double2;
+};
+template <>
+struct Pair2<float> {
+  using V = float2;
+};
+template <class T>
+__device__ __forceinline__ typename Pair2<T>::V ld2(const T* p) {
+  return *reinterpret_cast<const typename Pair2<T>::V*>(p);
+}
+template <class T>
+__device__ __forceinline__ void st2(T* p, T a, T b) {
+  typename Pair2<T>::V v;
+  v.x = a;
+  v.y = b;
+  *reinterpret_cast<typename Pair2<T>::V*>(p) = v;
+}
+// pair p of a (tile, shot) block: elements 2p, 2p + 1
+__device__ __forceinline__ size_t pair_index(int u) {
+  return size_t(blockIdx.x) * (kWideTile / 2) + size_t(u) * kWideThreads + threadIdx.x;
+}
 
 __device__ __forceinline__ bool wbit(const uint64_t* t, uint32_t i) { return (t[i >> 6] >> (i & 63)) & 1; }
 __device__ __forceinline__ void wset(uint64_t* t, uint32_t i, bool v) {
@@ -157,6 +185,21 @@ __global__ __launch_bounds__(kWideThreads) void wide_rotate_kernel(DevProgram P,
   const T c = T(in.c);
   if (in.op & kFlagDiagonal) {
     const T mr = T(in.me_re), mi = T(in.me_im);
+    if (n >= 2 && !(dim & 1)) {
+#pragma unroll
+      for (int u = 0; u < kWidePairs; ++u) {
+        const size_t b = 2 * pair_index(u);
+        if (b >= n) break;
+        const bool f0 = sign != parity64(uint64_t(b) & in.zmask);
+        const bool f1 = f0 != bool(in.zmask & 1);
+        const T fr0 = c + (f0 ? -mr : mr), fi0 = f0 ? -mi : mi;
+        const T fr1 = c + (f1 ? -mr : mr), fi1 = f1 ? -mi : mi;
+        const auto r = ld2(R + b), m = ld2(I + b);
+        st2(R + b, fr0 * r.x - fi0 * m.x, fr1 * r.y - fi1 * m.y);
+        st2(I + b, fr0 * m.x + fi0 * r.x, fr1 * m.y + fi1 * r.y);
+      }
+      return;
+    }
 #pragma unroll
     for (int q = 0; q < kWidePer; ++q) {
       size_t b = size_t(blockIdx.x) * kWideTile + size_t(q) * kWideThreads + threadIdx.x;
@@ -173,6 +216,54 @@ __global__ __launch_bounds__(kWideThreads) void wide_rotate_kernel(DevProgram P,
   const bool xz_odd = in.op & kFlagXzOdd;
   const T q = T(sign ? -in.me_im : in.me_im);
   const T mr = T(in.me_re), mi = T(in.me_im);
+  if (in.pivot >= 1 && !(dim & 1)) {
+    // pairs (idx, idx + 1) have adjacent left sources; the right sources are
+    // the aligned pair at i1 & ~1, swapped when xmask has bit 0
+    const bool swap = in.xmask & 1;
+#pragma unroll
+    for (int u = 0; u < kWidePairs; ++u) {
+      const size_t idx = 2 * pair_index(u);
+      if (idx >= (n >> 1)) break;
+      const size_t i0 = insert_zero_bit(idx, in.pivot), i1 = (i0 ^ size_t(in.xmask)) & ~size_t(1);
+      const auto a0 = ld2(R + i0), b0 = ld2(I + i0);
+      auto a1 = ld2(R + i1), b1 = ld2(I + i1);
+      if (swap) {
+        T t = a1.x; a1.x = a1.y; a1.y = t;
+        t = b1.x; b1.x = b1.y; b1.y = t;
+      }
+      T o0r[2], o0i[2], o1r[2], o1i[2];
+      const T r0s[2] = {a0.x, a0.y}, m0s[2] = {b0.x, b0.y}, r1s[2] = {a1.x, a1.y}, m1s[2] = {b1.x, b1.y};
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const T r0 = r0s[e], m0 = m0s[e], r1 = r1s[e], m1 = m1s[e];
+        if (uni) {
+          o0r[e] = c * r0 - q * m1;
+          o0i[e] = c * m0 + q * r1;
+          o1r[e] = c * r1 - q * m0;
+          o1i[e] = c * m1 + q * r0;
+        } else {
+          const bool lodd = parity64(uint64_t(i0 + size_t(e)) & in.zmask);
+          const T ld = (sign != lodd) ? T(-1) : T(1);
+          const T rd = xz_odd ? -ld : ld;
+          const T lre = ld * mr, lim = ld * mi, rre = rd * mr, rim = rd * mi;
+          o0r[e] = c * r0 + rre * r1 - rim * m1;
+          o0i[e] = c * m0 + rre * m1 + rim * r1;
+          o1r[e] = c * r1 + lre * r0 - lim * m0;
+          o1i[e] = c * m1 + lre * m0 + lim * r0;
+        }
+      }
+      st2(R + i0, o0r[0], o0r[1]);
+      st2(I + i0, o0i[0], o0i[1]);
+      if (swap) {
+        st2(R + i1, o1r[1], o1r[0]);
+        st2(I + i1, o1i[1], o1i[0]);
+      } else {
+        st2(R + i1, o1r[0], o1r[1]);
+        st2(I + i1, o1i[0], o1i[1]);
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int u = 0; u < kWidePer; ++u) {
     size_t idx = size_t(blockIdx.x) * kWideTile + size_t(u) * kWideThreads + threadIdx.x;
@@ -208,6 +299,19 @@ __global__ __launch_bounds__(kWideThreads) void wide_promote_kernel(DevProgram P
   T* I = im + size_t(j) * dim + in.comp_off;
   const size_t n = size_t(1) << k;
   const T c = T(in.c), q = T(sign ? in.s : -in.s);
+  if (n >= 2 && !(dim & 1)) {
+#pragma unroll
+    for (int u = 0; u < kWidePairs; ++u) {
+      const size_t b = 2 * pair_index(u);
+      if (b >= n) break;
+      const auto r = ld2(R + b), m = ld2(I + b);
+      st2(R + b, c * r.x, c * r.y);
+      st2(I + b, c * m.x, c * m.y);
+      st2(R + n + b, -q * m.x, -q * m.y);
+      st2(I + n + b, q * r.x, q * r.y);
+    }
+    return;
+  }
 #pragma unroll
   for (int u = 0; u < kWidePer; ++u) {
     size_t b = size_t(blockIdx.x) * kWideTile + size_t(u) * kWideThreads + threadIdx.x;
@@ -254,6 +358,19 @@ __global__ __launch_bounds__(kWideThreads) void wide_merge_kernel(DevProgram P, 
   const T* SI = sim + size_t(j) * dim + in.src_off;
   T* OR = ore + size_t(j) * dim + in.comp_off;
   T* OI = oim + size_t(j) * dim + in.comp_off;
+  if (!(dim & 1)) {  // td >= 2: outputs (o, o + 1) share the source entry
+#pragma unroll
+    for (int u = 0; u < kWidePairs; ++u) {
+      const size_t o = 2 * pair_index(u);
+      if (o >= total) break;
+      const size_t tb = o & (td - 1), sb = o >> tk;
+      const auto a = ld2(TR + tb), b = ld2(TI + tb);
+      const T cr = SR[sb], ci = SI[sb];
+      st2(OR + o, a.x * cr - b.x * ci, a.y * cr - b.y * ci);
+      st2(OI + o, a.x * ci + b.x * cr, a.y * ci + b.y * cr);
+    }
+    return;
+  }
 #pragma unroll
   for (int u = 0; u < kWidePer; ++u) {
     size_t o = size_t(blockIdx.x) * kWideTile + size_t(u) * kWideThreads + threadIdx.x;
@@ -274,6 +391,15 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return t;
 }
 
+// Whether a measurement's outputs (idx, idx + 1), idx even, read adjacent,
+// aligned source pairs: the pivot is not bit 0 and, for a diagonal body,
+// the parity mask does not include bit 0 (so both outputs pick the same
+// half); the state dimension is even so pairs stay aligned.
+__device__ __forceinline__ bool pairs_ok(const DevInstr& in, size_t out_dim, size_t dim) {
+  if (out_dim < 2 || (dim & 1) || in.pivot < 1) return false;
+  return !(in.op & kFlagDiagonal) || !(in.zwop & 1);
+}
+
 // FP64 partial sums of P(branch = 1) per (tile, shot).
 template <class T>
 __global__ __launch_bounds__(kWideThreads) void wide_prob_kernel(DevProgram P, int instr, const T* re,
@@ -287,6 +413,37 @@ __global__ __launch_bounds__(kWideThreads) void wide_prob_kernel(DevProgram P, i
   const size_t out_dim = (size_t(1) << k) >> 1;
   const bool diag = in.op & kFlagDiagonal;
   double acc = 0.0;
+  if (pairs_ok(in, out_dim, dim)) {
+#pragma unroll
+    for (int u = 0; u < kWidePairs; ++u) {
+      const size_t idx = 2 * pair_index(u);
+      if (idx >= out_dim) break;
+      if (diag) {
+        const size_t s0 = diagonal_source(in, idx, true);  // s0 + 1 is the source of idx + 1
+        const auto r = ld2(R + s0), m = ld2(I + s0);
+        acc += double(r.x) * double(r.x) + double(m.x) * double(m.x);
+        acc += double(r.y) * double(r.y) + double(m.y) * double(m.y);
+      } else {
+        const size_t s0 = insert_zero_bit(idx, in.pivot), s1 = (s0 ^ size_t(in.xmask)) & ~size_t(1);
+        const auto a0 = ld2(R + s0), b0 = ld2(I + s0), a1 = ld2(R + s1), b1 = ld2(I + s1);
+        const bool swap = in.xmask & 1;
+        const double r0[2] = {double(a0.x), double(a0.y)}, m0[2] = {double(b0.x), double(b0.y)};
+        const double r1[2] = {double(swap ? a1.y : a1.x), double(swap ? a1.x : a1.y)};
+        const double m1[2] = {double(swap ? b1.y : b1.x), double(swap ? b1.x : b1.y)};
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const double dir = parity64(uint64_t(s0 + size_t(e)) & in.zmask) ? 1.0 : -1.0;
+          const double c1r = dir * in.c1_re, c1i = dir * in.c1_im;
+          const double ar = kInvSqrt2 * r0[e] + c1r * r1[e] - c1i * m1[e];
+          const double ai = kInvSqrt2 * m0[e] + c1r * m1[e] + c1i * r1[e];
+          acc += ar * ar + ai * ai;
+        }
+      }
+    }
+    const double t = block_sum(acc, red);
+    if (threadIdx.x == 0) part[size_t(j) * max_tiles + blockIdx.x] = t;
+    return;
+  }
 #pragma unroll
   for (int u = 0; u < kWidePer; ++u) {
     size_t idx = size_t(blockIdx.x) * kWideTile + size_t(u) * kWideThreads + threadIdx.x;
@@ -363,6 +520,40 @@ __global__ __launch_bounds__(kWideThreads) void wide_project_kernel(DevProgram P
   T* OI = oim + size_t(j) * dim + in.comp_off;
   const size_t out_dim = (size_t(1) << k) >> 1;
   const bool diag = in.op & kFlagDiagonal;
+  if (pairs_ok(in, out_dim, dim)) {
+#pragma unroll
+    for (int u = 0; u < kWidePairs; ++u) {
+      const size_t idx = 2 * pair_index(u);
+      if (idx >= out_dim) break;
+      if (diag) {
+        const size_t s0 = diagonal_source(in, idx, branch);  // s0 + 1 feeds idx + 1
+        const auto r = ld2(R + s0), m = ld2(I + s0);
+        st2(OR + idx, T(double(r.x) * inv), T(double(r.y) * inv));
+        st2(OI + idx, T(double(m.x) * inv), T(double(m.y) * inv));
+      } else {
+        const size_t s0 = insert_zero_bit(idx, in.pivot), s1 = (s0 ^ size_t(in.xmask)) & ~size_t(1);
+        const auto a0 = ld2(R + s0), b0 = ld2(I + s0), a1 = ld2(R + s1), b1 = ld2(I + s1);
+        const bool swap = in.xmask & 1;
+        const double r0[2] = {double(a0.x), double(a0.y)}, m0[2] = {double(b0.x), double(b0.y)};
+        const double r1[2] = {double(swap ? a1.y : a1.x), double(swap ? a1.x : a1.y)};
+        const double m1[2] = {double(swap ? b1.y : b1.x), double(swap ? b1.x : b1.y)};
+        T outr[2], outi[2];
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const bool odd = parity64(uint64_t(s0 + size_t(e)) & in.zmask);
+          const double dir = (branch != odd) ? -1.0 : 1.0;
+          const double c1r = dir * in.c1_re, c1i = dir * in.c1_im;
+          const double ar = kInvSqrt2 * r0[e] + c1r * r1[e] - c1i * m1[e];
+          const double ai = kInvSqrt2 * m0[e] + c1r * m1[e] + c1i * r1[e];
+          outr[e] = T(ar * inv);
+          outi[e] = T(ai * inv);
+        }
+        st2(OR + idx, outr[0], outr[1]);
+        st2(OI + idx, outi[0], outi[1]);
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int u = 0; u < kWidePer; ++u) {
     size_t idx = size_t(blockIdx.x) * kWideTile + size_t(u) * kWideThreads + threadIdx.x;
@@ -378,6 +569,134 @@ __global__ __launch_bounds__(kWideThreads) void wide_project_kernel(DevProgram P
       double c1r = dir * in.c1_re, c1i = dir * in.c1_im;
       double ar = kInvSqrt2 * double(R[s0]) + c1r * double(R[s1]) - c1i * double(I[s1]);
       double ai = kInvSqrt2 * double(I[s0]) + c1r * double(I[s1]) + c1i * double(R[s1]);
+      OR[idx] = T(ar * inv);
+      OI[idx] = T(ai * inv);
+    }
+  }
+}
+
+// ---- Factored components -------------------------------------------------
+// A component about to be merged from factors f0 (target) and f1..fn-1
+// (sources, in merge order): amplitude x is
+// ((f0[x0] * f1[x1]) * f2[x2]) ..., x_f = (x >> shift_f) & mask_f, each
+// product in the order and formula of a merge, so every value equals the
+// materialized state's bit for bit. A merge chain followed by a
+// measurement of the merged component never writes the merged state: its
+// probability and projection are computed from the factors (which stay in
+// cache) and only the projected half is written.
+constexpr int kMaxFactors = 6;
+
+template <class T>
+struct Factors {
+  int n;
+  const T* re[kMaxFactors];  // shot 0 base + component offset
+  const T* im[kMaxFactors];
+  int shift[kMaxFactors];
+  uint32_t mask[kMaxFactors];
+};
+
+template <class T>
+__device__ __forceinline__ void factor_amp(const Factors<T>& F, size_t shot, size_t x, T& r, T& m) {
+  const size_t x0 = x & F.mask[0];
+  r = F.re[0][shot + x0];
+  m = F.im[0][shot + x0];
+  for (int f = 1; f < F.n; ++f) {
+    const size_t xf = (x >> F.shift[f]) & F.mask[f];
+    const T cr = F.re[f][shot + xf], ci = F.im[f][shot + xf];
+    const T nr = r * cr - m * ci, ni = r * ci + m * cr;
+    r = nr;
+    m = ni;
+  }
+}
+
+// The merged state written out (a chain of merges in one pass).
+template <class T>
+__global__ __launch_bounds__(kWideThreads) void wide_multimerge_kernel(Factors<T> F, T* ore, T* oim, size_t dim,
+                                                                       uint32_t out_off, int width) {
+  const size_t shot = size_t(blockIdx.y) * dim;
+  const size_t total = size_t(1) << width;
+  T* OR = ore + shot + out_off;
+  T* OI = oim + shot + out_off;
+#pragma unroll
+  for (int u = 0; u < kWidePer; ++u) {
+    const size_t o = size_t(blockIdx.x) * kWideTile + size_t(u) * kWideThreads + threadIdx.x;
+    if (o >= total) break;
+    T r, m;
+    factor_amp(F, shot, o, r, m);
+    OR[o] = r;
+    OI[o] = m;
+  }
+}
+
+// P(branch = 1) partial sums from the factors (as wide_prob_kernel).
+template <class T>
+__global__ __launch_bounds__(kWideThreads) void wide_prob_factored_kernel(DevProgram P, int instr, Factors<T> F,
+                                                                          size_t dim, int k, double* part,
+                                                                          uint32_t max_tiles) {
+  __shared__ double red[kWideThreads / 64];
+  const uint32_t j = blockIdx.y;
+  const size_t shot = size_t(j) * dim;
+  const DevInstr in = P.instrs[instr];
+  const size_t out_dim = (size_t(1) << k) >> 1;
+  const bool diag = in.op & kFlagDiagonal;
+  double acc = 0.0;
+#pragma unroll
+  for (int u = 0; u < kWidePer; ++u) {
+    const size_t idx = size_t(blockIdx.x) * kWideTile + size_t(u) * kWideThreads + threadIdx.x;
+    if (idx >= out_dim) break;
+    T r, m;
+    if (diag) {
+      factor_amp(F, shot, diagonal_source(in, idx, true), r, m);
+      acc += double(r) * double(r) + double(m) * double(m);
+    } else {
+      const size_t s0 = insert_zero_bit(idx, in.pivot), s1 = s0 ^ size_t(in.xmask);
+      T r1, m1;
+      factor_amp(F, shot, s0, r, m);
+      factor_amp(F, shot, s1, r1, m1);
+      const double dir = parity64(uint64_t(s0) & in.zmask) ? 1.0 : -1.0;
+      const double c1r = dir * in.c1_re, c1i = dir * in.c1_im;
+      const double ar = kInvSqrt2 * double(r) + c1r * double(r1) - c1i * double(m1);
+      const double ai = kInvSqrt2 * double(m) + c1r * double(m1) + c1i * double(r1);
+      acc += ar * ar + ai * ai;
+    }
+  }
+  const double t = block_sum(acc, red);
+  if (threadIdx.x == 0) part[size_t(j) * max_tiles + blockIdx.x] = t;
+}
+
+// The collapse from the factors into the output buffer (as wide_project_kernel).
+template <class T>
+__global__ __launch_bounds__(kWideThreads) void wide_project_factored_kernel(DevProgram P, int instr,
+                                                                             const WideCtl* ctl, Factors<T> F,
+                                                                             T* ore, T* oim, size_t dim, int k) {
+  const uint32_t j = blockIdx.y;
+  const size_t shot = size_t(j) * dim;
+  const DevInstr in = P.instrs[instr];
+  const bool branch = ctl[j].branch;
+  const double inv = ctl[j].invnorm;
+  T* OR = ore + shot + in.comp_off;
+  T* OI = oim + shot + in.comp_off;
+  const size_t out_dim = (size_t(1) << k) >> 1;
+  const bool diag = in.op & kFlagDiagonal;
+#pragma unroll
+  for (int u = 0; u < kWidePer; ++u) {
+    const size_t idx = size_t(blockIdx.x) * kWideTile + size_t(u) * kWideThreads + threadIdx.x;
+    if (idx >= out_dim) break;
+    T r, m;
+    if (diag) {
+      factor_amp(F, shot, diagonal_source(in, idx, branch), r, m);
+      OR[idx] = T(double(r) * inv);
+      OI[idx] = T(double(m) * inv);
+    } else {
+      const size_t s0 = insert_zero_bit(idx, in.pivot), s1 = s0 ^ size_t(in.xmask);
+      T r1, m1;
+      factor_amp(F, shot, s0, r, m);
+      factor_amp(F, shot, s1, r1, m1);
+      const bool odd = parity64(uint64_t(s0) & in.zmask);
+      const double dir = (branch != odd) ? -1.0 : 1.0;
+      const double c1r = dir * in.c1_re, c1i = dir * in.c1_im;
+      const double ar = kInvSqrt2 * double(r) + c1r * double(r1) - c1i * double(m1);
+      const double ai = kInvSqrt2 * double(m) + c1r * double(m1) + c1i * double(r1);
       OR[idx] = T(ar * inv);
       OI[idx] = T(ai * inv);
     }
@@ -574,13 +893,61 @@ static hipError_t run_wide(const DevProgram& P, const DevInstr* instrs, const De
         }
         WCK(hipGetLastError());
       } else if (op == kOpMerge) {
-        const int sbuf = cur[size_t(in.src_comp)];
-        size_t total = size_t(1) << (w + in.src_k);
-        dim3 g(unsigned((total + kWideTile - 1) / kWideTile), n);
-        hipLaunchKernelGGL(wide_merge_kernel<T>, g, dim3(kWideThreads), 0, stream, P, i, R, I, RE(sbuf), IM(sbuf),
-                           RE(cb ^ 1), IM(cb ^ 1), dim);
+        // the chain of merges into this component, as factors
+        Factors<T> F{};
+        F.n = 1;
+        F.re[0] = R + in.comp_off;
+        F.im[0] = I + in.comp_off;
+        F.shift[0] = 0;
+        F.mask[0] = (uint32_t(1) << w) - 1;
+        int W = w, e2 = i;
+        while (e2 < P.ninstr && F.n < kMaxFactors) {
+          const DevInstr& m = instrs[e2];
+          if ((m.op & 0xff) != kOpMerge || m.comp != in.comp || m.nq != W) break;
+          const int sb2 = cur[size_t(m.src_comp)];
+          F.re[F.n] = RE(sb2) + m.src_off;
+          F.im[F.n] = IM(sb2) + m.src_off;
+          F.shift[F.n] = W;
+          F.mask[F.n] = (uint32_t(1) << m.src_k) - 1;
+          W += m.src_k;
+          ++F.n;
+          ++e2;
+        }
+        const DevInstr* next = e2 < P.ninstr ? &instrs[e2] : nullptr;
+        if (next && (next->op & 0xff) == kOpMeasure && next->comp == in.comp && next->nq == W) {
+          // merge-and-measure: never materialize the merged state
+          size_t out_dim = (size_t(1) << W) >> 1;
+          unsigned tiles = unsigned((out_dim + kWideTile - 1) / kWideTile);
+          dim3 g(tiles, n);
+          hipLaunchKernelGGL(wide_prob_factored_kernel<T>, g, dim3(kWideThreads), 0, stream, P, e2, F, dim, W,
+                             ws.part, Y.max_tiles);
+          WCK(hipGetLastError());
+          hipLaunchKernelGGL(wide_decide_kernel, sg, sb, 0, stream, P, L, Y, start, n, e2, ws.bits, ws.ctl, ws.part,
+                             tiles, Y.max_tiles);
+          WCK(hipGetLastError());
+          hipLaunchKernelGGL(wide_project_factored_kernel<T>, g, dim3(kWideThreads), 0, stream, P, e2, ws.ctl, F,
+                             RE(cb ^ 1), IM(cb ^ 1), dim, W);
+          WCK(hipGetLastError());
+          cur[size_t(in.comp)] ^= 1;
+          i = e2 + 1;
+          continue;
+        }
+        if (F.n == 2) {
+          const int sbuf = cur[size_t(in.src_comp)];
+          size_t total = size_t(1) << W;
+          dim3 g(unsigned((total + kWideTile - 1) / kWideTile), n);
+          hipLaunchKernelGGL(wide_merge_kernel<T>, g, dim3(kWideThreads), 0, stream, P, i, R, I, RE(sbuf), IM(sbuf),
+                             RE(cb ^ 1), IM(cb ^ 1), dim);
+        } else {
+          size_t total = size_t(1) << W;
+          dim3 g(unsigned((total + kWideTile - 1) / kWideTile), n);
+          hipLaunchKernelGGL(wide_multimerge_kernel<T>, g, dim3(kWideThreads), 0, stream, F, RE(cb ^ 1), IM(cb ^ 1),
+                             dim, in.comp_off, W);
+        }
         WCK(hipGetLastError());
         cur[size_t(in.comp)] ^= 1;
+        i = e2;
+        continue;
       } else {  // Measure
         size_t out_dim = (size_t(1) << w) >> 1;
         unsigned tiles = unsigned((out_dim + kWideTile - 1) / kWideTile);
