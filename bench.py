@@ -76,7 +76,7 @@ def main() -> int:
     def step(seed: int) -> None:
         ticit_amd._capi.check(lib.ticit_hip_sample_device(
             sampler.handle, shots, seed, 0, 1, 1, meas.data_ptr(), det.data_ptr(), obs.data_ptr(),
-            ones.data_ptr(), ctypes_stream(stream)))
+            ones.data_ptr(), None, ctypes_stream(stream)))
 
     for i in range(args.warmup):
         step(1000 + i)

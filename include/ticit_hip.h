@@ -69,6 +69,10 @@ typedef struct ticit_hip_result {
   uint64_t* observable_ones;
   double sample_s; /* wall time of the sampling call */
   double kernel_s; /* device time of the sampling kernels */
+  /* SampleResult::exp_vals (src/sampler/prepared.rs:218,1020-1025):
+   * [record_rows][nexpvals] EXP_VAL values, row-major, kept with the rows */
+  uint64_t nexpvals;
+  double* exp_vals;
 } ticit_hip_result;
 
 /* Mirrors SamplerInfo (src/sampler/prepared.rs:161-189). */
@@ -97,6 +101,7 @@ typedef struct ticit_hip_program_info {
   uint32_t lds_bytes_per_block;  /* lane/group tier launch with bit-packed rows (0 wide) */
   uint32_t blocks_per_cu;        /* resident workgroups per CU of that launch */
   uint32_t table_words[4];       /* per-shot bit-table words: partials, slots, records, detectors */
+  uint64_t expectation_values;   /* EXP_VAL slots per shot (src/circuit/mod.rs:122) */
 } ticit_hip_program_info;
 
 /* Thread-local message of the last failing call on this thread. */
@@ -133,11 +138,13 @@ ticit_hip_status ticit_hip_sample(ticit_hip_program* program, uint64_t shots, ui
 /* Same sampling into caller-owned device buffers, enqueued on `stream`
  * (a hipStream_t, NULL = the default stream) and not synchronized:
  * meas/det/obs are [shots][row] bytes, observable_ones is [nobservables]
- * uint64 accumulated (not cleared) by the call. first_shot offsets the
- * per-shot streams so a large job can be split into launches. */
+ * uint64 accumulated (not cleared) by the call, exp_vals (NULL to skip) is
+ * [shots][expectation_values] doubles written with the rows. first_shot
+ * offsets the per-shot streams so a large job can be split into launches. */
 ticit_hip_status ticit_hip_sample_device(ticit_hip_program* program, uint64_t shots, uint64_t seed,
                                          uint64_t first_shot, int bit_packed, int keep_records, uint8_t* meas,
-                                         uint8_t* det, uint8_t* obs, uint64_t* observable_ones, void* stream);
+                                         uint8_t* det, uint8_t* obs, uint64_t* observable_ones, double* exp_vals,
+                                         void* stream);
 
 /* Injected randomness (host buffers): exo = symbol-major bit-planes
  * [nsymbols][ceil(shots/64)] (bit s of word w = shot 64w+s; only exogenous
@@ -179,6 +186,7 @@ typedef struct ticit_hip_batch_item {
   uint8_t* detectors;
   uint8_t* observables;
   uint64_t* observable_ones; /* [nobservables] device */
+  double* exp_vals;          /* [shots][expectation_values] device, NULL to skip */
 } ticit_hip_batch_item;
 
 typedef struct ticit_hip_batch_options {

@@ -35,11 +35,11 @@ def _load(name: str) -> ctypes.CDLL:
     lib.oracle_info.argtypes = [_u64p, ctypes.c_size_t, _i64p]
     lib.oracle_sample.argtypes = [_u64p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_uint64,
                                   ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
-                                  ctypes.c_int, _u8p, _u8p, _u8p, _u64p, _u64p, _f64p]
+                                  ctypes.c_int, _u8p, _u8p, _u8p, _u64p, _u64p, _f64p, _f64p]
     lib.oracle_sample_injected.argtypes = [_u64p, ctypes.c_size_t, ctypes.c_uint64, _u64p, _f64p,
                                            _u64p, ctypes.c_int, ctypes.c_int, _u8p, _u8p, _u8p,
                                            _u64p, _u64p, _f64p, _f64p, _f64p,
-                                           ctypes.POINTER(ctypes.c_int)]
+                                           ctypes.POINTER(ctypes.c_int), _f64p]
     lib.oracle_presample.argtypes = [_u64p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_uint64, _u64p]
     lib.oracle_splitmix_next.argtypes = [_u64p]
     lib.oracle_splitmix_next.restype = ctypes.c_uint64
@@ -86,10 +86,10 @@ def _check(rc: int, fast: bool = False) -> None:
 
 
 def info(blob: np.ndarray) -> dict:
-    out = np.zeros(10, dtype=np.int64)
+    out = np.zeros(11, dtype=np.int64)
     _check(lib().oracle_info(_ptr(blob, _u64p), blob.size, _ptr(out, _i64p)))
     keys = ["n", "initial_k", "max_k", "nsymbols", "nrecords", "ndetectors", "nobservables",
-            "ninstr", "nmeasure", "ncoin"]
+            "ninstr", "nmeasure", "ncoin", "nexpvals"]
     return dict(zip(keys, (int(v) for v in out)))
 
 
@@ -107,12 +107,13 @@ def sample(blob: np.ndarray, shots: int, seed: int, *, chunk_shots: int = 0, bat
     counts = np.zeros(4, dtype=np.uint64)
     ones = np.zeros(max(1, inf["nobservables"]), dtype=np.uint64)
     timing = np.zeros(1, dtype=np.float64)
+    ev = np.zeros((shots, inf["nexpvals"]), dtype=np.float64)
     _check(lib(fast).oracle_sample(_ptr(blob, _u64p), blob.size, shots, seed, chunk_shots, batch_size,
                                    threads, int(bit_packed), observable, _ptr(meas, _u8p), _ptr(det, _u8p),
                                    _ptr(obs, _u8p), _ptr(counts, _u64p), _ptr(ones, _u64p),
-                                   _ptr(timing, _f64p)), fast)
+                                   _ptr(timing, _f64p), _ptr(ev, _f64p)), fast)
     return dict(measurements=meas, detectors=det, observables=obs, counts=counts,
-                observable_ones=ones[:inf["nobservables"]], sample_s=float(timing[0]))
+                observable_ones=ones[:inf["nobservables"]], sample_s=float(timing[0]), exp_vals=ev)
 
 
 def sample_per_shot(blob: np.ndarray, shots: int, seed: int, **kw) -> dict:
@@ -142,13 +143,14 @@ def sample_injected(blob: np.ndarray, shots: int, exo: np.ndarray | None, unifor
     fre = np.zeros(dim) if want_final_state else None
     fim = np.zeros(dim) if want_final_state else None
     fk = ctypes.c_int(0)
+    ev = np.zeros((shots, inf["nexpvals"]), dtype=np.float64)
     _check(lib().oracle_sample_injected(_ptr(blob, _u64p), blob.size, shots, _ptr(exo, _u64p),
                                         _ptr(uniforms, _f64p), _ptr(coins, _u64p), int(bit_packed), observable,
                                         _ptr(meas, _u8p), _ptr(det, _u8p), _ptr(obs, _u8p), _ptr(counts, _u64p),
                                         _ptr(ones, _u64p), _ptr(probs, _f64p), _ptr(fre, _f64p), _ptr(fim, _f64p),
-                                        ctypes.byref(fk)))
+                                        ctypes.byref(fk), _ptr(ev, _f64p)))
     out = dict(measurements=meas, detectors=det, observables=obs, counts=counts,
-               observable_ones=ones[:inf["nobservables"]])
+               observable_ones=ones[:inf["nobservables"]], exp_vals=ev)
     if want_probs:
         out["probs"] = probs
     if want_final_state:

@@ -865,6 +865,7 @@ struct Outputs {
   std::vector<uint8_t> meas, det, obs;
   std::vector<uint64_t> obs_ones;
   uint64_t shots = 0, accepted = 0, logical = 0;
+  std::vector<double> expv;         // [shot][nexpvals] expectation values
   std::vector<double> probs;        // optional per (measure ordinal, global shot)
   std::vector<double> final_re, final_im;  // optional final state of shot 0
   int final_k = 0;
@@ -994,6 +995,13 @@ struct Runner {
     std::vector<char> assigned(size_t(p.nsymbols) + 1, 0);
     std::vector<uint64_t> rec(size_t(p.nrecords) * words, 0), det(size_t(p.ndetectors) * words, 0);
     std::vector<uint64_t> bits(words);
+    // src/sampler/batch/runtime.rs:46-75: expectation slots per shot
+    const size_t nexp = size_t(p.nexpvals);
+    std::vector<double> ev(shots * nexp, 0.0);
+    auto exp_slot = [&](int32_t slot) {
+      if (slot < 0 || size_t(slot) >= nexp) throw OracleError("expectation value index is out of range");
+      return size_t(slot);
+    };
     // src/sampler/batch/runtime.rs:97-118: a block reads its slice of the
     // chunk-wide exogenous rows starting at chunk-local shot `off`.
     const size_t sh = off & 63, w0 = off >> 6;
@@ -1114,6 +1122,12 @@ struct Runner {
         }
         case TICIT_OP_RECORD:
           eval(in.expr);
+          if (in.exp_val >= 0) {
+            // src/sampler/batch/runtime.rs:46-62: a deterministic probe is +-1
+            size_t e = exp_slot(in.exp_val);
+            for (size_t s = 0; s < shots; ++s) ev[s * nexp + e] = bit(bits, s) ? -1.0 : 1.0;
+            break;
+          }
           write_record(in.record, in.record_condition, bits);
           break;
         case TICIT_OP_DETECTOR: {
@@ -1133,6 +1147,20 @@ struct Runner {
         case TICIT_OP_MEASURE: {
           if (k == 0) throw OracleError("cannot measure an active Pauli when k == 0");
           if (in.nq != k) throw OracleError("measurement kernel dimension does not match batch active state");
+          if (in.exp_val >= 0) {
+            // src/sampler/batch/active.rs:818-850: +-(1 - 2 P(true)), no
+            // draw, no collapse (probe programs always run dense)
+            if (comp) throw OracleError("expectation probes need the dense layout");
+            size_t e = exp_slot(in.exp_val);
+            eval(in.expr);
+            for (size_t s = 0; s < shots; ++s) {
+              double pt = clamp01(prob_true(&re[s * stride], &im[s * stride], meas[i]));
+              if (probs_out) probs_out[measure_ordinal[i] * src.shots_total + src.first_shot + s] = pt;
+              double sign = bit(bits, s) ? -1.0 : 1.0;
+              ev[s * nexp + e] = sign * (1.0 - 2.0 * pt);
+            }
+            break;
+          }
           size_t coff = 0;
           if (comp) {
             const CompStep& st = cplan.steps[i];
@@ -1174,6 +1202,12 @@ struct Runner {
           break;
         }
         case TICIT_OP_COIN: {
+          if (in.exp_val >= 0) {
+            // src/sampler/batch/runtime.rs:353-356: an unbiased probe is 0
+            size_t e = exp_slot(in.exp_val);
+            for (size_t s = 0; s < shots; ++s) ev[s * nexp + e] = 0.0;
+            break;
+          }
           std::vector<uint64_t> br(words, 0);
           if (src.injected) {
             size_t wt = (src.shots_total + 63) / 64;
@@ -1219,6 +1253,7 @@ struct Runner {
     append_rows(out.meas, rec, size_t(p.nrecords), words, shots);
     append_rows(out.det, det, size_t(p.ndetectors), words, shots);
     append_rows(out.obs, ow, nobs, words, shots);
+    out.expv.insert(out.expv.end(), ev.begin(), ev.end());
   }
 
   void append_rows(std::vector<uint8_t>& out, const std::vector<uint64_t>& cols, size_t ncols, size_t words,
@@ -1248,7 +1283,9 @@ size_t default_batch(int max_k) {
   return b < 2048 ? b : 2048;
 }
 
-void copy_out(const Outputs& o, uint8_t* meas, uint8_t* det, uint8_t* obs, uint64_t* counts, uint64_t* ones) {
+void copy_out(const Outputs& o, uint8_t* meas, uint8_t* det, uint8_t* obs, uint64_t* counts, uint64_t* ones,
+              double* exp_vals) {
+  if (exp_vals && !o.expv.empty()) std::memcpy(exp_vals, o.expv.data(), o.expv.size() * sizeof(double));
   if (meas && !o.meas.empty()) std::memcpy(meas, o.meas.data(), o.meas.size());
   if (det && !o.det.empty()) std::memcpy(det, o.det.data(), o.det.size());
   if (obs && !o.obs.empty()) std::memcpy(obs, o.obs.data(), o.obs.size());
@@ -1313,8 +1350,8 @@ int oracle_component_plan(const uint64_t* blob, size_t nwords, int64_t* summary,
   }
 }
 
-// info[0..9] = n, initial_k, max_k, nsymbols, nrecords, ndetectors,
-// nobservables, ninstr, nmeasure, ncoin
+// info[0..10] = n, initial_k, max_k, nsymbols, nrecords, ndetectors,
+// nobservables, ninstr, nmeasure, ncoin, nexpvals
 int oracle_info(const uint64_t* blob, size_t nwords, int64_t* info) {
   try {
     Program p = decode(blob, nwords);
@@ -1323,8 +1360,8 @@ int oracle_info(const uint64_t* blob, size_t nwords, int64_t* info) {
       nm += in.op == TICIT_OP_MEASURE;
       nc += in.op == TICIT_OP_COIN;
     }
-    int64_t v[10] = {p.n, p.initial_k, p.max_k, p.nsymbols, p.nrecords, p.ndetectors,
-                     int64_t(p.observables.size()), int64_t(p.instrs.size()), nm, nc};
+    int64_t v[11] = {p.n, p.initial_k, p.max_k, p.nsymbols, p.nrecords, p.ndetectors,
+                     int64_t(p.observables.size()), int64_t(p.instrs.size()), nm, nc, p.nexpvals};
     std::memcpy(info, v, sizeof v);
     return 0;
   } catch (const std::exception& e) {
@@ -1344,10 +1381,11 @@ int64_t oracle_default_batch(int max_k) {
 // Reference-stream sampling: `chunk_shots`/`batch_size` of 0 select the
 // reference defaults. Rows are concatenated in worker order, exactly as
 // src/sampler/prepared.rs:834-852 does. timing[0] = wall seconds of the
-// sampling call.
+// sampling call; exp_vals (optional) receives the expectation values
+// [shots][nexpvals] in the same row order (src/sampler/prepared.rs:1020-1025).
 int oracle_sample(const uint64_t* blob, size_t nwords, uint64_t shots, uint64_t seed, uint64_t chunk_shots,
                   uint64_t batch_size, int threads, int bit_packed, int observable, uint8_t* meas, uint8_t* det,
-                  uint8_t* obs, uint64_t* counts, uint64_t* observable_ones, double* timing) {
+                  uint8_t* obs, uint64_t* counts, uint64_t* observable_ones, double* timing, double* exp_vals) {
   try {
     Program p = decode(blob, nwords);
     size_t batch = batch_size ? size_t(batch_size) : default_batch(p.max_k);
@@ -1399,12 +1437,13 @@ int oracle_sample(const uint64_t* blob, size_t nwords, uint64_t shots, uint64_t 
       all.meas.insert(all.meas.end(), o.meas.begin(), o.meas.end());
       all.det.insert(all.det.end(), o.det.begin(), o.det.end());
       all.obs.insert(all.obs.end(), o.obs.begin(), o.obs.end());
+      all.expv.insert(all.expv.end(), o.expv.begin(), o.expv.end());
       all.shots += o.shots;
       all.accepted += o.accepted;
       all.logical += o.logical;
       for (size_t i = 0; i < o.obs_ones.size(); ++i) all.obs_ones[i] += o.obs_ones[i];
     }
-    copy_out(all, meas, det, obs, counts, observable_ones);
+    copy_out(all, meas, det, obs, counts, observable_ones, exp_vals);
     if (timing) timing[0] = wall;
     return 0;
   } catch (const std::exception& e) {
@@ -1417,11 +1456,12 @@ int oracle_sample(const uint64_t* blob, size_t nwords, uint64_t shots, uint64_t 
 // (only exogenous symbols are read), uniforms = [nmeasure][shots], coins =
 // [ncoin][ceil(shots/64)]. probs (optional) receives P(branch = 1) per
 // (measure ordinal, shot); final_re/final_im (optional, length 2^max_k)
-// receive shot 0's final active state and final_k its width.
+// receive shot 0's final active state and final_k its width; exp_vals
+// (optional) receives [shots][nexpvals].
 int oracle_sample_injected(const uint64_t* blob, size_t nwords, uint64_t shots, const uint64_t* exo_planes,
                            const double* uniforms, const uint64_t* coins, int bit_packed, int observable,
                            uint8_t* meas, uint8_t* det, uint8_t* obs, uint64_t* counts, uint64_t* observable_ones,
-                           double* probs, double* final_re, double* final_im, int* final_k) {
+                           double* probs, double* final_re, double* final_im, int* final_k, double* exp_vals) {
   try {
     Program p = decode(blob, nwords);
     Runner run(p, bit_packed != 0, observable, g_component_mode);
@@ -1443,7 +1483,7 @@ int oracle_sample_injected(const uint64_t* blob, size_t nwords, uint64_t shots, 
       src.first_shot = s;
       run.run_block(exo, s, 1, src, o, probs, s == 0 && final_re != nullptr);
     }
-    copy_out(o, meas, det, obs, counts, observable_ones);
+    copy_out(o, meas, det, obs, counts, observable_ones, exp_vals);
     if (final_re && shots > 0) {
       std::memcpy(final_re, o.final_re.data(), o.final_re.size() * 8);
       std::memcpy(final_im, o.final_im.data(), o.final_im.size() * 8);

@@ -94,3 +94,82 @@ def record_values(rows: np.ndarray, nrec: int, packed: bool = True) -> np.ndarra
 
 def tvd(counts: np.ndarray, probs: np.ndarray) -> float:
     return 0.5 * float(np.abs(counts / counts.sum() - probs).sum())
+
+
+_PAULI = {"X": GATES1["X"], "Y": GATES1["Y"], "Z": GATES1["Z"]}
+
+
+def _apply_product(psi: np.ndarray, n: int, product: str) -> tuple[np.ndarray, float]:
+    """P|psi> for a product such as ``X0*!Z1``; returns (P psi, sign) where
+    each ``!`` flips the sign."""
+    sign = 1.0
+    out = psi
+    for f in product.split("*"):
+        while f.startswith("!"):
+            sign = -sign
+            f = f[1:]
+        out = apply1(out, n, int(f[1:]), _PAULI[f[0].upper()])
+    return out, sign
+
+
+def expectation_values(text: str) -> np.ndarray:
+    """Exact <psi|P|psi> of every EXP_VAL product, in slot order, for a
+    noiseless circuit whose probes all come before its measurements."""
+    lines = [l.split("#")[0].strip() for l in text.splitlines()]
+    lines = [l for l in lines if l and l != "TICK"]
+    n = 1
+    for l in lines:
+        for tok in l.split()[1:]:
+            for f in tok.split("*"):
+                f = f.lstrip("!")
+                if f[:1].isalpha():
+                    f = f[1:]
+                if f.isdigit():
+                    n = max(n, int(f) + 1)
+    psi = np.zeros(2 ** n, dtype=complex)
+    psi[0] = 1.0
+    vals = []
+    measured = False
+    for l in lines:
+        parts = l.split()
+        op = parts[0]
+        if op == "EXP_VAL":
+            if measured:
+                raise ValueError("probes after a measurement depend on the branch")
+            for prod in parts[1:]:
+                q, sign = _apply_product(psi, n, prod)
+                vals.append(sign * float(np.vdot(psi, q).real))
+            continue
+        if op == "M":
+            measured = True
+            continue
+        targets = [int(t) for t in parts[1:]]
+        if op in GATES1:
+            for q in targets:
+                psi = apply1(psi, n, q, GATES1[op])
+        elif op in ("CX", "CNOT"):
+            for i in range(0, len(targets), 2):
+                psi = apply_cx(psi, n, targets[i], targets[i + 1])
+        elif op == "CZ":
+            for i in range(0, len(targets), 2):
+                psi = apply_cz(psi, n, targets[i], targets[i + 1])
+        else:
+            raise ValueError(f"unsupported gate {op}")
+    return np.array(vals)
+
+
+def with_probes(text: str, rng: np.random.Generator, nprobes: int) -> str:
+    """Inserts EXP_VAL lines with random Pauli products (random inversions)
+    before random gate lines of a circuit whose last line is its only M."""
+    lines = [l for l in text.splitlines() if l.strip()]
+    n = 1 + max(int(t) for l in lines for t in l.split()[1:] if t.isdigit())
+    body, last = lines[:-1], lines[-1]
+    for _ in range(nprobes):
+        at = int(rng.integers(0, len(body) + 1))
+        prods = []
+        for _ in range(int(rng.integers(1, 4))):
+            qs = rng.choice(n, size=int(rng.integers(1, min(n, 3) + 1)), replace=False)
+            f = [("!" if rng.random() < 0.2 else "") + "XYZ"[int(rng.integers(3))] + str(int(q)) for q in qs]
+            prods.append("*".join(f))
+        body.insert(at, "EXP_VAL " + " ".join(prods))
+    return "\n".join(body + [last]) + "\n"

@@ -14,7 +14,7 @@ import pytest
 from oracle import oracle as O
 from ticit_amd import TicitError, plan_text
 from ticit_amd import workloads as w
-from tests.statevector import final_distribution, record_values, tvd
+from tests.statevector import expectation_values, final_distribution, record_values, tvd, with_probes
 
 FIX = os.path.join(os.path.dirname(__file__), "golden", "circuits")
 
@@ -115,3 +115,80 @@ def test_parse_errors_carry_kind():
 def test_repeat_blocks_flatten():
     a = O.info(plan_text("REPEAT 3 {\nH 0\nM 0\n}\n"))
     assert a["nrecords"] == 3
+
+
+def test_exp_val_allocates_one_slot_per_product():
+    # src/circuit/parser.rs:1863-1872: slots 0..2 then 3; probes take no records
+    inf = O.info(plan_text("EXP_VAL Z0*Z1 X0*X1 X2\nEXP_VAL Z2\n"))
+    assert inf["nexpvals"] == 4
+    assert inf["nrecords"] == 0
+    with pytest.raises(TicitError) as e:
+        plan_text("EXP_VAL\n")
+    assert e.value.kind == "Parse" and "at least one Pauli product" in str(e.value)
+
+
+def layered_probe_circuit(n: int, layers: int, rng: np.random.Generator) -> str:
+    """H on every qubit, then layers of T/T_DAG, CX and H with X/Y/Z probes
+    after each: most probes act on the active state with 0 < P < 1."""
+    lines = ["H " + " ".join(map(str, range(n)))]
+    for _ in range(layers):
+        for q in range(n):
+            r = rng.random()
+            if r < 0.4:
+                lines.append(("T " if rng.random() < 0.5 else "T_DAG ") + str(q))
+            elif r < 0.55:
+                lines.append("H " + str(q))
+            elif r < 0.65:
+                lines.append("S " + str(q))
+        perm = rng.permutation(n)
+        for a, b in zip(perm[0::2], perm[1::2]):
+            if rng.random() < 0.5:
+                lines.append(f"CX {int(a)} {int(b)}")
+        prods = []
+        for _ in range(int(rng.integers(2, 5))):
+            size = 1 if rng.random() < 0.6 else min(n, 2)
+            qs = rng.choice(n, size=size, replace=False)
+            prods.append("*".join(("!" if rng.random() < 0.2 else "") + "XYXYZ"[int(rng.integers(5))] + str(int(q))
+                                  for q in qs))
+        lines.append("EXP_VAL " + " ".join(prods))
+    lines.append("M " + " ".join(map(str, range(n))))
+    return "\n".join(lines) + "\n"
+
+
+def probe_circuits():
+    out = []
+    for i in range(6):
+        rng = w.SplitMix64(w.block_seed(0xE7A1, 3, i))
+        n = 2 + i % 4
+        base = w.random_clifford_t(n, 5 + i, 1 + i % 4, rng)
+        out.append(with_probes(base, np.random.default_rng(100 + i), 3 + i % 3))
+    for i in range(6):
+        out.append(layered_probe_circuit(2 + i % 4, 4 + i, np.random.default_rng(200 + i)))
+    return out
+
+
+@pytest.mark.parametrize("text", probe_circuits())
+def test_expectation_probes_match_statevector(text):
+    """Probes before any measurement read <psi|P|psi> of the deterministic
+    state, on every shot; the planner keeps them where they were written
+    (src/pending_optimizer.rs:230-242)."""
+    want = expectation_values(text)
+    blob = plan_text(text)
+    assert O.info(blob)["nexpvals"] == want.size
+    r = O.sample(blob, 64, 5, bit_packed=True)
+    assert r["exp_vals"].shape == (64, want.size)
+    if text.startswith("H 0 1"):  # the layered family: fractional probes exercise 0 < P < 1
+        assert np.sum(np.abs(np.abs(want) - np.round(np.abs(want))) > 1e-3) >= 2
+    assert np.abs(r["exp_vals"] - want[None, :]).max() <= 1e-12
+
+
+def test_probes_do_not_change_the_sampled_distribution():
+    """A probe is non-destructive: the final record distribution is the
+    probe-free circuit's exact one."""
+    text = small_circuits()[3]
+    qubits, probs = final_distribution(text)
+    probed = with_probes(text, np.random.default_rng(9), 4)
+    r = O.sample(plan_text(probed), 40000, 77)
+    vals = record_values(r["measurements"], len(qubits))
+    counts = np.bincount(vals, minlength=probs.size).astype(float)
+    assert tvd(counts, probs) < 0.02

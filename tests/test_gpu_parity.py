@@ -154,7 +154,7 @@ def test_device_entry_point_split_launches(hip):
     for first, n in ((0, half), (half, shots - half)):
         hip._capi.check(lib.ticit_hip_sample_device(
             s.handle, n, 4, first, 1, 1, meas[first:].data_ptr(), det[first:].data_ptr(),
-            obs[first:].data_ptr(), ones.data_ptr(), None))
+            obs[first:].data_ptr(), ones.data_ptr(), None, None))
     torch.cuda.synchronize()
     ref = s.sample(shots, seed=4, bit_packed=True)
     assert np.array_equal(meas.cpu().numpy(), ref.measurements)

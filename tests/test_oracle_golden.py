@@ -77,19 +77,33 @@ def test_default_batch_count():
 
 
 def test_exact_rows_of_deterministic_circuit():
-    # src/sampler/prepared.rs:1207-1245 (the trailing EXP_VAL probe is
-    # non-destructive and left out: this backend plans no expectation probes)
+    # src/sampler/prepared.rs:1207-1245
     blob = plan_text("M !0\nM 1\nDETECTOR rec[-2] rec[-1]\n"
-                     "OBSERVABLE_INCLUDE(0) rec[-2]\nOBSERVABLE_INCLUDE(2) rec[-1]\n")
+                     "OBSERVABLE_INCLUDE(0) rec[-2]\nOBSERVABLE_INCLUDE(2) rec[-1]\n"
+                     "EXP_VAL Z0\n")
     r = O.sample(blob, 3, 7, bit_packed=False)
     assert r["measurements"].reshape(-1).tolist() == [1, 0, 1, 0, 1, 0]
     assert r["detectors"].reshape(-1).tolist() == [1, 1, 1]
     assert r["observables"].reshape(-1).tolist() == [1, 0, 0, 1, 0, 0, 1, 0, 0]
     assert r["observable_ones"].tolist() == [3, 0, 0]
+    assert r["exp_vals"].reshape(-1).tolist() == [1.0] * 3
     p = O.sample(blob, 3, 7, bit_packed=True)
     assert p["measurements"].reshape(-1).tolist() == [1, 1, 1]
     assert p["detectors"].reshape(-1).tolist() == [1, 1, 1]
     assert p["observables"].reshape(-1).tolist() == [1, 1, 1]
+    assert p["exp_vals"].tolist() == r["exp_vals"].tolist()
+
+
+def test_expectation_probes_exact_and_non_destructive():
+    # src/sampler/batch/mod.rs:745-766: Z0 of |0> is 1, X0 is 0 (a dormant
+    # probe); after H T the X, Y, Z probes read 1/sqrt2, 1/sqrt2, 0; the final
+    # measurement still reads 0 (T_DAG H undo the state)
+    blob = plan_text("EXP_VAL Z0 X0\nH 0\nT 0\nEXP_VAL X0 Y0 Z0\nT_DAG 0\nH 0\nM 0\n")
+    r = O.sample(blob, 5, 71, chunk_shots=0, batch_size=0, bit_packed=False)
+    want = [1.0, 0.0, 2 ** -0.5, 2 ** -0.5, 0.0]
+    assert r["exp_vals"].shape == (5, 5)
+    assert np.abs(r["exp_vals"] - np.array(want)[None, :]).max() <= 1e-12
+    assert r["measurements"].reshape(-1).tolist() == [0] * 5
 
 
 def test_packed_rows_little_bit_order():

@@ -56,6 +56,8 @@ class SampleResult:
     sample_s: float
     kernel_s: float
     probabilities: np.ndarray | None = None
+    # [rows][num_expectation_values] EXP_VAL values (SampleResult::exp_vals)
+    exp_vals: np.ndarray | None = None
 
     # ticit_py aliases
     @property
@@ -91,13 +93,17 @@ def _convert(res: _capi.Result, keep_records: bool) -> SampleResult:
     nobs = int(res.nobservables)
     ones = (np.ctypeslib.as_array(res.observable_ones, shape=(nobs,)).copy()
             if nobs and res.observable_ones else np.zeros(0, dtype=np.uint64))
+    nexp = int(res.nexpvals)
+    ev = (np.ctypeslib.as_array(res.exp_vals, shape=(rows * nexp,)).copy().reshape(rows, nexp)
+          if rows and nexp and res.exp_vals else np.zeros((rows, nexp), dtype=np.float64))
     return SampleResult(
         shots=int(res.shots), discarded=int(res.discarded), accepted=int(res.accepted),
         logical_errors=int(res.logical_errors), bit_packed=packed,
         measurements=_rows(res.measurements, rows, cols(int(res.nmeasurements))),
         detectors=_rows(res.detectors, rows, cols(int(res.ndetectors))),
         observables=_rows(res.observables, rows, cols(nobs)),
-        observable_ones=ones.astype(np.uint64), sample_s=float(res.sample_s), kernel_s=float(res.kernel_s))
+        observable_ones=ones.astype(np.uint64), sample_s=float(res.sample_s), kernel_s=float(res.kernel_s),
+        exp_vals=ev)
 
 
 class CompiledSampler:
@@ -153,6 +159,7 @@ class CompiledSampler:
         self.lds_bytes_per_block = int(info.lds_bytes_per_block)
         self.blocks_per_cu = int(info.blocks_per_cu)
         self.table_words = [int(v) for v in info.table_words]
+        self.num_expectation_values = int(info.expectation_values)
         self.has_postselection = False
 
     def close(self) -> None:
@@ -188,12 +195,15 @@ class CompiledSampler:
     def sample_device(self, shots: int, seed: int, outputs, first_shot: int = 0, bit_packed: bool = True,
                       keep_records: bool = True, stream: int = 0) -> None:
         """Sampling into caller device buffers (ticit_hip_sample_device):
-        ``outputs`` = (measurements, detectors, observables, observable_ones)
-        objects with ``data_ptr()``; enqueued on ``stream``, not synchronized."""
+        ``outputs`` = (measurements, detectors, observables, observable_ones
+        [, exp_vals]) objects with ``data_ptr()``; enqueued on ``stream``,
+        not synchronized."""
         ptr = lambda t: ctypes.c_void_p(int(t.data_ptr())) if t is not None else None
+        ev = outputs[4] if len(outputs) > 4 else None
         _capi.check(self._lib.ticit_hip_sample_device(self._handle, shots, seed, first_shot, int(bit_packed),
                                                       int(keep_records), ptr(outputs[0]), ptr(outputs[1]),
-                                                      ptr(outputs[2]), ptr(outputs[3]), ctypes.c_void_p(stream or None)))
+                                                      ptr(outputs[2]), ptr(outputs[3]), ptr(ev),
+                                                      ctypes.c_void_p(stream or None)))
 
     def sample_injected(self, shots: int, exo: np.ndarray | None = None, uniforms: np.ndarray | None = None,
                         coins: np.ndarray | None = None, bit_packed: bool = True,
@@ -329,6 +339,7 @@ class Batch:
             items[i].detectors = ptr(out[1])
             items[i].observables = ptr(out[2])
             items[i].observable_ones = ptr(out[3])
+            items[i].exp_vals = ptr(out[4]) if len(out) > 4 else None
         opt = _capi.BatchOptions()
         opt.bit_packed = int(bit_packed)
         opt.keep_records = int(keep_records)

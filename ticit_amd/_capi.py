@@ -30,7 +30,7 @@ class Result(ctypes.Structure):
                 ("ndetectors", ctypes.c_uint64), ("nobservables", ctypes.c_uint64),
                 ("measurements", _u8p), ("detectors", _u8p), ("observables", _u8p),
                 ("observable_ones", _u64p), ("sample_s", ctypes.c_double),
-                ("kernel_s", ctypes.c_double)]
+                ("kernel_s", ctypes.c_double), ("nexpvals", ctypes.c_uint64), ("exp_vals", _f64p)]
 
 
 class ProgramInfo(ctypes.Structure):
@@ -45,13 +45,15 @@ class ProgramInfo(ctypes.Structure):
                 ("state_dim", ctypes.c_uint64), ("component_count", ctypes.c_uint64),
                 ("max_local_k", ctypes.c_int32), ("planner_source", ctypes.c_int32),
                 ("planner_p_components", ctypes.c_double), ("lds_bytes_per_block", ctypes.c_uint32),
-                ("blocks_per_cu", ctypes.c_uint32), ("table_words", ctypes.c_uint32 * 4)]
+                ("blocks_per_cu", ctypes.c_uint32), ("table_words", ctypes.c_uint32 * 4),
+                ("expectation_values", ctypes.c_uint64)]
 
 
 class BatchItem(ctypes.Structure):
     _fields_ = [("program", ctypes.c_void_p), ("shots", ctypes.c_uint64), ("seed", ctypes.c_uint64),
                 ("measurements", ctypes.c_void_p), ("detectors", ctypes.c_void_p),
-                ("observables", ctypes.c_void_p), ("observable_ones", ctypes.c_void_p)]
+                ("observables", ctypes.c_void_p), ("observable_ones", ctypes.c_void_p),
+                ("exp_vals", ctypes.c_void_p)]
 
 
 class BatchOptions(ctypes.Structure):
@@ -123,7 +125,7 @@ def load() -> ctypes.CDLL:
     lib.ticit_hip_sample_device.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64,
                                             ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-                                            ctypes.c_void_p]
+                                            ctypes.c_void_p, ctypes.c_void_p]
     lib.ticit_hip_sample_injected.argtypes = [ctypes.c_void_p, ctypes.c_uint64, _u64p, _f64p, _u64p,
                                               ctypes.c_int, ctypes.POINTER(Result), _f64p]
     lib.ticit_hip_final_state.argtypes = [ctypes.c_void_p, ctypes.c_uint64, _u64p, _f64p, _u64p, _f64p, _f64p,

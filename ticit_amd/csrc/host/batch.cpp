@@ -257,6 +257,7 @@ ticit_hip_status ticit_hip_batch_create(const ticit_hip_batch_item* items, size_
     L.det = items[i].detectors;
     L.obs = items[i].observables;
     L.obs_ones = reinterpret_cast<unsigned long long*>(items[i].observable_ones);
+    L.expv = b->keep_records && t.nexp > 0 ? items[i].exp_vals : nullptr;
     L.error = b->d_error;
     L.seed_offset = b->d_seed_offset;
     L.meas_row = h->row_bytes(uint64_t(t.nrec), b->bit_packed);
@@ -494,6 +495,8 @@ ticit_hip_status ticit_hip_sample_batch(ticit_hip_program* const* programs, cons
         items[i].detectors = static_cast<uint8_t*>(p);
       if (e == hipSuccess && (e = dev(shots[i] * h->row_bytes(uint64_t(t.nobs), bit_packed), &p)) == hipSuccess)
         items[i].observables = static_cast<uint8_t*>(p);
+      if (e == hipSuccess && t.nexp > 0 && (e = dev(shots[i] * size_t(t.nexp) * 8, &p)) == hipSuccess)
+        items[i].exp_vals = static_cast<double*>(p);
     }
     if (e == hipSuccess && (e = dev(size_t(std::max(1, t.nobs)) * 8, &p)) == hipSuccess) {
       items[i].observable_ones = static_cast<uint64_t*>(p);
@@ -536,6 +539,7 @@ ticit_hip_status ticit_hip_sample_batch(ticit_hip_program* const* programs, cons
     r.nmeasurements = uint64_t(t.nrec);
     r.ndetectors = uint64_t(t.ndet);
     r.nobservables = uint64_t(t.nobs);
+    r.nexpvals = uint64_t(t.nexp);
     r.observable_ones = static_cast<uint64_t*>(calloc(std::max(1, t.nobs), 8));
     hipError_t e = hipMemcpy(r.observable_ones, items[i].observable_ones, size_t(t.nobs) * 8, hipMemcpyDeviceToHost);
     if (t.nobs > 0 && h->opt.observable < uint32_t(t.nobs)) r.logical_errors = r.observable_ones[h->opt.observable];
@@ -550,6 +554,11 @@ ticit_hip_status ticit_hip_sample_batch(ticit_hip_program* const* programs, cons
       if (e == hipSuccess && mb) e = hipMemcpy(r.measurements, items[i].measurements, mb, hipMemcpyDeviceToHost);
       if (e == hipSuccess && db) e = hipMemcpy(r.detectors, items[i].detectors, db, hipMemcpyDeviceToHost);
       if (e == hipSuccess && ob) e = hipMemcpy(r.observables, items[i].observables, ob, hipMemcpyDeviceToHost);
+      if (t.nexp > 0) {
+        size_t eb = shots[i] * size_t(t.nexp) * 8;
+        r.exp_vals = static_cast<double*>(malloc(std::max<size_t>(8, eb)));
+        if (e == hipSuccess && eb) e = hipMemcpy(r.exp_vals, items[i].exp_vals, eb, hipMemcpyDeviceToHost);
+      }
     }
     r.kernel_s = double(ms) * 1e-3;
     r.sample_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

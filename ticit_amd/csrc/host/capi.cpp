@@ -119,6 +119,7 @@ ticit_hip_status create_from_program(Program prog, const ticit_hip_options* opti
   d.state_dim = t.state_dim;
   d.ncomp = int32_t(t.comp_off.size());
   d.max_local_k = t.max_local_k;
+  d.nexp = t.nexp;
   e = hipMalloc(reinterpret_cast<void**>(&h->d_error), sizeof(int));
   if (e != hipSuccess) {
     delete h;
@@ -233,6 +234,7 @@ ticit_hip_status run_host(ticit_hip_program* h, uint64_t shots, uint64_t seed, i
     out->nmeasurements = uint64_t(t.nrec);
     out->ndetectors = uint64_t(t.ndet);
     out->nobservables = uint64_t(t.nobs);
+    out->nexpvals = uint64_t(t.nexp);
     out->observable_ones = static_cast<uint64_t*>(calloc(std::max(1, t.nobs), 8));
     return TICIT_HIP_OK;
   }
@@ -244,6 +246,7 @@ ticit_hip_status run_host(ticit_hip_program* h, uint64_t shots, uint64_t seed, i
   double* d_uni = nullptr;
   uint64_t* d_coins = nullptr;
   double* d_probs = nullptr;
+  double* d_expv = nullptr;
 #define CK(expr, what)                        \
   do {                                        \
     e = (expr);                               \
@@ -253,6 +256,10 @@ ticit_hip_status run_host(ticit_hip_program* h, uint64_t shots, uint64_t seed, i
     CK(buf.alloc(&L.meas, size_t(shots) * L.meas_row), "allocating measurement rows");
     CK(buf.alloc(&L.det, size_t(shots) * L.det_row), "allocating detector rows");
     CK(buf.alloc(&L.obs, size_t(shots) * L.obs_row), "allocating observable rows");
+    if (t.nexp > 0) {
+      CK(buf.alloc(&d_expv, size_t(shots) * size_t(t.nexp)), "allocating expectation values");
+      L.expv = d_expv;
+    }
   }
   CK(buf.alloc(&L.obs_ones, size_t(std::max(1, t.nobs))), "allocating observable counts");
   CK(hipMemset(L.obs_ones, 0, size_t(std::max(1, t.nobs)) * 8), "clearing observable counts");
@@ -309,6 +316,7 @@ ticit_hip_status run_host(ticit_hip_program* h, uint64_t shots, uint64_t seed, i
 
   out->shots = shots;
   out->accepted = shots;
+  out->nexpvals = uint64_t(t.nexp);
   out->bit_packed = bit_packed != 0;
   out->nmeasurements = uint64_t(t.nrec);
   out->ndetectors = uint64_t(t.ndet);
@@ -324,6 +332,11 @@ ticit_hip_status run_host(ticit_hip_program* h, uint64_t shots, uint64_t seed, i
     if (L.meas_row) CK(hipMemcpy(out->measurements, L.meas, size_t(shots) * L.meas_row, hipMemcpyDeviceToHost), "reading rows");
     if (L.det_row) CK(hipMemcpy(out->detectors, L.det, size_t(shots) * L.det_row, hipMemcpyDeviceToHost), "reading rows");
     if (L.obs_row) CK(hipMemcpy(out->observables, L.obs, size_t(shots) * L.obs_row, hipMemcpyDeviceToHost), "reading rows");
+    if (t.nexp > 0) {
+      out->exp_vals = static_cast<double*>(malloc(size_t(shots) * size_t(t.nexp) * sizeof(double)));
+      CK(hipMemcpy(out->exp_vals, d_expv, size_t(shots) * size_t(t.nexp) * 8, hipMemcpyDeviceToHost),
+         "reading expectation values");
+    }
   }
   if (probs && t.nmeasure)
     CK(hipMemcpy(probs, d_probs, size_t(t.nmeasure) * shots * 8, hipMemcpyDeviceToHost), "reading probabilities");
@@ -463,6 +476,7 @@ ticit_hip_status ticit_hip_program_info_get(const ticit_hip_program* h, ticit_hi
   out->table_words[1] = h->lane.sw;
   out->table_words[2] = h->lane.rw;
   out->table_words[3] = h->lane.dw;
+  out->expectation_values = uint64_t(h->tab.nexp);
   if (h->tier == kTierLane) {
     uint32_t row = std::max({h->row_bytes(uint64_t(h->tab.nrec), true), h->row_bytes(uint64_t(h->tab.ndet), true),
                              h->row_bytes(uint64_t(h->tab.nobs), true)});
@@ -504,7 +518,8 @@ ticit_hip_status ticit_hip_final_state(ticit_hip_program* h, uint64_t shots, con
 
 ticit_hip_status ticit_hip_sample_device(ticit_hip_program* h, uint64_t shots, uint64_t seed, uint64_t first_shot,
                                          int bit_packed, int keep_records, uint8_t* meas, uint8_t* det,
-                                         uint8_t* obs, uint64_t* observable_ones, void* stream) {
+                                         uint8_t* obs, uint64_t* observable_ones, double* exp_vals,
+                                         void* stream) {
   if (!h || !observable_ones) return fail(TICIT_HIP_INVALID_INPUT, "null argument");
   if (keep_records && (!meas || !det || !obs)) return fail(TICIT_HIP_INVALID_INPUT, "null row buffer");
   if (shots == 0) return TICIT_HIP_OK;
@@ -522,6 +537,7 @@ ticit_hip_status ticit_hip_sample_device(ticit_hip_program* h, uint64_t shots, u
   L.det = det;
   L.obs = obs;
   L.obs_ones = reinterpret_cast<unsigned long long*>(observable_ones);
+  L.expv = keep_records && t.nexp > 0 ? exp_vals : nullptr;
   L.error = h->d_error;
   hipError_t e = launch(h, L, static_cast<hipStream_t>(stream));
   if (e != hipSuccess) return from_hip(e, "launching the sampling kernel");
@@ -534,8 +550,10 @@ void ticit_hip_result_free(ticit_hip_result* r) {
   free(r->detectors);
   free(r->observables);
   free(r->observable_ones);
+  free(r->exp_vals);
   r->measurements = r->detectors = r->observables = nullptr;
   r->observable_ones = nullptr;
+  r->exp_vals = nullptr;
 }
 
 int ticit_hip_device_query(int64_t* props) {

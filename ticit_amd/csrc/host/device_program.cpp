@@ -47,6 +47,7 @@ HostTables lower_to_device(const Program& p, bool components) {
   t.initial_k = p.initial_k;
   t.max_k = p.max_k;
   t.nrec = p.nrecords;
+  t.nexp = p.nexpvals;
   t.ndet = p.ndetectors;
   t.nobs = int(p.observables.size());
   t.nsym = p.nsymbols;
@@ -356,6 +357,14 @@ HostTables lower_to_device(const Program& p, bool components) {
         } else {
           fill_measure(d, in);
         }
+        if (in.exp_val >= 0) {
+          // src/sampler/batch/active.rs:818-850: the probability pass only;
+          // no draw, no collapse, the width stays
+          if (st) throw TicitError(4, "expectation probes need the dense layout");
+          t.amp_traffic += std::ldexp((d.op & kFlagDiagonal) ? 0.5 : 1.0, d.nq);
+          d.ordinal = t.nmeasure++;
+          break;
+        }
         if (st && chain) {
           // measured straight from the factors: they are read by the
           // probability and the projection passes, only the projected half
@@ -371,10 +380,20 @@ HostTables lower_to_device(const Program& p, bool components) {
       }
       case Op::Coin:
         d.ordinal = t.ncoin++;
-        d.branch_slot = assign(in.branch);
+        if (in.exp_val < 0) d.branch_slot = assign(in.branch);
         break;
     }
-    if (in.exp_val >= 0) throw TicitError(3, "expectation probes are not supported by the hip backend");
+    if (in.exp_val >= 0) {
+      // src/sampler/batch/runtime.rs:46-75,311-356: a probe writes its
+      // expectation slot instead of a record
+      if (in.op != Op::Record && in.op != Op::Measure && in.op != Op::Coin)
+        throw TicitError(1, "expectation probe on an instruction that cannot carry one");
+      if (in.exp_val >= p.nexpvals) throw TicitError(1, "expectation value index is out of range");
+      if (in.record != 0 || in.record_condition != 0)
+        throw TicitError(1, "expectation probes take no measurement record");
+      d.op |= kFlagExpVal;
+      d.record = in.exp_val;
+    }
     bool evaluates = !(in.op == Op::Detector && !in.det_records.empty());
     if (evaluates) {
       std::vector<int32_t> ex;

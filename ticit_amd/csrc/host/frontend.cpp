@@ -315,7 +315,7 @@ struct ActiveFrame {
 enum class K {
   Tick, H, S, SDag, SqrtX, SqrtXDag, SqrtY, SqrtYDag, X, Y, Z, CX, CY, CZ, Swap,
   T, TDag, Rot, MZ, MX, MY, MRZ, MRX, MRY, RZ, RX, RY, MPP, XErr, YErr, ZErr, Dep1, Dep2,
-  FeedbackX, FeedbackY, FeedbackZ,
+  FeedbackX, FeedbackY, FeedbackZ, ExpVal,
 };
 
 struct MTarget {
@@ -337,6 +337,7 @@ struct CInstr {
   double probability = 0.0;
   double angle = 0.0;
   int feedback_record = 0;  // classically controlled Pauli: record id
+  int exp_base = 0;         // EXP_VAL: slot of the first product
 };
 
 struct CDetector {
@@ -353,6 +354,7 @@ struct Circuit {
   int nqubits = 0;
   int nrecords = 0;
   int ticks = 0;  // TICK lines, REPEAT bodies expanded
+  int nexpvals = 0;  // EXP_VAL slots, one per product (src/circuit/parser.rs:1477-1487)
   std::vector<CInstr> instrs;
   std::vector<CDetector> detectors;
   std::vector<CObservable> observables;
@@ -661,6 +663,14 @@ Circuit parse_circuit(const std::string& text) {
       ci.probability = opt_prob(rl);
       ci.products = products_of(rl);
       c.nrecords += int(ci.products.size());
+    } else if (op == "EXP_VAL") {
+      // src/circuit/parser.rs:1477-1487: one slot per product, no records
+      need_no_args(rl);
+      ci.kind = K::ExpVal;
+      ci.products = products_of(rl);
+      if (ci.products.empty()) parse_error(rl.line, "EXP_VAL expects at least one Pauli product");
+      ci.exp_base = c.nexpvals;
+      c.nexpvals += int(ci.products.size());
     } else if (op == "X_ERROR" || op == "Y_ERROR" || op == "Z_ERROR") {
       ci.kind = op == "X_ERROR" ? K::XErr : op == "Y_ERROR" ? K::YErr : K::ZErr;
       ci.probability = one_prob(rl);
@@ -708,6 +718,7 @@ struct PendingOp {
   double angle = 0.0;
   int32_t record = 0;
   int32_t record_condition = 0;
+  int32_t exp_val = -1;  // expectation probe slot (kind 1), -1 = a measurement
   SymBool outcome;
   int32_t max_condition() const {
     if (kind == 0) return sign.max_condition();
@@ -763,6 +774,16 @@ struct Lowering {
     op.sign = sym_xor(pr.second, sign);
     op.record = record;
     op.record_condition = rc;
+    pending.push_back(std::move(op));
+  }
+  // src/factored.rs:604-622: a probe queues a record-less measurement
+  void expectation(const Pauli& p, int32_t slot) {
+    auto pr = prepare(p);
+    PendingOp op;
+    op.kind = 1;
+    op.pauli = pr.first;
+    op.sign = pr.second;
+    op.exp_val = slot;
     pending.push_back(std::move(op));
   }
   SymBool readout_sign(bool inverted, double p) {
@@ -848,6 +869,14 @@ struct Lowering {
           SymBool sign = readout_sign(pr.inverted, ci.probability);
           auto rr = reserve_record();
           measurement(pr.pauli, sign, rr.first, rr.second);
+        }
+        return;
+      case K::ExpVal:
+        // src/circuit/lowering.rs:676-685; an inverted product is -P
+        for (size_t i = 0; i < ci.products.size(); ++i) {
+          Pauli p = ci.products[i].pauli;
+          if (ci.products[i].inverted) p.set_phase(p.phase + 2);
+          expectation(p, int32_t(ci.exp_base + int(i)));
         }
         return;
       case K::XErr:
@@ -997,6 +1026,11 @@ std::vector<int> optimize_pending(std::vector<PendingOp>& pending, const std::ve
                                   CompileStats& st) {
   size_t count = pending.size();
   std::vector<int> remap(count + 1, -1);
+  // src/pending_optimizer.rs:230-242: expectation probes pin the schedule
+  if (std::any_of(pending.begin(), pending.end(), [](const PendingOp& op) { return op.exp_val >= 0; })) {
+    for (size_t i = 0; i <= count; ++i) remap[i] = int(i);
+    return remap;
+  }
   remap[0] = 0;
   std::vector<size_t> ends;
   for (size_t p : preserved) {
@@ -1445,6 +1479,14 @@ struct Planner {
     ctx.bump(std::max(op.sign.max_condition(), op.record_condition));
     Pauli body = project_active(op.pauli);
     int picked = highest_dormant_x(op.pauli);
+    if (picked >= 0 && op.exp_val >= 0) {
+      // src/planner.rs:1017-1028: an unbiased probe reads 0, no state change
+      Instr in;
+      in.op = Op::Coin;
+      in.exp_val = op.exp_val;
+      push(std::move(in));
+      return;
+    }
     if (picked >= 0) {
       int picked_q = k + picked;
       Clifford f = replacement_frame(op.pauli, picked);
@@ -1471,10 +1513,23 @@ struct Planner {
       Instr in;
       in.op = Op::Record;
       in.expr = base;
-      in.record = take_record(op.record, op.record_condition);
+      in.exp_val = op.exp_val;
+      // src/planner.rs:96-108: a probe takes no record slot
+      in.record = op.exp_val >= 0 ? 0 : take_record(op.record, op.record_condition);
       in.record_condition = op.record_condition;
       push(std::move(in));
       note_measurement_relation(op.record_condition, base);
+      return;
+    }
+    if (op.exp_val >= 0) {
+      // src/planner.rs:1101-1124: non-destructive, no branch, no basis change
+      if (!hermitian(body)) throw TicitError(4, "active expectation Pauli must square to identity");
+      Instr in;
+      in.op = Op::Measure;
+      fill_measure_kernel(in, body);
+      in.expr = base;
+      in.exp_val = op.exp_val;
+      push(std::move(in));
       return;
     }
     if (!hermitian(body)) throw TicitError(4, "active measurement Pauli must square to identity");

@@ -31,6 +31,9 @@ enum : uint32_t {
   kFlagDetRecords = 1u << 13,   // detector reads record bits, not its expression
   kFlagFresh = 1u << 14,        // promotion that starts a new one-qubit component
   kFlagZeroState = 1u << 15,    // fresh component in |0> (initial components)
+  kFlagExpVal = 1u << 16,       // expectation probe (Record/Measure/Coin): `record`
+                                // holds the expectation slot; no record, branch or
+                                // collapse (src/sampler/batch/runtime.rs:311-356)
 };
 
 struct alignas(16) DevInstr {
@@ -148,6 +151,7 @@ struct DevProgram {
   uint32_t state_dim;   // amplitudes per shot: 2^max_k dense, sum of component capacities otherwise
   int32_t ncomp;        // components (1 dense)
   int32_t max_local_k;  // widest component (max_k dense)
+  int32_t nexp;         // expectation slots per shot
 };
 
 // Per-launch sampling request.
@@ -174,6 +178,8 @@ struct DevLaunch {
   // optional device word added to `seed` at run time, so a captured graph
   // can be replayed with fresh seeds
   const uint64_t* seed_offset;
+  // optional [shots][nexp] expectation values (written when non-null)
+  double* expv;
 };
 
 // LDS table sizes of the two resident tiers (words of bits per shot).

@@ -97,6 +97,9 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
 
   uint64_t brng = seed_for(kBranchSeedBase, launch_seed(L), shot);  // identical in the group
   int err = 0;
+  // expectation probes (src/sampler/batch/runtime.rs:46-75): thread 0 of the
+  // shot writes its [nexp] row
+  double* expv = (L.expv && valid && t == 0) ? L.expv + local * uint64_t(P.nexp) : nullptr;
   for (int i = 0; i < P.ninstr; ++i) {
     // the compact step drives the loop; amplitude work reads the full record
     const DevStep st = P.steps[i];
@@ -196,7 +199,9 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
       }
       // One thread per shot writes; no reader of this instruction reads the
       // bit being written, and the barrier orders it before later readers.
-      if (t == 0) {
+      if (st.op & kFlagExpVal) {
+        if (expv) expv[st.a] = b ? -1.0 : 1.0;
+      } else if (t == 0) {
         if (op == kOpDetector) {
           set_bit(dbits, uint32_t(st.a), b);
         } else {
@@ -205,6 +210,8 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
         }
       }
       __syncthreads();
+    } else if (op == kOpCoin && (st.op & kFlagExpVal)) {
+      if (expv) expv[st.a] = 0.0;  // an unbiased probe reads 0 and draws nothing
     } else if (op == kOpMeasure || op == kOpCoin) {
       bool branch = false;
       if (op == kOpMeasure) {
@@ -230,6 +237,13 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
         double pt = 0.0;
         for (int w = 0; w < WPG; ++w) pt += red[g * WPG + w];
         pt = clamp01(pt);
+        if (st.op & kFlagExpVal) {
+          // src/sampler/batch/active.rs:818-850: +-(1 - 2 P(true)), no collapse
+          if (valid && L.want_probs && t == 0) L.probs[uint64_t(in.ordinal) * L.shots + local] = pt;
+          if (expv) expv[st.a] = (eval(st) ? -1.0 : 1.0) * (1.0 - 2.0 * pt);
+          __syncthreads();  // every thread has read `red` before the next reduction
+          continue;
+        }
         if (valid) {
           if (L.want_probs && t == 0) L.probs[uint64_t(in.ordinal) * L.shots + local] = pt;
           if (L.injected) {

@@ -80,6 +80,10 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
     if (st.b >= 0) Sw[st.b] = v;
   };
 
+  // expectation probes (src/sampler/batch/runtime.rs:46-75): one value per
+  // shot and slot, [shots][nexp] in global memory
+  double* expv = (L.expv && live) ? L.expv + local * uint64_t(P.nexp) : nullptr;
+
   int k = P.initial_k;
   uint64_t brng = seed_for(kBranchSeedBase, launch_seed(L), shot);
   int err = 0;
@@ -88,6 +92,10 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
     const DevStep st = P.steps[i];
     const uint32_t op = st.op & 0xff;
     if (op == kOpRecord) {
+      if (st.op & kFlagExpVal) {
+        if (expv) expv[st.a] = ((eval(st) >> lane) & 1) ? -1.0 : 1.0;
+        continue;
+      }
       write_outcome(st, eval(st));
       continue;
     }
@@ -102,6 +110,10 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
       continue;
     }
     if (op == kOpCoin) {
+      if (st.op & kFlagExpVal) {  // an unbiased probe reads 0 and draws nothing
+        if (expv) expv[st.a] = 0.0;
+        continue;
+      }
       uint64_t coins;
       if (L.injected) {
         const uint64_t words = (L.shots + 63) / 64;
@@ -187,6 +199,12 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
       }
       pt = clamp01(pt);
       if (L.want_probs && live) L.probs[uint64_t(in.ordinal) * L.shots + local] = pt;
+      if (st.op & kFlagExpVal) {
+        // src/sampler/batch/active.rs:818-850: +-(1 - 2 P(true)), no collapse
+        const double sign = ((eval(st) >> lane) & 1) ? -1.0 : 1.0;
+        if (expv) expv[st.a] = sign * (1.0 - 2.0 * pt);
+        continue;
+      }
       bool branch = false;
       if (L.injected) {
         if (live) {

@@ -140,9 +140,16 @@ __global__ __launch_bounds__(kWideThreads) void wide_bits_kernel(DevProgram P, D
   const uint64_t local = batch_start + j;
   WideShot w = wide_shot(bits, Y, j);
   uint64_t brng = ctl[j].brng;
+  double* expv = L.expv ? L.expv + local * uint64_t(P.nexp) : nullptr;
   for (int i = first; i < last; ++i) {
     const DevInstr in = P.instrs[i];
     const uint32_t op = in.op & 0xff;
+    if (in.op & kFlagExpVal) {
+      // src/sampler/batch/runtime.rs:46-75,311-356: a deterministic probe is
+      // +-1, an unbiased one 0 (no draw)
+      if (expv) expv[in.record] = op == kOpCoin ? 0.0 : (wide_eval(P, in, w) ? -1.0 : 1.0);
+      continue;
+    }
     if (op == kOpRecord) {
       bool b = wide_eval(P, in, w);
       if (in.record >= 0) wset(w.r, uint32_t(in.record), b);
@@ -480,6 +487,13 @@ __global__ __launch_bounds__(kWideThreads) void wide_decide_kernel(DevProgram P,
   for (uint32_t t = 0; t < ntiles; ++t) pt += part[size_t(j) * max_tiles + t];
   pt = clamp01(pt);
   if (L.want_probs) L.probs[uint64_t(in.ordinal) * L.shots + local] = pt;
+  if (in.op & kFlagExpVal) {
+    // src/sampler/batch/active.rs:818-850: +-(1 - 2 P(true)), no draw, no
+    // collapse
+    const double sign = wide_eval(P, in, wide_shot(bits, Y, j)) ? -1.0 : 1.0;
+    if (L.expv) L.expv[local * uint64_t(P.nexp) + uint32_t(in.record)] = sign * (1.0 - 2.0 * pt);
+    return;
+  }
   WideCtl c = ctl[j];
   bool branch = false;
   if (L.injected) {
@@ -958,6 +972,10 @@ static hipError_t run_wide(const DevProgram& P, const DevInstr* instrs, const De
         hipLaunchKernelGGL(wide_decide_kernel, sg, sb, 0, stream, P, L, Y, start, n, i, ws.bits, ws.ctl, ws.part,
                            tiles, Y.max_tiles);
         WCK(hipGetLastError());
+        if (in.op & kFlagExpVal) {  // a probe reads the state and leaves it
+          ++i;
+          continue;
+        }
         hipLaunchKernelGGL(wide_project_kernel<T>, g, dim3(kWideThreads), 0, stream, P, i, ws.ctl, R, I, RE(cb ^ 1),
                            IM(cb ^ 1), dim, w);
         WCK(hipGetLastError());
