@@ -971,7 +971,7 @@ struct Runner {
       }
     }
     // src/sampler/batch/mod.rs:597-680 for every shot of the block
-    auto do_merges = [&](const CompStep& st) {
+    auto do_merges = [&](const CompStep& st, bool round_merged = true) {
       if (st.merge_count == 0) throw OracleError("active component instruction has an invalid merge range");
       int t = cplan.merges[st.merge_off];
       if (t != st.comp || ck[size_t(t)] == 0) throw OracleError("active component merge has an invalid target");
@@ -983,7 +983,7 @@ struct Runner {
         for (size_t s = 0; s < shots; ++s) {
           size_t bt = s * stride + comp_off[size_t(t)], bs = s * stride + comp_off[size_t(src)];
           merge_into(&re[bt], &im[bt], ck[size_t(t)], &re[bs], &im[bs], ck[size_t(src)]);
-          round_state(bt, size_t(1) << (ck[size_t(t)] + ck[size_t(src)]));
+          if (round_merged) round_state(bt, size_t(1) << (ck[size_t(t)] + ck[size_t(src)]));
         }
         ck[size_t(t)] += ck[size_t(src)];
         ck[size_t(src)] = 0;
@@ -1162,10 +1162,15 @@ struct Runner {
             break;
           }
           size_t coff = 0;
+          // FP32 storage emulation: a merged state measured straight away is
+          // never stored, so probability and projection read the unrounded
+          // products; only the projected half is rounded (as stored)
+          bool round_late = false;
           if (comp) {
             const CompStep& st = cplan.steps[i];
             if (st.kind != kCompMeasurement) throw OracleError("component measurement step does not match its instruction");
-            do_merges(st);
+            round_late = fp32 && st.merge_count > 1;
+            do_merges(st, !round_late);
             coff = comp_off[size_t(st.comp)];
           }
           const MeasK& mk = comp ? cmeas[i] : meas[i];

@@ -162,9 +162,9 @@ def test_device_entry_point_split_launches(hip):
     assert int(ones.item()) == int(ref.observable_ones[0])
 
 
-@pytest.mark.parametrize("tps", [64, 128, 256])
+@pytest.mark.parametrize("tps", [64, 128, 256, 512])
 def test_group_threads_per_shot_give_identical_rows(hip, tps):
-    """Packing 4, 2 or 1 shots per workgroup changes the launch shape and
+    """Packing 4, 2 or 1 shots per workgroup (or one per 512 threads) changes the launch shape and
     the reduction tree of a shot's probability sums (last-bit differences),
     not the rows: they equal the oracle's."""
     c = hip.Circuit(w.config2_batch()[94][0])  # component layout, local k 6
@@ -174,3 +174,23 @@ def test_group_threads_per_shot_give_identical_rows(hip, tps):
     o = O.sample_per_shot(c.blob, shots, 5)
     assert np.array_equal(r.measurements, o["measurements"])
     assert r.observable_ones.tolist() == o["observable_ones"].tolist()
+
+
+def test_group_tier_holds_a_width_13_state(hip):
+    """wide_compact (max_k 13, a 128 KiB FP64 state) runs in LDS with 512
+    threads per shot when asked for: rows bit-exact, P(branch = 1) within
+    1e-12."""
+    text = open(__file__.replace("test_gpu_parity.py", "golden/circuits/wide_compact.stim")).read()
+    c = hip.Circuit(text)
+    s = c.compile(tier="group", group_threads=512)
+    assert s.tier == "group" and s.max_active_qubits == 13
+    shots = 500
+    r = s.sample(shots, seed=9, bit_packed=True)
+    o = O.sample_per_shot(c.blob, shots, 9)
+    assert np.array_equal(r.measurements, o["measurements"])
+    rng = np.random.default_rng(4)
+    uni = rng.random((s.num_measure_instructions, shots))
+    ri = s.sample_injected(shots, uniforms=uni, want_probs=True)
+    oi = O.sample_injected(c.blob, shots, None, uni, None, want_probs=True)
+    assert np.array_equal(ri.measurements, oi["measurements"])
+    assert np.max(np.abs(ri.probabilities - oi["probs"])) <= PROB_TOL

@@ -4,12 +4,12 @@
 // Programs are grouped by the tier they run on:
 //   * group tier: buckets of programs with similar per-shot LDS footprints,
 //     each bucket one persistent kernel that pulls (program, shot block)
-//     work units from an atomic counter; threads per shot (64, 128 or 256,
-//     i.e. 4, 2 or 1 shots per workgroup) are picked per bucket by timing
-//     each candidate on a sample of the bucket's shots;
+//     work units from an atomic counter; threads per shot (64, 128, 256 or
+//     512, one shot per workgroup) are picked per bucket by timing each
+//     candidate on a sample of the bucket's shots;
 //   * lane tier: one launch per program;
-//   * wide tier: each program's kernel sequence, on a second branch of the
-//     graph so the HBM-streaming wide work overlaps the LDS-bound group work.
+//   * wide tier: each program's kernel sequence, in a second graph replayed
+//     on a second stream.
 // The graph is captured once at creation and replayed by every launch; a
 // device word added to every program's seed lets replays draw fresh shots.
 #include <hip/hip_runtime.h>
@@ -35,7 +35,7 @@ struct ticit_hip_batch {
     std::vector<int> candidates;
     int tps = 256;
     int grid = 0;
-    double tuned_ms[3] = {0, 0, 0};  // sample time per candidate (64, 128, 256)
+    double tuned_ms[4] = {0, 0, 0, 0};  // sample time per candidate (64, 128, 256, 512)
     DevProgram* d_progs = nullptr;
     DevLaunch* d_launches = nullptr;
     GroupLayout* d_layouts = nullptr;
@@ -101,7 +101,7 @@ struct ticit_hip_batch {
 
 namespace {
 
-int tps_slot(int tps) { return tps == 64 ? 0 : (tps == 128 ? 1 : 2); }
+int tps_slot(int tps) { return tps == 64 ? 0 : (tps == 128 ? 1 : (tps == 256 ? 2 : 3)); }
 
 // Tuned threads-per-shot by bucket signature, kept for the process.
 std::mutex g_tune_mu;
@@ -115,12 +115,12 @@ uint32_t round_region(size_t bytes) {
 
 hipError_t unit_table(const ticit_hip_batch& b, const ticit_hip_batch::Bucket& k, int tps,
                       const std::vector<uint64_t>& shots, std::vector<uint64_t>& off, uint64_t& total) {
-  const int spb = 256 / tps;
+  const int spu = group_shots_per_unit(tps);
   off.clear();
   total = 0;
   for (size_t j = 0; j < k.items.size(); ++j) {
     off.push_back(total);
-    total += (shots[j] + spb - 1) / spb;
+    total += (shots[j] + spu - 1) / spu;
   }
   (void)b;
   return hipSuccess;
@@ -321,8 +321,9 @@ ticit_hip_status ticit_hip_batch_create(const ticit_hip_batch_item* items, size_
 
   // Group buckets: candidates, tables, tuning.
   for (auto& k : b->buckets) {
-    for (int tps : {64, 128, 256})
-      if (k.max_local_k <= group_max_local_k(tps) && 64 + size_t(256 / tps) * k.region <= size_t(lds_limit))
+    for (int tps : {64, 128, 256, 512})
+      if (k.max_local_k <= group_max_local_k(tps) &&
+          64 + size_t(group_shots_per_block(tps)) * k.region <= size_t(lds_limit))
         k.candidates.push_back(tps);
     if (k.candidates.empty()) {
       delete b;
@@ -454,7 +455,7 @@ ticit_hip_status ticit_hip_batch_info_get(const ticit_hip_batch* b, ticit_hip_ba
     out->bucket_items[j] = uint32_t(k.items.size());
     out->bucket_region_bytes[j] = k.region;
     out->bucket_grid[j] = uint32_t(k.grid);
-    for (int c = 0; c < 3; ++c) out->bucket_tuned_ms[j][c] = k.tuned_ms[c];
+    for (int c = 0; c < 4; ++c) out->bucket_tuned_ms[j][c] = k.tuned_ms[c];
   }
   return TICIT_HIP_OK;
 }

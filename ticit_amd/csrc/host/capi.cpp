@@ -137,15 +137,22 @@ ticit_hip_status create_from_program(Program prog, const ticit_hip_options* opti
   h->lane.stage_bytes = 64 * ((max_row + 15) / 16 * 16);
   h->wide = WideLayout{h->lane.pw, h->lane.sw, h->lane.rw, h->lane.dw,
                        h->lane.pw + h->lane.sw + h->lane.rw + h->lane.dw,
-                       uint32_t(((size_t(1) << t.max_local_k) / 2 + 2047) / 2048 + 1)};
+                       wide_partials_per_shot(t.max_local_k)};
   const bool lane_ok = !t.components && t.max_k <= 5 && lane_tier_lds_bytes(t.max_k, h->lane) <= size_t(lds_limit);
-  const int tps = h->opt.group_threads ? int(h->opt.group_threads) : 256;
-  if (tps != 64 && tps != 128 && tps != 256) {
+  auto group_fits = [&](int tps) {
+    return t.max_local_k <= group_max_local_k(tps) &&
+           group_tier_lds_bytes(t.state_dim, h->group, tps) <= size_t(lds_limit);
+  };
+  // default launch shape: one shot per 256-thread workgroup. Width-13
+  // states fit a 512-thread shot's LDS, but run slower there than in the
+  // wide tier (one shot per CU), so they go wide unless asked for.
+  int tps = h->opt.group_threads ? int(h->opt.group_threads) : 256;
+  if (tps != 64 && tps != 128 && tps != 256 && tps != 512) {
     delete h;
-    return fail(TICIT_HIP_INVALID_INPUT, "group_threads must be 64, 128 or 256");
+    return fail(TICIT_HIP_INVALID_INPUT, "group_threads must be 64, 128, 256 or 512");
   }
-  const bool group_ok =
-      t.max_local_k <= group_max_local_k(tps) && group_tier_lds_bytes(t.state_dim, h->group, tps) <= size_t(lds_limit);
+  h->group_tps = tps;
+  const bool group_ok = group_fits(tps);
   uint32_t want = h->opt.tier;
   if (want == 0 && h->opt.precision == 1) want = kTierWide;  // the FP32 path lives in the wide tier
   if (want == 0) want = lane_ok ? kTierLane : (group_ok ? kTierGroup : kTierWide);
@@ -182,7 +189,7 @@ hipError_t launch(ticit_hip_program* h, const DevLaunch& L, hipStream_t stream, 
                   double* dump_im = nullptr, int* dump_k = nullptr) {
   if (h->tier == kTierLane) return launch_lane_tier(h->tab.max_k, h->dev, L, h->lane, stream);
   if (h->tier == kTierGroup)
-    return launch_group_tier(h->dev, L, h->group, stream, h->opt.group_threads ? int(h->opt.group_threads) : 256);
+    return launch_group_tier(h->dev, L, h->group, stream, h->group_tps);
   uint32_t batch = uint32_t(std::min<uint64_t>(h->wide_batch, L.shots));
   // One wide-tier workspace per device, shared by every handle on it and
   // grown to the largest batch asked for; calls are serialized on it (host

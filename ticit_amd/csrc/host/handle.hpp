@@ -19,6 +19,8 @@ void lane_tier_occupancy(int kmax, const LaneLayout& Y, uint32_t row_bytes, uint
 size_t group_tier_lds_bytes(uint32_t state_dim, const GroupLayout& y, int tps);
 size_t group_region_bytes(uint32_t state_dim, const GroupLayout& y);
 int group_max_local_k(int tps);
+int group_shots_per_block(int tps);
+int group_shots_per_unit(int tps);
 int group_batch_occupancy(int tps, uint32_t region_bytes);
 hipError_t launch_lane_tier(int kmax, const DevProgram& P, const DevLaunch& L, const LaneLayout& Y,
                             hipStream_t stream);
@@ -30,6 +32,7 @@ hipError_t launch_group_batch(const DevProgram* progs, const DevLaunch* launches
                               hipStream_t stream);
 hipError_t launch_batch_prologue(int* error, uint64_t* seed_offset, uint64_t value, hipStream_t stream);
 size_t wide_bytes_per_shot(uint32_t state_dim, int fp32, const WideLayout& y);
+uint32_t wide_partials_per_shot(int max_local_k);
 struct WideWorkspace;
 WideWorkspace* wide_workspace_new();
 void wide_workspace_free(WideWorkspace* ws);
@@ -63,6 +66,7 @@ struct ticit_hip_program {
   ticit::LaneLayout lane{};
   ticit::GroupLayout group{};
   ticit::WideLayout wide{};
+  int group_tps = 256;  // group tier threads per shot of the plain launch
   uint32_t wide_batch = 0;
   int32_t planner_source = 0;
   float planner_p = -1.f;

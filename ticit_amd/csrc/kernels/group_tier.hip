@@ -1,11 +1,9 @@
-// Group tier: one shot per group of TPS threads, 256 / TPS shots per
-// workgroup, every shot's active state (split-complex FP64) and bit tables
-// resident in LDS.
-//
-// All shots of a workgroup run the same instruction stream in lockstep, so
-// the workgroup barriers are uniform; each shot's amplitude work is spread
-// over its own TPS threads. TPS = 256 is one shot per workgroup (the default
-// launch); 64 and 128 pack 4 or 2 shots per workgroup for small states.
+// Group tier: one shot per workgroup of TPS threads (64, 128, 256 or 512),
+// the shot's active state (split-complex FP64) and bit tables resident in
+// LDS. The shot's amplitude work is spread over its TPS threads; a 64-thread
+// shot is one wave, so its barriers never wait on another shot. TPS = 256 is
+// the default launch; 64 and 128 suit small states (more shots per CU), 512
+// holds states of 2^13 (128 KiB FP64).
 // Projections go through registers, kProjPer outputs per thread, so a shot
 // group of TPS threads handles local widths up to log2(2 * kProjPer * TPS).
 //
@@ -22,7 +20,12 @@
 
 namespace ticit {
 
-constexpr int kGroupBlock = 256;
+// Workgroup size for a shot of TPS threads: one shot per workgroup.
+constexpr int group_block(int tps) { return tps; }
+// Shots one work unit of the persistent batch kernel runs in sequence: as
+// many as a 256-thread group would hold, so the unit counter is taken once
+// per 256 threads' worth of shots.
+constexpr int group_unit_shots(int tps) { return tps < 256 ? 256 / tps : 1; }
 constexpr int kProjPer = 8;
 
 __device__ __forceinline__ double group_wave_sum(double v) {
@@ -35,12 +38,14 @@ __host__ __device__ inline size_t group_shot_bytes(uint32_t state_dim, const Gro
   return size_t(state_dim) * 16 + size_t(y.pw + y.sw + y.rw + y.dw) * 8;
 }
 
-// Runs shots [first_local, first_local + 256 / TPS) of launch L.
+// Runs shot first_local of launch L on the workgroup (group_block(TPS) / TPS
+// = 1 shot).
 template <int TPS>
 __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const GroupLayout& Y, uint64_t first_local,
                             unsigned char* smem, size_t region_bytes) {
-  constexpr int SPB = kGroupBlock / TPS;
+  constexpr int SPB = group_block(TPS) / TPS;
   constexpr int WPG = TPS / 64;  // waves per shot group
+  static_assert(SPB * WPG * sizeof(double) <= 64, "reduction slots exceed the reserved 64 bytes");
   const int tid = threadIdx.x;
   const int g = tid / TPS;       // shot group
   const int t = tid % TPS;       // thread within the group
@@ -348,17 +353,17 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
 }
 
 template <int TPS>
-__global__ __launch_bounds__(kGroupBlock) void group_tier_kernel(DevProgram P, DevLaunch L, GroupLayout Y,
+__global__ __launch_bounds__(group_block(TPS)) void group_tier_kernel(DevProgram P, DevLaunch L, GroupLayout Y,
                                                                  uint32_t region_bytes) {
   extern __shared__ __align__(16) unsigned char smem[];
-  group_shots<TPS>(P, L, Y, uint64_t(blockIdx.x) * (kGroupBlock / TPS), smem, region_bytes);
+  group_shots<TPS>(P, L, Y, uint64_t(blockIdx.x) * (group_block(TPS) / TPS), smem, region_bytes);
 }
 
 // Persistent kernel over a batch: unit u of item i covers shots
-// [(u - unit_off[i]) * SPB, ...) of that item; workgroups take units from
-// *counter until all `total_units` are done.
+// [(u - unit_off[i]) * SPU, ...) of that item, run one after another;
+// workgroups take units from *counter until all `total_units` are done.
 template <int TPS>
-__global__ __launch_bounds__(kGroupBlock) void group_batch_kernel(const DevProgram* progs, const DevLaunch* launches,
+__global__ __launch_bounds__(group_block(TPS)) __attribute__((amdgpu_waves_per_eu(4))) void group_batch_kernel(const DevProgram* progs, const DevLaunch* launches,
                                                                   const GroupLayout* layouts,
                                                                   const uint64_t* unit_off, uint32_t nitems,
                                                                   uint64_t total_units, unsigned long long* counter,
@@ -390,8 +395,13 @@ __global__ __launch_bounds__(kGroupBlock) void group_batch_kernel(const DevProgr
     const DevProgram& P = progs[item];
     const DevLaunch& L = launches[item];
     const GroupLayout& Y = layouts[item];
-    group_shots<TPS>(P, L, Y, (u - unit_off[item]) * (kGroupBlock / TPS), smem, region_bytes);
-    __syncthreads();
+    constexpr int SPU = group_unit_shots(TPS);
+    const uint64_t first = (u - unit_off[item]) * SPU;
+    for (int q = 0; q < SPU; ++q) {
+      if (first + uint64_t(q) >= L.shots) break;  // uniform: one item per unit
+      group_shots<TPS>(P, L, Y, first + uint64_t(q), smem, region_bytes);
+      __syncthreads();
+    }
   }
 }
 
@@ -417,8 +427,11 @@ size_t group_region_bytes(uint32_t state_dim, const GroupLayout& y) {
   return (group_shot_bytes(state_dim, y) + 15) / 16 * 16;
 }
 
+int group_shots_per_block(int tps) { return group_block(tps) / tps; }
+int group_shots_per_unit(int tps) { return group_unit_shots(tps); }
+
 size_t group_tier_lds_bytes(uint32_t state_dim, const GroupLayout& y, int tps) {
-  return 64 + size_t(kGroupBlock / tps) * group_region_bytes(state_dim, y);
+  return 64 + size_t(group_shots_per_block(tps)) * group_region_bytes(state_dim, y);
 }
 
 // Largest local width a shot group of `tps` threads projects through registers.
@@ -431,13 +444,13 @@ int group_max_local_k(int tps) {
 template <int TPS>
 static hipError_t launch_group_t(const DevProgram& P, const DevLaunch& L, const GroupLayout& Y, hipStream_t stream) {
   size_t region = group_region_bytes(P.state_dim, Y);
-  size_t lds = 64 + size_t(kGroupBlock / TPS) * region;
+  constexpr int SPB = group_block(TPS) / TPS;
+  size_t lds = 64 + size_t(SPB) * region;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&group_tier_kernel<TPS>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
   if (e != hipSuccess) return e;
-  constexpr int SPB = kGroupBlock / TPS;
   dim3 grid(unsigned((L.shots + SPB - 1) / SPB));
-  hipLaunchKernelGGL(group_tier_kernel<TPS>, grid, dim3(kGroupBlock), lds, stream, P, L, Y, uint32_t(region));
+  hipLaunchKernelGGL(group_tier_kernel<TPS>, grid, dim3(group_block(TPS)), lds, stream, P, L, Y, uint32_t(region));
   return hipGetLastError();
 }
 
@@ -449,6 +462,7 @@ hipError_t launch_group_tier(const DevProgram& P, const DevLaunch& L, const Grou
     case 64: return launch_group_t<64>(P, L, Y, stream);
     case 128: return launch_group_t<128>(P, L, Y, stream);
     case 256: return launch_group_t<256>(P, L, Y, stream);
+    case 512: return launch_group_t<512>(P, L, Y, stream);
     default: return hipErrorInvalidValue;
   }
 }
@@ -458,12 +472,13 @@ static hipError_t launch_group_batch_t(const DevProgram* progs, const DevLaunch*
                                        const uint64_t* unit_off, uint32_t nitems, uint64_t total_units,
                                        unsigned long long* counter, uint32_t region_bytes, int grid,
                                        hipStream_t stream) {
-  size_t lds = 64 + size_t(kGroupBlock / TPS) * region_bytes;
+  size_t lds = 64 + size_t(group_block(TPS) / TPS) * region_bytes;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&group_batch_kernel<TPS>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, counter);
-  hipLaunchKernelGGL(group_batch_kernel<TPS>, dim3(unsigned(grid)), dim3(kGroupBlock), lds, stream, progs, launches,
+  hipLaunchKernelGGL(group_batch_kernel<TPS>, dim3(unsigned(grid)), dim3(group_block(TPS)), lds, stream, progs,
+                     launches,
                      layouts, unit_off, nitems, total_units, counter, region_bytes);
   return hipGetLastError();
 }
@@ -482,6 +497,9 @@ hipError_t launch_group_batch(const DevProgram* progs, const DevLaunch* launches
     case 256:
       return launch_group_batch_t<256>(progs, launches, layouts, unit_off, nitems, total_units, counter,
                                        region_bytes, grid, stream);
+    case 512:
+      return launch_group_batch_t<512>(progs, launches, layouts, unit_off, nitems, total_units, counter,
+                                       region_bytes, grid, stream);
     default:
       return hipErrorInvalidValue;
   }
@@ -489,13 +507,14 @@ hipError_t launch_group_batch(const DevProgram* progs, const DevLaunch* launches
 
 // Workgroups of the persistent kernel that fit on the device at once.
 int group_batch_occupancy(int tps, uint32_t region_bytes) {
-  size_t lds = 64 + size_t(kGroupBlock / tps) * region_bytes;
+  size_t lds = 64 + size_t(group_shots_per_block(tps)) * region_bytes;
   int blocks = 0;
-  const void* fn = tps == 64 ? reinterpret_cast<const void*>(&group_batch_kernel<64>)
+  const void* fn = tps == 64    ? reinterpret_cast<const void*>(&group_batch_kernel<64>)
                    : tps == 128 ? reinterpret_cast<const void*>(&group_batch_kernel<128>)
-                                : reinterpret_cast<const void*>(&group_batch_kernel<256>);
+                   : tps == 256 ? reinterpret_cast<const void*>(&group_batch_kernel<256>)
+                                : reinterpret_cast<const void*>(&group_batch_kernel<512>);
   if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)) != hipSuccess) return 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, kGroupBlock, lds) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, group_block(tps), lds) != hipSuccess) return 0;
   return blocks;
 }
 

@@ -472,19 +472,10 @@ __global__ __launch_bounds__(kWideThreads) void wide_prob_kernel(DevProgram P, i
   if (threadIdx.x == 0) part[size_t(j) * max_tiles + blockIdx.x] = t;
 }
 
-// Per shot: fixed-order sum of the tile partials, clamp, draw, 1/sqrt,
-// branch symbol, outcome, record.
-__global__ __launch_bounds__(kWideThreads) void wide_decide_kernel(DevProgram P, DevLaunch L, WideLayout Y,
-                                                                   uint64_t batch_start, uint32_t nshots,
-                                                                   int instr, uint64_t* bits, WideCtl* ctl,
-                                                                   const double* part, uint32_t ntiles,
-                                                                   uint32_t max_tiles) {
-  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nshots) return;
-  const uint64_t local = batch_start + j;
-  const DevInstr in = P.instrs[instr];
-  double pt = 0.0;
-  for (uint32_t t = 0; t < ntiles; ++t) pt += part[size_t(j) * max_tiles + t];
+// Per shot, given P(branch = 1) before the clamp: clamp, draw, 1/sqrt,
+// branch symbol, outcome, record (or the expectation of a probe).
+__device__ void wide_decide_shot(const DevProgram& P, const DevLaunch& L, const WideLayout& Y, uint32_t j,
+                                 uint64_t local, const DevInstr& in, double pt, uint64_t* bits, WideCtl* ctl) {
   pt = clamp01(pt);
   if (L.want_probs) L.probs[uint64_t(in.ordinal) * L.shots + local] = pt;
   if (in.op & kFlagExpVal) {
@@ -517,6 +508,20 @@ __global__ __launch_bounds__(kWideThreads) void wide_decide_kernel(DevProgram P,
   bool b = wide_eval(P, in, w);
   if (in.record >= 0) wset(w.r, uint32_t(in.record), b);
   if (in.rc_slot >= 0) wset(w.s, uint32_t(in.rc_slot), b);
+}
+
+// Per shot: fixed-order sum of the tile partials, then the decision.
+__global__ __launch_bounds__(kWideThreads) void wide_decide_kernel(DevProgram P, DevLaunch L, WideLayout Y,
+                                                                   uint64_t batch_start, uint32_t nshots,
+                                                                   int instr, uint64_t* bits, WideCtl* ctl,
+                                                                   const double* part, uint32_t ntiles,
+                                                                   uint32_t max_tiles) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nshots) return;
+  const DevInstr in = P.instrs[instr];
+  double pt = 0.0;
+  for (uint32_t t = 0; t < ntiles; ++t) pt += part[size_t(j) * max_tiles + t];
+  wide_decide_shot(P, L, Y, j, batch_start + j, in, pt, bits, ctl);
 }
 
 // Collapse into the partner buffer (the renormalising multiply in FP64).
@@ -593,11 +598,12 @@ __global__ __launch_bounds__(kWideThreads) void wide_project_kernel(DevProgram P
 // A component about to be merged from factors f0 (target) and f1..fn-1
 // (sources, in merge order): amplitude x is
 // ((f0[x0] * f1[x1]) * f2[x2]) ..., x_f = (x >> shift_f) & mask_f, each
-// product in the order and formula of a merge, so every value equals the
-// materialized state's bit for bit. A merge chain followed by a
+// product in the order and formula of a merge (an FP64 state's values
+// equal the materialized state's bit for bit). A merge chain followed by a
 // measurement of the merged component never writes the merged state: its
-// probability and projection are computed from the factors (which stay in
-// cache) and only the projected half is written.
+// probability comes from the factors' expectations, its projection from
+// the factors (which stay in cache), and only the projected half is
+// written.
 constexpr int kMaxFactors = 6;
 
 template <class T>
@@ -607,17 +613,27 @@ struct Factors {
   const T* im[kMaxFactors];
   int shift[kMaxFactors];
   uint32_t mask[kMaxFactors];
+  int tile0[kMaxFactors + 1];  // expectation pass: first block of each factor
 };
 
-template <class T>
-__device__ __forceinline__ void factor_amp(const Factors<T>& F, size_t shot, size_t x, T& r, T& m) {
+// Tile table of a factor chain for the expectation pass and its decision.
+struct FactorTiles {
+  int n;
+  int tile0[kMaxFactors + 1];
+};
+
+// A = T forms the product as a merge pass stores it; A = double keeps a
+// never-stored merged amplitude unrounded (FP32 storage rounds only what
+// it writes).
+template <class T, class A>
+__device__ __forceinline__ void factor_amp(const Factors<T>& F, size_t shot, size_t x, A& r, A& m) {
   const size_t x0 = x & F.mask[0];
-  r = F.re[0][shot + x0];
-  m = F.im[0][shot + x0];
+  r = A(F.re[0][shot + x0]);
+  m = A(F.im[0][shot + x0]);
   for (int f = 1; f < F.n; ++f) {
     const size_t xf = (x >> F.shift[f]) & F.mask[f];
-    const T cr = F.re[f][shot + xf], ci = F.im[f][shot + xf];
-    const T nr = r * cr - m * ci, ni = r * ci + m * cr;
+    const A cr = A(F.re[f][shot + xf]), ci = A(F.im[f][shot + xf]);
+    const A nr = r * cr - m * ci, ni = r * ci + m * cr;
     r = nr;
     m = ni;
   }
@@ -636,46 +652,102 @@ __global__ __launch_bounds__(kWideThreads) void wide_multimerge_kernel(Factors<T
     const size_t o = size_t(blockIdx.x) * kWideTile + size_t(u) * kWideThreads + threadIdx.x;
     if (o >= total) break;
     T r, m;
-    factor_amp(F, shot, o, r, m);
+    factor_amp<T, T>(F, shot, o, r, m);
     OR[o] = r;
     OI[o] = m;
   }
 }
 
-// P(branch = 1) partial sums from the factors (as wide_prob_kernel).
+// P(branch = 1) of a measurement of a product state |f0> (x) |f1> (x) ...
+// from its factors alone. With the measured body's masks x, z and phase
+// i^ph, every measurement kernel's sum over the output half equals
+//   P(true) = (N - Re(i^-ph T)) / 2,  N = prod_f N_f,  T = prod_f T_f,
+//   N_f = sum_b |f[b]|^2,  T_f = sum_b (-1)^|b & z_f| conj(f[b]) f[b ^ x_f]
+// (x_f, z_f: the masks' bits in factor f's index field; a diagonal body has
+// x = 0 and i^-ph = (-1)^diag_phase), so the pass reads sum_f 2^k_f
+// amplitudes instead of gathering 2^W products. Block (b, shot) covers
+// kWideTile entries of one factor and writes (N, Re T, Im T) partials.
 template <class T>
-__global__ __launch_bounds__(kWideThreads) void wide_prob_factored_kernel(DevProgram P, int instr, Factors<T> F,
-                                                                          size_t dim, int k, double* part,
-                                                                          uint32_t max_tiles) {
-  __shared__ double red[kWideThreads / 64];
+__global__ __launch_bounds__(kWideThreads) void wide_expect_factored_kernel(DevProgram P, int instr, Factors<T> F,
+                                                                            size_t dim, double* part,
+                                                                            uint32_t max_tiles) {
+  __shared__ double red[3][kWideThreads / 64];
   const uint32_t j = blockIdx.y;
   const size_t shot = size_t(j) * dim;
   const DevInstr in = P.instrs[instr];
-  const size_t out_dim = (size_t(1) << k) >> 1;
-  const bool diag = in.op & kFlagDiagonal;
-  double acc = 0.0;
+  int f = 0;
+  while (f + 1 < F.n && int(blockIdx.x) >= F.tile0[f + 1]) ++f;
+  const size_t base = size_t(int(blockIdx.x) - F.tile0[f]) * kWideTile;
+  const size_t fmask = F.mask[f];
+  const uint64_t xf = (in.xmask >> F.shift[f]) & fmask, zf = (in.zmask >> F.shift[f]) & fmask;
+  const T* R = F.re[f] + shot;
+  const T* I = F.im[f] + shot;
+  double n = 0.0, tr = 0.0, ti = 0.0;
 #pragma unroll
   for (int u = 0; u < kWidePer; ++u) {
-    const size_t idx = size_t(blockIdx.x) * kWideTile + size_t(u) * kWideThreads + threadIdx.x;
-    if (idx >= out_dim) break;
-    T r, m;
-    if (diag) {
-      factor_amp(F, shot, diagonal_source(in, idx, true), r, m);
-      acc += double(r) * double(r) + double(m) * double(m);
-    } else {
-      const size_t s0 = insert_zero_bit(idx, in.pivot), s1 = s0 ^ size_t(in.xmask);
-      T r1, m1;
-      factor_amp(F, shot, s0, r, m);
-      factor_amp(F, shot, s1, r1, m1);
-      const double dir = parity64(uint64_t(s0) & in.zmask) ? 1.0 : -1.0;
-      const double c1r = dir * in.c1_re, c1i = dir * in.c1_im;
-      const double ar = kInvSqrt2 * double(r) + c1r * double(r1) - c1i * double(m1);
-      const double ai = kInvSqrt2 * double(m) + c1r * double(m1) + c1i * double(r1);
-      acc += ar * ar + ai * ai;
-    }
+    const size_t b = base + size_t(u) * kWideThreads + threadIdx.x;
+    if (b > fmask) break;
+    const double r = double(R[b]), m = double(I[b]);
+    const double r2 = double(R[b ^ xf]), m2 = double(I[b ^ xf]);
+    const double sg = parity64(uint64_t(b) & zf) ? -1.0 : 1.0;
+    n += r * r + m * m;
+    tr += sg * (r * r2 + m * m2);
+    ti += sg * (r * m2 - m * r2);
   }
-  const double t = block_sum(acc, red);
-  if (threadIdx.x == 0) part[size_t(j) * max_tiles + blockIdx.x] = t;
+  for (int off = 32; off > 0; off >>= 1) {
+    n += __shfl_xor(n, off, 64);
+    tr += __shfl_xor(tr, off, 64);
+    ti += __shfl_xor(ti, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = n;
+    red[1][threadIdx.x >> 6] = tr;
+    red[2][threadIdx.x >> 6] = ti;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double t = 0.0;
+    for (int w = 0; w < kWideThreads / 64; ++w) t += red[threadIdx.x][w];
+    part[size_t(j) * max_tiles + 3 * blockIdx.x + threadIdx.x] = t;
+  }
+}
+
+// Per shot: the factors' partials in fixed order, P(true) as above, then
+// the decision.
+__global__ __launch_bounds__(kWideThreads) void wide_decide_factored_kernel(DevProgram P, DevLaunch L,
+                                                                            WideLayout Y, uint64_t batch_start,
+                                                                            uint32_t nshots, int instr,
+                                                                            uint64_t* bits, WideCtl* ctl,
+                                                                            const double* part, FactorTiles ft,
+                                                                            uint32_t max_tiles) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nshots) return;
+  const DevInstr in = P.instrs[instr];
+  const double* pj = part + size_t(j) * max_tiles;
+  double n = 1.0, tr = 1.0, ti = 0.0;
+  for (int f = 0; f < ft.n; ++f) {
+    double nf = 0.0, rf = 0.0, jf = 0.0;
+    for (int t = ft.tile0[f]; t < ft.tile0[f + 1]; ++t) {
+      nf += pj[3 * t];
+      rf += pj[3 * t + 1];
+      jf += pj[3 * t + 2];
+    }
+    n *= nf;
+    const double r2 = tr * rf - ti * jf, i2 = tr * jf + ti * rf;
+    tr = r2;
+    ti = i2;
+  }
+  // i^-ph: a diagonal body's phase is +-1; otherwise conj(i^ph) is
+  // sqrt(2) * c1, one of +-1, +-i
+  double er = 1.0, ei = 0.0;
+  if (in.op & kFlagDiagonal) {
+    er = (in.op & kFlagDiagPhase) ? -1.0 : 1.0;
+  } else {
+    er = in.c1_re > 0.35 ? 1.0 : (in.c1_re < -0.35 ? -1.0 : 0.0);
+    ei = in.c1_im > 0.35 ? 1.0 : (in.c1_im < -0.35 ? -1.0 : 0.0);
+  }
+  const double pt = 0.5 * (n - (er * tr - ei * ti));
+  wide_decide_shot(P, L, Y, j, batch_start + j, in, pt, bits, ctl);
 }
 
 // The collapse from the factors into the output buffer (as wide_project_kernel).
@@ -696,14 +768,14 @@ __global__ __launch_bounds__(kWideThreads) void wide_project_factored_kernel(Dev
   for (int u = 0; u < kWidePer; ++u) {
     const size_t idx = size_t(blockIdx.x) * kWideTile + size_t(u) * kWideThreads + threadIdx.x;
     if (idx >= out_dim) break;
-    T r, m;
+    double r, m;
     if (diag) {
       factor_amp(F, shot, diagonal_source(in, idx, branch), r, m);
-      OR[idx] = T(double(r) * inv);
-      OI[idx] = T(double(m) * inv);
+      OR[idx] = T(r * inv);
+      OI[idx] = T(m * inv);
     } else {
       const size_t s0 = insert_zero_bit(idx, in.pivot), s1 = s0 ^ size_t(in.xmask);
-      T r1, m1;
+      double r1, m1;
       factor_amp(F, shot, s0, r, m);
       factor_amp(F, shot, s1, r1, m1);
       const bool odd = parity64(uint64_t(s0) & in.zmask);
@@ -835,6 +907,15 @@ static hipError_t wide_workspace_reserve(WideWorkspace& ws, size_t batch, size_t
   return hipSuccess;
 }
 
+// Probability partials per shot: one per tile of a measurement's output
+// half, or three per tile of a factor chain's expectation pass.
+uint32_t wide_partials_per_shot(int max_local_k) {
+  const size_t full = size_t(1) << max_local_k;
+  const size_t plain = (full / 2 + kWideTile - 1) / kWideTile + 1;
+  const size_t factored = 3 * ((full + kWideTile - 1) / kWideTile + kMaxFactors);
+  return uint32_t(std::max(plain, factored));
+}
+
 size_t wide_bytes_per_shot(uint32_t state_dim, int fp32, const WideLayout& y) {
   size_t dim = state_dim;
   size_t t = fp32 ? 4 : 8;
@@ -929,16 +1010,23 @@ static hipError_t run_wide(const DevProgram& P, const DevInstr* instrs, const De
         }
         const DevInstr* next = e2 < P.ninstr ? &instrs[e2] : nullptr;
         if (next && (next->op & 0xff) == kOpMeasure && next->comp == in.comp && next->nq == W) {
-          // merge-and-measure: never materialize the merged state
+          // merge-and-measure: never materialize the merged state; the
+          // probability comes from the factors' expectations
+          FactorTiles ft{};
+          ft.n = F.n;
+          for (int f = 0; f < F.n; ++f)
+            ft.tile0[f + 1] = ft.tile0[f] + int((size_t(F.mask[f]) + kWideTile) / kWideTile);
+          for (int f = 0; f <= F.n; ++f) F.tile0[f] = ft.tile0[f];
+          if (uint32_t(3 * ft.tile0[F.n]) > Y.max_tiles) return hipErrorInvalidValue;
+          hipLaunchKernelGGL(wide_expect_factored_kernel<T>, dim3(unsigned(ft.tile0[F.n]), n), dim3(kWideThreads), 0,
+                             stream, P, e2, F, dim, ws.part, Y.max_tiles);
+          WCK(hipGetLastError());
+          hipLaunchKernelGGL(wide_decide_factored_kernel, sg, sb, 0, stream, P, L, Y, start, n, e2, ws.bits, ws.ctl,
+                             ws.part, ft, Y.max_tiles);
+          WCK(hipGetLastError());
           size_t out_dim = (size_t(1) << W) >> 1;
           unsigned tiles = unsigned((out_dim + kWideTile - 1) / kWideTile);
           dim3 g(tiles, n);
-          hipLaunchKernelGGL(wide_prob_factored_kernel<T>, g, dim3(kWideThreads), 0, stream, P, e2, F, dim, W,
-                             ws.part, Y.max_tiles);
-          WCK(hipGetLastError());
-          hipLaunchKernelGGL(wide_decide_kernel, sg, sb, 0, stream, P, L, Y, start, n, e2, ws.bits, ws.ctl, ws.part,
-                             tiles, Y.max_tiles);
-          WCK(hipGetLastError());
           hipLaunchKernelGGL(wide_project_factored_kernel<T>, g, dim3(kWideThreads), 0, stream, P, e2, ws.ctl, F,
                              RE(cb ^ 1), IM(cb ^ 1), dim, W);
           WCK(hipGetLastError());
