@@ -82,6 +82,8 @@ ticit_hip_status create_from_program(Program prog, const ticit_hip_options* opti
   } while (0)
   UP(instrs, t.instrs);
   UP(steps, t.steps);
+  UP(lsteps, t.lsteps);
+  UP(bsteps, t.bsteps);
   UP(resid, t.resid);
   UP(det_recs, t.det_recs);
   UP(obs_off, t.obs_off);
@@ -120,6 +122,7 @@ ticit_hip_status create_from_program(Program prog, const ticit_hip_options* opti
   d.ncomp = int32_t(t.comp_off.size());
   d.max_local_k = t.max_local_k;
   d.nexp = t.nexp;
+  d.nlsteps = t.nlsteps;
   e = hipMalloc(reinterpret_cast<void**>(&h->d_error), sizeof(int));
   if (e != hipSuccess) {
     delete h;

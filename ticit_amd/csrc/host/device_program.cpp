@@ -7,6 +7,7 @@
 #include <cmath>
 #include <map>
 #include <set>
+#include <unordered_map>
 
 #include "component_plan.hpp"
 #include "frontend.hpp"
@@ -37,6 +38,81 @@ double gap_denominator(double p) {
 
 void check_probability(double p) {
   if (!(p >= 0.0 && p <= 1.0)) throw TicitError(1, "probability must be between 0 and 1");
+}
+
+// The lane tier's schedule (a list scheduler over the compact steps): a
+// record is deferred with level 1 + the highest level among the pending
+// records whose slots it reads; an in-order step that reads a pending
+// record's slot first flushes every pending record, level by level (a
+// level's records read only earlier levels and in-order steps, so they run
+// side by side); detectors read only records and slots and run last.
+void build_lane_schedule(HostTables& t) {
+  t.lsteps.clear();
+  t.bsteps.clear();
+  std::vector<std::pair<int, DevStep>> pending;  // (level, record)
+  std::unordered_map<int32_t, int> slot_level;   // slot -> level of its pending record
+  std::vector<DevStep> dets;
+  auto flush = [&]() {
+    if (pending.empty()) return;
+    std::stable_sort(pending.begin(), pending.end(),
+                     [](const std::pair<int, DevStep>& x, const std::pair<int, DevStep>& y) { return x.first < y.first; });
+    size_t i = 0;
+    while (i < pending.size()) {
+      size_t j = i;
+      while (j < pending.size() && pending[j].first == pending[i].first) ++j;
+      DevStep b{};
+      b.op = kOpRecBatch;
+      b.partial = -1;
+      b.a = int32_t(t.bsteps.size());
+      b.b = int32_t(j - i);
+      for (size_t q = i; q < j; ++q) t.bsteps.push_back(pending[q].second);
+      t.lsteps.push_back(b);
+      i = j;
+    }
+    pending.clear();
+    slot_level.clear();
+  };
+  auto reads_pending = [&](const DevStep& s, int* level) {
+    bool any = false;
+    *level = 0;
+    for (uint32_t j = 0; j < s.resid_len; ++j) {
+      auto it = slot_level.find(int32_t(t.resid[s.resid_off + j]));
+      if (it != slot_level.end()) {
+        any = true;
+        *level = std::max(*level, it->second);
+      }
+    }
+    return any;
+  };
+  for (const DevStep& s : t.steps) {
+    const uint32_t op = s.op & 0xff;
+    int level = 0;
+    if (op == kOpRecord && !(s.op & kFlagExpVal)) {
+      reads_pending(s, &level);
+      pending.emplace_back(level + 1, s);
+      if (s.b >= 0) slot_level[s.b] = level + 1;
+      continue;
+    }
+    if (op == kOpDetector) {
+      dets.push_back(s);
+      continue;
+    }
+    if (reads_pending(s, &level)) flush();
+    t.lsteps.push_back(s);
+  }
+  flush();
+  if (!dets.empty()) {
+    DevStep b{};
+    b.op = kOpDetBatch;
+    b.partial = -1;
+    b.a = int32_t(t.bsteps.size());
+    b.b = int32_t(dets.size());
+    t.bsteps.insert(t.bsteps.end(), dets.begin(), dets.end());
+    t.lsteps.push_back(b);
+  }
+  t.nlsteps = int32_t(t.lsteps.size());
+  if (t.lsteps.empty()) t.lsteps.push_back(DevStep{});
+  if (t.bsteps.empty()) t.bsteps.push_back(DevStep{});
 }
 
 }  // namespace
@@ -470,6 +546,7 @@ HostTables lower_to_device(const Program& p, bool components) {
     }
     t.steps.push_back(s);
   }
+  build_lane_schedule(t);
   // keep every table non-empty so device pointers are always valid
   if (t.resid.empty()) t.resid.push_back(0);
   if (t.det_recs.empty()) t.det_recs.push_back(0);

@@ -12,6 +12,13 @@ namespace ticit {
 struct HostTables {
   std::vector<DevInstr> instrs;
   std::vector<DevStep> steps;  // compact view, one per instr
+  // Lane-tier schedule: the steps in order, except that records wait until
+  // an in-order step reads a slot one of them assigns (then they run as
+  // dependency levels, 64 side by side) and detectors run last, together.
+  // Every runtime slot is assigned once and never reused, so a deferred
+  // step reads the values it would have read in order.
+  std::vector<DevStep> lsteps, bsteps;
+  int32_t nlsteps = 0;  // lsteps without the padding entry
   std::vector<uint32_t> resid, det_recs, obs_off, obs_recs, sym_off, sym_part;
   std::vector<int32_t> exo_syms;
   std::vector<DevCat> cats;

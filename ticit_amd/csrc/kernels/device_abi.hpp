@@ -20,6 +20,10 @@ enum : uint32_t {
   kOpMeasure = 4,
   kOpCoin = 5,
   kOpMerge = 6,  // device only: component src folded into component comp
+  // lane-tier schedule only: steps bsteps[a, a + b) evaluated side by side,
+  // one per lane (records of one dependency level, or the detectors)
+  kOpRecBatch = 7,
+  kOpDetBatch = 8,
 };
 
 enum : uint32_t {
@@ -129,6 +133,8 @@ struct DevLowp {      // low-probability Bernoulli group
 struct DevProgram {
   const DevInstr* instrs;
   const DevStep* steps;       // [ninstr] compact view of instrs
+  const DevStep* lsteps;      // [nlsteps] lane-tier schedule: steps and batch steps
+  const DevStep* bsteps;      // records and detectors the batch steps cover
   const uint32_t* resid;      // runtime slot ids
   const uint32_t* det_recs;   // 0-based record ids
   const uint32_t* obs_off;    // nobs + 1 offsets into obs_recs
@@ -152,6 +158,7 @@ struct DevProgram {
   int32_t ncomp;        // components (1 dense)
   int32_t max_local_k;  // widest component (max_k dense)
   int32_t nexp;         // expectation slots per shot
+  int32_t nlsteps;
 };
 
 // Per-launch sampling request.

@@ -87,26 +87,42 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
   int k = P.initial_k;
   uint64_t brng = seed_for(kBranchSeedBase, launch_seed(L), shot);
   int err = 0;
-  for (int i = 0; i < P.ninstr; ++i) {
-    // the compact step drives the loop; amplitude work reads the full record
-    const DevStep st = P.steps[i];
+  for (int i = 0; i < P.nlsteps; ++i) {
+    // the lane schedule drives the loop; amplitude work reads the full record
+    const DevStep st = P.lsteps[i];
     const uint32_t op = st.op & 0xff;
+    if (op == kOpRecBatch || op == kOpDetBatch) {
+      // independent records (one dependency level) or the detectors, one
+      // step per lane; each lane's step updates the words of all 64 shots
+      for (int32_t base = st.a; base < st.a + st.b; base += 64) {
+        const int32_t j = base + lane;
+        if (j < st.a + st.b) {
+          const DevStep q = P.bsteps[j];
+          uint64_t v = 0;
+          if (op == kOpDetBatch && (q.op & kFlagDetRecords)) {
+            for (int32_t r = 0; r < q.c; ++r) v ^= Rw[P.det_recs[q.b + r]];
+          } else {
+            v = (q.op & kFlagConstant) ? ~0ull : 0ull;
+            if (q.partial >= 0) v ^= Pw[q.partial];
+            for (uint32_t r = 0; r < q.resid_len; ++r) v ^= Sw[P.resid[q.resid_off + r]];
+          }
+          if (op == kOpDetBatch) {
+            Dw[q.a] = v;
+          } else {
+            if (q.a >= 0) Rw[q.a] = v;
+            if (q.b >= 0) Sw[q.b] = v;
+          }
+        }
+      }
+      __syncthreads();  // the next level reads these words
+      continue;
+    }
     if (op == kOpRecord) {
       if (st.op & kFlagExpVal) {
         if (expv) expv[st.a] = ((eval(st) >> lane) & 1) ? -1.0 : 1.0;
         continue;
       }
       write_outcome(st, eval(st));
-      continue;
-    }
-    if (op == kOpDetector) {
-      uint64_t v = 0;
-      if (st.op & kFlagDetRecords) {
-        for (int32_t j = 0; j < st.c; ++j) v ^= Rw[P.det_recs[st.b + j]];
-      } else {
-        v = eval(st);
-      }
-      Dw[st.a] = v;
       continue;
     }
     if (op == kOpCoin) {
