@@ -22,6 +22,22 @@ namespace ticit {
 // Lane tier
 // --------------------------------------------------------------------------
 
+// 64 x 64 bit transpose across a wave: lane j holds row j on entry; on exit
+// lane l holds column l (bit j = bit l of row j). Six butterfly stages, each
+// swapping the off-diagonal s x s blocks between lanes l and l ^ s.
+__device__ __forceinline__ uint64_t transpose64(uint64_t x, int lane) {
+  const uint64_t mask[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+                            0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+  for (int i = 0, sh = 32; i < 6; ++i, sh >>= 1) {
+    const uint32_t lo = uint32_t(__shfl_xor(int(uint32_t(x)), sh, 64));
+    const uint32_t hi = uint32_t(__shfl_xor(int(uint32_t(x >> 32)), sh, 64));
+    const uint64_t p = (uint64_t(hi) << 32) | lo;
+    x = (lane & sh) ? ((x & ~mask[i]) | ((p & ~mask[i]) >> sh)) : ((x & mask[i]) | ((p & mask[i]) << sh));
+  }
+  return x;
+}
+
 template <int KMAX>
 __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L, LaneLayout Y) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -277,31 +293,34 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
   }
   if (!L.keep_records) return;
 
-  // Row packing (src/sampler/prepared.rs:909-941): each lane builds its
-  // shot's row from the sliced words; rows are staged in LDS so the wave
-  // writes one contiguous block per table.
-  uint8_t* row = stage + size_t(lane) * (Y.stage_bytes / 64);
+  // Row packing (src/sampler/prepared.rs:909-941). Columns go 64 at a time:
+  // lane j loads column c0 + j's word (bit l = shot l), a 64 x 64 bit
+  // transpose across the wave leaves shot l's bits of those 64 columns in
+  // lane l (bit j = column c0 + j: eight packed row bytes, little bit
+  // order), and the rows are staged contiguously in LDS so the wave writes
+  // its block of rows as 8-byte words.
   auto emit = [&](uint8_t* out, uint32_t row_bytes, auto&& word_at, uint32_t ncols) {
     if (row_bytes == 0) return;
-    if (L.bit_packed) {
-      for (uint32_t byte = 0; byte < row_bytes; ++byte) {
-        uint32_t v = 0;
-        for (uint32_t bb = 0; bb < 8; ++bb) {
-          uint32_t col = byte * 8 + bb;
-          if (col < ncols && ((word_at(col) >> lane) & 1)) v |= 1u << bb;
-        }
-        row[byte] = uint8_t(v);
+    uint8_t* row = stage + size_t(lane) * row_bytes;
+    for (uint32_t c0 = 0; c0 < ncols; c0 += 64) {
+      const uint32_t c = c0 + uint32_t(lane);
+      const uint64_t x = transpose64(c < ncols ? word_at(c) : 0ull, lane);
+      if (L.bit_packed) {
+        const uint32_t b0 = c0 / 8;
+        for (uint32_t q = 0; q < 8 && b0 + q < row_bytes; ++q) row[b0 + q] = uint8_t(x >> (8 * q));
+      } else {
+        for (uint32_t q = 0; q < 64 && c0 + q < ncols; ++q) row[c0 + q] = uint8_t((x >> q) & 1);
       }
-    } else {
-      for (uint32_t col = 0; col < ncols; ++col) row[col] = uint8_t((word_at(col) >> lane) & 1);
     }
     __syncthreads();
     const size_t total = size_t(nlive) * row_bytes;
     uint8_t* dst = out + (uint64_t(blockIdx.x) * 64) * row_bytes;
-    const uint32_t pitch = Y.stage_bytes / 64;
-    for (size_t b = lane; b < total; b += 64) {
-      size_t r = b / row_bytes, cidx = b - r * row_bytes;
-      dst[b] = stage[r * pitch + cidx];
+    if (!(total & 7) && !(reinterpret_cast<uintptr_t>(dst) & 7)) {
+      const uint2* src8 = reinterpret_cast<const uint2*>(stage);
+      uint2* dst8 = reinterpret_cast<uint2*>(dst);
+      for (size_t b = lane; b < total / 8; b += 64) dst8[b] = src8[b];
+    } else {
+      for (size_t b = lane; b < total; b += 64) dst[b] = stage[b];
     }
     __syncthreads();
   };
@@ -327,7 +346,7 @@ static hipError_t launch_lane_k(const DevProgram& P, const DevLaunch& L, const L
   LaneLayout Y = Yp;
   uint32_t row = L.keep_records ? (L.meas_row > L.det_row ? L.meas_row : L.det_row) : 0;
   row = row > L.obs_row ? row : L.obs_row;
-  Y.stage_bytes = 64 * ((row + 15) / 16 * 16);
+  Y.stage_bytes = 64 * ((row + 7) / 8 * 8);
   size_t lds = lane_tier_lds_bytes(K, Y);
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&lane_tier_kernel<K>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
@@ -342,7 +361,7 @@ static hipError_t launch_lane_k(const DevProgram& P, const DevLaunch& L, const L
 template <int K>
 static void lane_occupancy_k(const LaneLayout& Yp, uint32_t row_bytes, uint32_t* lds, uint32_t* blocks) {
   LaneLayout Y = Yp;
-  Y.stage_bytes = 64 * ((row_bytes + 15) / 16 * 16);
+  Y.stage_bytes = 64 * ((row_bytes + 7) / 8 * 8);
   *lds = uint32_t(lane_tier_lds_bytes(K, Y));
   int b = 0;
   if (hipFuncSetAttribute(reinterpret_cast<const void*>(&lane_tier_kernel<K>),
