@@ -49,7 +49,7 @@ typedef struct ticit_hip_options {
   uint32_t components;   /* 0: as planned (use_active_components, src/planner.rs:1482-1484);
                             1: dense active state; 2: tensor components
                             (src/component_plan.rs, src/sampler/batch/runtime.rs:388-552) */
-  uint32_t group_threads; /* group tier threads per shot: 64, 128, 256 or 512 (0 = 256) */
+  uint32_t group_threads; /* group tier threads per shot: 8, 16, 32, 64, 128, 256 or 512 (0 = 256) */
   uint32_t reserved;
 } ticit_hip_options;
 
@@ -207,7 +207,8 @@ typedef struct ticit_hip_batch_info {
   uint32_t bucket_items[16];
   uint32_t bucket_region_bytes[16]; /* LDS bytes per shot */
   uint32_t bucket_grid[16];         /* persistent workgroups */
-  double bucket_tuned_ms[16][4];    /* timing-run ms for 64/128/256/512 threads per shot (0 = not run) */
+  double bucket_tuned_ms[16][8];    /* timing-run ms for 8, 16, ..., 512 threads per shot, index
+                                       log2(threads) - 3 (0 = not run) */
 } ticit_hip_batch_info;
 
 ticit_hip_status ticit_hip_batch_create(const ticit_hip_batch_item* items, size_t count,

@@ -162,11 +162,12 @@ def test_device_entry_point_split_launches(hip):
     assert int(ones.item()) == int(ref.observable_ones[0])
 
 
-@pytest.mark.parametrize("tps", [64, 128, 256, 512])
+@pytest.mark.parametrize("tps", [8, 16, 32, 64, 128, 256, 512])
 def test_group_threads_per_shot_give_identical_rows(hip, tps):
-    """Packing 4, 2 or 1 shots per workgroup (or one per 512 threads) changes the launch shape and
-    the reduction tree of a shot's probability sums (last-bit differences),
-    not the rows: they equal the oracle's."""
+    """Packing 8, 4 or 2 shots per wave, or one shot per workgroup of 64 to
+    512 threads, changes the launch shape and the reduction tree of a shot's
+    probability sums (last-bit differences), not the rows: they equal the
+    oracle's, and P(branch = 1) agrees to 1e-12."""
     c = hip.Circuit(w.config2_batch()[94][0])  # component layout, local k 6
     s = c.compile(tier="group", group_threads=tps)
     shots = 333  # not a multiple of any shots-per-workgroup
@@ -174,6 +175,13 @@ def test_group_threads_per_shot_give_identical_rows(hip, tps):
     o = O.sample_per_shot(c.blob, shots, 5)
     assert np.array_equal(r.measurements, o["measurements"])
     assert r.observable_ones.tolist() == o["observable_ones"].tolist()
+    rng = np.random.default_rng(tps)
+    uni = rng.random((s.num_measure_instructions, shots))
+    coins = rng.integers(0, 1 << 62, size=(max(1, s.num_coin_instructions), (shots + 63) // 64), dtype=np.uint64)
+    ri = s.sample_injected(shots, uniforms=uni, coins=coins, want_probs=True)
+    oi = O.sample_injected(c.blob, shots, None, uni, coins, want_probs=True)
+    assert np.array_equal(ri.measurements, oi["measurements"])
+    assert np.max(np.abs(ri.probabilities - oi["probs"])) <= PROB_TOL
 
 
 def test_group_tier_holds_a_width_13_state(hip):

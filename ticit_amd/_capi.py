@@ -66,7 +66,7 @@ class BatchInfo(ctypes.Structure):
     _fields_ = [("items", ctypes.c_uint64), ("lane_items", ctypes.c_uint64), ("wide_items", ctypes.c_uint64),
                 ("group_buckets", ctypes.c_uint64), ("bucket_tps", ctypes.c_uint32 * 16),
                 ("bucket_items", ctypes.c_uint32 * 16), ("bucket_region_bytes", ctypes.c_uint32 * 16),
-                ("bucket_grid", ctypes.c_uint32 * 16), ("bucket_tuned_ms", (ctypes.c_double * 4) * 16)]
+                ("bucket_grid", ctypes.c_uint32 * 16), ("bucket_tuned_ms", (ctypes.c_double * 8) * 16)]
 
 
 # ticit_hip_status -> exception kinds of TicitError (src/errors.rs:9-57)

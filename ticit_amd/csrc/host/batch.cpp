@@ -4,9 +4,10 @@
 // Programs are grouped by the tier they run on:
 //   * group tier: buckets of programs with similar per-shot LDS footprints,
 //     each bucket one persistent kernel that pulls (program, shot block)
-//     work units from an atomic counter; threads per shot (64, 128, 256 or
-//     512, one shot per workgroup) are picked per bucket by timing each
-//     candidate on a sample of the bucket's shots;
+//     work units from an atomic counter; threads per shot (8 to 512: 8, 4
+//     or 2 shots per wave below 64, else one shot per workgroup) are picked
+//     per bucket by timing each candidate on a sample of the bucket's
+//     shots;
 //   * lane tier: one launch per program;
 //   * wide tier: each program's kernel sequence, in a second graph replayed
 //     on a second stream.
@@ -35,7 +36,7 @@ struct ticit_hip_batch {
     std::vector<int> candidates;
     int tps = 256;
     int grid = 0;
-    double tuned_ms[4] = {0, 0, 0, 0};  // sample time per candidate (64, 128, 256, 512)
+    double tuned_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // sample time per candidate (8 ... 512, by log2 - 3)
     DevProgram* d_progs = nullptr;
     DevLaunch* d_launches = nullptr;
     GroupLayout* d_layouts = nullptr;
@@ -101,7 +102,11 @@ struct ticit_hip_batch {
 
 namespace {
 
-int tps_slot(int tps) { return tps == 64 ? 0 : (tps == 128 ? 1 : (tps == 256 ? 2 : 3)); }
+int tps_slot(int tps) {
+  int s = 0;
+  while ((8 << s) < tps) ++s;
+  return s;
+}
 
 // Tuned threads-per-shot by bucket signature, kept for the process.
 std::mutex g_tune_mu;
@@ -321,7 +326,7 @@ ticit_hip_status ticit_hip_batch_create(const ticit_hip_batch_item* items, size_
 
   // Group buckets: candidates, tables, tuning.
   for (auto& k : b->buckets) {
-    for (int tps : {64, 128, 256, 512})
+    for (int tps : {8, 16, 32, 64, 128, 256, 512})
       if (k.max_local_k <= group_max_local_k(tps) &&
           64 + size_t(group_shots_per_block(tps)) * k.region <= size_t(lds_limit))
         k.candidates.push_back(tps);
@@ -455,7 +460,7 @@ ticit_hip_status ticit_hip_batch_info_get(const ticit_hip_batch* b, ticit_hip_ba
     out->bucket_items[j] = uint32_t(k.items.size());
     out->bucket_region_bytes[j] = k.region;
     out->bucket_grid[j] = uint32_t(k.grid);
-    for (int c = 0; c < 4; ++c) out->bucket_tuned_ms[j][c] = k.tuned_ms[c];
+    for (int c = 0; c < 8; ++c) out->bucket_tuned_ms[j][c] = k.tuned_ms[c];
   }
   return TICIT_HIP_OK;
 }

@@ -150,9 +150,9 @@ ticit_hip_status create_from_program(Program prog, const ticit_hip_options* opti
   // states fit a 512-thread shot's LDS, but run slower there than in the
   // wide tier (one shot per CU), so they go wide unless asked for.
   int tps = h->opt.group_threads ? int(h->opt.group_threads) : 256;
-  if (tps != 64 && tps != 128 && tps != 256 && tps != 512) {
+  if (tps != 8 && tps != 16 && tps != 32 && tps != 64 && tps != 128 && tps != 256 && tps != 512) {
     delete h;
-    return fail(TICIT_HIP_INVALID_INPUT, "group_threads must be 64, 128, 256 or 512");
+    return fail(TICIT_HIP_INVALID_INPUT, "group_threads must be 8, 16, 32, 64, 128, 256 or 512");
   }
   h->group_tps = tps;
   const bool group_ok = group_fits(tps);

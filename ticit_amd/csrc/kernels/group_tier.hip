@@ -1,9 +1,11 @@
-// Group tier: one shot per workgroup of TPS threads (64, 128, 256 or 512),
-// the shot's active state (split-complex FP64) and bit tables resident in
-// LDS. The shot's amplitude work is spread over its TPS threads; a 64-thread
-// shot is one wave, so its barriers never wait on another shot. TPS = 256 is
-// the default launch; 64 and 128 suit small states (more shots per CU), 512
-// holds states of 2^13 (128 KiB FP64).
+// Group tier: every shot's active state (split-complex FP64) and bit tables
+// resident in LDS, its amplitude work spread over TPS threads. TPS of 64 to
+// 512 is one shot per workgroup (a 64-thread shot is one wave, so its
+// barriers never wait on another shot); TPS of 8, 16 or 32 packs 8, 4 or 2
+// shots into one wave, reduced with shuffles inside each lane group, which
+// spreads the interpreter's per-instruction work over several shots for
+// small states. TPS = 256 is the default launch; 512 holds states of 2^13
+// (128 KiB FP64).
 // Projections go through registers, kProjPer outputs per thread, so a shot
 // group of TPS threads handles local widths up to log2(2 * kProjPer * TPS).
 //
@@ -20,8 +22,9 @@
 
 namespace ticit {
 
-// Workgroup size for a shot of TPS threads: one shot per workgroup.
-constexpr int group_block(int tps) { return tps; }
+// Workgroup size for shots of TPS threads: one wave for sub-wave shots,
+// else one shot per workgroup.
+constexpr int group_block(int tps) { return tps < 64 ? 64 : tps; }
 // Shots one work unit of the persistent batch kernel runs in sequence: as
 // many as a 256-thread group would hold, so the unit counter is taken once
 // per 256 threads' worth of shots.
@@ -38,13 +41,13 @@ __host__ __device__ inline size_t group_shot_bytes(uint32_t state_dim, const Gro
   return size_t(state_dim) * 16 + size_t(y.pw + y.sw + y.rw + y.dw) * 8;
 }
 
-// Runs shot first_local of launch L on the workgroup (group_block(TPS) / TPS
-// = 1 shot).
+// Runs shots [first_local, first_local + group_block(TPS) / TPS) of launch L
+// on the workgroup (one shot unless TPS < 64).
 template <int TPS>
 __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const GroupLayout& Y, uint64_t first_local,
                             unsigned char* smem, size_t region_bytes) {
   constexpr int SPB = group_block(TPS) / TPS;
-  constexpr int WPG = TPS / 64;  // waves per shot group
+  constexpr int WPG = TPS / 64;  // waves per shot group (0: a group inside one wave)
   static_assert(SPB * WPG * sizeof(double) <= 64, "reduction slots exceed the reserved 64 bytes");
   const int tid = threadIdx.x;
   const int g = tid / TPS;       // shot group
@@ -236,11 +239,19 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
             part += ar * ar + ai * ai;
           }
         }
-        part = group_wave_sum(part);
-        if ((tid & 63) == 0) red[tid >> 6] = part;
-        __syncthreads();
         double pt = 0.0;
-        for (int w = 0; w < WPG; ++w) pt += red[g * WPG + w];
+        if constexpr (WPG == 0) {
+          // lanes [g * TPS, (g + 1) * TPS) of the wave: a butterfly inside
+          // the group leaves the sum in each of them
+          for (int off = TPS / 2; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+          pt = part;
+          __syncthreads();
+        } else {
+          part = group_wave_sum(part);
+          if ((tid & 63) == 0) red[tid >> 6] = part;
+          __syncthreads();
+          for (int w = 0; w < WPG; ++w) pt += red[g * WPG + w];
+        }
         pt = clamp01(pt);
         if (st.op & kFlagExpVal) {
           // src/sampler/batch/active.rs:818-850: +-(1 - 2 P(true)), no collapse
@@ -396,8 +407,9 @@ __global__ __launch_bounds__(group_block(TPS)) __attribute__((amdgpu_waves_per_e
     const DevLaunch& L = launches[item];
     const GroupLayout& Y = layouts[item];
     constexpr int SPU = group_unit_shots(TPS);
+    constexpr int SPB = group_block(TPS) / TPS;
     const uint64_t first = (u - unit_off[item]) * SPU;
-    for (int q = 0; q < SPU; ++q) {
+    for (int q = 0; q < SPU; q += SPB) {
       if (first + uint64_t(q) >= L.shots) break;  // uniform: one item per unit
       group_shots<TPS>(P, L, Y, first + uint64_t(q), smem, region_bytes);
       __syncthreads();
@@ -459,6 +471,9 @@ hipError_t launch_group_tier(const DevProgram& P, const DevLaunch& L, const Grou
   if (P.max_local_k > group_max_local_k(tps)) return hipErrorInvalidValue;
   if (L.shots == 0) return hipSuccess;
   switch (tps) {
+    case 8: return launch_group_t<8>(P, L, Y, stream);
+    case 16: return launch_group_t<16>(P, L, Y, stream);
+    case 32: return launch_group_t<32>(P, L, Y, stream);
     case 64: return launch_group_t<64>(P, L, Y, stream);
     case 128: return launch_group_t<128>(P, L, Y, stream);
     case 256: return launch_group_t<256>(P, L, Y, stream);
@@ -488,6 +503,15 @@ hipError_t launch_group_batch(const DevProgram* progs, const DevLaunch* launches
                               unsigned long long* counter, uint32_t region_bytes, int tps, int grid,
                               hipStream_t stream) {
   switch (tps) {
+    case 8:
+      return launch_group_batch_t<8>(progs, launches, layouts, unit_off, nitems, total_units, counter, region_bytes,
+                                     grid, stream);
+    case 16:
+      return launch_group_batch_t<16>(progs, launches, layouts, unit_off, nitems, total_units, counter, region_bytes,
+                                      grid, stream);
+    case 32:
+      return launch_group_batch_t<32>(progs, launches, layouts, unit_off, nitems, total_units, counter, region_bytes,
+                                      grid, stream);
     case 64:
       return launch_group_batch_t<64>(progs, launches, layouts, unit_off, nitems, total_units, counter, region_bytes,
                                       grid, stream);
@@ -509,7 +533,10 @@ hipError_t launch_group_batch(const DevProgram* progs, const DevLaunch* launches
 int group_batch_occupancy(int tps, uint32_t region_bytes) {
   size_t lds = 64 + size_t(group_shots_per_block(tps)) * region_bytes;
   int blocks = 0;
-  const void* fn = tps == 64    ? reinterpret_cast<const void*>(&group_batch_kernel<64>)
+  const void* fn = tps == 8     ? reinterpret_cast<const void*>(&group_batch_kernel<8>)
+                   : tps == 16  ? reinterpret_cast<const void*>(&group_batch_kernel<16>)
+                   : tps == 32  ? reinterpret_cast<const void*>(&group_batch_kernel<32>)
+                   : tps == 64  ? reinterpret_cast<const void*>(&group_batch_kernel<64>)
                    : tps == 128 ? reinterpret_cast<const void*>(&group_batch_kernel<128>)
                    : tps == 256 ? reinterpret_cast<const void*>(&group_batch_kernel<256>)
                                 : reinterpret_cast<const void*>(&group_batch_kernel<512>);
