@@ -315,6 +315,15 @@ def other_configs(ticit_amd, w, torch, np, stream, use_cpu=True) -> dict:
                  "regret_learned": ev["regret_learned"], "regret_heuristic": ev["regret_heuristic"],
                  "accuracy_learned": ev["accuracy_learned"], "accuracy_heuristic": ev["accuracy_heuristic"],
                  "train_programs": int(m["n_train"])}
+    if use_cpu:
+        sel = list(range(0, len(blobs), 32))
+        n_cpu, t_cpu = 0, 0.0
+        for i in sel:
+            r = O.sample(blobs[i], 2000, 5, threads=cores, fast=True)
+            n_cpu += 2000
+            t_cpu += r["sample_s"]
+        out["C4"]["cpu_baseline"] = {"value": n_cpu / t_cpu, "unit": "shots/s", "cores": cores, "kind": "port",
+                                     "sample": f"held-out programs {sel[0]}..{sel[-1]} step 32, 2000 shots each"}
     return out
 
 

@@ -594,6 +594,82 @@ __global__ __launch_bounds__(kWideThreads) void wide_project_kernel(DevProgram P
   }
 }
 
+// Collapse of measurement `instr` fused with the probability pass of the
+// measurement `next` that follows it on the same component (width k - 1):
+// block (b, shot) covers next's output pairs [b * kWideTile / 2, ...); for
+// each it forms the two collapsed amplitudes next pairs up (s0', s0' ^ x'
+// for a nondiagonal body, both halves' sources for a diagonal one), stores
+// them, and adds next's P(branch = 1) term from the stored values, so the
+// collapsed state is not read back for the next probability.
+template <class T>
+__global__ __launch_bounds__(kWideThreads) void wide_project_measure_kernel(DevProgram P, int instr, int next,
+                                                                            const WideCtl* ctl, const T* re,
+                                                                            const T* im, T* ore, T* oim,
+                                                                            size_t dim, int k, double* part,
+                                                                            uint32_t max_tiles) {
+  __shared__ double red[kWideThreads / 64];
+  const uint32_t j = blockIdx.y;
+  const DevInstr in = P.instrs[instr];
+  const DevInstr nx = P.instrs[next];
+  const bool branch = ctl[j].branch;
+  const double inv = ctl[j].invnorm;
+  const T* R = re + size_t(j) * dim + in.comp_off;
+  const T* I = im + size_t(j) * dim + in.comp_off;
+  T* OR = ore + size_t(j) * dim + in.comp_off;
+  T* OI = oim + size_t(j) * dim + in.comp_off;
+  const bool diag = in.op & kFlagDiagonal, ndiag = nx.op & kFlagDiagonal;
+  // collapsed amplitude idx of `in`, rounded as stored
+  auto collapsed = [&](size_t idx, T& r, T& m) {
+    if (diag) {
+      const size_t s = diagonal_source(in, idx, branch);
+      r = T(double(R[s]) * inv);
+      m = T(double(I[s]) * inv);
+    } else {
+      const size_t s0 = insert_zero_bit(idx, in.pivot), s1 = s0 ^ size_t(in.xmask);
+      const bool odd = parity64(uint64_t(s0) & in.zmask);
+      const double dir = (branch != odd) ? -1.0 : 1.0;
+      const double c1r = dir * in.c1_re, c1i = dir * in.c1_im;
+      const double ar = kInvSqrt2 * double(R[s0]) + c1r * double(R[s1]) - c1i * double(I[s1]);
+      const double ai = kInvSqrt2 * double(I[s0]) + c1r * double(I[s1]) + c1i * double(R[s1]);
+      r = T(ar * inv);
+      m = T(ai * inv);
+    }
+  };
+  const size_t pairs = ((size_t(1) << k) >> 1) >> 1;  // next's output half
+  double acc = 0.0;
+#pragma unroll
+  for (int u = 0; u < kWidePairs; ++u) {
+    const size_t q = size_t(blockIdx.x) * (kWideTile / 2) + size_t(u) * kWideThreads + threadIdx.x;
+    if (q >= pairs) break;
+    size_t a, b;  // a: the amplitude next's P(true) reads (nondiagonal: with b)
+    if (ndiag) {
+      a = diagonal_source(nx, q, true);
+      b = diagonal_source(nx, q, false);
+    } else {
+      a = insert_zero_bit(q, nx.pivot);
+      b = a ^ size_t(nx.xmask);
+    }
+    T ra, ma, rb, mb;
+    collapsed(a, ra, ma);
+    collapsed(b, rb, mb);
+    OR[a] = ra;
+    OI[a] = ma;
+    OR[b] = rb;
+    OI[b] = mb;
+    if (ndiag) {
+      acc += double(ra) * double(ra) + double(ma) * double(ma);
+    } else {
+      const double dir = parity64(uint64_t(a) & nx.zmask) ? 1.0 : -1.0;
+      const double c1r = dir * nx.c1_re, c1i = dir * nx.c1_im;
+      const double ar = kInvSqrt2 * double(ra) + c1r * double(rb) - c1i * double(mb);
+      const double ai = kInvSqrt2 * double(ma) + c1r * double(mb) + c1i * double(rb);
+      acc += ar * ar + ai * ai;
+    }
+  }
+  const double t = block_sum(acc, red);
+  if (threadIdx.x == 0) part[size_t(j) * max_tiles + blockIdx.x] = t;
+}
+
 // ---- Factored components -------------------------------------------------
 // A component about to be merged from factors f0 (target) and f1..fn-1
 // (sources, in merge order): amplitude x is
@@ -952,6 +1028,7 @@ static hipError_t run_wide(const DevProgram& P, const DevInstr* instrs, const De
                        dim);
     WCK(hipGetLastError());
     int k = P.initial_k;  // dense width, for the final-state dump
+    unsigned fused_tiles = 0;  // > 0: the next measurement's partials are in ws.part
     int i = 0;
     while (i < P.ninstr) {
       const DevInstr& in = instrs[i];
@@ -1054,9 +1131,14 @@ static hipError_t run_wide(const DevProgram& P, const DevInstr* instrs, const De
         size_t out_dim = (size_t(1) << w) >> 1;
         unsigned tiles = unsigned((out_dim + kWideTile - 1) / kWideTile);
         dim3 g(tiles, n);
-        hipLaunchKernelGGL(wide_prob_kernel<T>, g, dim3(kWideThreads), 0, stream, P, i, R, I, dim, w, ws.part,
-                           Y.max_tiles);
-        WCK(hipGetLastError());
+        if (fused_tiles) {  // the previous collapse already summed this probability
+          tiles = fused_tiles;
+          fused_tiles = 0;
+        } else {
+          hipLaunchKernelGGL(wide_prob_kernel<T>, g, dim3(kWideThreads), 0, stream, P, i, R, I, dim, w, ws.part,
+                             Y.max_tiles);
+          WCK(hipGetLastError());
+        }
         hipLaunchKernelGGL(wide_decide_kernel, sg, sb, 0, stream, P, L, Y, start, n, i, ws.bits, ws.ctl, ws.part,
                            tiles, Y.max_tiles);
         WCK(hipGetLastError());
@@ -1064,8 +1146,18 @@ static hipError_t run_wide(const DevProgram& P, const DevInstr* instrs, const De
           ++i;
           continue;
         }
-        hipLaunchKernelGGL(wide_project_kernel<T>, g, dim3(kWideThreads), 0, stream, P, i, ws.ctl, R, I, RE(cb ^ 1),
-                           IM(cb ^ 1), dim, w);
+        const DevInstr* nx = i + 1 < P.ninstr ? &instrs[i + 1] : nullptr;
+        const size_t pairs = out_dim >> 1;
+        const unsigned ftiles = unsigned((pairs + kWideTile / 2 - 1) / (kWideTile / 2));
+        if (nx && (nx->op & 0xff) == kOpMeasure && !(nx->op & kFlagExpVal) && nx->comp == in.comp &&
+            nx->nq == w - 1 && pairs > 0 && ftiles <= Y.max_tiles) {
+          hipLaunchKernelGGL(wide_project_measure_kernel<T>, dim3(ftiles, n), dim3(kWideThreads), 0, stream, P, i,
+                             i + 1, ws.ctl, R, I, RE(cb ^ 1), IM(cb ^ 1), dim, w, ws.part, Y.max_tiles);
+          fused_tiles = ftiles;
+        } else {
+          hipLaunchKernelGGL(wide_project_kernel<T>, g, dim3(kWideThreads), 0, stream, P, i, ws.ctl, R, I,
+                             RE(cb ^ 1), IM(cb ^ 1), dim, w);
+        }
         WCK(hipGetLastError());
         cur[size_t(in.comp)] ^= 1;
         --k;
