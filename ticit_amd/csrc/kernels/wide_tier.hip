@@ -715,6 +715,30 @@ __device__ __forceinline__ void factor_amp(const Factors<T>& F, size_t shot, siz
   }
 }
 
+// The merged amplitudes at x and x + 1 (x even): they share every factor
+// but the first, whose two entries are one aligned pair; each product in
+// the order of factor_amp.
+template <class T>
+__device__ __forceinline__ void factor_amp2(const Factors<T>& F, size_t shot, size_t x, double& r0, double& m0,
+                                            double& r1, double& m1) {
+  const size_t x0 = x & F.mask[0];
+  const auto a = ld2(F.re[0] + shot + x0), b = ld2(F.im[0] + shot + x0);
+  r0 = double(a.x);
+  m0 = double(b.x);
+  r1 = double(a.y);
+  m1 = double(b.y);
+  for (int f = 1; f < F.n; ++f) {
+    const size_t xf = (x >> F.shift[f]) & F.mask[f];
+    const double cr = double(F.re[f][shot + xf]), ci = double(F.im[f][shot + xf]);
+    const double n0r = r0 * cr - m0 * ci, n0i = r0 * ci + m0 * cr;
+    const double n1r = r1 * cr - m1 * ci, n1i = r1 * ci + m1 * cr;
+    r0 = n0r;
+    m0 = n0i;
+    r1 = n1r;
+    m1 = n1i;
+  }
+}
+
 // The merged state written out (a chain of merges in one pass).
 template <class T>
 __global__ __launch_bounds__(kWideThreads) void wide_multimerge_kernel(Factors<T> F, T* ore, T* oim, size_t dim,
@@ -840,6 +864,41 @@ __global__ __launch_bounds__(kWideThreads) void wide_project_factored_kernel(Dev
   T* OI = oim + shot + in.comp_off;
   const size_t out_dim = (size_t(1) << k) >> 1;
   const bool diag = in.op & kFlagDiagonal;
+  if (pairs_ok(in, out_dim, dim) && F.mask[0] >= 1) {
+    // outputs (idx, idx + 1) from source pairs (s, s + 1): the first factor
+    // read as aligned pairs, the others shared, the outputs stored as pairs
+    const bool swap = in.xmask & 1;
+#pragma unroll
+    for (int u = 0; u < kWidePairs; ++u) {
+      const size_t idx = 2 * pair_index(u);
+      if (idx >= out_dim) break;
+      if (diag) {
+        const size_t s0 = diagonal_source(in, idx, branch);  // s0 + 1 feeds idx + 1
+        double r0, m0, r1, m1;
+        factor_amp2(F, shot, s0, r0, m0, r1, m1);
+        st2(OR + idx, T(r0 * inv), T(r1 * inv));
+        st2(OI + idx, T(m0 * inv), T(m1 * inv));
+      } else {
+        const size_t s0 = insert_zero_bit(idx, in.pivot), s1 = (s0 ^ size_t(in.xmask)) & ~size_t(1);
+        double ar[2], ai[2], br[2], bi[2];
+        factor_amp2(F, shot, s0, ar[0], ai[0], ar[1], ai[1]);
+        factor_amp2(F, shot, s1, br[0], bi[0], br[1], bi[1]);
+        double orr[2], oii[2];
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const double r1 = br[e ^ int(swap)], m1 = bi[e ^ int(swap)];
+          const bool odd = parity64(uint64_t(s0 + size_t(e)) & in.zmask);
+          const double dir = (branch != odd) ? -1.0 : 1.0;
+          const double c1r = dir * in.c1_re, c1i = dir * in.c1_im;
+          orr[e] = (kInvSqrt2 * ar[e] + c1r * r1 - c1i * m1) * inv;
+          oii[e] = (kInvSqrt2 * ai[e] + c1r * m1 + c1i * r1) * inv;
+        }
+        st2(OR + idx, T(orr[0]), T(orr[1]));
+        st2(OI + idx, T(oii[0]), T(oii[1]));
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int u = 0; u < kWidePer; ++u) {
     const size_t idx = size_t(blockIdx.x) * kWideTile + size_t(u) * kWideThreads + threadIdx.x;
