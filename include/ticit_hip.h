@@ -102,6 +102,7 @@ typedef struct ticit_hip_program_info {
   uint32_t blocks_per_cu;        /* resident workgroups per CU of that launch */
   uint32_t table_words[4];       /* per-shot bit-table words: partials, slots, records, detectors */
   uint64_t expectation_values;   /* EXP_VAL slots per shot (src/circuit/mod.rs:122) */
+  uint64_t postselected_detectors; /* detectors whose firing discards a shot (src/circuit/mod.rs:329-333) */
 } ticit_hip_program_info;
 
 /* Thread-local message of the last failing call on this thread. */
@@ -112,6 +113,12 @@ const char* ticit_hip_last_error(void);
  * for hosts that do not serialize their own FactoredInstructionProgram. The
  * blob is freed with ticit_hip_blob_free. */
 ticit_hip_status ticit_hip_plan_text(const char* text, size_t len, uint64_t** blob, size_t* nwords);
+/* The same with detector postselection (plan_circuit's postselection_mask,
+ * src/circuit/mod.rs:273-327; SamplerOptions::postselection_mask,
+ * src/sampler/prepared.rs:108-109): mask[d] != 0 postselects detector d
+ * (missing entries are zero), unioned with the circuit's DISCARD lines. */
+ticit_hip_status ticit_hip_plan_text_postselected(const char* text, size_t len, const uint8_t* mask,
+                                                  size_t mask_len, uint64_t** blob, size_t* nwords);
 void ticit_hip_blob_free(uint64_t* blob);
 
 /* Sampler::new / from_input (src/sampler/prepared.rs:492-626): validates the

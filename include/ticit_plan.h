@@ -15,7 +15,8 @@
  *   nexpvals, flags (version 2 only), ninstr,
  *   ninstr x instruction:
  *     w0 = op (bits 0-7) | diagonal (8) | diag_phase_bit (9) |
- *          expr.constant (10) | nq (16-31) | pivot (32-47) | phase (48-49)
+ *          expr.constant (10) | postselect (11, detectors) | nq (16-31) |
+ *          pivot (32-47) | phase (48-49)
  *     xmask, zmask, f64 angle,
  *     branch (lo32) | record (hi32),
  *     record_condition (lo32) | detector (hi32),

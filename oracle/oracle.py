@@ -112,8 +112,9 @@ def sample(blob: np.ndarray, shots: int, seed: int, *, chunk_shots: int = 0, bat
                                    threads, int(bit_packed), observable, _ptr(meas, _u8p), _ptr(det, _u8p),
                                    _ptr(obs, _u8p), _ptr(counts, _u64p), _ptr(ones, _u64p),
                                    _ptr(timing, _f64p), _ptr(ev, _f64p)), fast)
-    return dict(measurements=meas, detectors=det, observables=obs, counts=counts,
-                observable_ones=ones[:inf["nobservables"]], sample_s=float(timing[0]), exp_vals=ev)
+    rows = int(counts[2])  # postselection keeps the accepted shots' rows
+    return dict(measurements=meas[:rows], detectors=det[:rows], observables=obs[:rows], counts=counts,
+                observable_ones=ones[:inf["nobservables"]], sample_s=float(timing[0]), exp_vals=ev[:rows])
 
 
 def sample_per_shot(blob: np.ndarray, shots: int, seed: int, **kw) -> dict:
@@ -149,8 +150,9 @@ def sample_injected(blob: np.ndarray, shots: int, exo: np.ndarray | None, unifor
                                         _ptr(meas, _u8p), _ptr(det, _u8p), _ptr(obs, _u8p), _ptr(counts, _u64p),
                                         _ptr(ones, _u64p), _ptr(probs, _f64p), _ptr(fre, _f64p), _ptr(fim, _f64p),
                                         ctypes.byref(fk), _ptr(ev, _f64p)))
-    out = dict(measurements=meas, detectors=det, observables=obs, counts=counts,
-               observable_ones=ones[:inf["nobservables"]], exp_vals=ev)
+    rows = int(counts[2])
+    out = dict(measurements=meas[:rows], detectors=det[:rows], observables=obs[:rows], counts=counts,
+               observable_ones=ones[:inf["nobservables"]], exp_vals=ev[:rows])
     if want_probs:
         out["probs"] = probs
     if want_final_state:

@@ -61,6 +61,7 @@ struct Instr {
   bool diagonal = false;
   int diag_phase_bit = 0;
   int32_t branch = 0, record = 0, record_condition = 0, detector = 0, exp_val = -1;
+  bool postselect = false;  // a firing detector discards the shot (src/factored.rs:227)
   std::vector<int32_t> det_records;
   Expr expr;
 };
@@ -141,6 +142,7 @@ Program decode(const uint64_t* w, size_t n) {
     in.diagonal = (w0 >> 8) & 1;
     in.diag_phase_bit = int((w0 >> 9) & 1);
     in.expr.constant = (w0 >> 10) & 1;
+    in.postselect = (w0 >> 11) & 1;
     in.nq = int((w0 >> 16) & 0xffff);
     in.pivot = int((w0 >> 32) & 0xffff);
     in.phase = int((w0 >> 48) & 3);
@@ -864,7 +866,7 @@ struct Source {
 struct Outputs {
   std::vector<uint8_t> meas, det, obs;
   std::vector<uint64_t> obs_ones;
-  uint64_t shots = 0, accepted = 0, logical = 0;
+  uint64_t shots = 0, accepted = 0, discarded = 0, logical = 0;
   std::vector<double> expv;         // [shot][nexpvals] expectation values
   std::vector<double> probs;        // optional per (measure ordinal, global shot)
   std::vector<double> final_re, final_im;  // optional final state of shot 0
@@ -994,6 +996,9 @@ struct Runner {
     std::vector<uint64_t> sym(size_t(p.nsymbols) * words, 0);
     std::vector<char> assigned(size_t(p.nsymbols) + 1, 0);
     std::vector<uint64_t> rec(size_t(p.nrecords) * words, 0), det(size_t(p.ndetectors) * words, 0);
+    // src/sampler/batch/postselect.rs: a fired postselected detector marks
+    // its shot dead; dead shots leave no rows and no counts
+    std::vector<uint64_t> dead(words, 0);
     std::vector<uint64_t> bits(words);
     // src/sampler/batch/runtime.rs:46-75: expectation slots per shot
     const size_t nexp = size_t(p.nexpvals);
@@ -1142,6 +1147,8 @@ struct Runner {
           }
           if (in.detector <= 0 || in.detector > p.ndetectors) throw OracleError("detector id out of range");
           for (size_t w = 0; w < words; ++w) det[size_t(in.detector - 1) * words + w] = bits[w] & live_mask(shots, w);
+          if (in.postselect)
+            for (size_t w = 0; w < words; ++w) dead[w] |= bits[w] & live_mask(shots, w);
           break;
         }
         case TICIT_OP_MEASURE: {
@@ -1247,27 +1254,37 @@ struct Runner {
         if (r <= 0 || r > p.nrecords) throw OracleError("observable references an out-of-range measurement record");
         for (size_t w = 0; w < words; ++w) ow[o * words + w] ^= rec[size_t(r - 1) * words + w];
       }
-    out.accepted += shots;
+    std::vector<uint64_t> keep(words);
+    uint64_t nkeep = 0;
+    for (size_t w = 0; w < words; ++w) {
+      keep[w] = live_mask(shots, w) & ~dead[w];
+      nkeep += __builtin_popcountll(keep[w]);
+    }
+    out.accepted += nkeep;
+    out.discarded += shots - nkeep;
     out.shots += shots;
     for (size_t o = 0; o < nobs; ++o) {
       uint64_t ones = 0;
-      for (size_t w = 0; w < words; ++w) ones += __builtin_popcountll(ow[o * words + w] & live_mask(shots, w));
+      for (size_t w = 0; w < words; ++w) ones += __builtin_popcountll(ow[o * words + w] & keep[w]);
       out.obs_ones[o] += ones;
       if (int(o) == selected_observable) out.logical += ones;
     }
-    append_rows(out.meas, rec, size_t(p.nrecords), words, shots);
-    append_rows(out.det, det, size_t(p.ndetectors), words, shots);
-    append_rows(out.obs, ow, nobs, words, shots);
-    out.expv.insert(out.expv.end(), ev.begin(), ev.end());
+    append_rows(out.meas, rec, size_t(p.nrecords), words, shots, keep);
+    append_rows(out.det, det, size_t(p.ndetectors), words, shots, keep);
+    append_rows(out.obs, ow, nobs, words, shots, keep);
+    for (size_t s = 0; s < shots; ++s)
+      if ((keep[s >> 6] >> (s & 63)) & 1) out.expv.insert(out.expv.end(), ev.begin() + s * nexp, ev.begin() + (s + 1) * nexp);
   }
 
+  // Rows of the shots whose keep bit is set, in shot order.
   void append_rows(std::vector<uint8_t>& out, const std::vector<uint64_t>& cols, size_t ncols, size_t words,
-                   size_t rows) {
+                   size_t rows, const std::vector<uint64_t>& keep) {
     size_t oc = bit_packed ? (ncols + 7) / 8 : ncols;
-    size_t base = out.size();
-    out.resize(base + rows * oc, 0);
     for (size_t s = 0; s < rows; ++s) {
-      uint8_t* row = &out[base + s * oc];
+      if (!((keep[s >> 6] >> (s & 63)) & 1)) continue;
+      size_t base = out.size();
+      out.resize(base + oc, 0);
+      uint8_t* row = &out[base];
       for (size_t c = 0; c < ncols; ++c) {
         bool b = (cols[c * words + (s >> 6)] >> (s & 63)) & 1;
         if (bit_packed) {
@@ -1296,7 +1313,7 @@ void copy_out(const Outputs& o, uint8_t* meas, uint8_t* det, uint8_t* obs, uint6
   if (obs && !o.obs.empty()) std::memcpy(obs, o.obs.data(), o.obs.size());
   if (counts) {
     counts[0] = o.shots;
-    counts[1] = 0;
+    counts[1] = o.discarded;
     counts[2] = o.accepted;
     counts[3] = o.logical;
   }
@@ -1395,6 +1412,17 @@ int oracle_sample(const uint64_t* blob, size_t nwords, uint64_t shots, uint64_t 
     Program p = decode(blob, nwords);
     size_t batch = batch_size ? size_t(batch_size) : default_batch(p.max_k);
     size_t chunk = chunk_shots ? size_t(chunk_shots) : (batch > 2048 ? batch : 2048);
+    // The reference compacts dead shots out of a block before every draw
+    // (src/sampler/batch/postselect.rs:1-5), which moves later draws of a
+    // multi-shot block; this restatement keeps dead shots in place, so it
+    // runs postselection only where no branch is drawn or one shot per block.
+    bool postselect = false, draws = false;
+    for (const auto& in : p.instrs) {
+      postselect |= in.op == TICIT_OP_DETECTOR && in.postselect;
+      draws |= in.op == TICIT_OP_MEASURE || in.op == TICIT_OP_COIN;
+    }
+    if (postselect && draws && batch != 1)
+      throw OracleError("postselection with branch draws is restated at one shot per block");
     uint64_t nchunks = (shots + chunk - 1) / chunk;
     uint64_t bpc = (chunk + batch - 1) / batch;
     if (threads < 1) threads = 1;
@@ -1445,6 +1473,7 @@ int oracle_sample(const uint64_t* blob, size_t nwords, uint64_t shots, uint64_t 
       all.expv.insert(all.expv.end(), o.expv.begin(), o.expv.end());
       all.shots += o.shots;
       all.accepted += o.accepted;
+      all.discarded += o.discarded;
       all.logical += o.logical;
       for (size_t i = 0; i < o.obs_ones.size(); ++i) all.obs_ones[i] += o.obs_ones[i];
     }

@@ -157,3 +157,52 @@ def test_presample_sparse_hit_rate():
     planes = O.presample(blob, 20000, 1234)
     hits = int(np.unpackbits(planes.view(np.uint8), bitorder="little").sum())
     assert 2300 <= hits <= 2700
+
+
+def test_postselection_returns_only_survivor_rows():
+    # src/sampler/prepared.rs:1248-1273 (batch 16: no branch is drawn)
+    blob = plan_text("EXP_VAL Z1\nM !1\nDETECTOR rec[-1]\nX_ERROR(0.5) 0\nM 0\nDETECTOR rec[-1]\n"
+                     "OBSERVABLE_INCLUDE(0) rec[-2]\n", postselection_mask=[0, 1])
+    r = O.sample(blob, 128, 9, batch_size=16, bit_packed=False)
+    rows = int(r["counts"][2])
+    assert 0 < rows < 128 and int(r["counts"][1]) == 128 - rows
+    assert r["measurements"].tolist() == [[1, 0]] * rows
+    assert r["detectors"].tolist() == [[1, 0]] * rows
+    assert r["observables"].reshape(-1).tolist() == [1] * rows
+    assert r["observable_ones"].tolist() == [rows]
+    assert r["exp_vals"].reshape(-1).tolist() == [1.0] * rows
+
+
+def test_source_discards_and_the_passed_mask_are_unioned():
+    # src/sampler/prepared.rs:1276-1307
+    text = "M !0\nDETECTOR rec[-1]\nM 0\nDISCARD rec[-1]\n"
+    c = O.sample(plan_text(text), 8, 1, bit_packed=False)["counts"]
+    assert (int(c[2]), int(c[1])) == (8, 0)
+    c = O.sample(plan_text(text, postselection_mask=[1]), 8, 1, bit_packed=False)["counts"]
+    assert (int(c[2]), int(c[1])) == (0, 8)
+    c = O.sample(plan_text("M !0\nDISCARD rec[-1]\n"), 8, 1, bit_packed=False)["counts"]
+    assert (int(c[2]), int(c[1])) == (0, 8)
+
+
+def test_an_always_fired_postselected_detector_discards_every_shot():
+    # src/sampler/prepared.rs:1111-1130: estimate_logical_error_rate
+    # postselects every detector (prepared.rs:1037)
+    r = O.sample(plan_text("M !0\nOBSERVABLE_INCLUDE(0) rec[-1]\n", postselection_mask=[]), 5, 1)
+    assert (int(r["counts"][1]), int(r["counts"][2]), int(r["counts"][3])) == (0, 5, 5)
+    r = O.sample(plan_text("M !0\nDETECTOR rec[-1]\nOBSERVABLE_INCLUDE(0) rec[-1]\n", postselection_mask=[1]), 5, 1)
+    assert (int(r["counts"][1]), int(r["counts"][2]), int(r["counts"][3])) == (5, 0, 0)
+    assert r["measurements"].shape[0] == 0
+
+
+def test_postselected_rows_are_the_unpostselected_rows_of_the_survivors():
+    """At one shot per stream, discarding a shot changes no other shot's
+    draws, so the survivors' rows are the plain run's rows of those shots."""
+    text = ("H 0\nT 0\nH 0\nX_ERROR(0.2) 1\nM 0 1\nDETECTOR rec[-1]\nH 2\nT 2\nM 2\n"
+            "DETECTOR rec[-1] rec[-2]\nOBSERVABLE_INCLUDE(0) rec[-3]\n")
+    plain = O.sample_per_shot(plan_text(text), 500, 3, bit_packed=False)
+    post = O.sample_per_shot(plan_text(text, postselection_mask=[1, 0]), 500, 3, bit_packed=False)
+    keep = plain["detectors"][:, 0] == 0
+    assert int(post["counts"][2]) == int(keep.sum()) and int(post["counts"][1]) == 500 - int(keep.sum())
+    assert np.array_equal(post["measurements"], plain["measurements"][keep])
+    assert np.array_equal(post["detectors"], plain["detectors"][keep])
+    assert int(post["observable_ones"][0]) == int(plain["observables"][keep, 0].sum())

@@ -27,13 +27,20 @@ PLANNERS = {"heuristic": 0, "learned": 1}
 COMPONENTS = {"planned": 0, "dense": 1, "components": 2}
 
 
-def plan_text(text: str) -> np.ndarray:
-    """Parse and plan ``.ticit`` text into a plan blob (include/ticit_plan.h)."""
+def plan_text(text: str, postselection_mask=None) -> np.ndarray:
+    """Parse and plan ``.ticit`` text into a plan blob (include/ticit_plan.h);
+    ``postselection_mask`` (flat detector flags) postselects detectors, as
+    SamplerOptions::postselection_mask does, unioned with DISCARD lines."""
     lib = _capi.load()
     data = text.encode()
     ptr = _capi._u64p()
     n = ctypes.c_size_t(0)
-    _capi.check(lib.ticit_hip_plan_text(data, len(data), ctypes.byref(ptr), ctypes.byref(n)))
+    if postselection_mask is None:
+        _capi.check(lib.ticit_hip_plan_text(data, len(data), ctypes.byref(ptr), ctypes.byref(n)))
+    else:
+        mask = np.ascontiguousarray([1 if int(v) else 0 for v in postselection_mask], dtype=np.uint8)
+        _capi.check(lib.ticit_hip_plan_text_postselected(data, len(data), mask.ctypes.data if mask.size else None,
+                                                         mask.size, ctypes.byref(ptr), ctypes.byref(n)))
     try:
         return np.ctypeslib.as_array(ptr, shape=(n.value,)).copy()
     finally:
@@ -160,7 +167,8 @@ class CompiledSampler:
         self.blocks_per_cu = int(info.blocks_per_cu)
         self.table_words = [int(v) for v in info.table_words]
         self.num_expectation_values = int(info.expectation_values)
-        self.has_postselection = False
+        self.num_postselected_detectors = int(info.postselected_detectors)
+        self.has_postselection = self.num_postselected_detectors > 0
 
     def close(self) -> None:
         if getattr(self, "_handle", None):
@@ -288,9 +296,10 @@ class Circuit:
         ``"hip"`` backend lives in this package."""
         if backend != "hip":
             raise ValueError(f"backend {backend!r} is not provided by ticit_amd; use backend='hip'")
+        blob = self.blob
         if postselection_mask is not None and any(int(v) for v in postselection_mask):
-            raise TicitError(3, "detector postselection is not supported by the hip backend")
-        return CompiledSampler(self.blob, observable=observable, precision=precision, tier=tier,
+            blob = plan_text(self.text, postselection_mask)
+        return CompiledSampler(blob, observable=observable, precision=precision, tier=tier,
                                planner=planner, wide_batch=wide_batch, components=components,
                                group_threads=group_threads)
 

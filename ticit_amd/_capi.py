@@ -46,7 +46,7 @@ class ProgramInfo(ctypes.Structure):
                 ("max_local_k", ctypes.c_int32), ("planner_source", ctypes.c_int32),
                 ("planner_p_components", ctypes.c_double), ("lds_bytes_per_block", ctypes.c_uint32),
                 ("blocks_per_cu", ctypes.c_uint32), ("table_words", ctypes.c_uint32 * 4),
-                ("expectation_values", ctypes.c_uint64)]
+                ("expectation_values", ctypes.c_uint64), ("postselected_detectors", ctypes.c_uint64)]
 
 
 class BatchItem(ctypes.Structure):
@@ -75,7 +75,7 @@ STATUS_NAMES = {1: "InvalidInput", 2: "Parse", 3: "Unsupported", 4: "Internal",
 
 # every symbol include/ticit_hip.h declares
 EXPORTS = [
-    "ticit_hip_last_error", "ticit_hip_plan_text", "ticit_hip_blob_free",
+    "ticit_hip_last_error", "ticit_hip_plan_text", "ticit_hip_plan_text_postselected", "ticit_hip_blob_free",
     "ticit_hip_program_create", "ticit_hip_compile_text", "ticit_hip_program_destroy",
     "ticit_hip_program_info_get", "ticit_hip_sample", "ticit_hip_sample_device",
     "ticit_hip_sample_injected", "ticit_hip_final_state", "ticit_hip_sample_batch",
@@ -111,6 +111,9 @@ def load() -> ctypes.CDLL:
         pass
     lib = ctypes.CDLL(path)
     lib.ticit_hip_last_error.restype = ctypes.c_char_p
+    lib.ticit_hip_plan_text_postselected.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_void_p,
+                                                    ctypes.c_size_t, ctypes.POINTER(_u64p),
+                                                    ctypes.POINTER(ctypes.c_size_t)]
     lib.ticit_hip_plan_text.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(_u64p),
                                         ctypes.POINTER(ctypes.c_size_t)]
     lib.ticit_hip_blob_free.argtypes = [_u64p]

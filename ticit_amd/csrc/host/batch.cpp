@@ -252,6 +252,10 @@ ticit_hip_status ticit_hip_batch_create(const ticit_hip_batch_item* items, size_
       delete b;
       return fail(TICIT_HIP_INVALID_INPUT, "batch programs must live on the current device");
     }
+    if (h->tab.npost > 0) {
+      delete b;
+      return fail(TICIT_HIP_UNSUPPORTED, "postselected programs return their surviving rows through ticit_hip_sample");
+    }
     const HostTables& t = h->tab;
     DevLaunch L{};
     L.shots = items[i].shots;

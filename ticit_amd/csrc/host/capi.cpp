@@ -86,6 +86,7 @@ ticit_hip_status create_from_program(Program prog, const ticit_hip_options* opti
   UP(bsteps, t.bsteps);
   UP(resid, t.resid);
   UP(det_recs, t.det_recs);
+  UP(post_dets, t.post_dets);
   UP(obs_off, t.obs_off);
   UP(obs_recs, t.obs_recs);
   UP(sym_off, t.sym_off);
@@ -123,6 +124,7 @@ ticit_hip_status create_from_program(Program prog, const ticit_hip_options* opti
   d.max_local_k = t.max_local_k;
   d.nexp = t.nexp;
   d.nlsteps = t.nlsteps;
+  d.npost = t.npost;
   e = hipMalloc(reinterpret_cast<void**>(&h->d_error), sizeof(int));
   if (e != hipSuccess) {
     delete h;
@@ -257,6 +259,7 @@ ticit_hip_status run_host(ticit_hip_program* h, uint64_t shots, uint64_t seed, i
   uint64_t* d_coins = nullptr;
   double* d_probs = nullptr;
   double* d_expv = nullptr;
+  uint8_t* d_keep = nullptr;
 #define CK(expr, what)                        \
   do {                                        \
     e = (expr);                               \
@@ -272,6 +275,10 @@ ticit_hip_status run_host(ticit_hip_program* h, uint64_t shots, uint64_t seed, i
     }
   }
   CK(buf.alloc(&L.obs_ones, size_t(std::max(1, t.nobs))), "allocating observable counts");
+  if (t.npost > 0) {
+    CK(buf.alloc(&d_keep, size_t(shots)), "allocating postselection flags");
+    L.keep = d_keep;
+  }
   CK(hipMemset(L.obs_ones, 0, size_t(std::max(1, t.nobs)) * 8), "clearing observable counts");
   CK(hipMemset(h->d_error, 0, sizeof(int)), "clearing the error flag");
   L.error = h->d_error;
@@ -327,6 +334,26 @@ ticit_hip_status run_host(ticit_hip_program* h, uint64_t shots, uint64_t seed, i
   out->shots = shots;
   out->accepted = shots;
   out->nexpvals = uint64_t(t.nexp);
+  // postselection: the kept shots' rows, in shot order
+  // (src/sampler/prepared.rs:881-1028 appends only survivors)
+  std::vector<uint8_t> keep;
+  if (t.npost > 0) {
+    keep.resize(size_t(shots));
+    CK(hipMemcpy(keep.data(), d_keep, size_t(shots), hipMemcpyDeviceToHost), "reading postselection flags");
+    uint64_t acc = 0;
+    for (uint8_t k : keep) acc += k;
+    out->accepted = acc;
+    out->discarded = shots - acc;
+  }
+  auto compact = [&](uint8_t* rows, size_t row_bytes) {
+    if (keep.empty() || row_bytes == 0) return;
+    size_t o = 0;
+    for (size_t s = 0; s < keep.size(); ++s)
+      if (keep[s]) {
+        if (o != s) std::memmove(rows + o * row_bytes, rows + s * row_bytes, row_bytes);
+        ++o;
+      }
+  };
   out->bit_packed = bit_packed != 0;
   out->nmeasurements = uint64_t(t.nrec);
   out->ndetectors = uint64_t(t.ndet);
@@ -335,17 +362,21 @@ ticit_hip_status run_host(ticit_hip_program* h, uint64_t shots, uint64_t seed, i
   CK(hipMemcpy(out->observable_ones, L.obs_ones, size_t(t.nobs) * 8, hipMemcpyDeviceToHost), "reading counts");
   if (t.nobs > 0 && h->opt.observable < uint32_t(t.nobs)) out->logical_errors = out->observable_ones[h->opt.observable];
   if (keep_records) {
-    out->record_rows = shots;
+    out->record_rows = out->accepted;
     out->measurements = static_cast<uint8_t*>(malloc(std::max<size_t>(1, size_t(shots) * L.meas_row)));
     out->detectors = static_cast<uint8_t*>(malloc(std::max<size_t>(1, size_t(shots) * L.det_row)));
     out->observables = static_cast<uint8_t*>(malloc(std::max<size_t>(1, size_t(shots) * L.obs_row)));
     if (L.meas_row) CK(hipMemcpy(out->measurements, L.meas, size_t(shots) * L.meas_row, hipMemcpyDeviceToHost), "reading rows");
     if (L.det_row) CK(hipMemcpy(out->detectors, L.det, size_t(shots) * L.det_row, hipMemcpyDeviceToHost), "reading rows");
     if (L.obs_row) CK(hipMemcpy(out->observables, L.obs, size_t(shots) * L.obs_row, hipMemcpyDeviceToHost), "reading rows");
+    compact(out->measurements, L.meas_row);
+    compact(out->detectors, L.det_row);
+    compact(out->observables, L.obs_row);
     if (t.nexp > 0) {
       out->exp_vals = static_cast<double*>(malloc(size_t(shots) * size_t(t.nexp) * sizeof(double)));
       CK(hipMemcpy(out->exp_vals, d_expv, size_t(shots) * size_t(t.nexp) * 8, hipMemcpyDeviceToHost),
          "reading expectation values");
+      compact(reinterpret_cast<uint8_t*>(out->exp_vals), size_t(t.nexp) * sizeof(double));
     }
   }
   if (probs && t.nmeasure)
@@ -369,9 +400,15 @@ extern "C" {
 const char* ticit_hip_last_error(void) { return g_error.c_str(); }
 
 ticit_hip_status ticit_hip_plan_text(const char* text, size_t len, uint64_t** blob, size_t* nwords) {
-  if (!text || !blob || !nwords) return fail(TICIT_HIP_INVALID_INPUT, "null argument");
+  return ticit_hip_plan_text_postselected(text, len, nullptr, 0, blob, nwords);
+}
+
+ticit_hip_status ticit_hip_plan_text_postselected(const char* text, size_t len, const uint8_t* mask,
+                                                  size_t mask_len, uint64_t** blob, size_t* nwords) {
+  if (!text || !blob || !nwords || (mask_len && !mask)) return fail(TICIT_HIP_INVALID_INPUT, "null argument");
   try {
-    Program p = compile_circuit_text(std::string(text, len));
+    Program p = compile_circuit_text(std::string(text, len), nullptr,
+                                     std::vector<uint8_t>(mask, mask + mask_len));
     std::vector<uint64_t> w = encode_program(p);
     *blob = static_cast<uint64_t*>(malloc(w.size() * 8));
     std::memcpy(*blob, w.data(), w.size() * 8);
@@ -487,6 +524,7 @@ ticit_hip_status ticit_hip_program_info_get(const ticit_hip_program* h, ticit_hi
   out->table_words[2] = h->lane.rw;
   out->table_words[3] = h->lane.dw;
   out->expectation_values = uint64_t(h->tab.nexp);
+  out->postselected_detectors = uint64_t(h->tab.npost);
   if (h->tier == kTierLane) {
     uint32_t row = std::max({h->row_bytes(uint64_t(h->tab.nrec), true), h->row_bytes(uint64_t(h->tab.ndet), true),
                              h->row_bytes(uint64_t(h->tab.nobs), true)});
@@ -532,6 +570,8 @@ ticit_hip_status ticit_hip_sample_device(ticit_hip_program* h, uint64_t shots, u
                                          void* stream) {
   if (!h || !observable_ones) return fail(TICIT_HIP_INVALID_INPUT, "null argument");
   if (keep_records && (!meas || !det || !obs)) return fail(TICIT_HIP_INVALID_INPUT, "null row buffer");
+  if (h->tab.npost > 0)
+    return fail(TICIT_HIP_UNSUPPORTED, "postselected programs return their surviving rows through ticit_hip_sample");
   if (shots == 0) return TICIT_HIP_OK;
   const HostTables& t = h->tab;
   DevLaunch L{};

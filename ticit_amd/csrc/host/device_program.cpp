@@ -402,6 +402,10 @@ HostTables lower_to_device(const Program& p, bool components) {
       case Op::Detector:
         if (in.detector <= 0 || in.detector > p.ndetectors) throw TicitError(1, "detector id out of range");
         d.detector = in.detector - 1;
+        if (in.postselect) {
+          d.op |= kFlagPostselect;
+          t.post_dets.push_back(uint32_t(d.detector));
+        }
         if (!in.det_records.empty()) {
           d.op |= kFlagDetRecords;
           d.det_off = uint32_t(t.det_recs.size());
@@ -550,6 +554,8 @@ HostTables lower_to_device(const Program& p, bool components) {
   // keep every table non-empty so device pointers are always valid
   if (t.resid.empty()) t.resid.push_back(0);
   if (t.det_recs.empty()) t.det_recs.push_back(0);
+  t.npost = int32_t(t.post_dets.size());
+  if (t.post_dets.empty()) t.post_dets.push_back(0);
   if (t.obs_recs.empty()) t.obs_recs.push_back(0);
   if (t.sym_part.empty()) t.sym_part.push_back(0);
   if (t.exo_syms.empty()) t.exo_syms.push_back(0);

@@ -19,7 +19,9 @@ struct HostTables {
   // step reads the values it would have read in order.
   std::vector<DevStep> lsteps, bsteps;
   int32_t nlsteps = 0;  // lsteps without the padding entry
+  int32_t npost = 0;    // post_dets without the padding entry
   std::vector<uint32_t> resid, det_recs, obs_off, obs_recs, sym_off, sym_part;
+  std::vector<uint32_t> post_dets;  // 0-based postselected detectors
   std::vector<int32_t> exo_syms;
   std::vector<DevCat> cats;
   std::vector<DevRare> rares;

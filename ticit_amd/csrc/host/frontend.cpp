@@ -343,6 +343,7 @@ struct CInstr {
 struct CDetector {
   std::vector<int> records;
   size_t after_instruction = 0;
+  bool discard = false;  // DISCARD: postselected (src/circuit/parser.rs:1361-1374)
 };
 
 struct CObservable {
@@ -684,8 +685,9 @@ Circuit parse_circuit(const std::string& text) {
       ci.probability = one_prob(rl);
       ci.qubits = qubits_of(rl);
       if (ci.qubits.size() % 2) parse_error(rl.line, "DEPOLARIZE2 requires paired targets");
-    } else if (op == "DETECTOR") {
+    } else if (op == "DETECTOR" || op == "DISCARD") {
       CDetector d;
+      d.discard = op == "DISCARD";
       d.records = records_of(rl);
       d.after_instruction = c.instrs.size();
       c.detectors.push_back(d);
@@ -1667,7 +1669,8 @@ Program freeze(int n, int initial_k, int max_k, std::vector<Instr> instrs, Conte
 
 }  // namespace
 
-Program compile_circuit_text(const std::string& text, CompileStats* stats_out) {
+Program compile_circuit_text(const std::string& text, CompileStats* stats_out,
+                             const std::vector<uint8_t>& postselection_mask) {
   CompileStats st;
   Circuit c = parse_circuit(text);
   st.ncircuit_instructions = int(c.instrs.size());
@@ -1710,6 +1713,9 @@ Program compile_circuit_text(const std::string& text, CompileStats* stats_out) {
       Instr d;
       d.op = Op::Detector;
       d.detector = int32_t(i + 1);
+      // src/circuit/mod.rs:236-243: DISCARD declarations union the mask
+      d.postselect = c.detectors[i].discard ||
+                     (i < postselection_mask.size() && postselection_mask[i] != 0);
       for (int r : c.detectors[i].records) {
         if (r <= 0 || r > int(low.records.size()))
           throw TicitError(1, "detector references an out-of-range measurement record");

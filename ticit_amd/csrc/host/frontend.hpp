@@ -32,6 +32,10 @@ struct CompileStats {
 
 // Parses and plans `text`. `postselection_mask` is accepted for interface
 // parity; detectors are never postselected by this backend yet.
-Program compile_circuit_text(const std::string& text, CompileStats* stats = nullptr);
+// postselection_mask: flat detector flags, nonzero postselects the detector
+// (SamplerOptions::postselection_mask, src/sampler/prepared.rs:108-109);
+// missing entries are zero.
+Program compile_circuit_text(const std::string& text, CompileStats* stats = nullptr,
+                             const std::vector<uint8_t>& postselection_mask = {});
 
 }  // namespace ticit

@@ -64,7 +64,7 @@ std::vector<uint64_t> encode_program(const Program& p) {
   o.push_back(uint64_t(p.instrs.size()));
   for (const auto& in : p.instrs) {
     uint64_t w0 = uint64_t(uint8_t(in.op)) | (uint64_t(in.diagonal) << 8) |
-                  (uint64_t(in.diag_phase_bit & 1) << 9) | (uint64_t(in.expr.constant) << 10) |
+                  (uint64_t(in.diag_phase_bit & 1) << 9) | (uint64_t(in.expr.constant) << 10) | (uint64_t(in.postselect) << 11) |
                   (uint64_t(uint32_t(in.nq) & 0xffff) << 16) | (uint64_t(uint32_t(in.pivot) & 0xffff) << 32) |
                   (uint64_t(uint32_t(in.phase) & 3) << 48);
     o.push_back(w0);
@@ -163,6 +163,7 @@ bool decode_program(const uint64_t* words, size_t nwords, Program& p, std::strin
     in.diagonal = (w0 >> 8) & 1;
     in.diag_phase_bit = int32_t((w0 >> 9) & 1);
     in.expr.constant = (w0 >> 10) & 1;
+    in.postselect = (w0 >> 11) & 1;
     in.nq = int32_t((w0 >> 16) & 0xffff);
     in.pivot = int32_t((w0 >> 32) & 0xffff);
     in.phase = int32_t((w0 >> 48) & 3);

@@ -56,6 +56,7 @@ struct Instr {
   int32_t record_condition = 0;  // symbol assigned the outcome, 0 = none
   int32_t detector = 0;    // 1-based detector id (Detector)
   int32_t exp_val = -1;    // expectation slot, -1 = none
+  bool postselect = false; // Detector: a firing discards the shot (src/factored.rs:227)
   std::vector<int32_t> det_records;  // records XORed by a detector
   SymBool expr;            // sign (Rotate/Promote) or outcome (others)
 };

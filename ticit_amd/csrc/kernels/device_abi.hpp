@@ -35,6 +35,7 @@ enum : uint32_t {
   kFlagDetRecords = 1u << 13,   // detector reads record bits, not its expression
   kFlagFresh = 1u << 14,        // promotion that starts a new one-qubit component
   kFlagZeroState = 1u << 15,    // fresh component in |0> (initial components)
+  kFlagPostselect = 1u << 17,   // detector whose firing discards the shot (src/factored.rs:227)
   kFlagExpVal = 1u << 16,       // expectation probe (Record/Measure/Coin): `record`
                                 // holds the expectation slot; no record, branch or
                                 // collapse (src/sampler/batch/runtime.rs:311-356)
@@ -137,6 +138,7 @@ struct DevProgram {
   const DevStep* bsteps;      // records and detectors the batch steps cover
   const uint32_t* resid;      // runtime slot ids
   const uint32_t* det_recs;   // 0-based record ids
+  const uint32_t* post_dets;  // [npost] 0-based postselected detectors
   const uint32_t* obs_off;    // nobs + 1 offsets into obs_recs
   const uint32_t* obs_recs;
   const uint32_t* sym_off;    // nsym + 1 offsets: symbol (c - 1) -> partial ids
@@ -159,6 +161,7 @@ struct DevProgram {
   int32_t max_local_k;  // widest component (max_k dense)
   int32_t nexp;         // expectation slots per shot
   int32_t nlsteps;
+  int32_t npost;        // postselected detectors (0: every shot is kept)
 };
 
 // Per-launch sampling request.
@@ -187,6 +190,9 @@ struct DevLaunch {
   const uint64_t* seed_offset;
   // optional [shots][nexp] expectation values (written when non-null)
   double* expv;
+  // postselection: keep[shot] = 1 when no postselected detector fired
+  // (written when non-null); observable counts only count kept shots
+  uint8_t* keep;
 };
 
 // LDS table sizes of the two resident tiers (words of bits per shot).

@@ -329,8 +329,13 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
   }
   if (!valid) return;
   if (err && t == 0) atomicOr(L.error, err);
+  // postselection (src/sampler/batch/postselect.rs): a fired postselected
+  // detector discards the shot, which is then not counted
+  bool kept = true;
+  for (int32_t d = 0; d < P.npost; ++d) kept = kept && !bit_of(dbits, P.post_dets[d]);
+  if (L.keep && t == 0 && P.npost > 0) L.keep[local] = uint8_t(kept);
 
-  for (int o = t; o < P.nobs; o += TPS) {
+  for (int o = t; o < P.nobs && kept; o += TPS) {
     bool b = false;
     for (uint32_t j = P.obs_off[o]; j < P.obs_off[o + 1]; ++j) b ^= bit_of(rbits, P.obs_recs[j]);
     if (b) atomicAdd(&L.obs_ones[o], 1ull);

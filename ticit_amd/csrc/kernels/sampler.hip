@@ -279,9 +279,17 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
   }
   if (err && live) atomicOr(L.error, err);
 
-  // Observables (src/sampler/prepared.rs:881-907) and their popcounts.
+  // Observables (src/sampler/prepared.rs:881-907) and their popcounts;
+  // shots a postselected detector fired on are discarded
+  // (src/sampler/batch/postselect.rs) and not counted.
   const uint32_t nlive = uint32_t(min<uint64_t>(64, L.shots - uint64_t(blockIdx.x) * 64));
-  const uint64_t live_mask = nlive >= 64 ? ~0ull : ((1ull << nlive) - 1);
+  uint64_t live_mask = nlive >= 64 ? ~0ull : ((1ull << nlive) - 1);
+  if (P.npost > 0) {
+    uint64_t dead = 0;
+    for (int32_t d = 0; d < P.npost; ++d) dead |= Dw[P.post_dets[d]];
+    live_mask &= ~dead;
+    if (L.keep && live) L.keep[local] = uint8_t((live_mask >> lane) & 1);
+  }
   auto obs_word = [&](uint32_t o) {
     uint64_t v = 0;
     for (uint32_t j = P.obs_off[o]; j < P.obs_off[o + 1]; ++j) v ^= Rw[P.obs_recs[j]];

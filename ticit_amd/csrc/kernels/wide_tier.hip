@@ -938,7 +938,12 @@ __global__ __launch_bounds__(kWideThreads) void wide_finish_kernel(DevProgram P,
     for (uint32_t q = P.obs_off[o]; q < P.obs_off[o + 1]; ++q) b ^= wbit(w.r, P.obs_recs[q]);
     return b;
   };
-  for (int o = 0; o < P.nobs; ++o)
+  // postselection (src/sampler/batch/postselect.rs): a fired postselected
+  // detector discards the shot, which is then not counted
+  bool kept = true;
+  for (int32_t d = 0; d < P.npost; ++d) kept = kept && !wbit(w.d, P.post_dets[d]);
+  if (L.keep && P.npost > 0) L.keep[local] = uint8_t(kept);
+  for (int o = 0; o < P.nobs && kept; ++o)
     if (obs_bit(uint32_t(o))) atomicAdd(&L.obs_ones[o], 1ull);
   if (!L.keep_records) return;
   auto emit = [&](uint8_t* out, uint32_t row_bytes, auto&& bit_at, uint32_t ncols) {
