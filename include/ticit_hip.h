@@ -126,6 +126,27 @@ void ticit_hip_blob_free(uint64_t* blob);
 ticit_hip_status ticit_hip_program_create(const uint64_t* blob, size_t nwords, const ticit_hip_options* options,
                                           ticit_hip_program** out);
 
+/* A pinned measurement parity (MeasurementParity, src/sampler/pinning.rs:35-48):
+ * the XOR of the zero-based measurement records must equal `value` in the
+ * noiseless circuit. */
+typedef struct ticit_hip_parity {
+  const uint64_t* records;
+  size_t nrecords;
+  int32_t value;
+  int32_t reserved;
+} ticit_hip_parity;
+
+/* ticit_hip_program_create with SamplerOptions::pin_measurements
+ * (src/sampler/prepared.rs:118-135, solved by plan_pinned_measurements,
+ * src/sampler/pinning.rs:227-289): each pinned branch is set to the XOR of
+ * earlier branches instead of being drawn. Fails (invalid input) when a
+ * constraint is decided the other way by the noiseless circuit
+ * ("deterministic") or names a record the circuit never writes; sampling
+ * fails when a pinned active branch is not a fair coin ("not one half"). */
+ticit_hip_status ticit_hip_program_create_pinned(const uint64_t* blob, size_t nwords,
+                                                const ticit_hip_options* options, const ticit_hip_parity* pins,
+                                                size_t npins, ticit_hip_program** out);
+
 /* Circuit::compile in one call: ticit_hip_plan_text + ticit_hip_program_create. */
 ticit_hip_status ticit_hip_compile_text(const char* text, size_t len, const ticit_hip_options* options,
                                         ticit_hip_program** out);

@@ -55,6 +55,7 @@ def _load(name: str) -> ctypes.CDLL:
     lib.oracle_set_fp32_storage.argtypes = [ctypes.c_int]
     lib.oracle_set_component_mode.argtypes = [ctypes.c_int]
     lib.oracle_set_exact_sums.argtypes = [ctypes.c_int]
+    lib.oracle_set_forced.argtypes = [_i64p, ctypes.c_size_t]
     lib.oracle_component_plan.argtypes = [_u64p, ctypes.c_size_t, _i64p, _f64p, _i64p, _i64p, _i64p,
                                           ctypes.c_size_t]
     lib.oracle_default_batch.argtypes = [ctypes.c_int]
@@ -248,3 +249,19 @@ def insert_zero_bit(packed: int, bit: int) -> int:
 
 def default_batch(max_k: int) -> int:
     return int(lib().oracle_default_batch(max_k))
+
+
+def set_pins(blob: np.ndarray | None, constraints=None) -> list:
+    """Pins (records, value) constraints for the following runs on this
+    process (oracle/pinning.py solves them); no constraints clears. Returns
+    the forced branches as (instruction, constant, branch symbols)."""
+    from oracle.pinning import plan_pinned
+    forced = plan_pinned(blob, constraints) if constraints else []
+    spec = []
+    for instr, c, conds in forced:
+        spec += [instr, int(c), len(conds), *conds]
+    arr = np.ascontiguousarray(spec or [0], dtype=np.int64)
+    for name in ("liboracle.so", "liboracle_fast.so"):
+        if name in _LIBS or name == "liboracle.so":
+            lib(name == "liboracle_fast.so").oracle_set_forced(_ptr(arr, _i64p), len(spec))
+    return forced

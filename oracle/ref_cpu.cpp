@@ -25,6 +25,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -39,6 +40,13 @@ thread_local std::string g_err;
 thread_local bool g_fp32_storage = false;
 int g_component_mode = -1;  // read by worker threads: not thread-local
 bool g_exact_sums = false;  // Neumaier-compensated probability sums (accuracy studies)
+// Pinned branches (src/sampler/pinning.rs ForcedBranch): instruction index ->
+// the XOR plan its branch takes instead of a draw.
+struct Forced {
+  bool constant = false;
+  std::vector<int32_t> conds;
+};
+std::map<size_t, Forced> g_forced;
 
 struct OracleError : std::runtime_error {
   using std::runtime_error::runtime_error;
@@ -1153,6 +1161,7 @@ struct Runner {
         }
         case TICIT_OP_MEASURE: {
           if (k == 0) throw OracleError("cannot measure an active Pauli when k == 0");
+          const auto forced = g_forced.find(i);
           if (in.nq != k) throw OracleError("measurement kernel dimension does not match batch active state");
           if (in.exp_val >= 0) {
             // src/sampler/batch/active.rs:818-850: +-(1 - 2 P(true)), no
@@ -1188,7 +1197,19 @@ struct Runner {
             double pt = clamp01(prob_true(r, m, mk));
             if (probs_out) probs_out[measure_ordinal[i] * src.shots_total + src.first_shot + s] = pt;
             bool b;
-            if (src.injected) {
+            if (forced != g_forced.end()) {
+              // src/sampler/batch/active.rs:538-554: only a fair coin is pinned
+              if (std::fabs(pt - 0.5) > 1e-9)
+                throw OracleError("pinned measurement branch has probability " + std::to_string(pt) +
+                                  ", not one half; pinning it would bias sampling");
+              bool f = forced->second.constant;
+              for (int32_t c : forced->second.conds) {
+                if (c <= 0 || c > p.nsymbols || exogenous[c] || !assigned[c])
+                  throw OracleError("pinned branch reads an unassigned branch symbol");
+                f ^= bool((sym[size_t(c - 1) * words + (s >> 6)] >> (s & 63)) & 1);
+              }
+              b = f;
+            } else if (src.injected) {
               double u = src.uniforms[measure_ordinal[i] * src.shots_total + src.first_shot + s];
               b = pt <= 0.0 ? false : (pt >= 1.0 ? true : u < pt);
             } else {
@@ -1214,6 +1235,20 @@ struct Runner {
           break;
         }
         case TICIT_OP_COIN: {
+          const auto forced = g_forced.find(i);
+          if (forced != g_forced.end() && in.exp_val < 0) {
+            // a pinned dormant branch copies its plan, no draw
+            // (src/sampler/batch/runtime.rs:353-367)
+            Expr fe;
+            fe.constant = forced->second.constant;
+            fe.conds = forced->second.conds;
+            eval(fe);
+            std::vector<uint64_t> br(bits.begin(), bits.end());
+            assign(in.branch, br);
+            eval(in.expr);
+            write_record(in.record, in.record_condition, bits);
+            break;
+          }
           if (in.exp_val >= 0) {
             // src/sampler/batch/runtime.rs:353-356: an unbiased probe is 0
             size_t e = exp_slot(in.exp_val);
@@ -1337,6 +1372,22 @@ void oracle_set_component_mode(int mode) { g_component_mode = mode < 0 ? -1 : (m
 // 1: probability sums with Neumaier compensation instead of the reference's
 // plain scalar order, to measure the summation error of either side.
 void oracle_set_exact_sums(int on) { g_exact_sums = on != 0; }
+
+// Pinned branches for the following runs: spec = repeated (instruction,
+// constant, ncond, conds...); n = 0 clears.
+void oracle_set_forced(const int64_t* spec, size_t n) {
+  g_forced.clear();
+  size_t i = 0;
+  while (i + 3 <= n) {
+    Forced f;
+    size_t instr = size_t(spec[i]);
+    f.constant = spec[i + 1] != 0;
+    size_t nc = size_t(spec[i + 2]);
+    for (size_t j = 0; j < nc && i + 3 + j < n; ++j) f.conds.push_back(int32_t(spec[i + 3 + j]));
+    g_forced[instr] = f;
+    i += 3 + nc;
+  }
+}
 
 // The component plan of src/component_plan.rs:301-522.
 // summary[0..8] = selected, initial_components, component_count,

@@ -121,7 +121,7 @@ class CompiledSampler:
 
     def __init__(self, blob: np.ndarray, observable: int = 0, precision: str = "fp64",
                  tier: str = "auto", planner: str = "heuristic", wide_batch: int = 0,
-                 components: str = "planned", group_threads: int = 0):
+                 components: str = "planned", group_threads: int = 0, pin_measurements=None):
         lib = _capi.load()
         self._lib = lib
         self.blob = np.ascontiguousarray(blob, dtype=np.uint64)
@@ -137,8 +137,20 @@ class CompiledSampler:
         opt.components = COMPONENTS[components]
         opt.group_threads = group_threads
         handle = ctypes.c_void_p()
-        _capi.check(lib.ticit_hip_program_create(self.blob.ctypes.data_as(_capi._u64p), self.blob.size,
-                                                 ctypes.byref(opt), ctypes.byref(handle)))
+        if pin_measurements:
+            # SamplerOptions::pin_measurements: (records, value) pairs
+            recs = [np.ascontiguousarray([int(r) for r in records], dtype=np.uint64)
+                    for records, _ in pin_measurements]
+            pins = (_capi.Parity * len(recs))()
+            for j, (r, (_, value)) in enumerate(zip(recs, pin_measurements)):
+                pins[j].records = r.ctypes.data_as(_capi._u64p)
+                pins[j].nrecords = r.size
+                pins[j].value = int(bool(value))
+            _capi.check(lib.ticit_hip_program_create_pinned(self.blob.ctypes.data_as(_capi._u64p), self.blob.size,
+                                                            ctypes.byref(opt), pins, len(recs), ctypes.byref(handle)))
+        else:
+            _capi.check(lib.ticit_hip_program_create(self.blob.ctypes.data_as(_capi._u64p), self.blob.size,
+                                                     ctypes.byref(opt), ctypes.byref(handle)))
         self._handle = handle
         info = _capi.ProgramInfo()
         _capi.check(lib.ticit_hip_program_info_get(handle, ctypes.byref(info)))
@@ -291,7 +303,7 @@ class Circuit:
     def compile(self, postselection_mask=None, *, observable: int = 0, backend: str = "hip",
                 precision: str = "fp64", tier: str = "auto", planner: str = "heuristic",
                 threads: int | None = None, wide_batch: int = 0,
-                components: str = "planned", group_threads: int = 0) -> CompiledSampler:
+                components: str = "planned", group_threads: int = 0, pin_measurements=None) -> CompiledSampler:
         """Circuit::compile with ticit_py's ``backend=`` selector; only the
         ``"hip"`` backend lives in this package."""
         if backend != "hip":
@@ -301,7 +313,7 @@ class Circuit:
             blob = plan_text(self.text, postselection_mask)
         return CompiledSampler(blob, observable=observable, precision=precision, tier=tier,
                                planner=planner, wide_batch=wide_batch, components=components,
-                               group_threads=group_threads)
+                               group_threads=group_threads, pin_measurements=pin_measurements)
 
 
 def sample_batch(samplers, shots, seeds, bit_packed: bool = False, keep_records: bool = True) -> list:

@@ -49,6 +49,11 @@ class ProgramInfo(ctypes.Structure):
                 ("expectation_values", ctypes.c_uint64), ("postselected_detectors", ctypes.c_uint64)]
 
 
+class Parity(ctypes.Structure):
+    _fields_ = [("records", ctypes.POINTER(ctypes.c_uint64)), ("nrecords", ctypes.c_size_t),
+                ("value", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
 class BatchItem(ctypes.Structure):
     _fields_ = [("program", ctypes.c_void_p), ("shots", ctypes.c_uint64), ("seed", ctypes.c_uint64),
                 ("measurements", ctypes.c_void_p), ("detectors", ctypes.c_void_p),
@@ -76,7 +81,8 @@ STATUS_NAMES = {1: "InvalidInput", 2: "Parse", 3: "Unsupported", 4: "Internal",
 # every symbol include/ticit_hip.h declares
 EXPORTS = [
     "ticit_hip_last_error", "ticit_hip_plan_text", "ticit_hip_plan_text_postselected", "ticit_hip_blob_free",
-    "ticit_hip_program_create", "ticit_hip_compile_text", "ticit_hip_program_destroy",
+    "ticit_hip_program_create", "ticit_hip_program_create_pinned", "ticit_hip_compile_text",
+    "ticit_hip_program_destroy",
     "ticit_hip_program_info_get", "ticit_hip_sample", "ticit_hip_sample_device",
     "ticit_hip_sample_injected", "ticit_hip_final_state", "ticit_hip_sample_batch",
     "ticit_hip_result_free", "ticit_hip_plan_features",
@@ -114,6 +120,9 @@ def load() -> ctypes.CDLL:
     lib.ticit_hip_plan_text_postselected.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_void_p,
                                                     ctypes.c_size_t, ctypes.POINTER(_u64p),
                                                     ctypes.POINTER(ctypes.c_size_t)]
+    lib.ticit_hip_program_create_pinned.argtypes = [_u64p, ctypes.c_size_t, ctypes.c_void_p,
+                                                    ctypes.POINTER(Parity), ctypes.c_size_t,
+                                                    ctypes.POINTER(ctypes.c_void_p)]
     lib.ticit_hip_plan_text.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(_u64p),
                                         ctypes.POINTER(ctypes.c_size_t)]
     lib.ticit_hip_blob_free.argtypes = [_u64p]

@@ -22,6 +22,7 @@ SOURCES = [
     "csrc/host/frontend.cpp",
     "csrc/host/program.cpp",
     "csrc/host/component_plan.cpp",
+    "csrc/host/pinning.cpp",
     "csrc/host/device_program.cpp",
     "csrc/host/capi.cpp",
     "csrc/host/batch.cpp",

@@ -446,6 +446,8 @@ ticit_hip_status ticit_hip_batch_wait(ticit_hip_batch* b) {
   if ((e = hipMemcpy(&err, b->d_error, sizeof(int), hipMemcpyDeviceToHost)) != hipSuccess)
     return from_hip(e, "reading the batch error flag");
   if (err & 2) return fail(TICIT_HIP_INVALID_INPUT, "probability must be between 0 and 1");
+  if (err & 4)
+    return fail(TICIT_HIP_INVALID_INPUT, "pinned measurement branch is not one half; pinning it would bias sampling");
   if (err & 1) return fail(TICIT_HIP_INTERNAL, "sampled an impossible active measurement branch");
   return TICIT_HIP_OK;
 }

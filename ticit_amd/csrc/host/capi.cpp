@@ -14,6 +14,7 @@
 
 #include "component_plan.hpp"
 #include "handle.hpp"
+#include "pinning.hpp"
 
 using namespace ticit;
 
@@ -35,7 +36,8 @@ ticit_hip_status from_hip(hipError_t e, const char* what) {
 
 namespace {
 
-ticit_hip_status create_from_program(Program prog, const ticit_hip_options* options, ticit_hip_program** out) {
+ticit_hip_status create_from_program(Program prog, const ticit_hip_options* options, ticit_hip_program** out,
+                                     const std::vector<MeasurementParity>* pins = nullptr) {
   auto* h = new ticit_hip_program();
   h->prog = std::move(prog);
   if (options) h->opt = *options;
@@ -58,7 +60,9 @@ ticit_hip_status create_from_program(Program prog, const ticit_hip_options* opti
     }
   }
   try {
-    h->tab = lower_to_device(h->prog, use_components);
+    std::vector<ForcedBranch> forced;
+    if (pins) forced = plan_pinned_measurements(h->prog, *pins);
+    h->tab = lower_to_device(h->prog, use_components, &forced);
   } catch (const TicitError& e) {
     delete h;
     return fail(ticit_hip_status(e.kind), e.what());
@@ -328,6 +332,10 @@ ticit_hip_status run_host(ticit_hip_program* h, uint64_t shots, uint64_t seed, i
   int derr = 0;
   CK(hipMemcpy(&derr, h->d_error, sizeof(int), hipMemcpyDeviceToHost), "reading the error flag");
   if (derr & kDevErrBadProbability) return fail(TICIT_HIP_INVALID_INPUT, "probability must be between 0 and 1");
+  if (derr & kDevErrUnfairPin)
+    return fail(TICIT_HIP_INVALID_INPUT,
+                "pinned measurement branch has a probability other than one half (not one half within 1e-9); "
+                "pinning it would bias sampling");
   if (derr & kDevErrImpossibleBranch)
     return fail(TICIT_HIP_INVALID_INPUT, "sampled an impossible active measurement branch");
 
@@ -479,6 +487,26 @@ ticit_hip_status ticit_hip_program_create(const uint64_t* blob, size_t nwords, c
   if (!decode_program(blob, nwords, p, err)) return fail(TICIT_HIP_INVALID_INPUT, err);
   try {
     return create_from_program(std::move(p), options, out);
+  } catch (const std::exception& e) {
+    return fail(TICIT_HIP_INTERNAL, e.what());
+  }
+}
+
+ticit_hip_status ticit_hip_program_create_pinned(const uint64_t* blob, size_t nwords,
+                                                const ticit_hip_options* options, const ticit_hip_parity* pins,
+                                                size_t npins, ticit_hip_program** out) {
+  if (!blob || !out || (npins && !pins)) return fail(TICIT_HIP_INVALID_INPUT, "null argument");
+  std::vector<MeasurementParity> ps(npins);
+  for (size_t i = 0; i < npins; ++i) {
+    if (pins[i].nrecords && !pins[i].records) return fail(TICIT_HIP_INVALID_INPUT, "null argument");
+    ps[i].records.assign(pins[i].records, pins[i].records + pins[i].nrecords);
+    ps[i].value = pins[i].value != 0;
+  }
+  Program p;
+  std::string err;
+  if (!decode_program(blob, nwords, p, err)) return fail(TICIT_HIP_INVALID_INPUT, err);
+  try {
+    return create_from_program(std::move(p), options, out, &ps);
   } catch (const std::exception& e) {
     return fail(TICIT_HIP_INTERNAL, e.what());
   }

@@ -11,6 +11,7 @@
 
 #include "component_plan.hpp"
 #include "frontend.hpp"
+#include "pinning.hpp"
 
 namespace ticit {
 
@@ -117,7 +118,10 @@ void build_lane_schedule(HostTables& t) {
 
 }  // namespace
 
-HostTables lower_to_device(const Program& p, bool components) {
+HostTables lower_to_device(const Program& p, bool components, const std::vector<ForcedBranch>* forced) {
+  std::map<int32_t, const SymBool*> forced_at;
+  if (forced)
+    for (const ForcedBranch& f : *forced) forced_at[f.instruction] = &f.plan;
   HostTables t;
   t.n = p.n;
   t.initial_k = p.initial_k;
@@ -463,6 +467,22 @@ HostTables lower_to_device(const Program& p, bool components) {
         if (in.exp_val < 0) d.branch_slot = assign(in.branch);
         break;
     }
+    // a pinned branch (src/sampler/pinning.rs:281-288): XOR of branch symbols
+    // assigned before it, plus a constant
+    auto fit = forced_at.find(int32_t(ii));
+    if (fit != forced_at.end()) {
+      if ((in.op != Op::Measure && in.op != Op::Coin) || in.exp_val >= 0)
+        throw TicitError(4, "a pinned branch must belong to a measurement");
+      d.op |= kFlagForced;
+      if (fit->second->constant) d.op |= kFlagForcedOne;
+      d.forced_off = uint32_t(t.resid.size());
+      for (int32_t c : fit->second->conds) {
+        if (c <= 0 || c > p.nsymbols || exo[c] || !assigned[c])
+          throw TicitError(4, "a pinned branch reads a symbol not assigned before it");
+        t.resid.push_back(uint32_t(slot_of[c]));
+      }
+      d.forced_len = uint32_t(t.resid.size()) - d.forced_off;
+    }
     if (in.exp_val >= 0) {
       // src/sampler/batch/runtime.rs:46-75,311-356: a probe writes its
       // expectation slot instead of a record
@@ -537,7 +557,7 @@ HostTables lower_to_device(const Program& p, bool components) {
         s.c = int32_t(d.det_len);
         break;
       case kOpCoin:
-        s.full = d.ordinal;
+        s.full = (d.op & kFlagForced) ? int32_t(i) : d.ordinal;
         s.a = d.record;
         s.b = d.rc_slot;
         s.c = d.branch_slot;

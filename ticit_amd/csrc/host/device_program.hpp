@@ -56,6 +56,9 @@ struct HostTables {
 // tables. Throws TicitError on failure.
 // With components = true the instruction stream is lowered through the
 // tensor-component plan (throws if the program has none).
-HostTables lower_to_device(const Program& p, bool components = false);
+// forced: pinned branches (pinning.hpp), by instruction index.
+struct ForcedBranch;
+HostTables lower_to_device(const Program& p, bool components = false,
+                           const std::vector<ForcedBranch>* forced = nullptr);
 
 }  // namespace ticit

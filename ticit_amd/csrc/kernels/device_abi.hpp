@@ -36,6 +36,10 @@ enum : uint32_t {
   kFlagFresh = 1u << 14,        // promotion that starts a new one-qubit component
   kFlagZeroState = 1u << 15,    // fresh component in |0> (initial components)
   kFlagPostselect = 1u << 17,   // detector whose firing discards the shot (src/factored.rs:227)
+  kFlagForced = 1u << 18,       // pinned Measure/Coin: the branch is forced_const XOR the
+                                // slots forced_off..+forced_len, not drawn
+                                // (src/sampler/pinning.rs, src/sampler/batch/active.rs:535-554)
+  kFlagForcedOne = 1u << 19,    // forced_const
   kFlagExpVal = 1u << 16,       // expectation probe (Record/Measure/Coin): `record`
                                 // holds the expectation slot; no record, branch or
                                 // collapse (src/sampler/batch/runtime.rs:311-356)
@@ -60,6 +64,8 @@ struct alignas(16) DevInstr {
   int32_t src_comp;     // Merge: component folded in
   uint32_t src_off;     // Merge: its amplitude offset
   int32_t src_k;        // Merge: its width
+  uint32_t forced_off;   // pinned Measure/Coin: runtime-slot list of the forced branch
+  uint32_t forced_len;
   uint64_t xmask;
   uint64_t zmask;
   uint64_t zwop;        // z mask without the pivot bit (diagonal measurement)
@@ -83,7 +89,7 @@ struct alignas(16) DevStep {
   int32_t a;            // Record/Measure/Coin: record (-1 none); Detector: detector
   int32_t b;            // Record/Measure/Coin: rc_slot (-1 none); Detector: det_off
   int32_t c;            // Measure/Coin: branch_slot; Detector: det_len
-  int32_t full;         // index of the DevInstr (Coin: its ordinal)
+  int32_t full;         // index of the DevInstr (Coin: its ordinal, unless pinned)
 };
 
 // Exogenous noise plan (src/factored.rs:444-462, src/planner.rs:1455-1480).
@@ -217,6 +223,11 @@ struct WideLayout {
 enum : int {
   kDevErrImpossibleBranch = 1,
   kDevErrBadProbability = 2,
+  kDevErrUnfairPin = 4,  // a pinned active branch whose P(true) is not 1/2 within 1e-9
 };
+
+// Pinning a branch conditions the shot on it; only a fair coin may be pinned
+// (src/sampler/batch/active.rs:525,547).
+constexpr double kFairBranchTolerance = 1e-9;
 
 }  // namespace ticit

@@ -262,7 +262,13 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
         }
         if (valid) {
           if (L.want_probs && t == 0) L.probs[uint64_t(in.ordinal) * L.shots + local] = pt;
-          if (L.injected) {
+          if (in.op & kFlagForced) {
+            // pinned: the XOR of earlier branches, no draw; only a fair coin
+            bool f = (in.op & kFlagForcedOne) != 0;
+            for (uint32_t j = 0; j < in.forced_len; ++j) f ^= bit_of(sbits, P.resid[in.forced_off + j]);
+            branch = f;
+            if (fabs(pt - 0.5) > kFairBranchTolerance) err |= kDevErrUnfairPin;
+          } else if (L.injected) {
             double u = L.uniforms[uint64_t(in.ordinal) * L.shots + local];
             branch = pt <= 0.0 ? false : (pt >= 1.0 ? true : u < pt);
           } else if (!(pt >= 0.0 && pt <= 1.0)) {
@@ -310,7 +316,12 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
           }
         }
       } else if (valid) {
-        if (L.injected) {
+        if (st.op & kFlagForced) {  // pinned: the XOR of earlier branches, no draw
+          const DevInstr& fin = P.instrs[st.full];
+          bool f = (fin.op & kFlagForcedOne) != 0;
+          for (uint32_t j = 0; j < fin.forced_len; ++j) f ^= bit_of(sbits, P.resid[fin.forced_off + j]);
+          branch = f;
+        } else if (L.injected) {
           const uint64_t words = (L.shots + 63) / 64;
           branch = (L.coins[uint64_t(st.full) * words + (local >> 6)] >> (local & 63)) & 1;
         } else {

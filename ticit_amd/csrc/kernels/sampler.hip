@@ -147,7 +147,11 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
         continue;
       }
       uint64_t coins;
-      if (L.injected) {
+      if (st.op & kFlagForced) {  // pinned: the XOR of earlier branches, no draw
+        const DevInstr& fin = P.instrs[st.full];
+        coins = (fin.op & kFlagForcedOne) ? ~0ull : 0ull;
+        for (uint32_t j = 0; j < fin.forced_len; ++j) coins ^= Sw[P.resid[fin.forced_off + j]];
+      } else if (L.injected) {
         const uint64_t words = (L.shots + 63) / 64;
         coins = L.coins[uint64_t(st.full) * words + blockIdx.x];
       } else {
@@ -238,7 +242,13 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
         continue;
       }
       bool branch = false;
-      if (L.injected) {
+      if (in.op & kFlagForced) {
+        // pinned: the XOR of earlier branches, no draw; only a fair coin
+        uint64_t f = (in.op & kFlagForcedOne) ? ~0ull : 0ull;
+        for (uint32_t j = 0; j < in.forced_len; ++j) f ^= Sw[P.resid[in.forced_off + j]];
+        branch = (f >> lane) & 1;
+        if (live && fabs(pt - 0.5) > kFairBranchTolerance) err |= kDevErrUnfairPin;
+      } else if (L.injected) {
         if (live) {
           double u = L.uniforms[uint64_t(in.ordinal) * L.shots + local];
           branch = pt <= 0.0 ? false : (pt >= 1.0 ? true : u < pt);

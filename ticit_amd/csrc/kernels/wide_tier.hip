@@ -164,7 +164,10 @@ __global__ __launch_bounds__(kWideThreads) void wide_bits_kernel(DevProgram P, D
       wset(w.d, uint32_t(in.detector), b);
     } else if (op == kOpCoin) {
       bool coin;
-      if (L.injected) {
+      if (in.op & kFlagForced) {  // pinned: the XOR of earlier branches, no draw
+        coin = (in.op & kFlagForcedOne) != 0;
+        for (uint32_t q = 0; q < in.forced_len; ++q) coin ^= wbit(w.s, P.resid[in.forced_off + q]);
+      } else if (L.injected) {
         const uint64_t words = (L.shots + 63) / 64;
         coin = (L.coins[uint64_t(in.ordinal) * words + (local >> 6)] >> (local & 63)) & 1;
       } else {
@@ -487,7 +490,14 @@ __device__ void wide_decide_shot(const DevProgram& P, const DevLaunch& L, const 
   }
   WideCtl c = ctl[j];
   bool branch = false;
-  if (L.injected) {
+  if (in.op & kFlagForced) {
+    // pinned: the XOR of earlier branches, no draw; only a fair coin
+    const WideShot w0 = wide_shot(bits, Y, j);
+    bool f = (in.op & kFlagForcedOne) != 0;
+    for (uint32_t q = 0; q < in.forced_len; ++q) f ^= wbit(w0.s, P.resid[in.forced_off + q]);
+    branch = f;
+    if (fabs(pt - 0.5) > kFairBranchTolerance) c.err |= kDevErrUnfairPin;
+  } else if (L.injected) {
     double u = L.uniforms[uint64_t(in.ordinal) * L.shots + local];
     branch = pt <= 0.0 ? false : (pt >= 1.0 ? true : u < pt);
   } else if (!(pt >= 0.0 && pt <= 1.0)) {
