@@ -147,6 +147,42 @@ ticit_hip_status ticit_hip_program_create_pinned(const uint64_t* blob, size_t nw
                                                 const ticit_hip_options* options, const ticit_hip_parity* pins,
                                                 size_t npins, ticit_hip_program** out);
 
+/* SamplerOptions::normalize_syndromes, expected_detectors and
+ * expected_observables (src/sampler/prepared.rs:111-116). A nonzero
+ * expected bit inverts that column in every row, in detector postselection
+ * (a postselected detector discards a shot when its value differs from the
+ * expected bit) and in the observable counts (src/sampler/prepared.rs:909-974,
+ * src/sampler/batch/postselect.rs:631-660). An empty list inverts nothing; a
+ * non-empty one has one entry per detector / observable.
+ * normalize_syndromes = 1 takes the bits from the noiseless reference sample
+ * (ticit_hip_reference_sample) and excludes explicit bits
+ * (src/sampler/prepared.rs:515-550). */
+typedef struct ticit_hip_reference_bits {
+  const uint8_t* detectors;
+  size_t ndetectors;
+  const uint8_t* observables;
+  size_t nobservables;
+  int32_t normalize_syndromes;
+  int32_t reserved;
+} ticit_hip_reference_bits;
+
+/* Sampler::new with every SamplerOptions field (src/sampler/prepared.rs:
+ * 492-626): pinned parities (may be 0 / null) and reference bits (null:
+ * none). Errors as the reference's: "normalize_syndromes cannot be combined
+ * with expected_detectors or expected_observables", "expected_detectors has
+ * length N, expected M", and those of ticit_hip_program_create_pinned. */
+ticit_hip_status ticit_hip_program_create_ex(const uint64_t* blob, size_t nwords, const ticit_hip_options* options,
+                                            const ticit_hip_parity* pins, size_t npins,
+                                            const ticit_hip_reference_bits* reference, ticit_hip_program** out);
+
+/* Circuit::reference_sample / reference_sample_for_program
+ * (src/circuit/mod.rs:89-91, src/sampler/prepared.rs:355-411): one
+ * noiseless shot (every noise symbol 0, branches drawn from the SplitMix64
+ * stream started at state 1, the program's pins obeyed, no postselection),
+ * run in FP64 on the device. Writes ndetectors and nobservables bytes
+ * (0 / 1), before any expected bits of the program. */
+ticit_hip_status ticit_hip_reference_sample(ticit_hip_program* program, uint8_t* detectors, uint8_t* observables);
+
 /* Circuit::compile in one call: ticit_hip_plan_text + ticit_hip_program_create. */
 ticit_hip_status ticit_hip_compile_text(const char* text, size_t len, const ticit_hip_options* options,
                                         ticit_hip_program** out);

@@ -56,6 +56,9 @@ def _load(name: str) -> ctypes.CDLL:
     lib.oracle_set_component_mode.argtypes = [ctypes.c_int]
     lib.oracle_set_exact_sums.argtypes = [ctypes.c_int]
     lib.oracle_set_forced.argtypes = [_i64p, ctypes.c_size_t]
+    lib.oracle_set_expected.argtypes = [_u8p, ctypes.c_size_t, _u8p, ctypes.c_size_t]
+    lib.oracle_set_expected.restype = None
+    lib.oracle_reference_sample.argtypes = [_u64p, ctypes.c_size_t, _u8p, _u8p]
     lib.oracle_component_plan.argtypes = [_u64p, ctypes.c_size_t, _i64p, _f64p, _i64p, _i64p, _i64p,
                                           ctypes.c_size_t]
     lib.oracle_default_batch.argtypes = [ctypes.c_int]
@@ -265,3 +268,26 @@ def set_pins(blob: np.ndarray | None, constraints=None) -> list:
         if name in _LIBS or name == "liboracle.so":
             lib(name == "liboracle_fast.so").oracle_set_forced(_ptr(arr, _i64p), len(spec))
     return forced
+
+
+def set_expected(detectors=None, observables=None) -> None:
+    """Expected detector / observable bits for the following runs on this
+    process (nonzero inverts the column); None or empty clears."""
+    dl = [int(bool(v)) for v in (detectors if detectors is not None else [])]
+    ol = [int(bool(v)) for v in (observables if observables is not None else [])]
+    det = np.ascontiguousarray(dl or [0], dtype=np.uint8)
+    obs = np.ascontiguousarray(ol or [0], dtype=np.uint8)
+    nd, no = len(dl), len(ol)
+    for name in ("liboracle.so", "liboracle_fast.so"):
+        if name in _LIBS or name == "liboracle.so":
+            lib(name == "liboracle_fast.so").oracle_set_expected(_ptr(det, _u8p), nd, _ptr(obs, _u8p), no)
+
+
+def reference_sample(blob: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """The noiseless reference sample: (detector bits, observable bits),
+    obeying the pins set with set_pins."""
+    inf = info(blob)
+    det = np.zeros(max(1, inf["ndetectors"]), dtype=np.uint8)
+    obs = np.zeros(max(1, inf["nobservables"]), dtype=np.uint8)
+    _check(lib().oracle_reference_sample(_ptr(blob, _u64p), blob.size, _ptr(det, _u8p), _ptr(obs, _u8p)))
+    return det[:inf["ndetectors"]], obs[:inf["nobservables"]]

@@ -47,6 +47,10 @@ struct Forced {
   std::vector<int32_t> conds;
 };
 std::map<size_t, Forced> g_forced;
+// SamplerOptions::expected_detectors / expected_observables
+// (src/sampler/prepared.rs:111-116): nonzero inverts that column; empty
+// inverts nothing. Read by worker threads: not thread-local.
+std::vector<uint8_t> g_expected_det, g_expected_obs;
 
 struct OracleError : std::runtime_error {
   using std::runtime_error::runtime_error;
@@ -891,6 +895,10 @@ struct Runner {
   std::vector<char> exogenous;
   std::vector<double> re, im, sre, sim;
   bool fp32 = false;  // emulate FP32 amplitude storage (precision studies)
+  // the reference shot (src/sampler/prepared.rs:355-405) runs without the
+  // expected bits and without postselection
+  bool apply_expected = true;
+  bool postselect = true;
   // tensor-component execution (src/sampler/batch/runtime.rs:388-552)
   CompPlan cplan;
   bool comp = false;
@@ -1154,8 +1162,13 @@ struct Runner {
             eval(in.expr);
           }
           if (in.detector <= 0 || in.detector > p.ndetectors) throw OracleError("detector id out of range");
+          // src/sampler/prepared.rs:926-936 XORs the expected bit into the
+          // row; src/sampler/batch/postselect.rs:631-660 discards a shot
+          // whose value differs from it
+          if (apply_expected && !g_expected_det.empty() && g_expected_det[size_t(in.detector - 1)])
+            for (size_t w = 0; w < words; ++w) bits[w] = ~bits[w];
           for (size_t w = 0; w < words; ++w) det[size_t(in.detector - 1) * words + w] = bits[w] & live_mask(shots, w);
-          if (in.postselect)
+          if (in.postselect && postselect)
             for (size_t w = 0; w < words; ++w) dead[w] |= bits[w] & live_mask(shots, w);
           break;
         }
@@ -1289,6 +1302,12 @@ struct Runner {
         if (r <= 0 || r > p.nrecords) throw OracleError("observable references an out-of-range measurement record");
         for (size_t w = 0; w < words; ++w) ow[o * words + w] ^= rec[size_t(r - 1) * words + w];
       }
+    // src/sampler/prepared.rs:965-974: an expected observable of 1 flips
+    // the column before counting
+    if (apply_expected && !g_expected_obs.empty())
+      for (size_t o = 0; o < nobs; ++o)
+        if (g_expected_obs[o])
+          for (size_t w = 0; w < words; ++w) ow[o * words + w] ^= live_mask(shots, w);
     std::vector<uint64_t> keep(words);
     uint64_t nkeep = 0;
     for (size_t w = 0; w < words; ++w) {
@@ -1331,6 +1350,16 @@ struct Runner {
     }
   }
 };
+
+// src/sampler/prepared.rs:523-540
+void check_expected(const Program& p) {
+  if (!g_expected_det.empty() && g_expected_det.size() != size_t(p.ndetectors))
+    throw OracleError("expected_detectors has length " + std::to_string(g_expected_det.size()) + ", expected " +
+                      std::to_string(p.ndetectors));
+  if (!g_expected_obs.empty() && g_expected_obs.size() != p.observables.size())
+    throw OracleError("expected_observables has length " + std::to_string(g_expected_obs.size()) + ", expected " +
+                      std::to_string(p.observables.size()));
+}
 
 size_t default_batch(int max_k) {
   if (max_k >= 62) throw OracleError("active qubit count is too large for machine basis indices");
@@ -1461,6 +1490,7 @@ int oracle_sample(const uint64_t* blob, size_t nwords, uint64_t shots, uint64_t 
                   uint8_t* obs, uint64_t* counts, uint64_t* observable_ones, double* timing, double* exp_vals) {
   try {
     Program p = decode(blob, nwords);
+    check_expected(p);
     size_t batch = batch_size ? size_t(batch_size) : default_batch(p.max_k);
     size_t chunk = chunk_shots ? size_t(chunk_shots) : (batch > 2048 ? batch : 2048);
     // The reference compacts dead shots out of a block before every draw
@@ -1549,6 +1579,7 @@ int oracle_sample_injected(const uint64_t* blob, size_t nwords, uint64_t shots, 
                            double* probs, double* final_re, double* final_im, int* final_k, double* exp_vals) {
   try {
     Program p = decode(blob, nwords);
+    check_expected(p);
     Runner run(p, bit_packed != 0, observable, g_component_mode);
     run.fp32 = g_fp32_storage;
     Outputs o;
@@ -1579,6 +1610,44 @@ int oracle_sample_injected(const uint64_t* blob, size_t nwords, uint64_t shots, 
     g_err = e.what();
     return 1;
   }
+}
+
+// reference_sample_for_program (src/sampler/prepared.rs:355-405): one
+// noiseless shot (every exogenous symbol 0) whose branches come from the
+// stream the reference's one-shot executor starts at state 1
+// (BatchFactoredExecutorState::new(program, 1, 1), src/sampler/batch/mod.rs:
+// 451-465), obeying the pinned branches, without postselection or expected
+// bits. det/obs receive ndetectors / nobservables bytes (0 / 1).
+int oracle_reference_sample(const uint64_t* blob, size_t nwords, uint8_t* det, uint8_t* obs) {
+  try {
+    Program p = decode(blob, nwords);
+    const size_t nobs = p.observables.size();
+    if (p.ndetectors == 0 && nobs == 0) return 0;
+    Runner run(p, false, 0, g_component_mode);
+    run.apply_expected = false;
+    run.postselect = false;
+    Outputs o;
+    o.obs_ones.assign(nobs, 0);
+    ExoTable exo;
+    exo.shots = 1;
+    exo.words = 1;
+    exo.vals.assign(size_t(p.nsymbols), 0);
+    Source src;
+    src.rng = 1;
+    run.run_block(exo, 0, 1, src, o, nullptr, false);
+    if (p.ndetectors) std::memcpy(det, o.det.data(), size_t(p.ndetectors));
+    if (nobs) std::memcpy(obs, o.obs.data(), nobs);
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+// Expected bits for the following runs (nonzero inverts); n = 0 clears.
+void oracle_set_expected(const uint8_t* det, size_t ndet, const uint8_t* obs, size_t nobs) {
+  g_expected_det.assign(det, det + (det ? ndet : 0));
+  g_expected_obs.assign(obs, obs + (obs ? nobs : 0));
 }
 
 // Exogenous presample of one chunk, exposed so tests can pin the noise

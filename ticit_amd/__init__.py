@@ -16,10 +16,11 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _capi
-from ._capi import TicitError
+from ._capi import InvalidInputError, ParseError, TicitError
 from .planner import planner_features
 
-__all__ = ["Batch", "Circuit", "CompiledSampler", "SampleResult", "TicitError", "plan_features", "plan_text"]
+__all__ = ["Batch", "Circuit", "CompiledSampler", "InvalidInputError", "ParseError", "ReferenceSample", "SampleResult",
+           "TicitError", "plan_features", "plan_text"]
 
 TIERS = {"auto": 0, "lane": 1, "group": 2, "wide": 3}
 PRECISIONS = {"fp64": 0, "fp32": 1}
@@ -113,6 +114,18 @@ def _convert(res: _capi.Result, keep_records: bool) -> SampleResult:
         exp_vals=ev)
 
 
+class ReferenceSample:
+    """Circuit::reference_sample's result (src/sampler/prepared.rs:221-228):
+    one bool per detector and per observable."""
+
+    def __init__(self, detectors: list, observables: list):
+        self.detectors = detectors
+        self.observables = observables
+
+    def __repr__(self) -> str:
+        return f"ReferenceSample(detectors={self.detectors}, observables={self.observables})"
+
+
 class CompiledSampler:
     """A compiled circuit bound to the current HIP device (Sampler,
     src/sampler/prepared.rs:467-478)."""
@@ -121,7 +134,8 @@ class CompiledSampler:
 
     def __init__(self, blob: np.ndarray, observable: int = 0, precision: str = "fp64",
                  tier: str = "auto", planner: str = "heuristic", wide_batch: int = 0,
-                 components: str = "planned", group_threads: int = 0, pin_measurements=None):
+                 components: str = "planned", group_threads: int = 0, pin_measurements=None,
+                 normalize_syndromes: bool = False, expected_detectors=None, expected_observables=None):
         lib = _capi.load()
         self._lib = lib
         self.blob = np.ascontiguousarray(blob, dtype=np.uint64)
@@ -137,17 +151,32 @@ class CompiledSampler:
         opt.components = COMPONENTS[components]
         opt.group_threads = group_threads
         handle = ctypes.c_void_p()
-        if pin_measurements:
+        reference = None
+        if normalize_syndromes or expected_detectors is not None or expected_observables is not None:
+            # SamplerOptions::normalize_syndromes / expected_detectors /
+            # expected_observables (src/sampler/prepared.rs:111-116)
+            det = np.ascontiguousarray([int(bool(v)) for v in (expected_detectors or [])], dtype=np.uint8)
+            obs = np.ascontiguousarray([int(bool(v)) for v in (expected_observables or [])], dtype=np.uint8)
+            reference = _capi.ReferenceBits()
+            reference.detectors = det.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+            reference.ndetectors = det.size
+            reference.observables = obs.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+            reference.nobservables = obs.size
+            reference.normalize_syndromes = int(bool(normalize_syndromes))
+            self._reference_arrays = (det, obs)
+        if pin_measurements or reference is not None:
             # SamplerOptions::pin_measurements: (records, value) pairs
+            pin_measurements = pin_measurements or []
             recs = [np.ascontiguousarray([int(r) for r in records], dtype=np.uint64)
                     for records, _ in pin_measurements]
-            pins = (_capi.Parity * len(recs))()
+            pins = (_capi.Parity * max(1, len(recs)))()
             for j, (r, (_, value)) in enumerate(zip(recs, pin_measurements)):
                 pins[j].records = r.ctypes.data_as(_capi._u64p)
                 pins[j].nrecords = r.size
                 pins[j].value = int(bool(value))
-            _capi.check(lib.ticit_hip_program_create_pinned(self.blob.ctypes.data_as(_capi._u64p), self.blob.size,
-                                                            ctypes.byref(opt), pins, len(recs), ctypes.byref(handle)))
+            _capi.check(lib.ticit_hip_program_create_ex(
+                self.blob.ctypes.data_as(_capi._u64p), self.blob.size, ctypes.byref(opt), pins, len(recs),
+                ctypes.byref(reference) if reference is not None else None, ctypes.byref(handle)))
         else:
             _capi.check(lib.ticit_hip_program_create(self.blob.ctypes.data_as(_capi._u64p), self.blob.size,
                                                      ctypes.byref(opt), ctypes.byref(handle)))
@@ -181,6 +210,16 @@ class CompiledSampler:
         self.num_expectation_values = int(info.expectation_values)
         self.num_postselected_detectors = int(info.postselected_detectors)
         self.has_postselection = self.num_postselected_detectors > 0
+
+    def reference_sample(self) -> "ReferenceSample":
+        """The noiseless reference sample of this program, obeying its pins
+        (reference_sample_for_program, src/sampler/prepared.rs:355-405),
+        computed on the device."""
+        det = np.zeros(max(1, self.num_detectors), dtype=np.uint8)
+        obs = np.zeros(max(1, self.num_observables), dtype=np.uint8)
+        _capi.check(self._lib.ticit_hip_reference_sample(self._handle, det.ctypes.data, obs.ctypes.data))
+        return ReferenceSample([bool(v) for v in det[:self.num_detectors]],
+                               [bool(v) for v in obs[:self.num_observables]])
 
     def close(self) -> None:
         if getattr(self, "_handle", None):
@@ -303,7 +342,9 @@ class Circuit:
     def compile(self, postselection_mask=None, *, observable: int = 0, backend: str = "hip",
                 precision: str = "fp64", tier: str = "auto", planner: str = "heuristic",
                 threads: int | None = None, wide_batch: int = 0,
-                components: str = "planned", group_threads: int = 0, pin_measurements=None) -> CompiledSampler:
+                components: str = "planned", group_threads: int = 0, pin_measurements=None,
+                normalize_syndromes: bool = False, expected_detectors=None,
+                expected_observables=None) -> CompiledSampler:
         """Circuit::compile with ticit_py's ``backend=`` selector; only the
         ``"hip"`` backend lives in this package."""
         if backend != "hip":
@@ -313,7 +354,18 @@ class Circuit:
             blob = plan_text(self.text, postselection_mask)
         return CompiledSampler(blob, observable=observable, precision=precision, tier=tier,
                                planner=planner, wide_batch=wide_batch, components=components,
-                               group_threads=group_threads, pin_measurements=pin_measurements)
+                               group_threads=group_threads, pin_measurements=pin_measurements,
+                               normalize_syndromes=normalize_syndromes, expected_detectors=expected_detectors,
+                               expected_observables=expected_observables)
+
+    def reference_sample(self) -> ReferenceSample:
+        """Circuit::reference_sample (src/circuit/mod.rs:89-91): the
+        noiseless detector and observable values, no pins."""
+        s = CompiledSampler(self.blob)
+        try:
+            return s.reference_sample()
+        finally:
+            s.close()
 
 
 def sample_batch(samplers, shots, seeds, bit_packed: bool = False, keep_records: bool = True) -> list:
@@ -388,6 +440,16 @@ class Batch:
                     buckets=[dict(threads_per_shot=int(inf.bucket_tps[j]), items=int(inf.bucket_items[j]),
                                   region_bytes=int(inf.bucket_region_bytes[j]), grid=int(inf.bucket_grid[j]),
                                   tuned_ms=[float(v) for v in inf.bucket_tuned_ms[j]]) for j in range(nb)])
+
+    def reference_sample(self) -> "ReferenceSample":
+        """The noiseless reference sample of this program, obeying its pins
+        (reference_sample_for_program, src/sampler/prepared.rs:355-405),
+        computed on the device."""
+        det = np.zeros(max(1, self.num_detectors), dtype=np.uint8)
+        obs = np.zeros(max(1, self.num_observables), dtype=np.uint8)
+        _capi.check(self._lib.ticit_hip_reference_sample(self._handle, det.ctypes.data, obs.ctypes.data))
+        return ReferenceSample([bool(v) for v in det[:self.num_detectors]],
+                               [bool(v) for v in obs[:self.num_observables]])
 
     def close(self) -> None:
         if getattr(self, "_handle", None):

@@ -54,6 +54,12 @@ class Parity(ctypes.Structure):
                 ("value", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
+class ReferenceBits(ctypes.Structure):
+    _fields_ = [("detectors", ctypes.POINTER(ctypes.c_uint8)), ("ndetectors", ctypes.c_size_t),
+                ("observables", ctypes.POINTER(ctypes.c_uint8)), ("nobservables", ctypes.c_size_t),
+                ("normalize_syndromes", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
 class BatchItem(ctypes.Structure):
     _fields_ = [("program", ctypes.c_void_p), ("shots", ctypes.c_uint64), ("seed", ctypes.c_uint64),
                 ("measurements", ctypes.c_void_p), ("detectors", ctypes.c_void_p),
@@ -81,7 +87,8 @@ STATUS_NAMES = {1: "InvalidInput", 2: "Parse", 3: "Unsupported", 4: "Internal",
 # every symbol include/ticit_hip.h declares
 EXPORTS = [
     "ticit_hip_last_error", "ticit_hip_plan_text", "ticit_hip_plan_text_postselected", "ticit_hip_blob_free",
-    "ticit_hip_program_create", "ticit_hip_program_create_pinned", "ticit_hip_compile_text",
+    "ticit_hip_program_create", "ticit_hip_program_create_pinned", "ticit_hip_program_create_ex",
+    "ticit_hip_reference_sample", "ticit_hip_compile_text",
     "ticit_hip_program_destroy",
     "ticit_hip_program_info_get", "ticit_hip_sample", "ticit_hip_sample_device",
     "ticit_hip_sample_injected", "ticit_hip_final_state", "ticit_hip_sample_batch",
@@ -123,6 +130,10 @@ def load() -> ctypes.CDLL:
     lib.ticit_hip_program_create_pinned.argtypes = [_u64p, ctypes.c_size_t, ctypes.c_void_p,
                                                     ctypes.POINTER(Parity), ctypes.c_size_t,
                                                     ctypes.POINTER(ctypes.c_void_p)]
+    lib.ticit_hip_program_create_ex.argtypes = [_u64p, ctypes.c_size_t, ctypes.c_void_p,
+                                                ctypes.POINTER(Parity), ctypes.c_size_t,
+                                                ctypes.POINTER(ReferenceBits), ctypes.POINTER(ctypes.c_void_p)]
+    lib.ticit_hip_reference_sample.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     lib.ticit_hip_plan_text.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(_u64p),
                                         ctypes.POINTER(ctypes.c_size_t)]
     lib.ticit_hip_blob_free.argtypes = [_u64p]
@@ -192,6 +203,19 @@ class TicitError(RuntimeError):
         self.kind = STATUS_NAMES.get(status, "Internal")
 
 
+class InvalidInputError(TicitError, ValueError):
+    """InvalidInput and Unsupported statuses: a ValueError, as ticit_py
+    raises for them (ticit_py/src/lib.rs:24-35)."""
+
+
+class ParseError(TicitError, ValueError):
+    """A circuit failed to parse (ticit_py/src/lib.rs:17-22)."""
+
+
+_ERROR_CLASSES = {1: InvalidInputError, 2: ParseError, 3: InvalidInputError}
+
+
 def check(status: int) -> None:
     if status != 0:
-        raise TicitError(status, load().ticit_hip_last_error().decode(errors="replace"))
+        cls = _ERROR_CLASSES.get(status, TicitError)
+        raise cls(status, load().ticit_hip_last_error().decode(errors="replace"))

@@ -36,8 +36,39 @@ ticit_hip_status from_hip(hipError_t e, const char* what) {
 
 namespace {
 
+// SamplerOptions::normalize_syndromes and expected_* (src/sampler/prepared.rs:111-116).
+struct ReferenceRequest {
+  ExpectedBits bits;
+  bool normalize = false;
+};
+
+ticit_hip_status reference_rows(ticit_hip_program* h, std::vector<uint8_t>& det, std::vector<uint8_t>& obs);
+
 ticit_hip_status create_from_program(Program prog, const ticit_hip_options* options, ticit_hip_program** out,
-                                     const std::vector<MeasurementParity>* pins = nullptr) {
+                                     const std::vector<MeasurementParity>* pins = nullptr,
+                                     const ReferenceRequest* ref = nullptr) {
+  // src/sampler/prepared.rs:515-550: the explicit bits and the noiseless
+  // reference exclude each other; the reference obeys the same pins
+  ExpectedBits expected;
+  if (ref) {
+    if (ref->normalize && (!ref->bits.detectors.empty() || !ref->bits.observables.empty()))
+      return fail(TICIT_HIP_INVALID_INPUT,
+                  "normalize_syndromes cannot be combined with expected_detectors or expected_observables");
+    expected = ref->bits;
+    if (ref->normalize && (prog.ndetectors > 0 || !prog.observables.empty())) {
+      // the reference shot runs in FP64 on the planned layout, whatever
+      // this sampler's precision and tier
+      ticit_hip_options ro = options ? *options : ticit_hip_options{};
+      ro.precision = 0;
+      ro.tier = 0;
+      ticit_hip_program* rh = nullptr;
+      ticit_hip_status st = create_from_program(prog, &ro, &rh, pins, nullptr);
+      if (st != TICIT_HIP_OK) return st;
+      st = reference_rows(rh, expected.detectors, expected.observables);
+      delete rh;
+      if (st != TICIT_HIP_OK) return st;
+    }
+  }
   auto* h = new ticit_hip_program();
   h->prog = std::move(prog);
   if (options) h->opt = *options;
@@ -62,7 +93,9 @@ ticit_hip_status create_from_program(Program prog, const ticit_hip_options* opti
   try {
     std::vector<ForcedBranch> forced;
     if (pins) forced = plan_pinned_measurements(h->prog, *pins);
-    h->tab = lower_to_device(h->prog, use_components, &forced);
+    h->tab = lower_to_device(h->prog, use_components, &forced, &expected);
+    h->expected = std::move(expected);
+    if (pins) h->pins = *pins;
   } catch (const TicitError& e) {
     delete h;
     return fail(ticit_hip_status(e.kind), e.what());
@@ -93,6 +126,7 @@ ticit_hip_status create_from_program(Program prog, const ticit_hip_options* opti
   UP(post_dets, t.post_dets);
   UP(obs_off, t.obs_off);
   UP(obs_recs, t.obs_recs);
+  UP(obs_flip, t.obs_flip);
   UP(sym_off, t.sym_off);
   UP(sym_part, t.sym_part);
   UP(exo_syms, t.exo_syms);
@@ -230,7 +264,7 @@ struct DeviceBuffers {
 ticit_hip_status run_host(ticit_hip_program* h, uint64_t shots, uint64_t seed, int bit_packed, int keep_records,
                           bool injected, const uint64_t* exo, const double* uniforms, const uint64_t* coins,
                           double* probs, ticit_hip_result* out, double* state_re = nullptr,
-                          double* state_im = nullptr, int* state_k = nullptr) {
+                          double* state_im = nullptr, int* state_k = nullptr, bool reference = false) {
   auto t0 = std::chrono::steady_clock::now();
   std::memset(out, 0, sizeof(*out));
   const HostTables& t = h->tab;
@@ -242,6 +276,7 @@ ticit_hip_status run_host(ticit_hip_program* h, uint64_t shots, uint64_t seed, i
   L.keep_records = keep_records != 0;
   L.injected = injected;
   L.want_probs = probs != nullptr;
+  L.reference = reference;
   L.meas_row = h->row_bytes(uint64_t(t.nrec), bit_packed);
   L.det_row = h->row_bytes(uint64_t(t.ndet), bit_packed);
   L.obs_row = h->row_bytes(uint64_t(t.nobs), bit_packed);
@@ -279,7 +314,7 @@ ticit_hip_status run_host(ticit_hip_program* h, uint64_t shots, uint64_t seed, i
     }
   }
   CK(buf.alloc(&L.obs_ones, size_t(std::max(1, t.nobs))), "allocating observable counts");
-  if (t.npost > 0) {
+  if (t.npost > 0 && !reference) {  // the reference shot is never discarded
     CK(buf.alloc(&d_keep, size_t(shots)), "allocating postselection flags");
     L.keep = d_keep;
   }
@@ -345,7 +380,7 @@ ticit_hip_status run_host(ticit_hip_program* h, uint64_t shots, uint64_t seed, i
   // postselection: the kept shots' rows, in shot order
   // (src/sampler/prepared.rs:881-1028 appends only survivors)
   std::vector<uint8_t> keep;
-  if (t.npost > 0) {
+  if (d_keep) {
     keep.resize(size_t(shots));
     CK(hipMemcpy(keep.data(), d_keep, size_t(shots), hipMemcpyDeviceToHost), "reading postselection flags");
     uint64_t acc = 0;
@@ -399,6 +434,41 @@ ticit_hip_status run_host(ticit_hip_program* h, uint64_t shots, uint64_t seed, i
   out->kernel_s = double(ms) * 1e-3;
   out->sample_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return TICIT_HIP_OK;
+}
+
+// The noiseless reference sample (reference_sample_for_program,
+// src/sampler/prepared.rs:355-405): one shot with every exogenous symbol 0
+// and the branch stream started from state 1, as the reference's one-shot
+// executor runs it; rows come back unpacked with the handle's own expected
+// bits taken out again.
+ticit_hip_status reference_rows(ticit_hip_program* h, std::vector<uint8_t>& det, std::vector<uint8_t>& obs) {
+  const size_t nd = size_t(h->tab.ndet), no = size_t(h->tab.nobs);
+  det.assign(nd, 0);
+  obs.assign(no, 0);
+  if (nd == 0 && no == 0) return TICIT_HIP_OK;
+  if (h->opt.precision != 0) {
+    // the reference runs in FP64: an FP64 twin with the same pins
+    ticit_hip_options ro = h->opt;
+    ro.precision = 0;
+    ro.tier = 0;
+    ticit_hip_program* rh = nullptr;
+    ticit_hip_status st = create_from_program(h->prog, &ro, &rh, &h->pins, nullptr);
+    if (st != TICIT_HIP_OK) return st;
+    st = reference_rows(rh, det, obs);
+    delete rh;
+    return st;
+  }
+  ticit_hip_result r{};
+  ticit_hip_status st = run_host(h, 1, 0, 0, 1, false, nullptr, nullptr, nullptr, nullptr, &r, nullptr, nullptr,
+                                 nullptr, true);
+  if (st == TICIT_HIP_OK) {
+    for (size_t d = 0; d < nd; ++d)
+      det[d] = uint8_t((r.detectors[d] != 0) != (!h->expected.detectors.empty() && h->expected.detectors[d] != 0));
+    for (size_t o = 0; o < no; ++o)
+      obs[o] = uint8_t((r.observables[o] != 0) != (!h->expected.observables.empty() && h->expected.observables[o] != 0));
+  }
+  ticit_hip_result_free(&r);
+  return st;
 }
 
 }  // namespace
@@ -495,7 +565,21 @@ ticit_hip_status ticit_hip_program_create(const uint64_t* blob, size_t nwords, c
 ticit_hip_status ticit_hip_program_create_pinned(const uint64_t* blob, size_t nwords,
                                                 const ticit_hip_options* options, const ticit_hip_parity* pins,
                                                 size_t npins, ticit_hip_program** out) {
+  return ticit_hip_program_create_ex(blob, nwords, options, pins, npins, nullptr, out);
+}
+
+ticit_hip_status ticit_hip_program_create_ex(const uint64_t* blob, size_t nwords, const ticit_hip_options* options,
+                                            const ticit_hip_parity* pins, size_t npins,
+                                            const ticit_hip_reference_bits* reference, ticit_hip_program** out) {
   if (!blob || !out || (npins && !pins)) return fail(TICIT_HIP_INVALID_INPUT, "null argument");
+  ReferenceRequest rq;
+  if (reference) {
+    if ((reference->ndetectors && !reference->detectors) || (reference->nobservables && !reference->observables))
+      return fail(TICIT_HIP_INVALID_INPUT, "null argument");
+    rq.bits.detectors.assign(reference->detectors, reference->detectors + reference->ndetectors);
+    rq.bits.observables.assign(reference->observables, reference->observables + reference->nobservables);
+    rq.normalize = reference->normalize_syndromes != 0;
+  }
   std::vector<MeasurementParity> ps(npins);
   for (size_t i = 0; i < npins; ++i) {
     if (pins[i].nrecords && !pins[i].records) return fail(TICIT_HIP_INVALID_INPUT, "null argument");
@@ -506,7 +590,7 @@ ticit_hip_status ticit_hip_program_create_pinned(const uint64_t* blob, size_t nw
   std::string err;
   if (!decode_program(blob, nwords, p, err)) return fail(TICIT_HIP_INVALID_INPUT, err);
   try {
-    return create_from_program(std::move(p), options, out, &ps);
+    return create_from_program(std::move(p), options, out, &ps, reference ? &rq : nullptr);
   } catch (const std::exception& e) {
     return fail(TICIT_HIP_INTERNAL, e.what());
   }
@@ -526,6 +610,21 @@ ticit_hip_status ticit_hip_compile_text(const char* text, size_t len, const tici
 }
 
 void ticit_hip_program_destroy(ticit_hip_program* program) { delete program; }
+
+ticit_hip_status ticit_hip_reference_sample(ticit_hip_program* h, uint8_t* detectors, uint8_t* observables) {
+  if (!h || (h->tab.ndet > 0 && !detectors) || (h->tab.nobs > 0 && !observables))
+    return fail(TICIT_HIP_INVALID_INPUT, "null argument");
+  std::vector<uint8_t> d, o;
+  try {
+    ticit_hip_status st = reference_rows(h, d, o);
+    if (st != TICIT_HIP_OK) return st;
+  } catch (const std::exception& e) {
+    return fail(TICIT_HIP_INTERNAL, e.what());
+  }
+  if (!d.empty()) std::memcpy(detectors, d.data(), d.size());
+  if (!o.empty()) std::memcpy(observables, o.data(), o.size());
+  return TICIT_HIP_OK;
+}
 
 ticit_hip_status ticit_hip_program_info_get(const ticit_hip_program* h, ticit_hip_program_info* out) {
   if (!h || !out) return fail(TICIT_HIP_INVALID_INPUT, "null argument");

@@ -118,7 +118,14 @@ void build_lane_schedule(HostTables& t) {
 
 }  // namespace
 
-HostTables lower_to_device(const Program& p, bool components, const std::vector<ForcedBranch>* forced) {
+HostTables lower_to_device(const Program& p, bool components, const std::vector<ForcedBranch>* forced,
+                           const ExpectedBits* expected) {
+  if (expected && !expected->detectors.empty() && expected->detectors.size() != size_t(p.ndetectors))
+    throw TicitError(1, "expected_detectors has length " + std::to_string(expected->detectors.size()) +
+                            ", expected " + std::to_string(p.ndetectors));
+  if (expected && !expected->observables.empty() && expected->observables.size() != p.observables.size())
+    throw TicitError(1, "expected_observables has length " + std::to_string(expected->observables.size()) +
+                            ", expected " + std::to_string(p.observables.size()));
   std::map<int32_t, const SymBool*> forced_at;
   if (forced)
     for (const ForcedBranch& f : *forced) forced_at[f.instruction] = &f.plan;
@@ -411,7 +418,9 @@ HostTables lower_to_device(const Program& p, bool components, const std::vector<
           t.post_dets.push_back(uint32_t(d.detector));
         }
         if (!in.det_records.empty()) {
-          d.op |= kFlagDetRecords;
+          // the value is the records' XOR; the constant flag carries only
+          // the expected bit below
+          d.op = (d.op & ~kFlagConstant) | kFlagDetRecords;
           d.det_off = uint32_t(t.det_recs.size());
           d.det_len = uint32_t(in.det_records.size());
           for (int32_t r : in.det_records) {
@@ -420,6 +429,8 @@ HostTables lower_to_device(const Program& p, bool components, const std::vector<
             t.det_recs.push_back(uint32_t(r - 1));
           }
         }
+        if (expected && !expected->detectors.empty() && expected->detectors[size_t(d.detector)] != 0)
+          d.op ^= kFlagConstant;
         break;
       case Op::Measure: {
         if (k == 0) throw TicitError(4, "cannot measure an active Pauli when k == 0");
@@ -541,6 +552,9 @@ HostTables lower_to_device(const Program& p, bool components, const std::vector<
     }
     t.obs_off.push_back(uint32_t(t.obs_recs.size()));
   }
+  t.obs_flip.assign(std::max<size_t>(1, p.observables.size()), 0);
+  if (expected && !expected->observables.empty())
+    for (size_t o = 0; o < p.observables.size(); ++o) t.obs_flip[o] = expected->observables[o] != 0;
   t.ninstr = int32_t(t.instrs.size());
   for (size_t i = 0; i < t.instrs.size(); ++i) {
     const DevInstr& d = t.instrs[i];

@@ -22,6 +22,7 @@ struct HostTables {
   int32_t npost = 0;    // post_dets without the padding entry
   std::vector<uint32_t> resid, det_recs, obs_off, obs_recs, sym_off, sym_part;
   std::vector<uint32_t> post_dets;  // 0-based postselected detectors
+  std::vector<uint8_t> obs_flip;    // [max(1, nobs)] inverted observables
   std::vector<int32_t> exo_syms;
   std::vector<DevCat> cats;
   std::vector<DevRare> rares;
@@ -58,7 +59,18 @@ struct HostTables {
 // tensor-component plan (throws if the program has none).
 // forced: pinned branches (pinning.hpp), by instruction index.
 struct ForcedBranch;
+
+// Expected detector and observable bits (SamplerOptions::expected_detectors
+// and expected_observables, src/sampler/prepared.rs:111-116): a nonzero
+// entry inverts that column in every row, in postselection and in the
+// observable counts (src/sampler/prepared.rs:909-974,
+// src/sampler/batch/postselect.rs:631-660). Empty: no inversion.
+struct ExpectedBits {
+  std::vector<uint8_t> detectors, observables;
+};
+
 HostTables lower_to_device(const Program& p, bool components = false,
-                           const std::vector<ForcedBranch>* forced = nullptr);
+                           const std::vector<ForcedBranch>* forced = nullptr,
+                           const ExpectedBits* expected = nullptr);
 
 }  // namespace ticit

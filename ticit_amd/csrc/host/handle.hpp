@@ -11,6 +11,7 @@
 #include "../../../include/ticit_hip.h"
 #include "device_program.hpp"
 #include "frontend.hpp"
+#include "pinning.hpp"
 
 namespace ticit {
 
@@ -71,6 +72,10 @@ struct ticit_hip_program {
   int32_t planner_source = 0;
   float planner_p = -1.f;
   uint32_t meas_row_packed = 0, det_row_packed = 0, obs_row_packed = 0;
+  // expected bits folded into the lowered tables (empty: none) and the
+  // pinned parities the program was built with
+  ticit::ExpectedBits expected;
+  std::vector<ticit::MeasurementParity> pins;
 
   ~ticit_hip_program() {
     for (void* p : allocations) (void)hipFree(p);

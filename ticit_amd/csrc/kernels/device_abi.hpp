@@ -30,7 +30,9 @@ enum : uint32_t {
   kFlagDiagonal = 1u << 8,      // rotation with xmask == 0, or diagonal measurement
   kFlagUniformImag = 1u << 9,   // rotation pairs with zmask == 0
   kFlagXzOdd = 1u << 10,        // popcount(xmask & zmask) odd
-  kFlagConstant = 1u << 11,     // expression constant
+  kFlagConstant = 1u << 11,     // expression constant; a detector XORs it into its
+                                // value however it is evaluated (an expected
+                                // detector bit of 1, src/sampler/prepared.rs:909-941)
   kFlagDiagPhase = 1u << 12,    // diagonal measurement of a -1 coefficient body
   kFlagDetRecords = 1u << 13,   // detector reads record bits, not its expression
   kFlagFresh = 1u << 14,        // promotion that starts a new one-qubit component
@@ -147,6 +149,8 @@ struct DevProgram {
   const uint32_t* post_dets;  // [npost] 0-based postselected detectors
   const uint32_t* obs_off;    // nobs + 1 offsets into obs_recs
   const uint32_t* obs_recs;
+  const uint8_t* obs_flip;    // [nobs] 1: the observable's bit is inverted (an expected
+                              // observable bit of 1, src/sampler/prepared.rs:965-974)
   const uint32_t* sym_off;    // nsym + 1 offsets: symbol (c - 1) -> partial ids
   const uint32_t* sym_part;
   const int32_t* exo_syms;    // every exogenous symbol id (injected mode scan)
@@ -190,7 +194,10 @@ struct DevLaunch {
   const uint64_t* exo;     // [nsym][ceil(shots/64)]
   const double* uniforms;  // [nmeasure][shots]
   const uint64_t* coins;   // [ncoin][ceil(shots/64)]
-  uint32_t meas_row, det_row, obs_row, pad;
+  uint32_t meas_row, det_row, obs_row;
+  // noiseless reference shot (src/sampler/prepared.rs:355-405): every
+  // exogenous symbol is 0 and the branch stream starts from state 1
+  uint32_t reference;
   // optional device word added to `seed` at run time, so a captured graph
   // can be replayed with fresh seeds
   const uint64_t* seed_offset;

@@ -85,7 +85,7 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
         int32_t c = P.exo_syms[i];
         if ((L.exo[uint64_t(c - 1) * words + (local >> 6)] >> (local & 63)) & 1) toggle(c);
       }
-    } else {
+    } else if (!L.reference) {
       sample_exogenous(P, seed_for(kExoSeedBase, launch_seed(L), shot), toggle);
     }
   }
@@ -103,7 +103,8 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
     return b;
   };
 
-  uint64_t brng = seed_for(kBranchSeedBase, launch_seed(L), shot);  // identical in the group
+  // identical in the group
+  uint64_t brng = L.reference ? 1ull : seed_for(kBranchSeedBase, launch_seed(L), shot);
   int err = 0;
   // expectation probes (src/sampler/batch/runtime.rs:46-75): thread 0 of the
   // shot writes its [nexp] row
@@ -201,6 +202,7 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
     } else if (op == kOpRecord || op == kOpDetector) {
       bool b = false;
       if (op == kOpDetector && (st.op & kFlagDetRecords)) {
+        b = (st.op & kFlagConstant) != 0;
         for (int32_t j = 0; j < st.c; ++j) b ^= bit_of(rbits, P.det_recs[st.b + j]);
       } else {
         b = eval(st);
@@ -347,7 +349,7 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
   if (L.keep && t == 0 && P.npost > 0) L.keep[local] = uint8_t(kept);
 
   for (int o = t; o < P.nobs && kept; o += TPS) {
-    bool b = false;
+    bool b = P.obs_flip[o] != 0;
     for (uint32_t j = P.obs_off[o]; j < P.obs_off[o + 1]; ++j) b ^= bit_of(rbits, P.obs_recs[j]);
     if (b) atomicAdd(&L.obs_ones[o], 1ull);
   }
@@ -372,7 +374,7 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
   emit(
       L.obs, L.obs_row,
       [&](uint32_t o) {
-        bool b = false;
+        bool b = P.obs_flip[o] != 0;
         for (uint32_t j = P.obs_off[o]; j < P.obs_off[o + 1]; ++j) b ^= bit_of(rbits, P.obs_recs[j]);
         return b;
       },

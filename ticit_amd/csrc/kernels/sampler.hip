@@ -72,7 +72,7 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
         for (uint32_t j = P.sym_off[c - 1]; j < P.sym_off[c]; ++j) Pw[P.sym_part[j]] ^= wv;  // same value in every lane
       __syncthreads();
     }
-  } else {
+  } else if (!L.reference) {
     const uint64_t me = 1ull << lane;
     sample_exogenous(P, seed_for(kExoSeedBase, launch_seed(L), shot), [&](int32_t c) {
       for (uint32_t j = P.sym_off[c - 1]; j < P.sym_off[c]; ++j) atomicXor(&Pw[P.sym_part[j]], me);
@@ -101,7 +101,7 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
   double* expv = (L.expv && live) ? L.expv + local * uint64_t(P.nexp) : nullptr;
 
   int k = P.initial_k;
-  uint64_t brng = seed_for(kBranchSeedBase, launch_seed(L), shot);
+  uint64_t brng = L.reference ? 1ull : seed_for(kBranchSeedBase, launch_seed(L), shot);
   int err = 0;
   for (int i = 0; i < P.nlsteps; ++i) {
     // the lane schedule drives the loop; amplitude work reads the full record
@@ -114,11 +114,10 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
         const int32_t j = base + lane;
         if (j < st.a + st.b) {
           const DevStep q = P.bsteps[j];
-          uint64_t v = 0;
+          uint64_t v = (q.op & kFlagConstant) ? ~0ull : 0ull;
           if (op == kOpDetBatch && (q.op & kFlagDetRecords)) {
             for (int32_t r = 0; r < q.c; ++r) v ^= Rw[P.det_recs[q.b + r]];
           } else {
-            v = (q.op & kFlagConstant) ? ~0ull : 0ull;
             if (q.partial >= 0) v ^= Pw[q.partial];
             for (uint32_t r = 0; r < q.resid_len; ++r) v ^= Sw[P.resid[q.resid_off + r]];
           }
@@ -301,7 +300,7 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
     if (L.keep && live) L.keep[local] = uint8_t((live_mask >> lane) & 1);
   }
   auto obs_word = [&](uint32_t o) {
-    uint64_t v = 0;
+    uint64_t v = P.obs_flip[o] ? ~0ull : 0ull;
     for (uint32_t j = P.obs_off[o]; j < P.obs_off[o + 1]; ++j) v ^= Rw[P.obs_recs[j]];
     return v;
   };

@@ -116,10 +116,10 @@ __global__ __launch_bounds__(kWideThreads) void wide_init_kernel(DevProgram P, D
       int32_t c = P.exo_syms[i];
       if ((L.exo[uint64_t(c - 1) * words + (local >> 6)] >> (local & 63)) & 1) toggle(c);
     }
-  } else {
+  } else if (!L.reference) {
     sample_exogenous(P, seed_for(kExoSeedBase, launch_seed(L), shot), toggle);
   }
-  ctl[j].brng = seed_for(kBranchSeedBase, launch_seed(L), shot);
+  ctl[j].brng = L.reference ? 1ull : seed_for(kBranchSeedBase, launch_seed(L), shot);
   ctl[j].err = 0;
   ctl[j].branch = 0;
   ctl[j].invnorm = 1.0;
@@ -157,6 +157,7 @@ __global__ __launch_bounds__(kWideThreads) void wide_bits_kernel(DevProgram P, D
     } else if (op == kOpDetector) {
       bool b = false;
       if (in.op & kFlagDetRecords) {
+        b = (in.op & kFlagConstant) != 0;
         for (uint32_t q = 0; q < in.det_len; ++q) b ^= wbit(w.r, P.det_recs[in.det_off + q]);
       } else {
         b = wide_eval(P, in, w);
@@ -944,7 +945,7 @@ __global__ __launch_bounds__(kWideThreads) void wide_finish_kernel(DevProgram P,
   WideShot w = wide_shot(bits, Y, j);
   if (ctl[j].err) atomicOr(L.error, ctl[j].err);
   auto obs_bit = [&](uint32_t o) {
-    bool b = false;
+    bool b = P.obs_flip[o] != 0;
     for (uint32_t q = P.obs_off[o]; q < P.obs_off[o + 1]; ++q) b ^= wbit(w.r, P.obs_recs[q]);
     return b;
   };
