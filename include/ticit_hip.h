@@ -210,6 +210,21 @@ ticit_hip_status ticit_hip_sample_device(ticit_hip_program* program, uint64_t sh
                                          uint8_t* det, uint8_t* obs, uint64_t* observable_ones, double* exp_vals,
                                          void* stream);
 
+/* ticit_hip_sample_device for programs with postselected detectors
+ * (src/sampler/batch/postselect.rs; src/sampler/prepared.rs:881-1028 appends
+ * only the survivors): the surviving shots' rows (and exp_vals) are written to
+ * the front of meas/det/obs/exp_vals in shot order and their number to
+ * *accepted (a device uint64, overwritten); observable_ones counts survivors
+ * only. Rows pass through a staging buffer the program owns (allocated on
+ * first use and grown to the largest call: not for graph capture); calls on
+ * one program are ordered on it by an event. Programs without postselection
+ * behave as ticit_hip_sample_device with *accepted = shots. */
+ticit_hip_status ticit_hip_sample_device_postselected(ticit_hip_program* program, uint64_t shots, uint64_t seed,
+                                                     uint64_t first_shot, int bit_packed, int keep_records,
+                                                     uint8_t* meas, uint8_t* det, uint8_t* obs,
+                                                     uint64_t* observable_ones, double* exp_vals,
+                                                     uint64_t* accepted, void* stream);
+
 /* Injected randomness (host buffers): exo = symbol-major bit-planes
  * [nsymbols][ceil(shots/64)] (bit s of word w = shot 64w+s; only exogenous
  * symbols are read), uniforms = [measure_instructions][shots] branch draws
@@ -251,6 +266,10 @@ typedef struct ticit_hip_batch_item {
   uint8_t* observables;
   uint64_t* observable_ones; /* [nobservables] device */
   double* exp_vals;          /* [shots][expectation_values] device, NULL to skip */
+  /* device uint64 receiving the survivor count of a postselected program
+   * (required then, ignored otherwise): its rows come first in the buffers,
+   * in shot order, as from ticit_hip_sample_device_postselected */
+  uint64_t* accepted;
 } ticit_hip_batch_item;
 
 typedef struct ticit_hip_batch_options {

@@ -5,7 +5,10 @@ the caller's postselection mask, src/circuit/mod.rs:236-243) is discarded
 (src/sampler/batch/postselect.rs): it leaves no rows, observable counts or
 expectation values, and counts as discarded. Bars: counts, survivor rows
 and observable counts bit-exact with the oracle at the per-shot stream
-contract, on every tier; injected runs likewise.
+contract, on every tier; injected runs likewise. Survivors are compacted on
+the device: ticit_hip_sample, ticit_hip_sample_device_postselected (rows at
+the front of the caller's buffers, the count in a device word) and the batch
+(compaction captured in its graph) give the same rows.
 """
 import numpy as np
 import pytest
@@ -60,11 +63,116 @@ def test_injected_postselection_matches_oracle(hip):
     assert np.array_equal(r.detectors, o["detectors"])
 
 
-def test_device_entry_point_refuses_postselected_programs(hip):
+def test_plain_device_entry_point_refuses_postselected_programs(hip):
     import torch
     s = hip.Circuit(NOISY).compile(postselection_mask=[0, 1])
     b = tuple(torch.zeros((10, 1), dtype=torch.uint8, device="cuda") for _ in range(3)) + (
         torch.zeros(1, dtype=torch.int64, device="cuda"),)
     with pytest.raises(hip.TicitError) as e:
         s.sample_device(10, 1, b)
-    assert "postselected" in str(e.value)
+    assert "ticit_hip_sample_device_postselected" in str(e.value)
+
+
+def device_buffers(s, n, fill=0xAB):
+    import torch
+    row = lambda c: max(1, (c + 7) // 8)
+    return (torch.full((n, row(s.num_measurements)), fill, dtype=torch.uint8, device="cuda"),
+            torch.full((n, row(s.num_detectors)), fill, dtype=torch.uint8, device="cuda"),
+            torch.full((n, row(s.num_observables)), fill, dtype=torch.uint8, device="cuda"),
+            torch.zeros(max(1, s.num_observables), dtype=torch.int64, device="cuda"),
+            torch.full((n, s.num_expectation_values), 7.0, dtype=torch.float64, device="cuda"),
+            torch.full((1,), -1, dtype=torch.int64, device="cuda"))
+
+
+@pytest.mark.parametrize("tier", ["lane", "group", "wide"])
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_device_survivors_match_host_survivors(hip, name, tier):
+    """ticit_hip_sample_device_postselected: survivors' rows at the front of
+    the caller's buffers, their number in the accepted word; two calls in a
+    row (the second smaller) reuse the program's staging."""
+    import torch
+    text, mask = CASES[name]
+    s = hip.Circuit(text).compile(postselection_mask=mask, tier=tier)
+    for shots, seed in ((3000, 13), (1100, 14)):
+        b = device_buffers(s, shots)
+        s.sample_device(shots, seed, b[:5], accepted=b[5])
+        torch.cuda.synchronize()
+        ref = s.sample(shots, seed=seed, bit_packed=True)
+        acc = int(b[5].item())
+        print(name, tier, shots, "accepted", acc, flush=True)
+        assert acc == ref.accepted and 0 < acc < shots
+        assert np.array_equal(b[0][:acc].cpu().numpy(), ref.measurements)
+        assert np.array_equal(b[1][:acc].cpu().numpy(), ref.detectors)
+        assert np.array_equal(b[2][:acc].cpu().numpy(), ref.observables)
+        assert np.array_equal(b[4][:acc].cpu().numpy(), ref.exp_vals)
+        assert b[3].cpu().numpy()[:s.num_observables].tolist() == ref.observable_ones.tolist()
+        # rows past the survivors are untouched
+        assert bool((b[0][acc:] == 0xAB).all())
+
+
+@pytest.mark.parametrize("group_threads", [8, 64, 256])
+def test_group_tier_shapes_agree_under_postselection(hip, group_threads):
+    """One shot per workgroup stops a shot at its first fired postselected
+    detector; packed sub-wave shots run on. Both give the oracle's
+    survivors."""
+    text, mask = CASES["surface_d3"]
+    s = hip.Circuit(text).compile(postselection_mask=mask, tier="group", group_threads=group_threads)
+    shots = 2500
+    r = s.sample(shots, seed=31, bit_packed=True)
+    o = O.sample_per_shot(s.blob, shots, 31, bit_packed=True)
+    assert (r.accepted, r.discarded) == (int(o["counts"][2]), int(o["counts"][1]))
+    for key in ("measurements", "detectors", "observables"):
+        assert np.array_equal(getattr(r, key), o[key]), key
+    assert r.observable_ones.tolist() == o["observable_ones"].tolist()
+
+
+def test_counts_only_and_edge_survivor_counts(hip):
+    import torch
+    always = hip.Circuit("X 0\nM 0\nDETECTOR rec[-1]\nOBSERVABLE_INCLUDE(0) rec[-1]\n").compile([1])
+    r = always.sample(5000, seed=1, keep_records=False)
+    assert (r.accepted, r.discarded, r.logical_errors) == (0, 5000, 0)
+    b = device_buffers(always, 5000)
+    always.sample_device(5000, 1, b[:4], accepted=b[5])
+    torch.cuda.synchronize()
+    assert int(b[5].item()) == 0 and bool((b[0] == 0xAB).all())
+    never = hip.Circuit("M 0\nDETECTOR rec[-1]\nOBSERVABLE_INCLUDE(0) rec[-1]\n").compile([1])
+    b = device_buffers(never, 4097)
+    never.sample_device(4097, 1, b[:4], accepted=b[5])
+    torch.cuda.synchronize()
+    assert int(b[5].item()) == 4097 and bool((b[0] == 0).all())
+    plain = hip.Circuit(CASES["surface_d3"][0]).compile()  # no DISCARD, no mask
+    assert not plain.has_postselection
+    b = device_buffers(plain, 300)
+    plain.sample_device(300, 2, b[:5], accepted=b[5])
+    torch.cuda.synchronize()
+    assert int(b[5].item()) == 300
+    ref = plain.sample(300, seed=2, bit_packed=True)
+    assert np.array_equal(b[0].cpu().numpy(), ref.measurements)
+
+
+def test_batch_compacts_postselected_items(hip):
+    import torch
+    samplers = [hip.Circuit(CASES[n][0]).compile(postselection_mask=CASES[n][1], tier=t)
+                for n, t in (("noisy_discard", "lane"), ("surface_d3", "group"), ("c1_first_round", "wide"))]
+    samplers.append(hip.Circuit(CASES["surface_d3"][0]).compile())  # no postselection in the same batch
+    shots = [3000, 2000, 700, 500]
+    seeds = [3, 4, 5, 6]
+    outs = [device_buffers(s, n) for s, n in zip(samplers, shots)]
+    bt = hip.Batch(samplers, shots, seeds, outs, autotune=False)
+    bt.launch(0)
+    bt.wait()
+    torch.cuda.synchronize()
+    for s, n, sd, b in zip(samplers, shots, seeds, outs):
+        ref = s.sample(n, seed=sd, bit_packed=True)
+        acc = int(b[5].item()) if s.has_postselection else n
+        assert acc == ref.accepted
+        assert np.array_equal(b[0][:acc].cpu().numpy(), ref.measurements)
+        assert np.array_equal(b[1][:acc].cpu().numpy(), ref.detectors)
+        assert np.array_equal(b[4][:acc].cpu().numpy(), ref.exp_vals)
+        assert b[3].cpu().numpy()[:s.num_observables].tolist() == ref.observable_ones.tolist()
+    res = hip.sample_batch(samplers, shots, seeds, bit_packed=True)
+    for s, n, sd, r in zip(samplers, shots, seeds, res):
+        ref = s.sample(n, seed=sd, bit_packed=True)
+        assert (r.accepted, r.discarded) == (ref.accepted, ref.discarded)
+        assert np.array_equal(r.measurements, ref.measurements)
+        assert np.array_equal(r.exp_vals, ref.exp_vals)

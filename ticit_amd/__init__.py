@@ -252,13 +252,21 @@ class CompiledSampler:
     sample_with_seed = sample
 
     def sample_device(self, shots: int, seed: int, outputs, first_shot: int = 0, bit_packed: bool = True,
-                      keep_records: bool = True, stream: int = 0) -> None:
+                      keep_records: bool = True, stream: int = 0, accepted=None) -> None:
         """Sampling into caller device buffers (ticit_hip_sample_device):
         ``outputs`` = (measurements, detectors, observables, observable_ones
         [, exp_vals]) objects with ``data_ptr()``; enqueued on ``stream``,
-        not synchronized."""
+        not synchronized. With ``accepted`` (a device uint64 / int64 word)
+        the call is ticit_hip_sample_device_postselected: the survivors' rows
+        come first, in shot order, and their number lands in ``accepted``."""
         ptr = lambda t: ctypes.c_void_p(int(t.data_ptr())) if t is not None else None
         ev = outputs[4] if len(outputs) > 4 else None
+        if accepted is not None:
+            _capi.check(self._lib.ticit_hip_sample_device_postselected(
+                self._handle, shots, seed, first_shot, int(bit_packed), int(keep_records), ptr(outputs[0]),
+                ptr(outputs[1]), ptr(outputs[2]), ptr(outputs[3]), ptr(ev), ptr(accepted),
+                ctypes.c_void_p(stream or None)))
+            return
         _capi.check(self._lib.ticit_hip_sample_device(self._handle, shots, seed, first_shot, int(bit_packed),
                                                       int(keep_records), ptr(outputs[0]), ptr(outputs[1]),
                                                       ptr(outputs[2]), ptr(outputs[3]), ptr(ev),
@@ -393,7 +401,9 @@ class Batch:
     (ticit_hip_batch_*). ``outputs[i]`` holds device buffers (anything with
     ``data_ptr()``, e.g. torch tensors) for item i: measurements,
     detectors, observables ([shots][row] uint8) and observable_ones
-    ([nobservables] 64-bit counts, accumulated across launches)."""
+    ([nobservables] 64-bit counts, accumulated across launches), then
+    optionally exp_vals and, for a postselected sampler, an ``accepted``
+    64-bit word that receives its survivor count (survivors' rows first)."""
 
     def __init__(self, samplers, shots, seeds, outputs, bit_packed: bool = True, keep_records: bool = True,
                  autotune: bool = True, autotune_shots: int = 0, group_grid_percent: int = 0,
@@ -413,6 +423,7 @@ class Batch:
             items[i].observables = ptr(out[2])
             items[i].observable_ones = ptr(out[3])
             items[i].exp_vals = ptr(out[4]) if len(out) > 4 else None
+            items[i].accepted = ptr(out[5]) if len(out) > 5 else None
         opt = _capi.BatchOptions()
         opt.bit_packed = int(bit_packed)
         opt.keep_records = int(keep_records)

@@ -30,6 +30,7 @@ SOURCES = [
     "csrc/kernels/sampler.hip",
     "csrc/kernels/group_tier.hip",
     "csrc/kernels/wide_tier.hip",
+    "csrc/kernels/compact.hip",
     "csrc/kernels/planner_mlp.hip",
 ]
 

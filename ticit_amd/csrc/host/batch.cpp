@@ -23,6 +23,7 @@
 #include <tuple>
 #include <vector>
 
+#include "../kernels/compact.hpp"
 #include "handle.hpp"
 
 using namespace ticit;
@@ -51,6 +52,17 @@ struct ticit_hip_batch {
   std::vector<DevLaunch> launches;
   std::vector<uint32_t> lane_items, wide_items;
   std::vector<Bucket> buckets;
+  // Postselected items: the kernels write full rows and keep flags into
+  // batch-owned staging; compaction kernels captured after the item's work
+  // move the survivors to the caller's buffers and write its count.
+  struct PostItem {
+    uint8_t* keep = nullptr;
+    uint64_t* tile_off = nullptr;
+    unsigned long long* accepted = nullptr;
+    CompactRows sets[4] = {};
+    int nsets = 0;
+  };
+  std::map<uint32_t, PostItem> post;
   std::vector<void*> allocs;
   unsigned long long* d_counters = nullptr;  // one per bucket
   uint64_t* d_seed_offset = nullptr;
@@ -252,9 +264,10 @@ ticit_hip_status ticit_hip_batch_create(const ticit_hip_batch_item* items, size_
       delete b;
       return fail(TICIT_HIP_INVALID_INPUT, "batch programs must live on the current device");
     }
-    if (h->tab.npost > 0) {
+    if (h->tab.npost > 0 && items[i].shots && !items[i].accepted) {
       delete b;
-      return fail(TICIT_HIP_UNSUPPORTED, "postselected programs return their surviving rows through ticit_hip_sample");
+      return fail(TICIT_HIP_INVALID_INPUT,
+                  "batch item " + std::to_string(i) + " is postselected and needs an accepted word");
     }
     const HostTables& t = h->tab;
     DevLaunch L{};
@@ -277,6 +290,28 @@ ticit_hip_status ticit_hip_batch_create(const ticit_hip_batch_item* items, size_
                     (t.nobs && !L.obs_ones))) {
       delete b;
       return fail(TICIT_HIP_INVALID_INPUT, "batch item " + std::to_string(i) + " lacks an output buffer");
+    }
+    if (t.npost > 0 && L.shots) {
+      // full rows and keep flags go to staging; the captured compaction
+      // fills the caller's buffers
+      ticit_hip_batch::PostItem pi;
+      pi.accepted = reinterpret_cast<unsigned long long*>(items[i].accepted);
+      if ((e = b->alloc(&pi.keep, size_t(L.shots))) != hipSuccess) return bail_hip(e, "allocating staging");
+      if ((e = b->alloc(&pi.tile_off, compaction_scratch_words(L.shots))) != hipSuccess)
+        return bail_hip(e, "allocating staging");
+      L.keep = pi.keep;
+      if (b->keep_records) {
+        uint8_t** slots[4] = {&L.meas, &L.det, &L.obs, reinterpret_cast<uint8_t**>(&L.expv)};
+        const uint32_t rbs[4] = {L.meas_row, L.det_row, L.obs_row, L.expv ? uint32_t(t.nexp) * 8u : 0u};
+        for (int q = 0; q < 4; ++q) {
+          if (!rbs[q] || !*slots[q]) continue;
+          uint8_t* stage = nullptr;
+          if ((e = b->alloc(&stage, size_t(L.shots) * rbs[q])) != hipSuccess) return bail_hip(e, "allocating staging");
+          pi.sets[pi.nsets++] = CompactRows{stage, *slots[q], rbs[q], 0};
+          *slots[q] = stage;
+        }
+      }
+      b->post[uint32_t(i)] = pi;
     }
     b->progs.push_back(h);
     b->launches.push_back(L);
@@ -386,6 +421,15 @@ ticit_hip_status ticit_hip_batch_create(const ticit_hip_batch_item* items, size_
     ticit_hip_program* h = b->progs[i];
     step(launch_lane_tier(h->tab.max_k, h->dev, b->launches[i], h->lane, b->main));
   }
+  auto compact_item = [&](uint32_t i, hipStream_t st) {
+    auto it = b->post.find(i);
+    if (it == b->post.end()) return;
+    const auto& pi = it->second;
+    step(launch_compaction(pi.keep, b->launches[i].shots, pi.tile_off, pi.accepted, pi.sets, pi.nsets, st));
+  };
+  for (const auto& k : b->buckets)
+    for (uint32_t i : k.items) compact_item(i, b->main);
+  for (uint32_t i : b->lane_items) compact_item(i, b->main);
   e = hipStreamEndCapture(b->main, &b->graph);
   if (ce != hipSuccess) return bail_hip(ce, "capturing the batch");
   if (e != hipSuccess) return bail_hip(e, "ending graph capture");
@@ -399,6 +443,7 @@ ticit_hip_status ticit_hip_batch_create(const ticit_hip_batch_item* items, size_
       uint32_t batch = uint32_t(std::min<uint64_t>(h->wide_batch, b->launches[i].shots));
       step(launch_wide_tier(h->dev, h->tab.instrs.data(), b->launches[i], h->wide, batch, h->opt.precision == 1,
                             b->ws, b->side, nullptr, nullptr, nullptr, false));
+      compact_item(i, b->side);
     }
     e = hipStreamEndCapture(b->side, &b->wide_graph);
     if (ce != hipSuccess) return bail_hip(ce, "capturing the batch");
@@ -514,6 +559,7 @@ ticit_hip_status ticit_hip_sample_batch(ticit_hip_program* const* programs, cons
       items[i].observable_ones = static_cast<uint64_t*>(p);
       e = hipMemset(p, 0, size_t(std::max(1, t.nobs)) * 8);
     }
+    if (e == hipSuccess && t.npost > 0 && (e = dev(8, &p)) == hipSuccess) items[i].accepted = static_cast<uint64_t*>(p);
     if (e != hipSuccess) {
       release();
       return from_hip(e, "allocating batch outputs");
@@ -555,11 +601,17 @@ ticit_hip_status ticit_hip_sample_batch(ticit_hip_program* const* programs, cons
     r.observable_ones = static_cast<uint64_t*>(calloc(std::max(1, t.nobs), 8));
     hipError_t e = hipMemcpy(r.observable_ones, items[i].observable_ones, size_t(t.nobs) * 8, hipMemcpyDeviceToHost);
     if (t.nobs > 0 && h->opt.observable < uint32_t(t.nobs)) r.logical_errors = r.observable_ones[h->opt.observable];
+    uint64_t rows = shots[i];
+    if (e == hipSuccess && items[i].accepted && shots[i]) {
+      e = hipMemcpy(&rows, items[i].accepted, 8, hipMemcpyDeviceToHost);
+      r.accepted = rows;
+      r.discarded = shots[i] - rows;
+    }
     if (keep_records) {
-      r.record_rows = shots[i];
-      size_t mb = shots[i] * h->row_bytes(uint64_t(t.nrec), bit_packed);
-      size_t db = shots[i] * h->row_bytes(uint64_t(t.ndet), bit_packed);
-      size_t ob = shots[i] * h->row_bytes(uint64_t(t.nobs), bit_packed);
+      r.record_rows = rows;
+      size_t mb = rows * h->row_bytes(uint64_t(t.nrec), bit_packed);
+      size_t db = rows * h->row_bytes(uint64_t(t.ndet), bit_packed);
+      size_t ob = rows * h->row_bytes(uint64_t(t.nobs), bit_packed);
       r.measurements = static_cast<uint8_t*>(malloc(std::max<size_t>(1, mb)));
       r.detectors = static_cast<uint8_t*>(malloc(std::max<size_t>(1, db)));
       r.observables = static_cast<uint8_t*>(malloc(std::max<size_t>(1, ob)));
@@ -567,7 +619,7 @@ ticit_hip_status ticit_hip_sample_batch(ticit_hip_program* const* programs, cons
       if (e == hipSuccess && db) e = hipMemcpy(r.detectors, items[i].detectors, db, hipMemcpyDeviceToHost);
       if (e == hipSuccess && ob) e = hipMemcpy(r.observables, items[i].observables, ob, hipMemcpyDeviceToHost);
       if (t.nexp > 0) {
-        size_t eb = shots[i] * size_t(t.nexp) * 8;
+        size_t eb = rows * size_t(t.nexp) * 8;
         r.exp_vals = static_cast<double*>(malloc(std::max<size_t>(8, eb)));
         if (e == hipSuccess && eb) e = hipMemcpy(r.exp_vals, items[i].exp_vals, eb, hipMemcpyDeviceToHost);
       }

@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include "../kernels/compact.hpp"
 #include "component_plan.hpp"
 #include "handle.hpp"
 #include "pinning.hpp"
@@ -377,26 +378,40 @@ ticit_hip_status run_host(ticit_hip_program* h, uint64_t shots, uint64_t seed, i
   out->shots = shots;
   out->accepted = shots;
   out->nexpvals = uint64_t(t.nexp);
-  // postselection: the kept shots' rows, in shot order
-  // (src/sampler/prepared.rs:881-1028 appends only survivors)
-  std::vector<uint8_t> keep;
+  // postselection: the kept shots' rows move to the front on the device, in
+  // shot order (src/sampler/prepared.rs:881-1028 appends only survivors),
+  // and only those are copied back
   if (d_keep) {
-    keep.resize(size_t(shots));
-    CK(hipMemcpy(keep.data(), d_keep, size_t(shots), hipMemcpyDeviceToHost), "reading postselection flags");
-    uint64_t acc = 0;
-    for (uint8_t k : keep) acc += k;
+    uint64_t* tile_off = nullptr;
+    unsigned long long* d_acc = nullptr;
+    CK(buf.alloc(&tile_off, compaction_scratch_words(shots)), "allocating compaction offsets");
+    CK(buf.alloc(&d_acc, 1), "allocating the survivor count");
+    CompactRows sets[4] = {};
+    int nsets = 0;
+    if (keep_records) {
+      uint8_t* srcs[4] = {L.meas, L.det, L.obs, reinterpret_cast<uint8_t*>(d_expv)};
+      uint32_t rbs[4] = {L.meas_row, L.det_row, L.obs_row, uint32_t(t.nexp) * 8u};
+      for (int i = 0; i < 4; ++i) {
+        if (!srcs[i] || rbs[i] == 0) continue;
+        uint8_t* dst = nullptr;
+        CK(buf.alloc(&dst, size_t(shots) * rbs[i]), "allocating survivor rows");
+        sets[nsets++] = CompactRows{srcs[i], dst, rbs[i], 0};
+      }
+    }
+    CK(launch_compaction(d_keep, shots, tile_off, d_acc, sets, nsets, nullptr), "compacting survivor rows");
+    unsigned long long acc = 0;
+    CK(hipMemcpy(&acc, d_acc, 8, hipMemcpyDeviceToHost), "reading the survivor count");
     out->accepted = acc;
     out->discarded = shots - acc;
+    // the compacted sets replace the full ones
+    for (int i = 0; i < nsets; ++i) {
+      if (sets[i].src == L.meas) L.meas = sets[i].dst;
+      else if (sets[i].src == L.det) L.det = sets[i].dst;
+      else if (sets[i].src == L.obs) L.obs = sets[i].dst;
+      else d_expv = reinterpret_cast<double*>(sets[i].dst);
+    }
   }
-  auto compact = [&](uint8_t* rows, size_t row_bytes) {
-    if (keep.empty() || row_bytes == 0) return;
-    size_t o = 0;
-    for (size_t s = 0; s < keep.size(); ++s)
-      if (keep[s]) {
-        if (o != s) std::memmove(rows + o * row_bytes, rows + s * row_bytes, row_bytes);
-        ++o;
-      }
-  };
+  const uint64_t rows = out->accepted;
   out->bit_packed = bit_packed != 0;
   out->nmeasurements = uint64_t(t.nrec);
   out->ndetectors = uint64_t(t.ndet);
@@ -405,21 +420,21 @@ ticit_hip_status run_host(ticit_hip_program* h, uint64_t shots, uint64_t seed, i
   CK(hipMemcpy(out->observable_ones, L.obs_ones, size_t(t.nobs) * 8, hipMemcpyDeviceToHost), "reading counts");
   if (t.nobs > 0 && h->opt.observable < uint32_t(t.nobs)) out->logical_errors = out->observable_ones[h->opt.observable];
   if (keep_records) {
-    out->record_rows = out->accepted;
-    out->measurements = static_cast<uint8_t*>(malloc(std::max<size_t>(1, size_t(shots) * L.meas_row)));
-    out->detectors = static_cast<uint8_t*>(malloc(std::max<size_t>(1, size_t(shots) * L.det_row)));
-    out->observables = static_cast<uint8_t*>(malloc(std::max<size_t>(1, size_t(shots) * L.obs_row)));
-    if (L.meas_row) CK(hipMemcpy(out->measurements, L.meas, size_t(shots) * L.meas_row, hipMemcpyDeviceToHost), "reading rows");
-    if (L.det_row) CK(hipMemcpy(out->detectors, L.det, size_t(shots) * L.det_row, hipMemcpyDeviceToHost), "reading rows");
-    if (L.obs_row) CK(hipMemcpy(out->observables, L.obs, size_t(shots) * L.obs_row, hipMemcpyDeviceToHost), "reading rows");
-    compact(out->measurements, L.meas_row);
-    compact(out->detectors, L.det_row);
-    compact(out->observables, L.obs_row);
+    out->record_rows = rows;
+    out->measurements = static_cast<uint8_t*>(malloc(std::max<size_t>(1, size_t(rows) * L.meas_row)));
+    out->detectors = static_cast<uint8_t*>(malloc(std::max<size_t>(1, size_t(rows) * L.det_row)));
+    out->observables = static_cast<uint8_t*>(malloc(std::max<size_t>(1, size_t(rows) * L.obs_row)));
+    if (rows && L.meas_row)
+      CK(hipMemcpy(out->measurements, L.meas, size_t(rows) * L.meas_row, hipMemcpyDeviceToHost), "reading rows");
+    if (rows && L.det_row)
+      CK(hipMemcpy(out->detectors, L.det, size_t(rows) * L.det_row, hipMemcpyDeviceToHost), "reading rows");
+    if (rows && L.obs_row)
+      CK(hipMemcpy(out->observables, L.obs, size_t(rows) * L.obs_row, hipMemcpyDeviceToHost), "reading rows");
     if (t.nexp > 0) {
-      out->exp_vals = static_cast<double*>(malloc(size_t(shots) * size_t(t.nexp) * sizeof(double)));
-      CK(hipMemcpy(out->exp_vals, d_expv, size_t(shots) * size_t(t.nexp) * 8, hipMemcpyDeviceToHost),
-         "reading expectation values");
-      compact(reinterpret_cast<uint8_t*>(out->exp_vals), size_t(t.nexp) * sizeof(double));
+      out->exp_vals = static_cast<double*>(malloc(std::max<size_t>(1, size_t(rows) * size_t(t.nexp)) * sizeof(double)));
+      if (rows)
+        CK(hipMemcpy(out->exp_vals, d_expv, size_t(rows) * size_t(t.nexp) * 8, hipMemcpyDeviceToHost),
+           "reading expectation values");
     }
   }
   if (probs && t.nmeasure)
@@ -698,7 +713,9 @@ ticit_hip_status ticit_hip_sample_device(ticit_hip_program* h, uint64_t shots, u
   if (!h || !observable_ones) return fail(TICIT_HIP_INVALID_INPUT, "null argument");
   if (keep_records && (!meas || !det || !obs)) return fail(TICIT_HIP_INVALID_INPUT, "null row buffer");
   if (h->tab.npost > 0)
-    return fail(TICIT_HIP_UNSUPPORTED, "postselected programs return their surviving rows through ticit_hip_sample");
+    return fail(TICIT_HIP_UNSUPPORTED,
+                "postselected programs return their surviving rows through ticit_hip_sample or "
+                "ticit_hip_sample_device_postselected");
   if (shots == 0) return TICIT_HIP_OK;
   const HostTables& t = h->tab;
   DevLaunch L{};
@@ -718,6 +735,92 @@ ticit_hip_status ticit_hip_sample_device(ticit_hip_program* h, uint64_t shots, u
   L.error = h->d_error;
   hipError_t e = launch(h, L, static_cast<hipStream_t>(stream));
   if (e != hipSuccess) return from_hip(e, "launching the sampling kernel");
+  return TICIT_HIP_OK;
+}
+
+ticit_hip_status ticit_hip_sample_device_postselected(ticit_hip_program* h, uint64_t shots, uint64_t seed,
+                                                     uint64_t first_shot, int bit_packed, int keep_records,
+                                                     uint8_t* meas, uint8_t* det, uint8_t* obs,
+                                                     uint64_t* observable_ones, double* exp_vals,
+                                                     uint64_t* accepted, void* stream) {
+  if (!h || !observable_ones || !accepted) return fail(TICIT_HIP_INVALID_INPUT, "null argument");
+  if (keep_records && (!meas || !det || !obs)) return fail(TICIT_HIP_INVALID_INPUT, "null row buffer");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  auto* acc = reinterpret_cast<unsigned long long*>(accepted);
+  const HostTables& t = h->tab;
+  if (t.npost == 0 || shots == 0) {
+    ticit_hip_status s0 = ticit_hip_sample_device(h, shots, seed, first_shot, bit_packed, keep_records, meas, det,
+                                                  obs, observable_ones, exp_vals, stream);
+    if (s0 != TICIT_HIP_OK) return s0;
+    hipError_t e = launch_store_u64(acc, shots, st);
+    return e == hipSuccess ? TICIT_HIP_OK : from_hip(e, "writing the survivor count");
+  }
+  DevLaunch L{};
+  L.shots = shots;
+  L.seed = seed;
+  L.first_shot = first_shot;
+  L.bit_packed = bit_packed != 0;
+  L.keep_records = keep_records != 0;
+  L.meas_row = h->row_bytes(uint64_t(t.nrec), bit_packed);
+  L.det_row = h->row_bytes(uint64_t(t.ndet), bit_packed);
+  L.obs_row = h->row_bytes(uint64_t(t.nobs), bit_packed);
+  L.obs_ones = reinterpret_cast<unsigned long long*>(observable_ones);
+  L.error = h->d_error;
+  const bool want_exp = keep_records && t.nexp > 0 && exp_vals;
+  // staging: keep flags, tile offsets, then the full rows of every shot
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t n = size_t(shots);
+  const size_t keep_b = up(n), off_b = up(compaction_scratch_words(shots) * 8);
+  const uint32_t rbs[4] = {L.meas_row, L.det_row, L.obs_row, want_exp ? uint32_t(t.nexp) * 8u : 0u};
+  size_t rows_b[4] = {0, 0, 0, 0};
+  size_t need = keep_b + off_b;
+  if (keep_records)
+    for (int i = 0; i < 4; ++i) need += rows_b[i] = up(n * rbs[i]);
+  hipError_t e = hipSuccess;
+  if (!h->post_done) {
+    e = hipEventCreateWithFlags(&h->post_done, hipEventDisableTiming);
+    if (e != hipSuccess) return from_hip(e, "creating the staging event");
+  }
+  if (need > h->post_bytes) {
+    // the previous call may still read the old buffer
+    (void)hipEventSynchronize(h->post_done);
+    if (h->post_buf) (void)hipFree(h->post_buf);
+    h->post_buf = nullptr;
+    h->post_bytes = 0;
+    e = hipMalloc(&h->post_buf, need);
+    if (e != hipSuccess) return from_hip(e, "allocating the postselection staging");
+    h->post_bytes = need;
+  } else if (h->post_pending) {
+    e = hipStreamWaitEvent(st, h->post_done, 0);
+    if (e != hipSuccess) return from_hip(e, "ordering the staging reuse");
+  }
+  uint8_t* base = static_cast<uint8_t*>(h->post_buf);
+  L.keep = base;
+  uint64_t* tile_off = reinterpret_cast<uint64_t*>(base + keep_b);
+  uint8_t* p = base + keep_b + off_b;
+  uint8_t* stage[4] = {nullptr, nullptr, nullptr, nullptr};
+  if (keep_records)
+    for (int i = 0; i < 4; ++i) {
+      stage[i] = rows_b[i] ? p : nullptr;
+      p += rows_b[i];
+    }
+  L.meas = stage[0];
+  L.det = stage[1];
+  L.obs = stage[2];
+  L.expv = reinterpret_cast<double*>(stage[3]);
+  e = launch(h, L, st);
+  if (e != hipSuccess) return from_hip(e, "launching the sampling kernel");
+  CompactRows sets[4] = {};
+  int nsets = 0;
+  uint8_t* dsts[4] = {meas, det, obs, reinterpret_cast<uint8_t*>(exp_vals)};
+  if (keep_records)
+    for (int i = 0; i < 4; ++i)
+      if (stage[i] && rbs[i]) sets[nsets++] = CompactRows{stage[i], dsts[i], rbs[i], 0};
+  e = launch_compaction(L.keep, shots, tile_off, acc, sets, nsets, st);
+  if (e != hipSuccess) return from_hip(e, "compacting survivor rows");
+  e = hipEventRecord(h->post_done, st);
+  if (e != hipSuccess) return from_hip(e, "recording the staging event");
+  h->post_pending = true;
   return TICIT_HIP_OK;
 }
 

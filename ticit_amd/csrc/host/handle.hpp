@@ -77,7 +77,20 @@ struct ticit_hip_program {
   ticit::ExpectedBits expected;
   std::vector<ticit::MeasurementParity> pins;
 
+  // Staging of postselected device launches (full rows, keep flags, tile
+  // offsets): grown to the largest call, reused in stream order through
+  // `post_done`.
+  void* post_buf = nullptr;
+  size_t post_bytes = 0;
+  hipEvent_t post_done = nullptr;
+  bool post_pending = false;
+
   ~ticit_hip_program() {
+    if (post_done) {
+      (void)hipEventSynchronize(post_done);
+      (void)hipEventDestroy(post_done);
+    }
+    if (post_buf) (void)hipFree(post_buf);
     for (void* p : allocations) (void)hipFree(p);
   }
 

@@ -220,6 +220,13 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
         }
       }
       __syncthreads();
+      // A fired postselected detector discards the shot: nothing later can
+      // reach its outputs (src/sampler/batch/postselect.rs removes it from
+      // the block). With one shot per workgroup `b` is uniform, so the
+      // whole group leaves the loop together; packed sub-wave shots run on.
+      if constexpr (TPS >= 64) {
+        if (b && (st.op & kFlagPostselect) && op == kOpDetector) break;
+      }
     } else if (op == kOpCoin && (st.op & kFlagExpVal)) {
       if (expv) expv[st.a] = 0.0;  // an unbiased probe reads 0 and draws nothing
     } else if (op == kOpMeasure || op == kOpCoin) {
