@@ -37,6 +37,18 @@ double gap_denominator(double p) {
   return std::log1p(-p);
 }
 
+// Largest uniform below which floor(ln(u) / denom) >= trials is certain
+// (no event among `trials`): (1-p)^trials shrunk by a relative 1e-9. The
+// kernels' ln and division err by a few ulps, i.e. a relative 1e-15 of a
+// gap below 700 / |denom|, while the shrink moves the exact gap up by
+// 1e-9 / |denom|; beyond trials * |denom| = 700 the bound would approach
+// the DBL_MIN clamp of the uniform, so the shortcut is off there.
+double no_event_below(double trials, double denom) {
+  const double x = trials * denom;
+  if (!(denom < 0.0) || !(x > -700.0)) return 0.0;
+  return std::exp(x) * (1.0 - 1e-9);
+}
+
 void check_probability(double p) {
   if (!(p >= 0.0 && p <= 1.0)) throw TicitError(1, "probability must be between 0 and 1");
 }
@@ -53,7 +65,20 @@ void build_lane_schedule(HostTables& t) {
   std::vector<std::pair<int, DevStep>> pending;  // (level, record)
   std::unordered_map<int32_t, int> slot_level;   // slot -> level of its pending record
   std::vector<DevStep> dets;
+  std::vector<DevStep> coin_run;  // coin draws not yet emitted
+  auto emit_coins = [&]() {
+    if (coin_run.empty()) return;
+    DevStep b{};
+    b.op = kOpCoinBatch;
+    b.partial = -1;
+    b.a = int32_t(t.bsteps.size());
+    b.b = int32_t(coin_run.size());
+    t.bsteps.insert(t.bsteps.end(), coin_run.begin(), coin_run.end());
+    t.lsteps.push_back(b);
+    coin_run.clear();
+  };
   auto flush = [&]() {
+    emit_coins();  // pending records may read the coins' branch slots
     if (pending.empty()) return;
     std::stable_sort(pending.begin(), pending.end(),
                      [](const std::pair<int, DevStep>& x, const std::pair<int, DevStep>& y) { return x.first < y.first; });
@@ -98,6 +123,20 @@ void build_lane_schedule(HostTables& t) {
       dets.push_back(s);
       continue;
     }
+    if (op == kOpCoin && !(s.op & (kFlagExpVal | kFlagForced))) {
+      // a dormant coin: its draw joins the run of consecutive draws, its
+      // outcome record is deferred like any record (it reads the branch
+      // slot the draw writes, which every flush emits first)
+      coin_run.push_back(s);
+      DevStep r = s;
+      r.op = (s.op & ~0xffu) | kOpRecord;
+      reads_pending(r, &level);
+      pending.emplace_back(level + 1, r);
+      if (r.b >= 0) slot_level[r.b] = level + 1;
+      continue;
+    }
+    // draws stay in program order, and the step may read a coin's branch
+    emit_coins();
     if (reads_pending(s, &level)) flush();
     t.lsteps.push_back(s);
   }
@@ -175,6 +214,7 @@ HostTables lower_to_device(const Program& p, bool components, const std::vector<
     dr.enabled = g.event_p > 0.0 && !g.sets.empty();
     dr.denom = dr.enabled ? gap_denominator(g.event_p) : 0.0;
     dr.nsets = uint32_t(g.sets.size());
+    dr.none_below = dr.enabled ? no_event_below(double(dr.nsets), dr.denom) : 0.0;
     dr.nbits = uint32_t(g.nbits);
     dr.nev = uint32_t(g.event_rows.size());
     dr.cond_off = uint32_t(t.i32pool.size());
@@ -214,6 +254,7 @@ HostTables lower_to_device(const Program& p, bool components, const std::vector<
       } else if (q < 0.02) {
         b.mode = kBernLow;
         b.denom = gap_denominator(q);
+        b.miss_below = no_event_below(1.0, b.denom);
       } else {
         b.mode = kBernCoins;
         uint64_t raw = uint64_t(q * 256.0);
@@ -222,6 +263,7 @@ HostTables lower_to_device(const Program& p, bool components, const std::vector<
         b.top_bits = uint32_t(top);
         double left = q - pt;
         b.left_denom = left > 0.0 ? gap_denominator(left / (1.0 - pt)) : 0.0;
+        b.left_miss_below = left > 0.0 ? no_event_below(1.0, b.left_denom) : 0.0;
       }
     }
     t.berns.push_back(b);
@@ -231,6 +273,7 @@ HostTables lower_to_device(const Program& p, bool components, const std::vector<
     l.enabled = g.p > 0.0 && !g.conds.empty();
     l.denom = l.enabled ? gap_denominator(g.p) : 0.0;
     l.nconds = uint32_t(g.conds.size());
+    l.none_below = l.enabled ? no_event_below(double(l.nconds), l.denom) : 0.0;
     l.cond_off = uint32_t(t.i32pool.size());
     for (int32_t c : g.conds) t.i32pool.push_back(c);
     t.lowps.push_back(l);

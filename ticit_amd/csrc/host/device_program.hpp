@@ -15,8 +15,11 @@ struct HostTables {
   // Lane-tier schedule: the steps in order, except that records wait until
   // an in-order step reads a slot one of them assigns (then they run as
   // dependency levels, 64 side by side) and detectors run last, together.
-  // Every runtime slot is assigned once and never reused, so a deferred
-  // step reads the values it would have read in order.
+  // A dormant coin splits into its draw, merged with the draws of the
+  // coins next to it into one step that keeps their program order (and
+  // runs before any later draw), and its outcome record, deferred like the
+  // others. Every runtime slot is assigned once and never reused, so a
+  // deferred step reads the values it would have read in order.
   std::vector<DevStep> lsteps, bsteps;
   int32_t nlsteps = 0;  // lsteps without the padding entry
   int32_t npost = 0;    // post_dets without the padding entry

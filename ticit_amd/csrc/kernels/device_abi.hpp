@@ -24,6 +24,9 @@ enum : uint32_t {
   // one per lane (records of one dependency level, or the detectors)
   kOpRecBatch = 7,
   kOpDetBatch = 8,
+  // lane-tier schedule only: the draws of consecutive dormant coins
+  // bsteps[a, a + b), in program order (their records join the batches)
+  kOpCoinBatch = 9,
 };
 
 enum : uint32_t {
@@ -103,8 +106,14 @@ struct DevCat {       // categorical distribution drawn once per shot
   uint32_t pad;
 };
 
+// Geometric-gap walks: a first uniform below `none_below` is a gap of at
+// least the walk's trial count (no event), decided without the logarithm;
+// 0 disables the shortcut. The host sets it to (1-p)^trials * (1 - 1e-9),
+// far enough below the exact boundary that the rounded ln(u)/denom the
+// kernels would compute lands on the same side.
 struct DevRare {      // rare categorical group walked by geometric gaps
   double denom;       // ln(1 - event_p)
+  double none_below;  // first-draw shortcut over nsets trials
   uint32_t enabled;   // event_p > 0
   uint32_t nsets, nbits, nev;
   uint32_t cond_off;  // i32 pool: nsets * nbits conditions
@@ -128,10 +137,13 @@ struct DevBern {      // src/sampler/exogenous.rs:185-266 at one shot per stream
   uint32_t top_bits;
   double denom;       // kBernLow: ln(1 - p)
   double left_denom;  // kBernCoins leftover: ln(1 - p'), 0 when no leftover
+  double miss_below;       // shortcut of the kBernLow draw over one trial
+  double left_miss_below;  // shortcut of the leftover draw
 };
 
 struct DevLowp {      // low-probability Bernoulli group
   double denom;
+  double none_below;  // first-draw shortcut over nconds trials
   uint32_t enabled;
   uint32_t nconds;
   uint32_t cond_off;

@@ -44,13 +44,14 @@ __device__ __forceinline__ uint64_t seed_for(uint64_t base, uint64_t seed, uint6
 }
 
 // src/random.rs:64-73: one draw per gap, clamped to i32::MAX
-__device__ __forceinline__ double sample_gap(uint64_t& s, double denom) {
-  double u = sm_uniform(s);
+__device__ __forceinline__ double gap_of(double u, double denom) {
   u = u < 2.2250738585072014e-308 ? 2.2250738585072014e-308 : u;
   double gap = floor(log(u) / denom);
   if (!isfinite(gap) || gap >= 2147483647.0) return 2147483647.0;
   return gap;
 }
+
+__device__ __forceinline__ double sample_gap(uint64_t& s, double denom) { return gap_of(sm_uniform(s), denom); }
 
 // src/random.rs:40-50: inclusive cumulative walk, falls through to the last row
 __device__ __forceinline__ uint32_t categorical_row(uint64_t& s, const double* probs, uint32_t n) {
@@ -64,10 +65,11 @@ __device__ __forceinline__ uint32_t categorical_row(uint64_t& s, const double* p
 }
 
 // or_low_probability_bits over a single shot (src/sampler/exogenous.rs:150-175):
-// a miss costs one draw, a hit two (the second one ends the walk).
-__device__ __forceinline__ bool low_probability_bit(uint64_t& s, double denom) {
-  double gap = sample_gap(s, denom);
-  if (gap >= 1.0) return false;
+// a miss costs one draw, a hit two (the second one ends the walk). A first
+// uniform below `miss_below` is a miss without the logarithm.
+__device__ __forceinline__ bool low_probability_bit(uint64_t& s, double denom, double miss_below) {
+  const double u = sm_uniform(s);
+  if (u < miss_below || gap_of(u, denom) >= 1.0) return false;
   (void)sample_gap(s, denom);
   return true;
 }
@@ -89,8 +91,10 @@ __device__ void sample_exogenous(const DevProgram& P, uint64_t rng, Toggle&& tog
     if (!g.enabled) continue;
     const int64_t total = int64_t(g.nsets);
     int64_t draw = 0;
-    for (;;) {
-      int64_t gap = int64_t(sample_gap(rng, g.denom));
+    const double u0 = sm_uniform(rng);
+    if (u0 < g.none_below) continue;  // the first gap spans every set
+    for (bool first = true;; first = false) {
+      int64_t gap = int64_t(first ? gap_of(u0, g.denom) : sample_gap(rng, g.denom));
       if (gap >= total - draw) break;
       draw += gap;
       uint32_t set = uint32_t(draw);  // one shot per stream: draw / nsets == 0
@@ -116,7 +120,7 @@ __device__ void sample_exogenous(const DevProgram& P, uint64_t rng, Toggle&& tog
         bit = sm_next(rng) & 1;
         break;
       case kBernLow:
-        bit = low_probability_bit(rng, b.denom);
+        bit = low_probability_bit(rng, b.denom, b.miss_below);
         break;
       default: {  // kBernCoins: src/sampler/exogenous.rs:225-257
         uint64_t alive = sm_next(rng);
@@ -127,7 +131,7 @@ __device__ void sample_exogenous(const DevProgram& P, uint64_t rng, Toggle&& tog
           alive &= ~shoot;
         }
         bit = result & 1;
-        if (b.left_denom != 0.0) bit = low_probability_bit(rng, b.left_denom) || bit;
+        if (b.left_denom != 0.0) bit = low_probability_bit(rng, b.left_denom, b.left_miss_below) || bit;
         break;
       }
     }
@@ -139,8 +143,10 @@ __device__ void sample_exogenous(const DevProgram& P, uint64_t rng, Toggle&& tog
     if (!g.enabled) continue;
     const int64_t total = int64_t(g.nconds);
     int64_t draw = 0;
-    for (;;) {
-      int64_t gap = int64_t(sample_gap(rng, g.denom));
+    const double u0 = sm_uniform(rng);
+    if (u0 < g.none_below) continue;  // the first gap spans every condition
+    for (bool first = true;; first = false) {
+      int64_t gap = int64_t(first ? gap_of(u0, g.denom) : sample_gap(rng, g.denom));
       if (gap >= total - draw) break;
       draw += gap;
       toggle(P.i32pool[g.cond_off + uint32_t(draw)]);

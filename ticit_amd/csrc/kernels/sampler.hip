@@ -107,6 +107,22 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
     // the lane schedule drives the loop; amplitude work reads the full record
     const DevStep st = P.lsteps[i];
     const uint32_t op = st.op & 0xff;
+    if (op == kOpCoinBatch) {
+      // consecutive dormant coins (src/sampler/batch/mod.rs:254-274): one
+      // draw per coin from every shot's stream, in program order
+      for (int32_t j = st.a; j < st.a + st.b; ++j) {
+        const DevStep q = P.bsteps[j];
+        uint64_t coins;
+        if (L.injected) {
+          const uint64_t words = (L.shots + 63) / 64;
+          coins = L.coins[uint64_t(q.full) * words + blockIdx.x];
+        } else {
+          coins = __ballot(sm_next(brng) & 1);
+        }
+        Sw[q.c] = coins;
+      }
+      continue;
+    }
     if (op == kOpRecBatch || op == kOpDetBatch) {
       // independent records (one dependency level) or the detectors, one
       // step per lane; each lane's step updates the words of all 64 shots
