@@ -43,21 +43,26 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int DIM = 1 << KMAX;
   constexpr int SDIM = KMAX > 0 ? DIM / 2 : 0;
+  // the collapse's output half: registers up to two amplitudes, LDS above
+  constexpr bool kRegScratch = KMAX <= 2;
+  constexpr int SDIM_LDS = kRegScratch ? 0 : SDIM;
   const int lane = threadIdx.x;
   const uint64_t local = uint64_t(blockIdx.x) * 64 + lane;
   const bool live = local < L.shots;
   const uint64_t shot = L.first_shot + local;
 
   double2* amp = reinterpret_cast<double2*>(smem);  // [DIM][64]
-  double2* scr = amp + DIM * 64;                    // [SDIM][64]
+  double2* scr = amp + DIM * 64;                    // [SDIM_LDS][64]
   // Bit-sliced tables: word i holds item i of all 64 shots (bit l = lane l),
   // so a record, detector or expression update is one uniform word
   // operation for the whole wave.
-  uint64_t* Pw = reinterpret_cast<uint64_t*>(scr + SDIM * 64);  // exogenous partials
+  uint64_t* Pw = reinterpret_cast<uint64_t*>(scr + SDIM_LDS * 64);  // exogenous partials
   uint64_t* Sw = Pw + Y.np;                                     // runtime slots
   uint64_t* Rw = Sw + Y.ns;                                     // records
   uint64_t* Dw = Rw + Y.nr;                                     // detectors
-  uint8_t* stage = reinterpret_cast<uint8_t*>(Dw + Y.nd);
+  // Row staging lives past the tables, or over the amplitudes, which are
+  // dead once the rows are emitted, when it fits there.
+  uint8_t* stage = Y.stage_bytes <= uint32_t((DIM + SDIM_LDS) * 64 * 16) ? smem : reinterpret_cast<uint8_t*>(Dw + Y.nd);
 
   for (uint32_t w = lane; w < Y.np + Y.ns + Y.nr + Y.nd; w += 64) Pw[w] = 0;
   __syncthreads();
@@ -285,7 +290,7 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
           amp[idx * 64 + lane] = make_double2(v.x * inv, v.y * inv);
         }
       } else {
-        for (int idx = 0; idx < out_dim; ++idx) {
+        auto mixed = [&](int idx) {
           int s0 = int(insert_zero_bit(size_t(idx), in.pivot)), s1 = s0 ^ int(in.xmask);
           bool odd = parity64(uint64_t(s0) & in.zmask);
           double dir = (branch != odd) ? -1.0 : 1.0;
@@ -293,9 +298,20 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
           double2 v0 = amp[s0 * 64 + lane], v1 = amp[s1 * 64 + lane];
           double ar = kInvSqrt2 * v0.x + c1r * v1.x - c1i * v1.y;
           double ai = kInvSqrt2 * v0.y + c1r * v1.y + c1i * v1.x;
-          scr[idx * 64 + lane] = make_double2(ar * inv, ai * inv);
+          return make_double2(ar * inv, ai * inv);
+        };
+        if constexpr (kRegScratch) {
+          double2 out[SDIM > 0 ? SDIM : 1];
+#pragma unroll
+          for (int idx = 0; idx < SDIM; ++idx)
+            if (idx < out_dim) out[idx] = mixed(idx);
+#pragma unroll
+          for (int idx = 0; idx < SDIM; ++idx)
+            if (idx < out_dim) amp[idx * 64 + lane] = out[idx];
+        } else {
+          for (int idx = 0; idx < out_dim; ++idx) scr[idx * 64 + lane] = mixed(idx);
+          for (int idx = 0; idx < out_dim; ++idx) amp[idx * 64 + lane] = scr[idx * 64 + lane];
         }
-        for (int idx = 0; idx < out_dim; ++idx) amp[idx * 64 + lane] = scr[idx * 64 + lane];
       }
       --k;
       Sw[st.c] = __ballot(branch);
@@ -368,8 +384,9 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
 
 size_t lane_tier_lds_bytes(int kmax, const LaneLayout& y) {
   size_t dim = size_t(1) << kmax;
-  size_t sdim = kmax > 0 ? dim / 2 : 0;
-  return (dim + sdim) * 64 * 16 + size_t(y.np + y.ns + y.nr + y.nd) * 8 + y.stage_bytes;
+  size_t sdim = kmax > 2 ? dim / 2 : 0;  // widths up to 2 collapse in registers
+  size_t amps = (dim + sdim) * 64 * 16;
+  return amps + size_t(y.np + y.ns + y.nr + y.nd) * 8 + (y.stage_bytes <= amps ? 0 : y.stage_bytes);
 }
 
 template <int K>
