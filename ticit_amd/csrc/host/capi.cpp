@@ -175,7 +175,8 @@ ticit_hip_status create_from_program(Program prog, const ticit_hip_options* opti
   // workgroup while the FP64 state fits the LDS, otherwise unsupported.
   auto words = [](int bits) { return uint32_t((bits + 63) / 64); };
   h->lane = LaneLayout{words(t.npartials), words(t.nslots), words(t.nrec), words(t.ndet), 0,
-                       uint32_t(t.npartials), uint32_t(t.nslots), uint32_t(t.nrec), uint32_t(t.ndet)};
+                       uint32_t(t.npartials), uint32_t(t.nslots), uint32_t(t.nrec), uint32_t(t.ndet),
+                       uint32_t(t.dets_once), 0};
   h->group = GroupLayout{h->lane.pw, h->lane.sw, h->lane.rw, h->lane.dw};
   uint32_t max_row = std::max<uint32_t>({uint32_t(t.nrec), uint32_t(t.ndet), uint32_t(t.nobs), 1});
   h->lane.stage_bytes = 64 * ((max_row + 15) / 16 * 16);

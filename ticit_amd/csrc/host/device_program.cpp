@@ -628,6 +628,16 @@ HostTables lower_to_device(const Program& p, bool components, const std::vector<
     t.steps.push_back(s);
   }
   build_lane_schedule(t);
+  {
+    std::vector<int> writes(size_t(std::max(0, t.ndet)), 0);
+    bool ok = true;
+    for (const DevInstr& d : t.instrs)
+      if ((d.op & 0xff) == kOpDetector) {
+        if (d.detector < 0 || d.detector >= t.ndet || writes[size_t(d.detector)]++) ok = false;
+      }
+    for (int w : writes) ok = ok && w == 1;
+    t.dets_once = ok;
+  }
   // keep every table non-empty so device pointers are always valid
   if (t.resid.empty()) t.resid.push_back(0);
   if (t.det_recs.empty()) t.det_recs.push_back(0);

@@ -35,6 +35,7 @@ struct HostTables {
   std::vector<uint64_t> u64pool;
   std::vector<int32_t> i32pool;
   int n = 0, initial_k = 0, max_k = 0, nrec = 0, ndet = 0, nobs = 0, nslots = 0;
+  bool dets_once = false;  // each detector id has exactly one Detector instruction
   int npartials = 0, nsym = 0, nmeasure = 0, ncoin = 0, nexp = 0;
   // planning features used by the tier planner
   size_t expression_terms = 0;

@@ -227,6 +227,10 @@ struct LaneLayout {
   // bit-sliced tables of the lane tier: one 64-bit word per item, bit l =
   // the shot of lane l
   uint32_t np, ns, nr, nd;  // partials, runtime slots, records, detectors
+  // every detector is written exactly once, by the schedule's last step:
+  // its table may share the amplitudes' LDS, dead by then
+  uint32_t dets_once;
+  uint32_t dets_in_amps;    // per launch: the detector table lives there
 };
 
 struct GroupLayout {

@@ -59,12 +59,17 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
   uint64_t* Pw = reinterpret_cast<uint64_t*>(scr + SDIM_LDS * 64);  // exogenous partials
   uint64_t* Sw = Pw + Y.np;                                     // runtime slots
   uint64_t* Rw = Sw + Y.ns;                                     // records
-  uint64_t* Dw = Rw + Y.nr;                                     // detectors
-  // Row staging lives past the tables, or over the amplitudes, which are
-  // dead once the rows are emitted, when it fits there.
-  uint8_t* stage = Y.stage_bytes <= uint32_t((DIM + SDIM_LDS) * 64 * 16) ? smem : reinterpret_cast<uint8_t*>(Dw + Y.nd);
+  // The amplitudes are dead once the schedule's last step (the detectors)
+  // runs: the detector table and the row staging share their LDS when they
+  // fit (the host decides, lane_tier_lds_bytes counts the same way).
+  const uint32_t amp_bytes = uint32_t((DIM + SDIM_LDS) * 64 * 16);
+  const bool stage_in_amps = Y.stage_bytes <= amp_bytes;
+  uint64_t* Dw = Y.dets_in_amps ? reinterpret_cast<uint64_t*>(smem + (stage_in_amps ? Y.stage_bytes : 0u))
+                                : Rw + Y.nr;  // detectors
+  uint8_t* stage = stage_in_amps ? smem : reinterpret_cast<uint8_t*>(Rw + Y.nr + (Y.dets_in_amps ? 0u : Y.nd));
 
-  for (uint32_t w = lane; w < Y.np + Y.ns + Y.nr + Y.nd; w += 64) Pw[w] = 0;
+  const uint32_t nzero = Y.np + Y.ns + Y.nr + (Y.dets_in_amps ? 0u : Y.nd);
+  for (uint32_t w = lane; w < nzero; w += 64) Pw[w] = 0;
   __syncthreads();
 
   // Frame: noise events toggle the exogenous partial bits they feed.
@@ -382,11 +387,23 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
 // Host launchers
 // --------------------------------------------------------------------------
 
-size_t lane_tier_lds_bytes(int kmax, const LaneLayout& y) {
+static size_t lane_amp_bytes(int kmax) {
   size_t dim = size_t(1) << kmax;
   size_t sdim = kmax > 2 ? dim / 2 : 0;  // widths up to 2 collapse in registers
-  size_t amps = (dim + sdim) * 64 * 16;
-  return amps + size_t(y.np + y.ns + y.nr + y.nd) * 8 + (y.stage_bytes <= amps ? 0 : y.stage_bytes);
+  return (dim + sdim) * 64 * 16;
+}
+
+// Whether the detector table fits the amplitudes' LDS beside the row staging.
+static uint32_t lane_dets_in_amps(int kmax, const LaneLayout& y) {
+  const size_t amps = lane_amp_bytes(kmax);
+  const size_t staged = y.stage_bytes <= amps ? y.stage_bytes : 0;
+  return uint32_t(y.dets_once && y.nd > 0 && staged % 8 == 0 && staged + size_t(y.nd) * 8 <= amps);
+}
+
+size_t lane_tier_lds_bytes(int kmax, const LaneLayout& y) {
+  const size_t amps = lane_amp_bytes(kmax);
+  const size_t dets = lane_dets_in_amps(kmax, y) ? 0 : size_t(y.nd) * 8;
+  return amps + size_t(y.np + y.ns + y.nr) * 8 + dets + (y.stage_bytes <= amps ? 0 : y.stage_bytes);
 }
 
 template <int K>
@@ -397,6 +414,7 @@ static hipError_t launch_lane_k(const DevProgram& P, const DevLaunch& L, const L
   uint32_t row = L.keep_records ? (L.meas_row > L.det_row ? L.meas_row : L.det_row) : 0;
   row = row > L.obs_row ? row : L.obs_row;
   Y.stage_bytes = 64 * ((row + 7) / 8 * 8);
+  Y.dets_in_amps = lane_dets_in_amps(K, Y);
   size_t lds = lane_tier_lds_bytes(K, Y);
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&lane_tier_kernel<K>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
@@ -412,6 +430,7 @@ template <int K>
 static void lane_occupancy_k(const LaneLayout& Yp, uint32_t row_bytes, uint32_t* lds, uint32_t* blocks) {
   LaneLayout Y = Yp;
   Y.stage_bytes = 64 * ((row_bytes + 7) / 8 * 8);
+  Y.dets_in_amps = lane_dets_in_amps(K, Y);
   *lds = uint32_t(lane_tier_lds_bytes(K, Y));
   int b = 0;
   if (hipFuncSetAttribute(reinterpret_cast<const void*>(&lane_tier_kernel<K>),
