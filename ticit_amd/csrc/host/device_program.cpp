@@ -150,6 +150,28 @@ void build_lane_schedule(HostTables& t) {
     t.bsteps.insert(t.bsteps.end(), dets.begin(), dets.end());
     t.lsteps.push_back(b);
   }
+  // Batch entries carry short id lists inline, saving each lane a dependent
+  // global load (the coin draws keep their fields).
+  for (const DevStep& l : t.lsteps) {
+    const uint32_t lop = l.op & 0xff;
+    if (lop != kOpRecBatch && lop != kOpDetBatch) continue;
+    for (int32_t j = l.a; j < l.a + l.b; ++j) {
+      DevStep& q = t.bsteps[size_t(j)];
+      if (lop == kOpDetBatch && (q.op & kFlagDetRecords)) {
+        if (q.c < 0 || q.c > 3) continue;
+        const int32_t off = q.b;
+        q.partial = q.c > 0 ? int32_t(t.det_recs[size_t(off)]) : 0;
+        q.resid_off = q.c > 1 ? t.det_recs[size_t(off) + 1] : 0u;
+        q.full = q.c > 2 ? int32_t(t.det_recs[size_t(off) + 2]) : 0;
+      } else {
+        if (q.resid_len > 2) continue;
+        const uint32_t off = q.resid_off;
+        q.c = q.resid_len > 0 ? int32_t(t.resid[off]) : 0;
+        q.full = q.resid_len > 1 ? int32_t(t.resid[off + 1]) : 0;
+      }
+      q.op |= kFlagInline;
+    }
+  }
   t.nlsteps = int32_t(t.lsteps.size());
   if (t.lsteps.empty()) t.lsteps.push_back(DevStep{});
   if (t.bsteps.empty()) t.bsteps.push_back(DevStep{});

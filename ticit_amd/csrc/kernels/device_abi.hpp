@@ -45,6 +45,10 @@ enum : uint32_t {
                                 // slots forced_off..+forced_len, not drawn
                                 // (src/sampler/pinning.rs, src/sampler/batch/active.rs:535-554)
   kFlagForcedOne = 1u << 19,    // forced_const
+  kFlagInline = 1u << 20,       // lane-tier batch entry with its short id list inline:
+                                // a record-list detector's <= 3 record ids in
+                                // partial, resid_off, full (count in c); otherwise
+                                // <= 2 runtime-slot ids in c, full (count in resid_len)
   kFlagExpVal = 1u << 16,       // expectation probe (Record/Measure/Coin): `record`
                                 // holds the expectation slot; no record, branch or
                                 // collapse (src/sampler/batch/runtime.rs:311-356)

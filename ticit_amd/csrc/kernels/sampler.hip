@@ -142,10 +142,21 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
           const DevStep q = P.bsteps[j];
           uint64_t v = (q.op & kFlagConstant) ? ~0ull : 0ull;
           if (op == kOpDetBatch && (q.op & kFlagDetRecords)) {
-            for (int32_t r = 0; r < q.c; ++r) v ^= Rw[P.det_recs[q.b + r]];
+            if (q.op & kFlagInline) {
+              if (q.c > 0) v ^= Rw[q.partial];
+              if (q.c > 1) v ^= Rw[q.resid_off];
+              if (q.c > 2) v ^= Rw[q.full];
+            } else {
+              for (int32_t r = 0; r < q.c; ++r) v ^= Rw[P.det_recs[q.b + r]];
+            }
           } else {
             if (q.partial >= 0) v ^= Pw[q.partial];
-            for (uint32_t r = 0; r < q.resid_len; ++r) v ^= Sw[P.resid[q.resid_off + r]];
+            if (q.op & kFlagInline) {
+              if (q.resid_len > 0) v ^= Sw[q.c];
+              if (q.resid_len > 1) v ^= Sw[q.full];
+            } else {
+              for (uint32_t r = 0; r < q.resid_len; ++r) v ^= Sw[P.resid[q.resid_off + r]];
+            }
           }
           if (op == kOpDetBatch) {
             Dw[q.a] = v;
