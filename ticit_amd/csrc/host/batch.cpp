@@ -130,6 +130,17 @@ uint32_t round_region(size_t bytes) {
   return r;
 }
 
+// LDS bytes per shot of a bucket at `tps` threads per shot (each wave of a
+// shot keeps its own bit tables, so the region grows with the waves).
+uint32_t bucket_region(const ticit_hip_batch& b, const ticit_hip_batch::Bucket& k, int tps) {
+  size_t r = 0;
+  for (uint32_t i : k.items) {
+    const ticit_hip_program* h = b.progs[i];
+    r = std::max(r, group_region_bytes(h->tab.state_dim, h->group, tps));
+  }
+  return uint32_t(r);
+}
+
 hipError_t unit_table(const ticit_hip_batch& b, const ticit_hip_batch::Bucket& k, int tps,
                       const std::vector<uint64_t>& shots, std::vector<uint64_t>& off, uint64_t& total) {
   const int spu = group_shots_per_unit(tps);
@@ -146,7 +157,8 @@ hipError_t unit_table(const ticit_hip_batch& b, const ticit_hip_batch::Bucket& k
 hipError_t run_bucket(ticit_hip_batch& b, ticit_hip_batch::Bucket& k, int tps, const DevLaunch* d_launches,
                       const uint64_t* d_unit_off, uint64_t total_units, unsigned long long* counter,
                       hipStream_t stream) {
-  int occ = group_batch_occupancy(tps, k.region);
+  const uint32_t region = bucket_region(b, k, tps);
+  int occ = group_batch_occupancy(tps, region, k.max_local_k);
   int cus = 0;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b.device);
   if (b.split && b.group_cus > 0) cus = b.group_cus;
@@ -155,7 +167,7 @@ hipError_t run_bucket(ticit_hip_batch& b, ticit_hip_batch::Bucket& k, int tps, c
     grid = std::max(cus, grid * b.group_grid_percent / 100);
   k.grid = grid;
   return launch_group_batch(k.d_progs, d_launches, k.d_layouts, d_unit_off, uint32_t(k.items.size()), total_units,
-                            counter, k.region, tps, grid, stream);
+                            counter, region, tps, k.max_local_k, grid, stream);
 }
 
 // Times every candidate on up to `sample` shots per program and keeps the
@@ -321,7 +333,7 @@ ticit_hip_status ticit_hip_batch_create(const ticit_hip_batch_item* items, size_
     } else if (h->tier == kTierWide) {
       b->wide_items.push_back(uint32_t(i));
     } else {
-      const uint32_t region = uint32_t(group_region_bytes(t.state_dim, h->group));
+      const uint32_t region = uint32_t(group_region_bytes(t.state_dim, h->group, 64));
       const uint32_t cls = round_region(region);
       auto it = std::find_if(b->buckets.begin(), b->buckets.end(),
                              [&](const ticit_hip_batch::Bucket& k) { return k.size_class == cls; });
@@ -367,7 +379,7 @@ ticit_hip_status ticit_hip_batch_create(const ticit_hip_batch_item* items, size_
   for (auto& k : b->buckets) {
     for (int tps : {8, 16, 32, 64, 128, 256, 512})
       if (k.max_local_k <= group_max_local_k(tps) &&
-          64 + size_t(group_shots_per_block(tps)) * k.region <= size_t(lds_limit))
+          64 + size_t(group_shots_per_block(tps)) * bucket_region(*b, k, tps) <= size_t(lds_limit))
         k.candidates.push_back(tps);
     if (k.candidates.empty()) {
       delete b;
@@ -509,7 +521,7 @@ ticit_hip_status ticit_hip_batch_info_get(const ticit_hip_batch* b, ticit_hip_ba
     const auto& k = b->buckets[j];
     out->bucket_tps[j] = uint32_t(k.tps);
     out->bucket_items[j] = uint32_t(k.items.size());
-    out->bucket_region_bytes[j] = k.region;
+    out->bucket_region_bytes[j] = bucket_region(*b, k, k.tps);
     out->bucket_grid[j] = uint32_t(k.grid);
     for (int c = 0; c < 8; ++c) out->bucket_tuned_ms[j][c] = k.tuned_ms[c];
   }

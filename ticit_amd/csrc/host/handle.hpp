@@ -18,18 +18,18 @@ namespace ticit {
 size_t lane_tier_lds_bytes(int kmax, const LaneLayout& y);
 void lane_tier_occupancy(int kmax, const LaneLayout& Y, uint32_t row_bytes, uint32_t* lds, uint32_t* blocks);
 size_t group_tier_lds_bytes(uint32_t state_dim, const GroupLayout& y, int tps);
-size_t group_region_bytes(uint32_t state_dim, const GroupLayout& y);
+size_t group_region_bytes(uint32_t state_dim, const GroupLayout& y, int tps);
 int group_max_local_k(int tps);
 int group_shots_per_block(int tps);
 int group_shots_per_unit(int tps);
-int group_batch_occupancy(int tps, uint32_t region_bytes);
+int group_batch_occupancy(int tps, uint32_t region_bytes, int max_local_k);
 hipError_t launch_lane_tier(int kmax, const DevProgram& P, const DevLaunch& L, const LaneLayout& Y,
                             hipStream_t stream);
 hipError_t launch_group_tier(const DevProgram& P, const DevLaunch& L, const GroupLayout& Y, hipStream_t stream,
                              int tps);
 hipError_t launch_group_batch(const DevProgram* progs, const DevLaunch* launches, const GroupLayout* layouts,
                               const uint64_t* unit_off, uint32_t nitems, uint64_t total_units,
-                              unsigned long long* counter, uint32_t region_bytes, int tps, int grid,
+                              unsigned long long* counter, uint32_t region_bytes, int tps, int max_local_k, int grid,
                               hipStream_t stream);
 hipError_t launch_batch_prologue(int* error, uint64_t* seed_offset, uint64_t value, hipStream_t stream);
 size_t wide_bytes_per_shot(uint32_t state_dim, int fp32, const WideLayout& y);

@@ -6,8 +6,11 @@
 // spreads the interpreter's per-instruction work over several shots for
 // small states. TPS = 256 is the default launch; 512 holds states of 2^13
 // (128 KiB FP64).
-// Projections go through registers, kProjPer outputs per thread, so a shot
-// group of TPS threads handles local widths up to log2(2 * kProjPer * TPS).
+// Projections go through registers, PROJ outputs per thread (kProjPer at
+// most), so a shot group of TPS threads handles local widths up to
+// log2(2 * PROJ * TPS). Each launch picks the smallest PROJ that covers its
+// program: the staging registers set the kernel's VGPR count, and with it
+// how many waves a SIMD holds.
 //
 // Two entry points share the body: a plain grid over one program's shots,
 // and a persistent kernel that pulls (program, shot block) work units of a
@@ -30,6 +33,13 @@ constexpr int group_block(int tps) { return tps < 64 ? 64 : tps; }
 // per 256 threads' worth of shots.
 constexpr int group_unit_shots(int tps) { return tps < 256 ? 256 / tps : 1; }
 constexpr int kProjPer = 8;
+constexpr int kProjNarrow = 2;
+// Copies of a shot's runtime-slot, record and detector bits: one per wave of
+// the shot, so bit-only steps need no workgroup barrier (every lane of a
+// wave writes the same value to its own copy; a wave's LDS operations
+// complete in order). Exogenous partial bits are written once, before the
+// first barrier, and shared.
+constexpr int group_bit_copies(int tps) { return tps < 64 ? 1 : tps / 64; }
 
 __device__ __forceinline__ double group_wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -37,13 +47,13 @@ __device__ __forceinline__ double group_wave_sum(double v) {
 }
 
 // Bytes of LDS one shot needs.
-__host__ __device__ inline size_t group_shot_bytes(uint32_t state_dim, const GroupLayout& y) {
-  return size_t(state_dim) * 16 + size_t(y.pw + y.sw + y.rw + y.dw) * 8;
+__host__ __device__ inline size_t group_shot_bytes(uint32_t state_dim, const GroupLayout& y, int tps) {
+  return size_t(state_dim) * 16 + size_t(y.pw + uint32_t(group_bit_copies(tps)) * (y.sw + y.rw + y.dw)) * 8;
 }
 
 // Runs shots [first_local, first_local + group_block(TPS) / TPS) of launch L
 // on the workgroup (one shot unless TPS < 64).
-template <int TPS>
+template <int TPS, int PROJ>
 __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const GroupLayout& Y, uint64_t first_local,
                             unsigned char* smem, size_t region_bytes) {
   constexpr int SPB = group_block(TPS) / TPS;
@@ -60,12 +70,17 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
   unsigned char* region = smem + 64 + size_t(g) * region_bytes;
   double* const re0 = reinterpret_cast<double*>(region);
   double* const im0 = re0 + DIM;
+  constexpr int COPIES = group_bit_copies(TPS);
+  const uint32_t bit_words = Y.sw + Y.rw + Y.dw;
   uint64_t* pbits = reinterpret_cast<uint64_t*>(im0 + DIM);
-  uint64_t* sbits = pbits + Y.pw;
+  uint64_t* sbits = pbits + Y.pw + uint32_t(COPIES > 1 ? t / 64 : 0) * bit_words;  // this wave's copy
   uint64_t* rbits = sbits + Y.sw;
   uint64_t* dbits = rbits + Y.rw;
+  // the first lane of each wave (of each shot group below 64 threads)
+  // writes that wave's copy; the wave's later reads follow in issue order
+  const bool bit_writer = (t & ((TPS < 64 ? TPS : 64) - 1)) == 0;
 
-  for (uint32_t w = t; w < Y.pw + Y.sw + Y.rw + Y.dw; w += TPS) pbits[w] = 0;
+  for (uint32_t w = t; w < Y.pw + uint32_t(COPIES) * bit_words; w += TPS) pbits[w] = 0;
   const int dim0 = 1 << P.initial_k;
   for (int a = t; a < dim0; a += TPS) {
     re0[a] = a == 0 ? 1.0 : 0.0;
@@ -207,23 +222,22 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
       } else {
         b = eval(st);
       }
-      // One thread per shot writes; no reader of this instruction reads the
-      // bit being written, and the barrier orders it before later readers.
+      // One lane per wave writes its wave's copy; later readers are lanes
+      // of the same wave.
       if (st.op & kFlagExpVal) {
         if (expv) expv[st.a] = b ? -1.0 : 1.0;
-      } else if (t == 0) {
-        if (op == kOpDetector) {
-          set_bit(dbits, uint32_t(st.a), b);
-        } else {
-          if (st.a >= 0) set_bit(rbits, uint32_t(st.a), b);
-          if (st.b >= 0) set_bit(sbits, uint32_t(st.b), b);
-        }
+      } else if (!bit_writer) {
+      } else if (op == kOpDetector) {
+        set_bit(dbits, uint32_t(st.a), b);
+      } else {
+        if (st.a >= 0) set_bit(rbits, uint32_t(st.a), b);
+        if (st.b >= 0) set_bit(sbits, uint32_t(st.b), b);
       }
-      __syncthreads();
       // A fired postselected detector discards the shot: nothing later can
       // reach its outputs (src/sampler/batch/postselect.rs removes it from
-      // the block). With one shot per workgroup `b` is uniform, so the
-      // whole group leaves the loop together; packed sub-wave shots run on.
+      // the block). With one shot per workgroup `b` is the same in every
+      // thread, so the whole group leaves the loop at this step; packed
+      // sub-wave shots run on.
       if constexpr (TPS >= 64) {
         if (b && (st.op & kFlagPostselect) && op == kOpDetector) break;
       }
@@ -292,9 +306,9 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
           ps = 1.0;
         }
         const double inv = 1.0 / sqrt(ps);
-        double nr[kProjPer], ni[kProjPer];
+        double nr[PROJ], ni[PROJ];
 #pragma unroll
-        for (int j = 0; j < kProjPer; ++j) {
+        for (int j = 0; j < PROJ; ++j) {
           int idx = t + j * TPS;
           nr[j] = 0.0;
           ni[j] = 0.0;
@@ -317,7 +331,7 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
         }
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < kProjPer; ++j) {
+        for (int j = 0; j < PROJ; ++j) {
           int idx = t + j * TPS;
           if (idx < out_dim) {
             re[idx] = nr[j];
@@ -337,14 +351,15 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
           branch = sm_next(brng) & 1;
         }
       }
-      if (t == 0) set_bit(sbits, uint32_t(st.c), branch);
-      __syncthreads();
+      // `branch` is the same in every thread of the shot: one lane per wave
+      // writes the wave's copy
+      if (bit_writer) set_bit(sbits, uint32_t(st.c), branch);
       bool b = eval(st);
-      if (t == 0) {
+      if (bit_writer) {
         if (st.a >= 0) set_bit(rbits, uint32_t(st.a), b);
         if (st.b >= 0) set_bit(sbits, uint32_t(st.b), b);
       }
-      __syncthreads();
+      if (op == kOpMeasure) __syncthreads();  // the collapsed state before its next reader
     }
   }
   if (!valid) return;
@@ -388,19 +403,21 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
       uint32_t(P.nobs));
 }
 
-template <int TPS>
-__global__ __launch_bounds__(group_block(TPS)) void group_tier_kernel(DevProgram P, DevLaunch L, GroupLayout Y,
-                                                                 uint32_t region_bytes) {
+// Narrow staging fits 80 VGPRs without spilling (six waves per SIMD), the
+// full staging 96 (five).
+template <int TPS, int PROJ>
+__global__ __launch_bounds__(group_block(TPS)) __attribute__((amdgpu_waves_per_eu(PROJ == kProjNarrow ? 6 : 5)))
+void group_tier_kernel(DevProgram P, DevLaunch L, GroupLayout Y, uint32_t region_bytes) {
   extern __shared__ __align__(16) unsigned char smem[];
-  group_shots<TPS>(P, L, Y, uint64_t(blockIdx.x) * (group_block(TPS) / TPS), smem, region_bytes);
+  group_shots<TPS, PROJ>(P, L, Y, uint64_t(blockIdx.x) * (group_block(TPS) / TPS), smem, region_bytes);
 }
 
 // Persistent kernel over a batch: unit u of item i covers shots
 // [(u - unit_off[i]) * SPU, ...) of that item, run one after another;
 // workgroups take units from *counter until all `total_units` are done.
-template <int TPS>
-__global__ __launch_bounds__(group_block(TPS)) __attribute__((amdgpu_waves_per_eu(4))) void group_batch_kernel(const DevProgram* progs, const DevLaunch* launches,
-                                                                  const GroupLayout* layouts,
+template <int TPS, int PROJ>
+__global__ __launch_bounds__(group_block(TPS)) __attribute__((amdgpu_waves_per_eu(4))) void group_batch_kernel(
+    const DevProgram* __restrict__ progs, const DevLaunch* __restrict__ launches, const GroupLayout* __restrict__ layouts,
                                                                   const uint64_t* unit_off, uint32_t nitems,
                                                                   uint64_t total_units, unsigned long long* counter,
                                                                   uint32_t region_bytes) {
@@ -436,7 +453,7 @@ __global__ __launch_bounds__(group_block(TPS)) __attribute__((amdgpu_waves_per_e
     const uint64_t first = (u - unit_off[item]) * SPU;
     for (int q = 0; q < SPU; q += SPB) {
       if (first + uint64_t(q) >= L.shots) break;  // uniform: one item per unit
-      group_shots<TPS>(P, L, Y, first + uint64_t(q), smem, region_bytes);
+      group_shots<TPS, PROJ>(P, L, Y, first + uint64_t(q), smem, region_bytes);
       __syncthreads();
     }
   }
@@ -460,15 +477,15 @@ hipError_t launch_batch_prologue(int* error, uint64_t* seed_offset, uint64_t val
 // Host side
 // --------------------------------------------------------------------------
 
-size_t group_region_bytes(uint32_t state_dim, const GroupLayout& y) {
-  return (group_shot_bytes(state_dim, y) + 15) / 16 * 16;
+size_t group_region_bytes(uint32_t state_dim, const GroupLayout& y, int tps) {
+  return (group_shot_bytes(state_dim, y, tps) + 15) / 16 * 16;
 }
 
 int group_shots_per_block(int tps) { return group_block(tps) / tps; }
 int group_shots_per_unit(int tps) { return group_unit_shots(tps); }
 
 size_t group_tier_lds_bytes(uint32_t state_dim, const GroupLayout& y, int tps) {
-  return 64 + size_t(group_shots_per_block(tps)) * group_region_bytes(state_dim, y);
+  return 64 + size_t(group_shots_per_block(tps)) * group_region_bytes(state_dim, y, tps);
 }
 
 // Largest local width a shot group of `tps` threads projects through registers.
@@ -478,17 +495,29 @@ int group_max_local_k(int tps) {
   return k;
 }
 
-template <int TPS>
-static hipError_t launch_group_t(const DevProgram& P, const DevLaunch& L, const GroupLayout& Y, hipStream_t stream) {
-  size_t region = group_region_bytes(P.state_dim, Y);
+// Whether `max_local_k` fits the narrow projection staging at `tps`.
+static bool group_narrow(int tps, int max_local_k) {
+  return max_local_k <= 1 || (size_t(1) << (max_local_k - 1)) <= size_t(kProjNarrow) * size_t(tps);
+}
+
+template <int TPS, int PROJ>
+static hipError_t launch_group_tp(const DevProgram& P, const DevLaunch& L, const GroupLayout& Y, hipStream_t stream) {
+  size_t region = group_region_bytes(P.state_dim, Y, TPS);
   constexpr int SPB = group_block(TPS) / TPS;
   size_t lds = 64 + size_t(SPB) * region;
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&group_tier_kernel<TPS>),
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&group_tier_kernel<TPS, PROJ>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
   if (e != hipSuccess) return e;
   dim3 grid(unsigned((L.shots + SPB - 1) / SPB));
-  hipLaunchKernelGGL(group_tier_kernel<TPS>, grid, dim3(group_block(TPS)), lds, stream, P, L, Y, uint32_t(region));
+  hipLaunchKernelGGL((group_tier_kernel<TPS, PROJ>), grid, dim3(group_block(TPS)), lds, stream, P, L, Y,
+                     uint32_t(region));
   return hipGetLastError();
+}
+
+template <int TPS>
+static hipError_t launch_group_t(const DevProgram& P, const DevLaunch& L, const GroupLayout& Y, hipStream_t stream) {
+  return group_narrow(TPS, P.max_local_k) ? launch_group_tp<TPS, kProjNarrow>(P, L, Y, stream)
+                                          : launch_group_tp<TPS, kProjPer>(P, L, Y, stream);
 }
 
 hipError_t launch_group_tier(const DevProgram& P, const DevLaunch& L, const GroupLayout& Y, hipStream_t stream,
@@ -507,64 +536,80 @@ hipError_t launch_group_tier(const DevProgram& P, const DevLaunch& L, const Grou
   }
 }
 
-template <int TPS>
-static hipError_t launch_group_batch_t(const DevProgram* progs, const DevLaunch* launches, const GroupLayout* layouts,
-                                       const uint64_t* unit_off, uint32_t nitems, uint64_t total_units,
-                                       unsigned long long* counter, uint32_t region_bytes, int grid,
-                                       hipStream_t stream) {
+template <int TPS, int PROJ>
+static hipError_t launch_group_batch_tp(const DevProgram* progs, const DevLaunch* launches,
+                                        const GroupLayout* layouts, const uint64_t* unit_off, uint32_t nitems,
+                                        uint64_t total_units, unsigned long long* counter, uint32_t region_bytes,
+                                        int grid, hipStream_t stream) {
   size_t lds = 64 + size_t(group_block(TPS) / TPS) * region_bytes;
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&group_batch_kernel<TPS>),
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&group_batch_kernel<TPS, PROJ>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, counter);
-  hipLaunchKernelGGL(group_batch_kernel<TPS>, dim3(unsigned(grid)), dim3(group_block(TPS)), lds, stream, progs,
-                     launches,
-                     layouts, unit_off, nitems, total_units, counter, region_bytes);
+  hipLaunchKernelGGL((group_batch_kernel<TPS, PROJ>), dim3(unsigned(grid)), dim3(group_block(TPS)), lds, stream,
+                     progs, launches, layouts, unit_off, nitems, total_units, counter, region_bytes);
   return hipGetLastError();
+}
+
+template <int TPS>
+static hipError_t launch_group_batch_t(const DevProgram* progs, const DevLaunch* launches, const GroupLayout* layouts,
+                                       const uint64_t* unit_off, uint32_t nitems, uint64_t total_units,
+                                       unsigned long long* counter, uint32_t region_bytes, int max_local_k, int grid,
+                                       hipStream_t stream) {
+  return group_narrow(TPS, max_local_k)
+             ? launch_group_batch_tp<TPS, kProjNarrow>(progs, launches, layouts, unit_off, nitems, total_units,
+                                                       counter, region_bytes, grid, stream)
+             : launch_group_batch_tp<TPS, kProjPer>(progs, launches, layouts, unit_off, nitems, total_units, counter,
+                                                    region_bytes, grid, stream);
 }
 
 hipError_t launch_group_batch(const DevProgram* progs, const DevLaunch* launches, const GroupLayout* layouts,
                               const uint64_t* unit_off, uint32_t nitems, uint64_t total_units,
-                              unsigned long long* counter, uint32_t region_bytes, int tps, int grid,
+                              unsigned long long* counter, uint32_t region_bytes, int tps, int max_local_k, int grid,
                               hipStream_t stream) {
   switch (tps) {
     case 8:
       return launch_group_batch_t<8>(progs, launches, layouts, unit_off, nitems, total_units, counter, region_bytes,
-                                     grid, stream);
+                                     max_local_k, grid, stream);
     case 16:
       return launch_group_batch_t<16>(progs, launches, layouts, unit_off, nitems, total_units, counter, region_bytes,
-                                      grid, stream);
+                                      max_local_k, grid, stream);
     case 32:
       return launch_group_batch_t<32>(progs, launches, layouts, unit_off, nitems, total_units, counter, region_bytes,
-                                      grid, stream);
+                                      max_local_k, grid, stream);
     case 64:
       return launch_group_batch_t<64>(progs, launches, layouts, unit_off, nitems, total_units, counter, region_bytes,
-                                      grid, stream);
+                                      max_local_k, grid, stream);
     case 128:
       return launch_group_batch_t<128>(progs, launches, layouts, unit_off, nitems, total_units, counter,
-                                       region_bytes, grid, stream);
+                                       region_bytes, max_local_k, grid, stream);
     case 256:
       return launch_group_batch_t<256>(progs, launches, layouts, unit_off, nitems, total_units, counter,
-                                       region_bytes, grid, stream);
+                                       region_bytes, max_local_k, grid, stream);
     case 512:
       return launch_group_batch_t<512>(progs, launches, layouts, unit_off, nitems, total_units, counter,
-                                       region_bytes, grid, stream);
+                                       region_bytes, max_local_k, grid, stream);
     default:
       return hipErrorInvalidValue;
   }
 }
 
+template <int PROJ>
+static const void* group_batch_fn(int tps) {
+  return tps == 8     ? reinterpret_cast<const void*>(&group_batch_kernel<8, PROJ>)
+         : tps == 16  ? reinterpret_cast<const void*>(&group_batch_kernel<16, PROJ>)
+         : tps == 32  ? reinterpret_cast<const void*>(&group_batch_kernel<32, PROJ>)
+         : tps == 64  ? reinterpret_cast<const void*>(&group_batch_kernel<64, PROJ>)
+         : tps == 128 ? reinterpret_cast<const void*>(&group_batch_kernel<128, PROJ>)
+         : tps == 256 ? reinterpret_cast<const void*>(&group_batch_kernel<256, PROJ>)
+                      : reinterpret_cast<const void*>(&group_batch_kernel<512, PROJ>);
+}
+
 // Workgroups of the persistent kernel that fit on the device at once.
-int group_batch_occupancy(int tps, uint32_t region_bytes) {
+int group_batch_occupancy(int tps, uint32_t region_bytes, int max_local_k) {
   size_t lds = 64 + size_t(group_shots_per_block(tps)) * region_bytes;
   int blocks = 0;
-  const void* fn = tps == 8     ? reinterpret_cast<const void*>(&group_batch_kernel<8>)
-                   : tps == 16  ? reinterpret_cast<const void*>(&group_batch_kernel<16>)
-                   : tps == 32  ? reinterpret_cast<const void*>(&group_batch_kernel<32>)
-                   : tps == 64  ? reinterpret_cast<const void*>(&group_batch_kernel<64>)
-                   : tps == 128 ? reinterpret_cast<const void*>(&group_batch_kernel<128>)
-                   : tps == 256 ? reinterpret_cast<const void*>(&group_batch_kernel<256>)
-                                : reinterpret_cast<const void*>(&group_batch_kernel<512>);
+  const void* fn = group_narrow(tps, max_local_k) ? group_batch_fn<kProjNarrow>(tps) : group_batch_fn<kProjPer>(tps);
   if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)) != hipSuccess) return 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, fn, group_block(tps), lds) != hipSuccess) return 0;
   return blocks;
