@@ -130,6 +130,8 @@ ticit_hip_status create_from_program(Program prog, const ticit_hip_options* opti
   UP(obs_flip, t.obs_flip);
   UP(sym_off, t.sym_off);
   UP(sym_part, t.sym_part);
+  UP(tl_off, t.tl_off);
+  UP(tl_part, t.tl_part);
   UP(exo_syms, t.exo_syms);
   UP(cats, t.cats);
   UP(rares, t.rares);

@@ -224,8 +224,13 @@ HostTables lower_to_device(const Program& p, bool components, const std::vector<
     dc.cond_off = uint32_t(t.i32pool.size());
     if (d.probs.empty() || d.conds.size() != size_t(d.nbits))
       throw TicitError(1, "invalid categorical symbolic distribution");
+    // cumulative sums in row order: the kernels binary-search the first
+    // row whose running sum reaches the uniform, which is the row the
+    // sequential walk stops at (src/random.rs:40-50)
+    double cum = 0.0;
     for (size_t r = 0; r < d.probs.size(); ++r) {
-      t.f64pool.push_back(d.probs[r]);
+      cum += d.probs[r];
+      t.f64pool.push_back(cum);
       t.u64pool.push_back(d.assignments[r]);
     }
     for (int32_t c : d.conds) t.i32pool.push_back(c);
@@ -252,7 +257,11 @@ HostTables lower_to_device(const Program& p, bool components, const std::vector<
       t.i32pool.push_back(r);
     }
     dr.ev_prob_off = uint32_t(t.f64pool.size());
-    for (double q : g.event_probs) t.f64pool.push_back(q);
+    double cum = 0.0;
+    for (double q : g.event_probs) {
+      cum += q;
+      t.f64pool.push_back(cum);
+    }
     if (dr.enabled && dr.nev == 0) throw TicitError(1, "rare categorical group has no event rows");
     t.rares.push_back(dr);
   }
@@ -609,6 +618,49 @@ HostTables lower_to_device(const Program& p, bool components, const std::vector<
     t.sym_off.push_back(uint32_t(t.sym_part.size()));
   }
 
+  // Toggle lists: the partials one noise event flips, with partials named
+  // by two of its symbols cancelled, so an event costs one list walk.
+  auto begin_lists = [&]() {
+    t.tl_off.push_back(uint32_t(t.tl_part.size()));
+    return uint32_t(t.tl_off.size() - 1);
+  };
+  auto push_list = [&](const std::vector<int32_t>& syms) {
+    std::vector<uint32_t> parts;
+    for (int32_t c : syms) {
+      if (c <= 0 || c > p.nsymbols) throw TicitError(1, "noise condition out of range");
+      parts.insert(parts.end(), by_sym[size_t(c - 1)].begin(), by_sym[size_t(c - 1)].end());
+    }
+    std::sort(parts.begin(), parts.end());
+    for (size_t i = 0; i < parts.size();) {
+      size_t j = i;
+      while (j < parts.size() && parts[j] == parts[i]) ++j;
+      if ((j - i) & 1) t.tl_part.push_back(parts[i]);
+      i = j;
+    }
+    t.tl_off.push_back(uint32_t(t.tl_part.size()));
+  };
+  auto set_bits = [](const std::vector<int32_t>& conds, uint64_t a) {
+    std::vector<int32_t> out;
+    for (size_t b = 0; b < conds.size(); ++b)
+      if ((a >> b) & 1) out.push_back(conds[b]);
+    return out;
+  };
+  for (size_t i = 0; i < p.categorical.size(); ++i) {
+    const auto& d = p.categorical[i];
+    t.cats[i].tl_base = begin_lists();
+    for (uint64_t a : d.assignments) push_list(set_bits(d.conds, a));
+  }
+  for (size_t i = 0; i < p.rare.size(); ++i) {
+    const auto& g = p.rare[i];
+    t.rares[i].tl_base = begin_lists();
+    for (const auto& set : g.sets)
+      for (int32_t row : g.event_rows) push_list(set_bits(set, g.assignments[size_t(row)]));
+  }
+  for (size_t i = 0; i < p.lowp.size(); ++i) {
+    t.lowps[i].tl_base = begin_lists();
+    for (int32_t c : p.lowp[i].conds) push_list({c});
+  }
+
   t.obs_off.push_back(0);
   for (const auto& ob : p.observables) {
     for (int32_t r : ob) {
@@ -667,6 +719,8 @@ HostTables lower_to_device(const Program& p, bool components, const std::vector<
   if (t.post_dets.empty()) t.post_dets.push_back(0);
   if (t.obs_recs.empty()) t.obs_recs.push_back(0);
   if (t.sym_part.empty()) t.sym_part.push_back(0);
+  if (t.tl_off.empty()) t.tl_off.push_back(0);
+  if (t.tl_part.empty()) t.tl_part.push_back(0);
   if (t.exo_syms.empty()) t.exo_syms.push_back(0);
   if (t.f64pool.empty()) t.f64pool.push_back(0);
   if (t.u64pool.empty()) t.u64pool.push_back(0);

@@ -24,6 +24,7 @@ struct HostTables {
   int32_t nlsteps = 0;  // lsteps without the padding entry
   int32_t npost = 0;    // post_dets without the padding entry
   std::vector<uint32_t> resid, det_recs, obs_off, obs_recs, sym_off, sym_part;
+  std::vector<uint32_t> tl_off, tl_part;  // noise-event toggle lists
   std::vector<uint32_t> post_dets;  // 0-based postselected detectors
   std::vector<uint8_t> obs_flip;    // [max(1, nobs)] inverted observables
   std::vector<int32_t> exo_syms;

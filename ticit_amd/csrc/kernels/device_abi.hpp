@@ -102,12 +102,15 @@ struct alignas(16) DevStep {
 };
 
 // Exogenous noise plan (src/factored.rs:444-462, src/planner.rs:1455-1480).
+// Noise events flip partials through toggle lists: entry k of a family
+// covers partials tl_part[tl_off[tl_base + k] .. tl_off[tl_base + k + 1]),
+// the XOR of the partial lists of the symbols the event sets.
 struct DevCat {       // categorical distribution drawn once per shot
   uint32_t nbits, nrows;
-  uint32_t prob_off;  // f64 pool
+  uint32_t prob_off;  // f64 pool: cumulative probabilities, summed in row order
   uint32_t assign_off;  // u64 pool
   uint32_t cond_off;  // i32 pool
-  uint32_t pad;
+  uint32_t tl_base;   // toggle list of row r: entry r
 };
 
 // Geometric-gap walks: a first uniform below `none_below` is a gap of at
@@ -123,7 +126,8 @@ struct DevRare {      // rare categorical group walked by geometric gaps
   uint32_t cond_off;  // i32 pool: nsets * nbits conditions
   uint32_t assign_off;  // u64 pool: one assignment per row
   uint32_t ev_row_off;  // i32 pool
-  uint32_t ev_prob_off;  // f64 pool
+  uint32_t ev_prob_off;  // f64 pool: cumulative event probabilities
+  uint32_t tl_base;      // toggle list of (set s, event e): entry s * nev + e
 };
 
 enum : uint32_t {
@@ -151,7 +155,7 @@ struct DevLowp {      // low-probability Bernoulli group
   uint32_t enabled;
   uint32_t nconds;
   uint32_t cond_off;
-  uint32_t pad;
+  uint32_t tl_base;   // toggle list of condition j: entry j
 };
 
 // Device pointers of one uploaded program.
@@ -169,6 +173,8 @@ struct DevProgram {
                               // observable bit of 1, src/sampler/prepared.rs:965-974)
   const uint32_t* sym_off;    // nsym + 1 offsets: symbol (c - 1) -> partial ids
   const uint32_t* sym_part;
+  const uint32_t* tl_off;     // noise-event toggle lists (see DevCat)
+  const uint32_t* tl_part;
   const int32_t* exo_syms;    // every exogenous symbol id (injected mode scan)
   const DevCat* cats;
   const DevRare* rares;

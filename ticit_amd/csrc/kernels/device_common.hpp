@@ -53,15 +53,19 @@ __device__ __forceinline__ double gap_of(double u, double denom) {
 
 __device__ __forceinline__ double sample_gap(uint64_t& s, double denom) { return gap_of(sm_uniform(s), denom); }
 
-// src/random.rs:40-50: inclusive cumulative walk, falls through to the last row
-__device__ __forceinline__ uint32_t categorical_row(uint64_t& s, const double* probs, uint32_t n) {
-  double r = sm_uniform(s);
-  double cum = 0.0;
-  for (uint32_t i = 0; i < n; ++i) {
-    cum += probs[i];
-    if (r <= cum) return i;
+// src/random.rs:40-50: the first row whose running sum reaches the uniform,
+// else the last row. `cum` holds the running sums as the sequential walk
+// forms them (host-summed in row order); they never decrease, so a binary
+// search finds the same row.
+__device__ __forceinline__ uint32_t categorical_row(uint64_t& s, const double* cum, uint32_t n) {
+  const double r = sm_uniform(s);
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (r <= cum[mid]) hi = mid;
+    else lo = mid + 1;
   }
-  return n - 1;
+  return lo < n ? lo : n - 1;
 }
 
 // or_low_probability_bits over a single shot (src/sampler/exogenous.rs:150-175):
@@ -75,16 +79,19 @@ __device__ __forceinline__ bool low_probability_bit(uint64_t& s, double denom, d
 }
 
 // Walks one shot's exogenous noise in the reference family order
-// (src/sampler/exogenous.rs:536-625) and calls toggle(c) for every symbol c
-// that comes out true (XOR semantics: a symbol toggled twice is false).
-template <class Toggle>
-__device__ void sample_exogenous(const DevProgram& P, uint64_t rng, Toggle&& toggle) {
+// (src/sampler/exogenous.rs:536-625) and calls flip(p) for every exogenous
+// partial p an event changes (XOR semantics: a partial flipped twice is
+// unchanged). Events reach their partials through the toggle lists.
+template <class Flip>
+__device__ void sample_exogenous(const DevProgram& P, uint64_t rng, Flip&& flip) {
+  auto flip_list = [&](uint32_t k) {
+    const uint32_t o1 = P.tl_off[k + 1];
+    for (uint32_t j = P.tl_off[k]; j < o1; ++j) flip(P.tl_part[j]);
+  };
   for (int i = 0; i < P.ncat; ++i) {
     const DevCat d = P.cats[i];
     uint32_t row = categorical_row(rng, P.f64pool + d.prob_off, d.nrows);
-    uint64_t a = P.u64pool[d.assign_off + row];
-    for (uint32_t b = 0; b < d.nbits; ++b)
-      if ((a >> b) & 1) toggle(P.i32pool[d.cond_off + b]);
+    flip_list(d.tl_base + row);
   }
   for (int i = 0; i < P.nrare; ++i) {
     const DevRare g = P.rares[i];
@@ -99,11 +106,7 @@ __device__ void sample_exogenous(const DevProgram& P, uint64_t rng, Toggle&& tog
       draw += gap;
       uint32_t set = uint32_t(draw);  // one shot per stream: draw / nsets == 0
       uint32_t er = categorical_row(rng, P.f64pool + g.ev_prob_off, g.nev);
-      uint32_t row = uint32_t(P.i32pool[g.ev_row_off + er]);
-      uint64_t a = P.u64pool[g.assign_off + row];
-      const int32_t* conds = P.i32pool + g.cond_off + size_t(set) * g.nbits;
-      for (uint32_t b = 0; b < g.nbits; ++b)
-        if ((a >> b) & 1) toggle(conds[b]);
+      flip_list(g.tl_base + set * g.nev + er);
       draw += 1;
     }
   }
@@ -136,7 +139,8 @@ __device__ void sample_exogenous(const DevProgram& P, uint64_t rng, Toggle&& tog
       }
     }
     if (b.invert) bit = !bit;
-    if (bit) toggle(b.cond);
+    if (bit)
+      for (uint32_t j = P.sym_off[b.cond - 1]; j < P.sym_off[b.cond]; ++j) flip(P.sym_part[j]);
   }
   for (int i = 0; i < P.nlowp; ++i) {
     const DevLowp g = P.lowps[i];
@@ -149,7 +153,7 @@ __device__ void sample_exogenous(const DevProgram& P, uint64_t rng, Toggle&& tog
       int64_t gap = int64_t(first ? gap_of(u0, g.denom) : sample_gap(rng, g.denom));
       if (gap >= total - draw) break;
       draw += gap;
-      toggle(P.i32pool[g.cond_off + uint32_t(draw)]);
+      flip_list(g.tl_base + uint32_t(draw));
       draw += 1;
     }
   }

@@ -101,7 +101,8 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
         if ((L.exo[uint64_t(c - 1) * words + (local >> 6)] >> (local & 63)) & 1) toggle(c);
       }
     } else if (!L.reference) {
-      sample_exogenous(P, seed_for(kExoSeedBase, launch_seed(L), shot), toggle);
+      sample_exogenous(P, seed_for(kExoSeedBase, launch_seed(L), shot),
+                       [&](uint32_t part) { pbits[part >> 6] ^= 1ull << (part & 63); });
     }
   }
   __syncthreads();

@@ -84,9 +84,8 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
     }
   } else if (!L.reference) {
     const uint64_t me = 1ull << lane;
-    sample_exogenous(P, seed_for(kExoSeedBase, launch_seed(L), shot), [&](int32_t c) {
-      for (uint32_t j = P.sym_off[c - 1]; j < P.sym_off[c]; ++j) atomicXor(&Pw[P.sym_part[j]], me);
-    });
+    sample_exogenous(P, seed_for(kExoSeedBase, launch_seed(L), shot),
+                     [&](uint32_t part) { atomicXor(&Pw[part], me); });
   }
   __syncthreads();
 

@@ -117,7 +117,8 @@ __global__ __launch_bounds__(kWideThreads) void wide_init_kernel(DevProgram P, D
       if ((L.exo[uint64_t(c - 1) * words + (local >> 6)] >> (local & 63)) & 1) toggle(c);
     }
   } else if (!L.reference) {
-    sample_exogenous(P, seed_for(kExoSeedBase, launch_seed(L), shot), toggle);
+    sample_exogenous(P, seed_for(kExoSeedBase, launch_seed(L), shot),
+                     [&](uint32_t part) { w.p[part >> 6] ^= 1ull << (part & 63); });
   }
   ctl[j].brng = L.reference ? 1ull : seed_for(kBranchSeedBase, launch_seed(L), shot);
   ctl[j].err = 0;
