@@ -21,7 +21,9 @@ object with the other workloads: C0 (random Clifford+T, group tier), C2
 (256 circuits: one-at-a-time versus one batched graph launch), C3 (dense
 stress: FP64 versus FP32 amplitudes, probability error against the FP64
 oracle), C4 (held-out planner batch: heuristic versus learned layouts,
-one-at-a-time versus batched), each with a bounded CPU-oracle sample.
+one-at-a-time versus batched) and P (a cultivation circuit with every
+detector postselected and normalized syndromes), each with a bounded
+CPU-oracle sample.
 """
 from __future__ import annotations
 
@@ -178,7 +180,8 @@ def other_configs(ticit_amd, w, torch, np, stream, use_cpu=True) -> dict:
     """The other benchmark configurations: C0 (random Clifford+T, group
     tier), C2 (256 circuits, plain versus one batched launch), C3 (dense
     stress, FP64 versus FP32 amplitudes, error against FP64), C4 (held-out
-    planner batch: heuristic versus learned layouts, plain versus batched)."""
+    planner batch: heuristic versus learned layouts, plain versus batched),
+    P (postselected cultivation circuit, survivors compacted on the device)."""
     from oracle import oracle as O
     out = {}
     cores = _cores()
@@ -324,6 +327,35 @@ def other_configs(ticit_amd, w, torch, np, stream, use_cpu=True) -> dict:
             t_cpu += r["sample_s"]
         out["C4"]["cpu_baseline"] = {"value": n_cpu / t_cpu, "unit": "shots/s", "cores": cores, "kind": "port",
                                      "sample": f"held-out programs {sel[0]}..{sel[-1]} step 32, 2000 shots each"}
+
+    # Postselection: the cultivation fixture as the reference's benchmark
+    # driver runs it (normalize_syndromes, a postselection mask), every
+    # detector postselected; survivors compacted on the device.
+    path = os.path.join(ROOT, "tests", "golden", "circuits", "msc_d3_inject_cultivate_p1e-3.stim")
+    with open(path) as f:
+        cp = ticit_amd.Circuit(f.read())
+    mask = [1] * cp.num_detectors
+    sp = cp.compile(mask, normalize_syndromes=True)
+    npost = 1_000_000
+    bp = _device_buffers(torch, sp, npost)
+    acc = torch.zeros(1, dtype=torch.int64, device="cuda:0")
+    t = _time(torch, stream, lambda r: sp.sample_device(npost, 20 + r, bp, stream=st, accepted=acc), reps=5)
+    torch.cuda.synchronize()
+    kept = int(acc.item())
+    out["P"] = {"workload": "magic-state cultivation d=3 (injection + cultivation, p=1e-3), every detector "
+                            "postselected, normalized syndromes; 1e6 shots, survivors compacted on the device",
+                "tier": sp.tier, "shots_per_s": npost / t, "ms": t * 1e3, "accepted_fraction": kept / npost}
+    if use_cpu:
+        n_cpu = 20000
+        O.lib(fast=True)  # loaded before the expected bits are set in both builds
+        O.set_expected(*O.reference_sample(sp.blob))
+        try:
+            r = O.sample(sp.blob, n_cpu, 3, batch_size=1, threads=cores, fast=True)
+        finally:
+            O.set_expected(None, None)
+        out["P"]["cpu_baseline"] = {"value": n_cpu / r["sample_s"], "unit": "shots/s", "cores": cores,
+                                    "kind": "port", "sample": f"{n_cpu} shots of the same postselected program, "
+                                                              "one shot per block"}
     return out
 
 
