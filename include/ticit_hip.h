@@ -217,7 +217,8 @@ ticit_hip_status ticit_hip_sample_device(ticit_hip_program* program, uint64_t sh
  * *accepted (a device uint64, overwritten); observable_ones counts survivors
  * only. Rows pass through a staging buffer the program owns (allocated on
  * first use and grown to the largest call: not for graph capture); calls on
- * one program are ordered on it by an event. Programs without postselection
+ * one program are ordered on it by an event (and a lock between host
+ * threads). Programs without postselection
  * behave as ticit_hip_sample_device with *accepted = shots. */
 ticit_hip_status ticit_hip_sample_device_postselected(ticit_hip_program* program, uint64_t shots, uint64_t seed,
                                                      uint64_t first_shot, int bit_packed, int keep_records,

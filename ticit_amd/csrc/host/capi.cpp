@@ -770,6 +770,7 @@ ticit_hip_status ticit_hip_sample_device_postselected(ticit_hip_program* h, uint
   L.obs_ones = reinterpret_cast<unsigned long long*>(observable_ones);
   L.error = h->d_error;
   const bool want_exp = keep_records && t.nexp > 0 && exp_vals;
+  std::lock_guard<std::mutex> lock(h->post_mu);
   // staging: keep flags, tile offsets, then the full rows of every shot
   auto up = [](size_t b) { return (b + 255) / 256 * 256; };
   const size_t n = size_t(shots);

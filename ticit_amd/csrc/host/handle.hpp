@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -84,6 +85,7 @@ struct ticit_hip_program {
   size_t post_bytes = 0;
   hipEvent_t post_done = nullptr;
   bool post_pending = false;
+  std::mutex post_mu;  // host threads sharing the program take turns on the staging
 
   ~ticit_hip_program() {
     if (post_done) {
