@@ -176,3 +176,34 @@ def test_batch_compacts_postselected_items(hip):
         assert (r.accepted, r.discarded) == (ref.accepted, ref.discarded)
         assert np.array_equal(r.measurements, ref.measurements)
         assert np.array_equal(r.exp_vals, ref.exp_vals)
+
+
+def test_wide_tier_skips_discarded_shots(hip):
+    """A postselected detector that fires in about 90 % of shots sits before
+    a width-12 section (14 qubits, 8 layers): the wide tier stops those shots at the detector
+    (src/sampler/batch/postselect.rs removes them), and the survivors' rows
+    and counts still equal the oracle's."""
+    rng = np.random.default_rng(12)
+    lines = ["X_ERROR(0.9) 20", "M 20", "DETECTOR rec[-1]"]
+    for layer in range(8):
+        for q in range(14):
+            lines.append(f"{rng.choice(['H', 'S'])} {q}")
+            if rng.random() < 0.5:
+                lines.append(f"T {q}")
+        for q in range(layer % 2, 13, 2):
+            lines.append(f"CX {q} {q + 1}")
+    lines.append("M " + " ".join(str(q) for q in range(14)))
+    lines.append("OBSERVABLE_INCLUDE(0) rec[-1] rec[-2]")
+    text = "\n".join(lines) + "\n"
+    c = hip.Circuit(text)
+    s = c.compile([1], tier="wide")
+    assert s.tier == "wide" and s.max_active_qubits >= 10
+    shots = 400
+    r = s.sample(shots, seed=3, bit_packed=True)
+    o = O.sample_per_shot(s.blob, shots, 3, bit_packed=True)
+    print("accepted", r.accepted, "of", shots, "max_k", s.max_active_qubits, flush=True)
+    assert (r.accepted, r.discarded) == (int(o["counts"][2]), int(o["counts"][1]))
+    assert 0 < r.accepted < shots // 4
+    for key in ("measurements", "detectors", "observables"):
+        assert np.array_equal(getattr(r, key), o[key]), key
+    assert r.observable_ones.tolist() == o["observable_ones"].tolist()

@@ -28,6 +28,10 @@ struct WideCtl {
   double invnorm;
   int32_t branch;
   int32_t err;
+  // a postselected detector fired: the shot is discarded and every later
+  // kernel skips it (src/sampler/batch/postselect.rs drops it from the block)
+  int32_t dead;
+  int32_t pad;
 };
 
 constexpr int kWideThreads = 256;
@@ -124,6 +128,7 @@ __global__ __launch_bounds__(kWideThreads) void wide_init_kernel(DevProgram P, D
   ctl[j].err = 0;
   ctl[j].branch = 0;
   ctl[j].invnorm = 1.0;
+  ctl[j].dead = 0;
   const size_t d0 = size_t(1) << P.initial_k;
   for (size_t a = 0; a < d0; ++a) {
     re[j * dim + a] = a == 0 ? T(1) : T(0);
@@ -137,7 +142,7 @@ __global__ __launch_bounds__(kWideThreads) void wide_bits_kernel(DevProgram P, D
                                                                  uint64_t* bits, WideCtl* ctl, int first,
                                                                  int last) {
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nshots) return;
+  if (j >= nshots || ctl[j].dead) return;
   const uint64_t local = batch_start + j;
   WideShot w = wide_shot(bits, Y, j);
   uint64_t brng = ctl[j].brng;
@@ -164,6 +169,10 @@ __global__ __launch_bounds__(kWideThreads) void wide_bits_kernel(DevProgram P, D
         b = wide_eval(P, in, w);
       }
       wset(w.d, uint32_t(in.detector), b);
+      if (b && (in.op & kFlagPostselect)) {
+        ctl[j].dead = 1;
+        break;
+      }
     } else if (op == kOpCoin) {
       bool coin;
       if (in.op & kFlagForced) {  // pinned: the XOR of earlier branches, no draw
@@ -186,9 +195,10 @@ __global__ __launch_bounds__(kWideThreads) void wide_bits_kernel(DevProgram P, D
 
 template <class T>
 __global__ __launch_bounds__(kWideThreads) void wide_rotate_kernel(DevProgram P, WideLayout Y, int instr,
-                                                                   uint64_t* bits, T* re, T* im, size_t dim,
-                                                                   int k) {
+                                                                   uint64_t* bits, const WideCtl* ctl, T* re, T* im,
+                                                                   size_t dim, int k) {
   const uint32_t j = blockIdx.y;
+  if (ctl[j].dead) return;
   const DevInstr in = P.instrs[instr];
   const bool sign = wide_eval(P, in, wide_shot(bits, Y, j));
   T* R = re + size_t(j) * dim + in.comp_off;
@@ -302,9 +312,10 @@ __global__ __launch_bounds__(kWideThreads) void wide_rotate_kernel(DevProgram P,
 
 template <class T>
 __global__ __launch_bounds__(kWideThreads) void wide_promote_kernel(DevProgram P, WideLayout Y, int instr,
-                                                                    uint64_t* bits, T* re, T* im, size_t dim,
-                                                                    int k) {
+                                                                    uint64_t* bits, const WideCtl* ctl, T* re, T* im,
+                                                                    size_t dim, int k) {
   const uint32_t j = blockIdx.y;
+  if (ctl[j].dead) return;
   const DevInstr in = P.instrs[instr];
   const bool sign = wide_eval(P, in, wide_shot(bits, Y, j));
   T* R = re + size_t(j) * dim + in.comp_off;
@@ -340,10 +351,10 @@ __global__ __launch_bounds__(kWideThreads) void wide_promote_kernel(DevProgram P
 // component (src/sampler/batch/runtime.rs:511-550, batch/mod.rs:376-402).
 template <class T>
 __global__ __launch_bounds__(kWideThreads) void wide_fresh_kernel(DevProgram P, WideLayout Y, int instr,
-                                                                  uint32_t nshots, uint64_t* bits, T* re, T* im,
-                                                                  size_t dim) {
+                                                                  uint32_t nshots, uint64_t* bits,
+                                                                  const WideCtl* ctl, T* re, T* im, size_t dim) {
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nshots) return;
+  if (j >= nshots || ctl[j].dead) return;
   const DevInstr in = P.instrs[instr];
   const bool zero = in.op & kFlagZeroState;
   const bool sign = zero ? false : wide_eval(P, in, wide_shot(bits, Y, j));
@@ -358,10 +369,11 @@ __global__ __launch_bounds__(kWideThreads) void wide_fresh_kernel(DevProgram P, 
 // target (x) source of src/sampler/batch/mod.rs:597-680, written into the
 // target's partner buffer: out[(sb << tk) + tb] = t[tb] * s[sb].
 template <class T>
-__global__ __launch_bounds__(kWideThreads) void wide_merge_kernel(DevProgram P, int instr, const T* tre,
-                                                                  const T* tim, const T* sre, const T* sim,
-                                                                  T* ore, T* oim, size_t dim) {
+__global__ __launch_bounds__(kWideThreads) void wide_merge_kernel(DevProgram P, int instr, const WideCtl* ctl,
+                                                                  const T* tre, const T* tim, const T* sre,
+                                                                  const T* sim, T* ore, T* oim, size_t dim) {
   const uint32_t j = blockIdx.y;
+  if (ctl[j].dead) return;
   const DevInstr in = P.instrs[instr];
   const size_t tk = size_t(in.nq), td = size_t(1) << tk, total = size_t(1) << (tk + size_t(in.src_k));
   const T* TR = tre + size_t(j) * dim + in.comp_off;
@@ -414,11 +426,12 @@ __device__ __forceinline__ bool pairs_ok(const DevInstr& in, size_t out_dim, siz
 
 // FP64 partial sums of P(branch = 1) per (tile, shot).
 template <class T>
-__global__ __launch_bounds__(kWideThreads) void wide_prob_kernel(DevProgram P, int instr, const T* re,
-                                                                 const T* im, size_t dim, int k, double* part,
-                                                                 uint32_t max_tiles) {
+__global__ __launch_bounds__(kWideThreads) void wide_prob_kernel(DevProgram P, int instr, const WideCtl* ctl,
+                                                                 const T* re, const T* im, size_t dim, int k,
+                                                                 double* part, uint32_t max_tiles) {
   __shared__ double red[kWideThreads / 64];
   const uint32_t j = blockIdx.y;
+  if (ctl[j].dead) return;
   const DevInstr in = P.instrs[instr];
   const T* R = re + size_t(j) * dim + in.comp_off;
   const T* I = im + size_t(j) * dim + in.comp_off;
@@ -481,6 +494,7 @@ __global__ __launch_bounds__(kWideThreads) void wide_prob_kernel(DevProgram P, i
 // branch symbol, outcome, record (or the expectation of a probe).
 __device__ void wide_decide_shot(const DevProgram& P, const DevLaunch& L, const WideLayout& Y, uint32_t j,
                                  uint64_t local, const DevInstr& in, double pt, uint64_t* bits, WideCtl* ctl) {
+  if (ctl[j].dead) return;
   pt = clamp01(pt);
   if (L.want_probs) L.probs[uint64_t(in.ordinal) * L.shots + local] = pt;
   if (in.op & kFlagExpVal) {
@@ -542,6 +556,7 @@ __global__ __launch_bounds__(kWideThreads) void wide_project_kernel(DevProgram P
                                                                     const T* re, const T* im, T* ore, T* oim,
                                                                     size_t dim, int k) {
   const uint32_t j = blockIdx.y;
+  if (ctl[j].dead) return;
   const DevInstr in = P.instrs[instr];
   const bool branch = ctl[j].branch;
   const double inv = ctl[j].invnorm;
@@ -621,6 +636,7 @@ __global__ __launch_bounds__(kWideThreads) void wide_project_measure_kernel(DevP
                                                                             uint32_t max_tiles) {
   __shared__ double red[kWideThreads / 64];
   const uint32_t j = blockIdx.y;
+  if (ctl[j].dead) return;
   const DevInstr in = P.instrs[instr];
   const DevInstr nx = P.instrs[next];
   const bool branch = ctl[j].branch;
@@ -753,8 +769,9 @@ __device__ __forceinline__ void factor_amp2(const Factors<T>& F, size_t shot, si
 
 // The merged state written out (a chain of merges in one pass).
 template <class T>
-__global__ __launch_bounds__(kWideThreads) void wide_multimerge_kernel(Factors<T> F, T* ore, T* oim, size_t dim,
-                                                                       uint32_t out_off, int width) {
+__global__ __launch_bounds__(kWideThreads) void wide_multimerge_kernel(Factors<T> F, const WideCtl* ctl, T* ore, T* oim,
+                                                                       size_t dim, uint32_t out_off, int width) {
+  if (ctl[blockIdx.y].dead) return;
   const size_t shot = size_t(blockIdx.y) * dim;
   const size_t total = size_t(1) << width;
   T* OR = ore + shot + out_off;
@@ -781,10 +798,11 @@ __global__ __launch_bounds__(kWideThreads) void wide_multimerge_kernel(Factors<T
 // kWideTile entries of one factor and writes (N, Re T, Im T) partials.
 template <class T>
 __global__ __launch_bounds__(kWideThreads) void wide_expect_factored_kernel(DevProgram P, int instr, Factors<T> F,
-                                                                            size_t dim, double* part,
-                                                                            uint32_t max_tiles) {
+                                                                            const WideCtl* ctl, size_t dim,
+                                                                            double* part, uint32_t max_tiles) {
   __shared__ double red[3][kWideThreads / 64];
   const uint32_t j = blockIdx.y;
+  if (ctl[j].dead) return;
   const size_t shot = size_t(j) * dim;
   const DevInstr in = P.instrs[instr];
   int f = 0;
@@ -868,6 +886,7 @@ __global__ __launch_bounds__(kWideThreads) void wide_project_factored_kernel(Dev
                                                                              const WideCtl* ctl, Factors<T> F,
                                                                              T* ore, T* oim, size_t dim, int k) {
   const uint32_t j = blockIdx.y;
+  if (ctl[j].dead) return;
   const size_t shot = size_t(j) * dim;
   const DevInstr in = P.instrs[instr];
   const bool branch = ctl[j].branch;
@@ -1128,15 +1147,16 @@ static hipError_t run_wide(const DevProgram& P, const DevInstr* instrs, const De
       if (op == kOpRotate) {
         size_t work = in.op & kFlagDiagonal ? (size_t(1) << w) : (size_t(1) << w) >> 1;
         dim3 g(unsigned((work + kWideTile - 1) / kWideTile), n);
-        hipLaunchKernelGGL(wide_rotate_kernel<T>, g, dim3(kWideThreads), 0, stream, P, Y, i, ws.bits, R, I, dim, w);
+        hipLaunchKernelGGL(wide_rotate_kernel<T>, g, dim3(kWideThreads), 0, stream, P, Y, i, ws.bits, ws.ctl, R, I,
+                           dim, w);
         WCK(hipGetLastError());
       } else if (op == kOpPromote) {
         if (in.op & kFlagFresh) {
-          hipLaunchKernelGGL(wide_fresh_kernel<T>, sg, sb, 0, stream, P, Y, i, n, ws.bits, R, I, dim);
+          hipLaunchKernelGGL(wide_fresh_kernel<T>, sg, sb, 0, stream, P, Y, i, n, ws.bits, ws.ctl, R, I, dim);
         } else {
           dim3 g(unsigned(((size_t(1) << w) + kWideTile - 1) / kWideTile), n);
-          hipLaunchKernelGGL(wide_promote_kernel<T>, g, dim3(kWideThreads), 0, stream, P, Y, i, ws.bits, R, I, dim,
-                             w);
+          hipLaunchKernelGGL(wide_promote_kernel<T>, g, dim3(kWideThreads), 0, stream, P, Y, i, ws.bits, ws.ctl, R,
+                             I, dim, w);
           ++k;
         }
         WCK(hipGetLastError());
@@ -1172,7 +1192,7 @@ static hipError_t run_wide(const DevProgram& P, const DevInstr* instrs, const De
           for (int f = 0; f <= F.n; ++f) F.tile0[f] = ft.tile0[f];
           if (uint32_t(3 * ft.tile0[F.n]) > Y.max_tiles) return hipErrorInvalidValue;
           hipLaunchKernelGGL(wide_expect_factored_kernel<T>, dim3(unsigned(ft.tile0[F.n]), n), dim3(kWideThreads), 0,
-                             stream, P, e2, F, dim, ws.part, Y.max_tiles);
+                             stream, P, e2, F, ws.ctl, dim, ws.part, Y.max_tiles);
           WCK(hipGetLastError());
           hipLaunchKernelGGL(wide_decide_factored_kernel, sg, sb, 0, stream, P, L, Y, start, n, e2, ws.bits, ws.ctl,
                              ws.part, ft, Y.max_tiles);
@@ -1191,13 +1211,13 @@ static hipError_t run_wide(const DevProgram& P, const DevInstr* instrs, const De
           const int sbuf = cur[size_t(in.src_comp)];
           size_t total = size_t(1) << W;
           dim3 g(unsigned((total + kWideTile - 1) / kWideTile), n);
-          hipLaunchKernelGGL(wide_merge_kernel<T>, g, dim3(kWideThreads), 0, stream, P, i, R, I, RE(sbuf), IM(sbuf),
-                             RE(cb ^ 1), IM(cb ^ 1), dim);
+          hipLaunchKernelGGL(wide_merge_kernel<T>, g, dim3(kWideThreads), 0, stream, P, i, ws.ctl, R, I, RE(sbuf),
+                             IM(sbuf), RE(cb ^ 1), IM(cb ^ 1), dim);
         } else {
           size_t total = size_t(1) << W;
           dim3 g(unsigned((total + kWideTile - 1) / kWideTile), n);
-          hipLaunchKernelGGL(wide_multimerge_kernel<T>, g, dim3(kWideThreads), 0, stream, F, RE(cb ^ 1), IM(cb ^ 1),
-                             dim, in.comp_off, W);
+          hipLaunchKernelGGL(wide_multimerge_kernel<T>, g, dim3(kWideThreads), 0, stream, F, ws.ctl, RE(cb ^ 1),
+                             IM(cb ^ 1), dim, in.comp_off, W);
         }
         WCK(hipGetLastError());
         cur[size_t(in.comp)] ^= 1;
@@ -1211,8 +1231,8 @@ static hipError_t run_wide(const DevProgram& P, const DevInstr* instrs, const De
           tiles = fused_tiles;
           fused_tiles = 0;
         } else {
-          hipLaunchKernelGGL(wide_prob_kernel<T>, g, dim3(kWideThreads), 0, stream, P, i, R, I, dim, w, ws.part,
-                             Y.max_tiles);
+          hipLaunchKernelGGL(wide_prob_kernel<T>, g, dim3(kWideThreads), 0, stream, P, i, ws.ctl, R, I, dim, w,
+                             ws.part, Y.max_tiles);
           WCK(hipGetLastError());
         }
         hipLaunchKernelGGL(wide_decide_kernel, sg, sb, 0, stream, P, L, Y, start, n, i, ws.bits, ws.ctl, ws.part,
