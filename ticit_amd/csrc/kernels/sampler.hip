@@ -43,8 +43,8 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int DIM = 1 << KMAX;
   constexpr int SDIM = KMAX > 0 ? DIM / 2 : 0;
-  // the collapse's output half: registers up to two amplitudes, LDS above
-  constexpr bool kRegScratch = KMAX <= 2;
+  // the collapse's output half: registers up to four amplitudes, LDS above
+  constexpr bool kRegScratch = KMAX <= 3;
   constexpr int SDIM_LDS = kRegScratch ? 0 : SDIM;
   const int lane = threadIdx.x;
   const uint64_t local = uint64_t(blockIdx.x) * 64 + lane;
@@ -399,7 +399,7 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
 
 static size_t lane_amp_bytes(int kmax) {
   size_t dim = size_t(1) << kmax;
-  size_t sdim = kmax > 2 ? dim / 2 : 0;  // widths up to 2 collapse in registers
+  size_t sdim = kmax > 3 ? dim / 2 : 0;  // widths up to 3 collapse in registers
   return (dim + sdim) * 64 * 16;
 }
 
