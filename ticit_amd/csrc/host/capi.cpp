@@ -190,10 +190,15 @@ ticit_hip_status create_from_program(Program prog, const ticit_hip_options* opti
     return t.max_local_k <= group_max_local_k(tps) &&
            group_tier_lds_bytes(t.state_dim, h->group, tps) <= size_t(lds_limit);
   };
-  // default launch shape: one shot per 256-thread workgroup. Width-13
-  // states fit a 512-thread shot's LDS, but run slower there than in the
-  // wide tier (one shot per CU), so they go wide unless asked for.
-  int tps = h->opt.group_threads ? int(h->opt.group_threads) : 256;
+  // Default launch shape: one shot per workgroup, its threads sized by the
+  // width as the reference's one-shot-per-block kernels size their tiles
+  // (128, 1024 and 4096 amplitudes up to widths 7, 10 and 12): 64 threads
+  // up to width 7, 128 up to 10, 256 above. Width-13 states fit a
+  // 512-thread shot's LDS, but run slower there than in the wide tier (one
+  // shot per CU), so they go wide unless asked for. Packing several shots
+  // into a wave (8-32 threads per shot) is left to the batch autotuner.
+  const int plain_tps = t.max_local_k <= 7 ? 64 : (t.max_local_k <= 10 ? 128 : 256);
+  int tps = h->opt.group_threads ? int(h->opt.group_threads) : plain_tps;
   if (tps != 8 && tps != 16 && tps != 32 && tps != 64 && tps != 128 && tps != 256 && tps != 512) {
     delete h;
     return fail(TICIT_HIP_INVALID_INPUT, "group_threads must be 8, 16, 32, 64, 128, 256 or 512");
@@ -675,7 +680,7 @@ ticit_hip_status ticit_hip_program_info_get(const ticit_hip_program* h, ticit_hi
                              h->row_bytes(uint64_t(h->tab.nobs), true)});
     lane_tier_occupancy(h->tab.max_k, h->lane, row, &out->lds_bytes_per_block, &out->blocks_per_cu);
   } else if (h->tier == kTierGroup) {
-    out->lds_bytes_per_block = uint32_t(group_tier_lds_bytes(h->tab.state_dim, h->group, 256));
+    out->lds_bytes_per_block = uint32_t(group_tier_lds_bytes(h->tab.state_dim, h->group, h->group_tps));
   }
   out->planner_p_components = h->planner_p;
   out->state_dim = h->tab.state_dim;
