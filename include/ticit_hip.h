@@ -44,7 +44,8 @@ typedef struct ticit_hip_options {
   uint32_t observable;   /* observable counted as a logical error */
   uint32_t precision;    /* 0: FP64 amplitudes; 1: FP32 amplitudes, FP64 reductions */
   uint32_t planner_mode; /* 0: heuristic tier choice; 1: learned planner */
-  uint32_t tier;         /* 0: automatic; 1: lane tier; 2: group tier; 3: wide tier */
+  uint32_t tier;         /* 0: automatic (lane tier when it fits, group tier up to active
+                            width 10, wide tier above); 1: lane; 2: group; 3: wide */
   uint32_t wide_batch;   /* cap on wide-tier shots in flight (0 = memory budget) */
   uint32_t components;   /* 0: as planned (use_active_components, src/planner.rs:1482-1484);
                             1: dense active state; 2: tensor components

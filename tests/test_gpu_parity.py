@@ -202,3 +202,26 @@ def test_group_tier_holds_a_width_13_state(hip):
     oi = O.sample_injected(c.blob, shots, None, uni, None, want_probs=True)
     assert np.array_equal(ri.measurements, oi["measurements"])
     assert np.max(np.abs(ri.probabilities - oi["probs"])) <= PROB_TOL
+
+
+def chain_circuit(n: int) -> str:
+    """H T on every qubit, a CX chain, H T again, measure all: a dense active
+    state of width n."""
+    lines = [f"H {q}\nT {q}" for q in range(n)] + [f"CX {q} {q + 1}" for q in range(n - 1)]
+    lines += [f"H {q}\nT {q}" for q in range(n)] + ["M " + " ".join(map(str, range(n)))]
+    return "\n".join(lines) + "\n"
+
+
+@pytest.mark.parametrize("n,tier", [(10, "group"), (11, "wide")])
+def test_automatic_tier_keeps_the_group_tier_up_to_width_10(hip, n, tier):
+    """The automatic choice runs dense widths up to 10 in the group tier and
+    wider ones in the wide tier. An explicit group request still runs width
+    11, at 256 threads per shot; every tier's rows equal the oracle's."""
+    c = hip.Circuit(chain_circuit(n))
+    s = c.compile(components="dense")
+    assert s.max_local_k == n and s.tier == tier
+    shots = 300
+    o = O.sample_per_shot(c.blob, shots, 11)
+    for smp in (s, c.compile(components="dense", tier="group"), c.compile(components="dense", tier="wide")):
+        r = smp.sample(shots, seed=11, bit_packed=True)
+        assert np.array_equal(r.measurements, o["measurements"]), smp.tier

@@ -37,6 +37,9 @@ ticit_hip_status from_hip(hipError_t e, const char* what) {
 
 namespace {
 
+// Widest active state the automatic tier choice gives the group tier.
+constexpr int kGroupAutoMaxK = 10;
+
 // SamplerOptions::normalize_syndromes and expected_* (src/sampler/prepared.rs:111-116).
 struct ReferenceRequest {
   ExpectedBits bits;
@@ -174,7 +177,7 @@ ticit_hip_status create_from_program(Program prog, const ticit_hip_options* opti
   h->allocations.push_back(h->d_error);
 
   // Tier choice: per-lane resident state up to k = 5, one shot per
-  // workgroup while the FP64 state fits the LDS, otherwise unsupported.
+  // workgroup up to width 10, the HBM-resident wide tier above.
   auto words = [](int bits) { return uint32_t((bits + 63) / 64); };
   h->lane = LaneLayout{words(t.npartials), words(t.nslots), words(t.nrec), words(t.ndet), 0,
                        uint32_t(t.npartials), uint32_t(t.nslots), uint32_t(t.nrec), uint32_t(t.ndet),
@@ -193,10 +196,16 @@ ticit_hip_status create_from_program(Program prog, const ticit_hip_options* opti
   // Default launch shape: one shot per workgroup, its threads sized by the
   // width as the reference's one-shot-per-block kernels size their tiles
   // (128, 1024 and 4096 amplitudes up to widths 7, 10 and 12): 64 threads
-  // up to width 7, 128 up to 10, 256 above. Width-13 states fit a
-  // 512-thread shot's LDS, but run slower there than in the wide tier (one
-  // shot per CU), so they go wide unless asked for. Packing several shots
-  // into a wave (8-32 threads per shot) is left to the batch autotuner.
+  // up to width 7, 128 up to 10, 256 above. Packing several shots into a
+  // wave (8-32 threads per shot) is left to the batch autotuner.
+  //
+  // The automatic choice stops the group tier at width 10. Widths 11 to 13
+  // fit the LDS of a 256- or 512-thread shot, but a 32-128 KiB region
+  // leaves one or two workgroups per CU, and the amplitude passes then wait
+  // on LDS latency: on 30 random Clifford+T programs of widths 11 and 12,
+  // 1e4-1e6 shots each, the wide tier took 0.39-1.14x the time of the best
+  // group shape (median 0.67x; faster on 28). At width 10 and below the two
+  // are within 2x either way and the group tier keeps the rows in LDS.
   const int plain_tps = t.max_local_k <= 7 ? 64 : (t.max_local_k <= 10 ? 128 : 256);
   int tps = h->opt.group_threads ? int(h->opt.group_threads) : plain_tps;
   if (tps != 8 && tps != 16 && tps != 32 && tps != 64 && tps != 128 && tps != 256 && tps != 512) {
@@ -207,7 +216,7 @@ ticit_hip_status create_from_program(Program prog, const ticit_hip_options* opti
   const bool group_ok = group_fits(tps);
   uint32_t want = h->opt.tier;
   if (want == 0 && h->opt.precision == 1) want = kTierWide;  // the FP32 path lives in the wide tier
-  if (want == 0) want = lane_ok ? kTierLane : (group_ok ? kTierGroup : kTierWide);
+  if (want == 0) want = lane_ok ? kTierLane : (group_ok && t.max_local_k <= kGroupAutoMaxK ? kTierGroup : kTierWide);
   if (want == kTierLane && !lane_ok) {
     delete h;
     return fail(TICIT_HIP_UNSUPPORTED, "lane tier needs a dense max_k <= 5 and its tables in LDS");
