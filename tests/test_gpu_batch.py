@@ -85,3 +85,32 @@ def test_sample_batch_host_results_match_the_oracle(hip):
         o = O.sample_per_shot(s.blob, n, sd)
         assert np.array_equal(r.measurements, o["measurements"])
         assert r.shots == n
+
+
+def test_small_width_11_program_joins_a_group_bucket(hip):
+    """A dense width-11 program runs wide on its own; in a batch it joins a
+    group bucket when it has at most 5e4 shots and stays wide above. Both
+    ways its rows equal the single-program run's and the oracle's."""
+    n = 11
+    lines = [f"H {q}\nT {q}" for q in range(n)] + [f"CX {q} {q + 1}" for q in range(n - 1)]
+    lines += [f"H {q}\nT {q}" for q in range(n)] + ["M " + " ".join(map(str, range(n)))]
+    c = hip.Circuit("\n".join(lines) + "\n")
+    s = c.compile(components="dense")
+    assert s.tier == "wide" and s.max_local_k == n
+    for shots, wide in ((3000, 0), (50001, 1)):
+        out = buffers(s, shots)
+        b = hip.Batch([s], [shots], [21], [out], autotune=True, autotune_shots=256)
+        b.launch(0)
+        b.wait()
+        info = b.info()
+        print(shots, info, flush=True)
+        assert info["wide_items"] == wide and len(info["buckets"]) == 1 - wide
+        ref = buffers(s, shots)
+        s.sample_device(shots, 21, ref)
+        torch.cuda.synchronize()
+        for a, r in zip(out, ref):
+            assert torch.equal(a, r)
+        b.close()
+    o = O.sample_per_shot(c.blob, 3000, 21)
+    r = s.sample(3000, seed=21, bit_packed=True)
+    assert np.array_equal(r.measurements, o["measurements"])
