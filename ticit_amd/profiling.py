@@ -1,6 +1,6 @@
 """Probe workloads for rocprofv3 runs.
 
-``python -m ticit_amd.profiling [c1|c0|c3|c2]`` runs one device copy of a
+``python -m ticit_amd.profiling [c1|c0|c3|c2|p]`` runs one device copy of a
 known size (256 MiB read, 256 MiB written) and then the workload's sampling
 launches into device memory:
 
@@ -8,7 +8,9 @@ launches into device memory:
   * c0: random Clifford+T n=16, 12 T (group tier), two launches of 1e5 shots;
   * c3: dense stress n=30, 24 T (wide tier, tensor components, FP64), one
     warm launch and one measured launch of 4096 shots;
-  * c2: the 256-circuit batch as one batched graph launch (learned layouts).
+  * c2: the 256-circuit batch as one batched graph launch (learned layouts);
+  * p: the d=3 cultivation fixture, every detector postselected, normalized,
+    two launches of 1e6 shots with the survivors compacted on the device.
 
 Profiled with ``rocprofv3 --pmc FETCH_SIZE`` and ``--pmc WRITE_SIZE`` (one
 counter per run), the copy calibrates each counter and the sampling
@@ -60,6 +62,22 @@ def main(argv) -> int:
         bt.launch(0)
         bt.wait()
         torch.cuda.synchronize()
+    elif which == "p":
+        # the cultivation fixture, every detector postselected, normalized;
+        # survivors compacted on the device
+        import os
+        path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tests", "golden",
+                            "circuits", "msc_d3_inject_cultivate_p1e-3.stim")
+        with open(path) as f:
+            c = ticit_amd.Circuit(f.read())
+        s = c.compile([1] * c.num_detectors, normalize_syndromes=True)
+        shots = 1_000_000
+        out = _buffers(torch, s, shots)
+        acc = torch.zeros(1, dtype=torch.int64, device="cuda")
+        for seed in (1, 2):
+            s.sample_device(shots, seed, out, accepted=acc)
+        torch.cuda.synchronize()
+        print(f"accepted {int(acc.item())} of {shots}", flush=True)
     else:
         print(f"unknown probe {which!r}", file=sys.stderr)
         return 2
