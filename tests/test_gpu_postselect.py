@@ -207,3 +207,33 @@ def test_wide_tier_skips_discarded_shots(hip):
     for key in ("measurements", "detectors", "observables"):
         assert np.array_equal(getattr(r, key), o[key]), key
     assert r.observable_ones.tolist() == o["observable_ones"].tolist()
+
+
+def test_batch_replays_match_eager_runs_between_them(hip):
+    """Four replays of one captured batch (lane, group and wide items, three
+    of them postselected with their compaction inside the graph), with eager
+    sampling and synchronization between the replays: every replay's
+    survivors, rows and counts equal an eager run at the same seeds."""
+    import torch
+    samplers = [hip.Circuit(CASES[n][0]).compile(postselection_mask=CASES[n][1], tier=t)
+                for n, t in (("noisy_discard", "lane"), ("surface_d3", "group"), ("c1_first_round", "wide"))]
+    samplers.append(hip.Circuit(CASES["surface_d3"][0]).compile())
+    shots = [3000, 2000, 700, 500]
+    seeds = [3, 4, 5, 6]
+    outs = [device_buffers(s, n) for s, n in zip(samplers, shots)]
+    bt = hip.Batch(samplers, shots, seeds, outs, autotune=True, autotune_shots=256)
+    for off in (0, 5, 5, 12):
+        for b in outs:
+            b[3].zero_()
+        bt.launch(off)
+        bt.wait()
+        torch.cuda.synchronize()
+        print("replay", off, flush=True)
+        for s, n, sd, b in zip(samplers, shots, seeds, outs):
+            ref = s.sample(n, seed=sd + off, bit_packed=True)
+            acc = int(b[5].item()) if s.has_postselection else n
+            assert acc == ref.accepted, (off, s.tier)
+            assert np.array_equal(b[0][:acc].cpu().numpy(), ref.measurements), (off, s.tier)
+            assert np.array_equal(b[1][:acc].cpu().numpy(), ref.detectors), (off, s.tier)
+            assert b[3].cpu().numpy()[:s.num_observables].tolist() == ref.observable_ones.tolist(), (off, s.tier)
+    bt.close()
