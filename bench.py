@@ -211,15 +211,15 @@ def other_configs(ticit_amd, w, torch, np, stream, use_cpu=True) -> dict:
         for s, n, sd, b in zip(samplers, shots, seeds, bufs):
             s.sample_device(n, sd + 100 * (r + 2), b, stream=st)
 
-    tp = _time(torch, stream, plain)
+    tp = _time(torch, stream, plain, reps=3)
     bh = ticit_amd.Batch(samplers, shots, seeds, bufs, autotune=True, autotune_shots=4096)
-    tbh = _time(torch, stream, lambda r: bh.launch(100 * (r + 2), st))
+    tbh = _time(torch, stream, lambda r: bh.launch(100 * (r + 2), st), reps=3)
     bh.close()
     learned = [ticit_amd.CompiledSampler(s.blob, planner="learned") for s in samplers]
     t0 = time.perf_counter()
     bt = ticit_amd.Batch(learned, shots, seeds, bufs, autotune=True, autotune_shots=4096)
     setup = time.perf_counter() - t0
-    tb = _time(torch, stream, lambda r: bt.launch(100 * (r + 2), st))
+    tb = _time(torch, stream, lambda r: bt.launch(100 * (r + 2), st), reps=3)
     info = bt.info()
     total = sum(shots)
     out["C2"] = {"workload": "256 random Clifford+T circuits, 8-40 qubits, 4-20 T, DEPOLARIZE1 5e-4, "
@@ -229,6 +229,7 @@ def other_configs(ticit_amd, w, torch, np, stream, use_cpu=True) -> dict:
                  "batched_learned_shots_per_s": total / tb,
                  "batched_learned_over_plain": tp / tb, "batched_heuristic_over_plain": tp / tbh,
                  "plain_s": tp, "batched_learned_s": tb, "batch_setup_s": setup,
+                 "timing": "each leg: one warm run, then the best of 3 timed runs",
                  "components_chosen": {"heuristic": int(sum(s.components for s in samplers)),
                                        "learned": int(sum(s.components for s in learned))},
                  "tiers": {"lane": info["lane_items"], "wide": info["wide_items"],
@@ -300,11 +301,11 @@ def other_configs(ticit_amd, w, torch, np, stream, use_cpu=True) -> dict:
         return lambda r: [s.sample_device(n4, 7 + i + 1000 * (r + 2), b, stream=st)
                           for i, (s, b) in enumerate(zip(samps, bufs4))]
 
-    th = _time(torch, stream, run_each(heur))
-    tl = _time(torch, stream, run_each(lear))
+    th = _time(torch, stream, run_each(heur), reps=3)
+    tl = _time(torch, stream, run_each(lear), reps=3)
     b4 = ticit_amd.Batch(lear, [n4] * len(lear), [7 + i for i in range(len(lear))], bufs4, autotune=True,
                          autotune_shots=4096)
-    tbl = _time(torch, stream, lambda r: b4.launch(1000 * (r + 2), st))
+    tbl = _time(torch, stream, lambda r: b4.launch(1000 * (r + 2), st), reps=3)
     b4.close()
     ev = PL.evaluate(m["ho_p_components"], m["ho_heuristic"], m["ho_td"], m["ho_tc"], float(m["margin"]))
     total4 = n4 * len(blobs)
