@@ -32,6 +32,7 @@ struct ticit_hip_batch {
   struct Bucket {
     std::vector<uint32_t> items;
     uint32_t size_class = 0;    // power-of-two class of the per-shot LDS bytes
+    int width_class = 0;        // 0: widths up to 10; 1: 11-12 (256+ threads); 2: 13 (512)
     uint32_t region = 0;        // LDS bytes per shot: the largest of the bucket
     int max_local_k = 0;
     std::vector<int> candidates;
@@ -351,12 +352,20 @@ ticit_hip_status ticit_hip_batch_create(const ticit_hip_batch_item* items, size_
     } else {
       const uint32_t region = uint32_t(group_region_bytes(t.state_dim, h->group, 64));
       const uint32_t cls = round_region(region);
+      // Buckets also split by the threads per shot a width needs, so every
+      // item of a bucket fits the shape its widest item forces: an item
+      // fits at the shape it was created for (or, moved from the wide
+      // tier, at 256), and a narrower shape only shrinks its region.
+      const int wcls = t.max_local_k > group_max_local_k(256) ? 2 : (t.max_local_k > group_max_local_k(128) ? 1 : 0);
       auto it = std::find_if(b->buckets.begin(), b->buckets.end(),
-                             [&](const ticit_hip_batch::Bucket& k) { return k.size_class == cls; });
+                             [&](const ticit_hip_batch::Bucket& k) {
+                               return k.size_class == cls && k.width_class == wcls;
+                             });
       if (it == b->buckets.end()) {
         b->buckets.push_back(ticit_hip_batch::Bucket{});
         it = b->buckets.end() - 1;
         it->size_class = cls;
+        it->width_class = wcls;
       }
       it->region = std::max(it->region, region);
       it->items.push_back(uint32_t(i));
