@@ -114,3 +114,33 @@ def test_small_width_11_program_joins_a_group_bucket(hip):
     o = O.sample_per_shot(c.blob, 3000, 21)
     r = s.sample(3000, seed=21, bit_packed=True)
     assert np.array_equal(r.measurements, o["measurements"])
+
+
+def test_moved_and_explicit_wide_group_shapes_share_a_batch(hip):
+    """A width-12 program moved from the wide tier into the group buckets, a
+    width-13 program asked to run at 512 threads per shot and a width-10
+    program batched together: every item's rows equal its own run's."""
+    n = 12
+    lines = [f"H {q}\nT {q}" for q in range(n)] + [f"CX {q} {q + 1}" for q in range(n - 1)]
+    lines += [f"H {q}\nT {q}" for q in range(n)] + ["M " + " ".join(map(str, range(n)))]
+    moved = hip.Circuit("\n".join(lines) + "\n").compile(components="dense")
+    text13 = open(__file__.replace("test_gpu_batch.py", "golden/circuits/wide_compact.stim")).read()
+    wide13 = hip.Circuit(text13).compile(tier="group", group_threads=512, components="dense")
+    narrow = hip.Circuit(W.config0_circuit()).compile()
+    assert moved.tier == "wide" and moved.max_local_k == 12
+    assert wide13.tier == "group" and wide13.max_local_k == 13 and narrow.tier == "group"
+    samplers, shots, seeds = [moved, wide13, narrow], [2000, 300, 1500], [31, 32, 33]
+    outs = [buffers(s, k) for s, k in zip(samplers, shots)]
+    b = hip.Batch(samplers, shots, seeds, outs, autotune=True, autotune_shots=256)
+    b.launch(0)
+    b.wait()
+    info = b.info()
+    print(info, flush=True)
+    assert info["wide_items"] == 0
+    for s, k, sd, out in zip(samplers, shots, seeds, outs):
+        ref = buffers(s, k)
+        s.sample_device(k, sd, ref)
+        torch.cuda.synchronize()
+        for a, r in zip(out, ref):
+            assert torch.equal(a, r), s.max_local_k
+    b.close()
