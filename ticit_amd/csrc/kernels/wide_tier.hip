@@ -838,8 +838,11 @@ __global__ __launch_bounds__(kWideThreads) void wide_expect_factored_kernel(DevP
   if (threadIdx.x < 3) {
     double t = 0.0;
     for (int w = 0; w < kWideThreads / 64; ++w) t += red[threadIdx.x][w];
-    part[size_t(j) * max_tiles + 3 * blockIdx.x + threadIdx.x] = t;
+    // tile-major, shots fastest: the per-shot decision reads these
+    // coalesced; 3 * tiles * gridDim.y <= max_tiles * batch (checked at launch)
+    part[(size_t(3) * blockIdx.x + threadIdx.x) * gridDim.y + j] = t;
   }
+  (void)max_tiles;
 }
 
 // Per shot: the factors' partials in fixed order, P(true) as above, then
@@ -853,14 +856,17 @@ __global__ __launch_bounds__(kWideThreads) void wide_decide_factored_kernel(DevP
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nshots) return;
   const DevInstr in = P.instrs[instr];
-  const double* pj = part + size_t(j) * max_tiles;
+  (void)max_tiles;
+  // partial (tile t, component c) of shot j sits at (3t + c) * nshots + j
+  const double* pj = part + j;
+  const size_t step = nshots;
   double n = 1.0, tr = 1.0, ti = 0.0;
   for (int f = 0; f < ft.n; ++f) {
     double nf = 0.0, rf = 0.0, jf = 0.0;
     for (int t = ft.tile0[f]; t < ft.tile0[f + 1]; ++t) {
-      nf += pj[3 * t];
-      rf += pj[3 * t + 1];
-      jf += pj[3 * t + 2];
+      nf += pj[(size_t(3) * t) * step];
+      rf += pj[(size_t(3) * t + 1) * step];
+      jf += pj[(size_t(3) * t + 2) * step];
     }
     n *= nf;
     const double r2 = tr * rf - ti * jf, i2 = tr * jf + ti * rf;
