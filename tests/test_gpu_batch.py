@@ -116,20 +116,26 @@ def test_small_width_11_program_joins_a_group_bucket(hip):
     assert np.array_equal(r.measurements, o["measurements"])
 
 
-def test_moved_and_explicit_wide_group_shapes_share_a_batch(hip):
-    """A width-12 program moved from the wide tier into the group buckets, a
-    width-13 program asked to run at 512 threads per shot and a width-10
-    program batched together: every item's rows equal its own run's."""
-    n = 12
+def dense_chain(hip, n):
     lines = [f"H {q}\nT {q}" for q in range(n)] + [f"CX {q} {q + 1}" for q in range(n - 1)]
     lines += [f"H {q}\nT {q}" for q in range(n)] + ["M " + " ".join(map(str, range(n)))]
-    moved = hip.Circuit("\n".join(lines) + "\n").compile(components="dense")
+    return hip.Circuit("\n".join(lines) + "\n").compile(components="dense")
+
+
+def test_moved_and_explicit_wide_group_shapes_share_a_batch(hip):
+    """A width-12 and a width-11 program moved from the wide tier into the
+    group buckets, a width-13 program asked to run at 512 threads per shot
+    and a width-10 program batched together: every item's rows equal its
+    own run's, and the buckets split by width class (up to 10, 11-12, 13)."""
+    moved = dense_chain(hip, 12)
+    moved11 = dense_chain(hip, 11)
     text13 = open(__file__.replace("test_gpu_batch.py", "golden/circuits/wide_compact.stim")).read()
     wide13 = hip.Circuit(text13).compile(tier="group", group_threads=512, components="dense")
     narrow = hip.Circuit(W.config0_circuit()).compile()
     assert moved.tier == "wide" and moved.max_local_k == 12
+    assert moved11.tier == "wide" and moved11.max_local_k == 11
     assert wide13.tier == "group" and wide13.max_local_k == 13 and narrow.tier == "group"
-    samplers, shots, seeds = [moved, wide13, narrow], [2000, 300, 1500], [31, 32, 33]
+    samplers, shots, seeds = [moved, wide13, narrow, moved11], [2000, 300, 1500, 900], [31, 32, 33, 34]
     outs = [buffers(s, k) for s, k in zip(samplers, shots)]
     b = hip.Batch(samplers, shots, seeds, outs, autotune=True, autotune_shots=256)
     b.launch(0)
@@ -137,6 +143,9 @@ def test_moved_and_explicit_wide_group_shapes_share_a_batch(hip):
     info = b.info()
     print(info, flush=True)
     assert info["wide_items"] == 0
+    # width classes: C0's width-10 bucket keeps its own shape; the width-11
+    # and width-12 items need at least 128 and 256 threads per shot
+    assert len(info["buckets"]) >= 3, info
     for s, k, sd, out in zip(samplers, shots, seeds, outs):
         ref = buffers(s, k)
         s.sample_device(k, sd, ref)

@@ -228,3 +228,28 @@ def test_device_entry_point_and_batch_apply_the_bits(hip):
     torch.cuda.synchronize()
     for b, o in zip(singles, outs):
         assert all(torch.equal(x, y) for x, y in zip(b, o))
+
+
+# A postselected detector whose noiseless value is 1, ahead of an entangled
+# width-7 block with its own detectors and observables: the reference shot
+# must record every later detector (src/sampler/prepared.rs:355-405 runs it
+# without postselection), on the tiers that stop a discarded shot early.
+POST_FIRST = "X 12\nM 12\nDETECTOR rec[-1]\n" + ENTANGLED
+
+
+@pytest.mark.parametrize("tier,threads", [("group", 0), ("group", 64), ("group", 256), ("wide", 0)])
+def test_reference_shot_runs_past_a_fired_postselected_detector(hip, tier, threads):
+    c = hip.Circuit(POST_FIRST)
+    mask = [1] + [0] * (c.num_detectors - 1)
+    s = c.compile(mask, tier=tier, group_threads=threads, normalize_syndromes=True)
+    want = oracle_reference(c.blob)
+    assert want[0][0] and sum(want[0]) > 1, want
+    ref = s.reference_sample()
+    print(tier, threads, "reference ones", sum(ref.detectors), flush=True)
+    assert (ref.detectors, ref.observables) == want
+    shots = 1500
+    r = s.sample(shots, seed=29, bit_packed=True)
+    O.set_expected(*want)
+    o = O.sample_per_shot(s.blob, shots, 29, bit_packed=True)
+    assert_matches(r, o, shots)
+    assert r.discarded == 0  # the normalized detector never fires

@@ -32,7 +32,7 @@ struct ticit_hip_batch {
   struct Bucket {
     std::vector<uint32_t> items;
     uint32_t size_class = 0;    // power-of-two class of the per-shot LDS bytes
-    int width_class = 0;        // 0: widths up to 10; 1: 11-12 (256+ threads); 2: 13 (512)
+    int width_class = 0;        // 0: widths up to 10; 1: 11-12 (128+ threads); 2: 13 (512)
     uint32_t region = 0;        // LDS bytes per shot: the largest of the bucket
     int max_local_k = 0;
     std::vector<int> candidates;
@@ -356,7 +356,7 @@ ticit_hip_status ticit_hip_batch_create(const ticit_hip_batch_item* items, size_
       // item of a bucket fits the shape its widest item forces: an item
       // fits at the shape it was created for (or, moved from the wide
       // tier, at 256), and a narrower shape only shrinks its region.
-      const int wcls = t.max_local_k > group_max_local_k(256) ? 2 : (t.max_local_k > group_max_local_k(128) ? 1 : 0);
+      const int wcls = t.max_local_k > group_max_local_k(256) ? 2 : (t.max_local_k > group_max_local_k(64) ? 1 : 0);
       auto it = std::find_if(b->buckets.begin(), b->buckets.end(),
                              [&](const ticit_hip_batch::Bucket& k) {
                                return k.size_class == cls && k.width_class == wcls;

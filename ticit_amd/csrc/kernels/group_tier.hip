@@ -238,9 +238,11 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
       // reach its outputs (src/sampler/batch/postselect.rs removes it from
       // the block). With one shot per workgroup `b` is the same in every
       // thread, so the whole group leaves the loop at this step; packed
-      // sub-wave shots run on.
+      // sub-wave shots run on. The noiseless reference shot records every
+      // detector (src/sampler/prepared.rs:355-405 runs it without
+      // postselection).
       if constexpr (TPS >= 64) {
-        if (b && (st.op & kFlagPostselect) && op == kOpDetector) break;
+        if (b && (st.op & kFlagPostselect) && op == kOpDetector && !L.reference) break;
       }
     } else if (op == kOpCoin && (st.op & kFlagExpVal)) {
       if (expv) expv[st.a] = 0.0;  // an unbiased probe reads 0 and draws nothing

@@ -169,7 +169,9 @@ __global__ __launch_bounds__(kWideThreads) void wide_bits_kernel(DevProgram P, D
         b = wide_eval(P, in, w);
       }
       wset(w.d, uint32_t(in.detector), b);
-      if (b && (in.op & kFlagPostselect)) {
+      // the noiseless reference shot records every detector
+      // (src/sampler/prepared.rs:355-405 runs it without postselection)
+      if (b && (in.op & kFlagPostselect) && !L.reference) {
         ctl[j].dead = 1;
         break;
       }
@@ -799,7 +801,7 @@ __global__ __launch_bounds__(kWideThreads) void wide_multimerge_kernel(Factors<T
 template <class T>
 __global__ __launch_bounds__(kWideThreads) void wide_expect_factored_kernel(DevProgram P, int instr, Factors<T> F,
                                                                             const WideCtl* ctl, size_t dim,
-                                                                            double* part, uint32_t max_tiles) {
+                                                                            double* part, uint32_t nshots) {
   __shared__ double red[3][kWideThreads / 64];
   const uint32_t j = blockIdx.y;
   if (ctl[j].dead) return;
@@ -838,11 +840,11 @@ __global__ __launch_bounds__(kWideThreads) void wide_expect_factored_kernel(DevP
   if (threadIdx.x < 3) {
     double t = 0.0;
     for (int w = 0; w < kWideThreads / 64; ++w) t += red[threadIdx.x][w];
-    // tile-major, shots fastest: the per-shot decision reads these
-    // coalesced; 3 * tiles * gridDim.y <= max_tiles * batch (checked at launch)
-    part[(size_t(3) * blockIdx.x + threadIdx.x) * gridDim.y + j] = t;
+    // tile-major, shots fastest (stride nshots, the decision kernel's
+    // stride): the per-shot decision reads these coalesced;
+    // 3 * tiles * nshots <= max_tiles * batch (checked at launch)
+    part[(size_t(3) * blockIdx.x + threadIdx.x) * nshots + j] = t;
   }
-  (void)max_tiles;
 }
 
 // Per shot: the factors' partials in fixed order, P(true) as above, then
@@ -851,12 +853,10 @@ __global__ __launch_bounds__(kWideThreads) void wide_decide_factored_kernel(DevP
                                                                             WideLayout Y, uint64_t batch_start,
                                                                             uint32_t nshots, int instr,
                                                                             uint64_t* bits, WideCtl* ctl,
-                                                                            const double* part, FactorTiles ft,
-                                                                            uint32_t max_tiles) {
+                                                                            const double* part, FactorTiles ft) {
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nshots) return;
   const DevInstr in = P.instrs[instr];
-  (void)max_tiles;
   // partial (tile t, component c) of shot j sits at (3t + c) * nshots + j
   const double* pj = part + j;
   const size_t step = nshots;
@@ -1198,10 +1198,10 @@ static hipError_t run_wide(const DevProgram& P, const DevInstr* instrs, const De
           for (int f = 0; f <= F.n; ++f) F.tile0[f] = ft.tile0[f];
           if (uint32_t(3 * ft.tile0[F.n]) > Y.max_tiles) return hipErrorInvalidValue;
           hipLaunchKernelGGL(wide_expect_factored_kernel<T>, dim3(unsigned(ft.tile0[F.n]), n), dim3(kWideThreads), 0,
-                             stream, P, e2, F, ws.ctl, dim, ws.part, Y.max_tiles);
+                             stream, P, e2, F, ws.ctl, dim, ws.part, n);
           WCK(hipGetLastError());
           hipLaunchKernelGGL(wide_decide_factored_kernel, sg, sb, 0, stream, P, L, Y, start, n, e2, ws.bits, ws.ctl,
-                             ws.part, ft, Y.max_tiles);
+                             ws.part, ft);
           WCK(hipGetLastError());
           size_t out_dim = (size_t(1) << W) >> 1;
           unsigned tiles = unsigned((out_dim + kWideTile - 1) / kWideTile);
