@@ -228,6 +228,26 @@ ticit_hip_status ticit_hip_sample_device_postselected(ticit_hip_program* program
                                                      uint64_t* observable_ones, double* exp_vals,
                                                      uint64_t* accepted, void* stream);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): ticit_hip_sample_device_postselected with the same
+ * arguments and results, but its staging (keep flags, compaction offsets and
+ * the rows of every shot) lives in a 32 MiB device array the library holds
+ * for the whole process, one per GPU, and the call allocates and frees
+ * nothing. A call whose staging exceeds 32 MiB (about 1 byte per shot plus
+ * its packed row bytes) fails with TICIT_HIP_INVALID_INPUT: split the shots
+ * with first_shot. Lane- and group-tier programs only (the wide tier's
+ * per-shot states live in a workspace the library allocates:
+ * TICIT_HIP_UNSUPPORTED). The program's device must be current. Every call
+ * on one GPU shares that array and leaves its contents behind; calls that
+ * overlap on one GPU (other host threads, or streams not ordered with each
+ * other) race on it. */
+ticit_hip_status ticit_hip_sample_device_postselected_noalloc(ticit_hip_program* program, uint64_t shots,
+                                                             uint64_t seed, uint64_t first_shot, int bit_packed,
+                                                             int keep_records, uint8_t* meas, uint8_t* det,
+                                                             uint8_t* obs, uint64_t* observable_ones,
+                                                             double* exp_vals, uint64_t* accepted, void* stream);
+
 /* Injected randomness (host buffers): exo = symbol-major bit-planes
  * [nsymbols][ceil(shots/64)] (bit s of word w = shot 64w+s; only exogenous
  * symbols are read), uniforms = [measure_instructions][shots] branch draws

@@ -91,7 +91,7 @@ EXPORTS = [
     "ticit_hip_reference_sample", "ticit_hip_compile_text",
     "ticit_hip_program_destroy",
     "ticit_hip_program_info_get", "ticit_hip_sample", "ticit_hip_sample_device",
-    "ticit_hip_sample_device_postselected",
+    "ticit_hip_sample_device_postselected", "ticit_hip_sample_device_postselected_noalloc",
     "ticit_hip_sample_injected", "ticit_hip_final_state", "ticit_hip_sample_batch",
     "ticit_hip_result_free", "ticit_hip_plan_features",
     "ticit_hip_batch_create", "ticit_hip_batch_launch", "ticit_hip_batch_wait", "ticit_hip_batch_info_get",
@@ -155,6 +155,8 @@ def load() -> ctypes.CDLL:
                                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                                          ctypes.c_void_p]
+    lib.ticit_hip_sample_device_postselected_noalloc.argtypes = \
+        lib.ticit_hip_sample_device_postselected.argtypes
     lib.ticit_hip_sample_injected.argtypes = [ctypes.c_void_p, ctypes.c_uint64, _u64p, _f64p, _u64p,
                                               ctypes.c_int, ctypes.POINTER(Result), _f64p]
     lib.ticit_hip_final_state.argtypes = [ctypes.c_void_p, ctypes.c_uint64, _u64p, _f64p, _u64p, _f64p, _f64p,
@@ -187,7 +189,7 @@ def load() -> ctypes.CDLL:
     lib.ticit_hip_device_query.restype = ctypes.c_int
     for name in ("ticit_hip_plan_text", "ticit_hip_program_create", "ticit_hip_compile_text",
                  "ticit_hip_program_info_get", "ticit_hip_sample", "ticit_hip_sample_device",
-    "ticit_hip_sample_device_postselected",
+                 "ticit_hip_sample_device_postselected", "ticit_hip_sample_device_postselected_noalloc",
                  "ticit_hip_sample_injected", "ticit_hip_sample_batch", "ticit_hip_final_state",
                  "ticit_hip_plan_features", "ticit_hip_batch_create", "ticit_hip_batch_launch",
                  "ticit_hip_batch_wait", "ticit_hip_batch_info_get", "ticit_hip_planner_features",

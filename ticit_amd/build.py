@@ -31,6 +31,7 @@ SOURCES = [
     "csrc/kernels/group_tier.hip",
     "csrc/kernels/wide_tier.hip",
     "csrc/kernels/compact.hip",
+    "csrc/kernels/static_scratch.hip",
     "csrc/kernels/planner_mlp.hip",
 ]
 

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../kernels/compact.hpp"
+#include "../kernels/static_scratch.hpp"
 #include "component_plan.hpp"
 #include "handle.hpp"
 #include "pinning.hpp"
@@ -503,6 +504,75 @@ ticit_hip_status reference_rows(ticit_hip_program* h, std::vector<uint8_t>& det,
   return st;
 }
 
+// Staging layout of one postselected device launch: keep flags, compaction
+// tile offsets, then the full rows of every shot (each region 256-byte
+// aligned).
+struct PostStaging {
+  size_t keep_b = 0, off_b = 0, rows_b[4] = {0, 0, 0, 0};
+  uint32_t rbs[4] = {0, 0, 0, 0};
+  size_t total() const { return keep_b + off_b + rows_b[0] + rows_b[1] + rows_b[2] + rows_b[3]; }
+};
+
+DevLaunch post_launch_request(const ticit_hip_program* h, uint64_t shots, uint64_t seed, uint64_t first_shot,
+                              int bit_packed, int keep_records, uint64_t* observable_ones, double* exp_vals,
+                              PostStaging& stg) {
+  const HostTables& t = h->tab;
+  DevLaunch L{};
+  L.shots = shots;
+  L.seed = seed;
+  L.first_shot = first_shot;
+  L.bit_packed = bit_packed != 0;
+  L.keep_records = keep_records != 0;
+  L.meas_row = h->row_bytes(uint64_t(t.nrec), bit_packed);
+  L.det_row = h->row_bytes(uint64_t(t.ndet), bit_packed);
+  L.obs_row = h->row_bytes(uint64_t(t.nobs), bit_packed);
+  L.obs_ones = reinterpret_cast<unsigned long long*>(observable_ones);
+  L.error = h->d_error;
+  const bool want_exp = keep_records && t.nexp > 0 && exp_vals;
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t n = size_t(shots);
+  stg.keep_b = up(n);
+  stg.off_b = up(compaction_scratch_words(shots) * 8);
+  const uint32_t rbs[4] = {L.meas_row, L.det_row, L.obs_row, want_exp ? uint32_t(t.nexp) * 8u : 0u};
+  for (int i = 0; i < 4; ++i) {
+    stg.rbs[i] = rbs[i];
+    stg.rows_b[i] = keep_records ? up(n * rbs[i]) : 0;
+  }
+  return L;
+}
+
+// Samples into the staging at `base` (stg.total() bytes of device memory),
+// then moves the survivors' rows to the caller's buffers and their count to
+// *acc, all on `st`.
+ticit_hip_status post_launch_into(ticit_hip_program* h, DevLaunch& L, const PostStaging& stg, uint8_t* base,
+                                  int keep_records, uint8_t* meas, uint8_t* det, uint8_t* obs, double* exp_vals,
+                                  unsigned long long* acc, hipStream_t st) {
+  L.keep = base;
+  uint64_t* tile_off = reinterpret_cast<uint64_t*>(base + stg.keep_b);
+  uint8_t* p = base + stg.keep_b + stg.off_b;
+  uint8_t* stage[4] = {nullptr, nullptr, nullptr, nullptr};
+  if (keep_records)
+    for (int i = 0; i < 4; ++i) {
+      stage[i] = stg.rows_b[i] ? p : nullptr;
+      p += stg.rows_b[i];
+    }
+  L.meas = stage[0];
+  L.det = stage[1];
+  L.obs = stage[2];
+  L.expv = reinterpret_cast<double*>(stage[3]);
+  hipError_t e = launch(h, L, st);
+  if (e != hipSuccess) return from_hip(e, "launching the sampling kernel");
+  CompactRows sets[4] = {};
+  int nsets = 0;
+  uint8_t* dsts[4] = {meas, det, obs, reinterpret_cast<uint8_t*>(exp_vals)};
+  if (keep_records)
+    for (int i = 0; i < 4; ++i)
+      if (stage[i] && stg.rbs[i]) sets[nsets++] = CompactRows{stage[i], dsts[i], stg.rbs[i], 0};
+  e = launch_compaction(L.keep, L.shots, tile_off, acc, sets, nsets, st);
+  if (e != hipSuccess) return from_hip(e, "compacting survivor rows");
+  return TICIT_HIP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -772,28 +842,11 @@ ticit_hip_status ticit_hip_sample_device_postselected(ticit_hip_program* h, uint
     hipError_t e = launch_store_u64(acc, shots, st);
     return e == hipSuccess ? TICIT_HIP_OK : from_hip(e, "writing the survivor count");
   }
-  DevLaunch L{};
-  L.shots = shots;
-  L.seed = seed;
-  L.first_shot = first_shot;
-  L.bit_packed = bit_packed != 0;
-  L.keep_records = keep_records != 0;
-  L.meas_row = h->row_bytes(uint64_t(t.nrec), bit_packed);
-  L.det_row = h->row_bytes(uint64_t(t.ndet), bit_packed);
-  L.obs_row = h->row_bytes(uint64_t(t.nobs), bit_packed);
-  L.obs_ones = reinterpret_cast<unsigned long long*>(observable_ones);
-  L.error = h->d_error;
-  const bool want_exp = keep_records && t.nexp > 0 && exp_vals;
+  PostStaging stg;
+  DevLaunch L = post_launch_request(h, shots, seed, first_shot, bit_packed, keep_records, observable_ones, exp_vals,
+                                    stg);
   std::lock_guard<std::mutex> lock(h->post_mu);
-  // staging: keep flags, tile offsets, then the full rows of every shot
-  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
-  const size_t n = size_t(shots);
-  const size_t keep_b = up(n), off_b = up(compaction_scratch_words(shots) * 8);
-  const uint32_t rbs[4] = {L.meas_row, L.det_row, L.obs_row, want_exp ? uint32_t(t.nexp) * 8u : 0u};
-  size_t rows_b[4] = {0, 0, 0, 0};
-  size_t need = keep_b + off_b;
-  if (keep_records)
-    for (int i = 0; i < 4; ++i) need += rows_b[i] = up(n * rbs[i]);
+  const size_t need = stg.total();
   hipError_t e = hipSuccess;
   if (!h->post_done) {
     e = hipEventCreateWithFlags(&h->post_done, hipEventDisableTiming);
@@ -812,34 +865,57 @@ ticit_hip_status ticit_hip_sample_device_postselected(ticit_hip_program* h, uint
     e = hipStreamWaitEvent(st, h->post_done, 0);
     if (e != hipSuccess) return from_hip(e, "ordering the staging reuse");
   }
-  uint8_t* base = static_cast<uint8_t*>(h->post_buf);
-  L.keep = base;
-  uint64_t* tile_off = reinterpret_cast<uint64_t*>(base + keep_b);
-  uint8_t* p = base + keep_b + off_b;
-  uint8_t* stage[4] = {nullptr, nullptr, nullptr, nullptr};
-  if (keep_records)
-    for (int i = 0; i < 4; ++i) {
-      stage[i] = rows_b[i] ? p : nullptr;
-      p += rows_b[i];
-    }
-  L.meas = stage[0];
-  L.det = stage[1];
-  L.obs = stage[2];
-  L.expv = reinterpret_cast<double*>(stage[3]);
-  e = launch(h, L, st);
-  if (e != hipSuccess) return from_hip(e, "launching the sampling kernel");
-  CompactRows sets[4] = {};
-  int nsets = 0;
-  uint8_t* dsts[4] = {meas, det, obs, reinterpret_cast<uint8_t*>(exp_vals)};
-  if (keep_records)
-    for (int i = 0; i < 4; ++i)
-      if (stage[i] && rbs[i]) sets[nsets++] = CompactRows{stage[i], dsts[i], rbs[i], 0};
-  e = launch_compaction(L.keep, shots, tile_off, acc, sets, nsets, st);
-  if (e != hipSuccess) return from_hip(e, "compacting survivor rows");
+  ticit_hip_status s1 = post_launch_into(h, L, stg, static_cast<uint8_t*>(h->post_buf), keep_records, meas, det, obs,
+                                         exp_vals, acc, st);
+  if (s1 != TICIT_HIP_OK) return s1;
   e = hipEventRecord(h->post_done, st);
   if (e != hipSuccess) return from_hip(e, "recording the staging event");
   h->post_pending = true;
   return TICIT_HIP_OK;
+}
+
+ticit_hip_status ticit_hip_sample_device_postselected_noalloc(ticit_hip_program* h, uint64_t shots, uint64_t seed,
+                                                             uint64_t first_shot, int bit_packed, int keep_records,
+                                                             uint8_t* meas, uint8_t* det, uint8_t* obs,
+                                                             uint64_t* observable_ones, double* exp_vals,
+                                                             uint64_t* accepted, void* stream) {
+  if (!h || !observable_ones || !accepted) return fail(TICIT_HIP_INVALID_INPUT, "null argument");
+  if (keep_records && (!meas || !det || !obs)) return fail(TICIT_HIP_INVALID_INPUT, "null row buffer");
+  if (h->tier == kTierWide)
+    return fail(TICIT_HIP_UNSUPPORTED,
+                "the wide tier keeps its per-shot states in a workspace the library allocates; "
+                "ticit_hip_sample_device_postselected_noalloc serves lane- and group-tier programs");
+  int current = -1;
+  hipError_t e = hipGetDevice(&current);
+  if (e != hipSuccess) return from_hip(e, "reading the current device");
+  if (current != h->device) return fail(TICIT_HIP_INVALID_INPUT, "the program's device must be the current device");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  auto* acc = reinterpret_cast<unsigned long long*>(accepted);
+  const HostTables& t = h->tab;
+  if (t.npost == 0 || shots == 0) {
+    // lane and group launches into caller buffers allocate nothing
+    ticit_hip_status s0 = ticit_hip_sample_device(h, shots, seed, first_shot, bit_packed, keep_records, meas, det,
+                                                  obs, observable_ones, exp_vals, stream);
+    if (s0 != TICIT_HIP_OK) return s0;
+    e = launch_store_u64(acc, shots, st);
+    return e == hipSuccess ? TICIT_HIP_OK : from_hip(e, "writing the survivor count");
+  }
+  PostStaging stg;
+  DevLaunch L = post_launch_request(h, shots, seed, first_shot, bit_packed, keep_records, observable_ones, exp_vals,
+                                    stg);
+  const size_t need = stg.total();
+  if (need > kStaticScratchBytes)
+    return fail(TICIT_HIP_INVALID_INPUT,
+                "this call stages " + std::to_string(need) + " bytes, more than the " +
+                    std::to_string(kStaticScratchBytes) +
+                    "-byte static scratch; split the shots into calls with first_shot");
+  void* base = nullptr;
+  e = static_scratch_address(h->device, &base);
+  if (e != hipSuccess) return from_hip(e, "looking up the static scratch");
+  // every staged byte the compaction reads is written by this launch first
+  // (keep flags and rows of every shot, tile offsets by the count pass), so
+  // nothing left by an earlier call needs clearing
+  return post_launch_into(h, L, stg, static_cast<uint8_t*>(base), keep_records, meas, det, obs, exp_vals, acc, st);
 }
 
 void ticit_hip_result_free(ticit_hip_result* r) {
