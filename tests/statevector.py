@@ -1,8 +1,10 @@
-"""Exact statevector probabilities for small noiseless circuits (test-only).
+"""Exact statevector probabilities for small circuits (test-only).
 
 Used to pin the front end and both samplers against the mathematics: the
 distribution of the final measurement record of a circuit whose only
-measurements are the last ``M`` line is |<b|U|0>|^2 over basis states b.
+measurements are the last ``M`` line is |<b|U|0>|^2 over basis states b;
+``record_distribution`` extends this to mid-circuit measurements, resets and
+a few Pauli-noise sites as an exact mixture over noise and branch outcomes.
 """
 from __future__ import annotations
 
@@ -173,3 +175,181 @@ def with_probes(text: str, rng: np.random.Generator, nprobes: int) -> str:
             prods.append("*".join(f))
         body.insert(at, "EXP_VAL " + " ".join(prods))
     return "\n".join(body + [last]) + "\n"
+
+
+# ---------------------------------------------------------------------------
+# Noisy circuits with mid-circuit measurements: an exact mixture
+# ---------------------------------------------------------------------------
+
+_NOISE1 = {"X_ERROR": "X", "Y_ERROR": "Y", "Z_ERROR": "Z"}
+_ONE_Q_PAULIS = [None, "X", "Y", "Z"]
+
+
+def _pauli_terms(op: str, p: float, targets: list[int]) -> list[list[tuple[float, list[tuple[int, str]]]]]:
+    """Per noise site of one line: the (probability, [(qubit, Pauli)]) terms
+    of its channel (p/3 per Pauli for DEPOLARIZE1, p/15 per non-identity pair
+    for DEPOLARIZE2)."""
+    sites = []
+    if op in _NOISE1:
+        for q in targets:
+            sites.append([(1.0 - p, []), (p, [(q, _NOISE1[op])])])
+    elif op == "DEPOLARIZE1":
+        for q in targets:
+            sites.append([(1.0 - p, [])] + [(p / 3.0, [(q, a)]) for a in "XYZ"])
+    elif op == "DEPOLARIZE2":
+        for i in range(0, len(targets), 2):
+            a, b = targets[i], targets[i + 1]
+            terms = [(1.0 - p, [])]
+            for pa in _ONE_Q_PAULIS:
+                for pb in _ONE_Q_PAULIS:
+                    if pa is None and pb is None:
+                        continue
+                    ops = ([(a, pa)] if pa else []) + ([(b, pb)] if pb else [])
+                    terms.append((p / 15.0, ops))
+            sites.append(terms)
+    else:
+        raise ValueError(op)
+    return sites
+
+
+def record_distribution(text: str, max_noise_sites: int = 3) -> np.ndarray:
+    """Exact probability of every measurement-record value (bit j = record j)
+    of a small circuit with mid-circuit M, R, MR, Pauli-error channels
+    (X/Y/Z_ERROR, DEPOLARIZE1, DEPOLARIZE2) and unitary gates, as a mixture:
+    every noise site splits each branch by its Pauli terms, every
+    measurement by its Born probabilities. Detector and observable lines are
+    ignored. At most `max_noise_sites` noise sites (one per target, or per
+    target pair for DEPOLARIZE2)."""
+    lines = [l.split("#")[0].strip() for l in text.splitlines()]
+    lines = [l for l in lines if l and l != "TICK"]
+    n = 1
+    for l in lines:
+        head = l.split()[0]
+        if head.startswith(("DETECTOR", "OBSERVABLE_INCLUDE")):
+            continue
+        for tok in l.split()[1:]:
+            if tok.isdigit():
+                n = max(n, int(tok) + 1)
+    psi0 = np.zeros(2 ** n, dtype=complex)
+    psi0[0] = 1.0
+    branches = [(1.0, psi0, 0, 0)]  # (weight, normalized state, record bits, record count)
+    nsites = 0
+    idx = np.arange(2 ** n)
+
+    def measure(brs, q, reset):
+        out = []
+        for w, psi, rec, nr in brs:
+            one = ((idx >> q) & 1) == 1
+            p1 = float(np.sum(np.abs(psi[one]) ** 2))
+            for bit, pb in ((0, 1.0 - p1), (1, p1)):
+                if pb <= 1e-15:
+                    continue
+                phi = np.where(one if bit else ~one, psi, 0) / np.sqrt(pb)
+                if reset and bit:
+                    phi = apply1(phi, n, q, GATES1["X"])
+                out.append((w * pb, phi, rec | (bit << nr), nr + 1))
+        return out
+
+    for l in lines:
+        parts = l.split()
+        head = parts[0]
+        if head.startswith(("DETECTOR", "OBSERVABLE_INCLUDE")):
+            continue
+        op, _, arg = head.partition("(")
+        targets = [int(t) for t in parts[1:]]
+        if op in ("M", "MR"):
+            for q in targets:
+                branches = measure(branches, q, op == "MR")
+        elif op == "R":
+            for q in targets:
+                # reset = measure without a record, then flip a 1 back to 0
+                out = []
+                for w, psi, rec, nr in branches:
+                    for w2, phi, _, _ in measure([(w, psi, 0, 0)], q, True):
+                        out.append((w2, phi, rec, nr))
+                branches = out
+        elif op in _NOISE1 or op in ("DEPOLARIZE1", "DEPOLARIZE2"):
+            p = float(arg.rstrip(")"))
+            for terms in _pauli_terms(op, p, targets):
+                nsites += 1
+                if nsites > max_noise_sites:
+                    raise ValueError("too many noise sites for an exact mixture")
+                out = []
+                for w, psi, rec, nr in branches:
+                    for pt, ops in terms:
+                        if pt <= 0.0:
+                            continue
+                        phi = psi
+                        for q, a in ops:
+                            phi = apply1(phi, n, q, _PAULI[a])
+                        out.append((w * pt, phi, rec, nr))
+                branches = out
+        elif op in GATES1:
+            branches = [(w, psi_apply(psi, n, op, targets), rec, nr) for w, psi, rec, nr in branches]
+        elif op in ("CX", "CNOT", "CZ"):
+            branches = [(w, psi_apply(psi, n, op, targets), rec, nr) for w, psi, rec, nr in branches]
+        else:
+            raise ValueError(f"unsupported instruction {op}")
+    nrec = branches[0][3] if branches else 0
+    dist = np.zeros(2 ** nrec)
+    for w, _, rec, nr in branches:
+        assert nr == nrec
+        dist[rec] += w
+    return dist
+
+
+def psi_apply(psi: np.ndarray, n: int, op: str, targets: list[int]) -> np.ndarray:
+    if op in GATES1:
+        for q in targets:
+            psi = apply1(psi, n, q, GATES1[op])
+        return psi
+    for i in range(0, len(targets), 2):
+        psi = (apply_cz if op == "CZ" else apply_cx)(psi, n, targets[i], targets[i + 1])
+    return psi
+
+
+def chi2_pvalue(counts: np.ndarray, probs: np.ndarray, min_expected: float = 5.0) -> float:
+    """Pearson chi-square goodness of fit of `counts` to `probs`; outcomes
+    whose expected count is below `min_expected` are pooled into one bin."""
+    from scipy.stats import chi2
+    counts = np.asarray(counts, dtype=float)
+    exp = np.asarray(probs, dtype=float) * counts.sum()
+    big = exp >= min_expected
+    obs_b, exp_b = list(counts[big]), list(exp[big])
+    if (~big).any() and exp[~big].sum() > 0:
+        obs_b.append(counts[~big].sum())
+        exp_b.append(exp[~big].sum())
+    obs_b, exp_b = np.array(obs_b), np.array(exp_b)
+    stat = float(((obs_b - exp_b) ** 2 / exp_b).sum())
+    dof = max(1, len(obs_b) - 1)
+    return float(chi2.sf(stat, dof))
+
+
+def noisy_midcircuit_circuits(count: int = 6, seed: int = 31) -> list[str]:
+    """Small random circuits: one-qubit Cliffords and T, CX layers, a
+    mid-circuit M (or MR) on some qubits, three noise sites drawn from
+    X_ERROR, Y_ERROR, DEPOLARIZE1 and DEPOLARIZE2 at probabilities that
+    visibly move the distribution, and a final M on every qubit."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for c in range(count):
+        n = 4 + c % 2
+        sites = 0
+        lines = []
+        for layer in range(6):
+            for q in range(n):
+                g = rng.choice(["H", "S", "T", "T_DAG", "H", ""])
+                if g:
+                    lines.append(f"{g} {q}")
+            perm = rng.permutation(n)
+            lines.append("CX " + " ".join(f"{perm[2 * i]} {perm[2 * i + 1]}" for i in range(n // 2)))
+            if layer in (1, 3, 4) and sites < 3:
+                kind = ["X_ERROR(0.2)", "Y_ERROR(0.15)", "DEPOLARIZE1(0.25)", "DEPOLARIZE2(0.3)"][(c + layer) % 4]
+                qs = rng.choice(n, size=2, replace=False)
+                lines.append(f"{kind} {qs[0]} {qs[1]}" if kind.startswith("DEPOLARIZE2") else f"{kind} {qs[0]}")
+                sites += 1
+            if layer == 2:
+                lines.append(("MR " if c % 3 == 2 else "M ") + " ".join(str(q) for q in range(0, n, 2)))
+        lines.append("M " + " ".join(str(q) for q in range(n)))
+        out.append("\n".join(lines) + "\n")
+    return out
