@@ -26,9 +26,18 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+def build_native() -> str:
+    """Builds the -march=native timing build for this host (seconds of g++);
+    returns its path."""
+    subprocess.run(["make", "-s", "-C", HERE, "native"], check=True)
+    return os.path.join(BUILD, "native", "liboracle_native.so")
+
+
 def _load(name: str) -> ctypes.CDLL:
     path = os.path.join(BUILD, name)
-    if not os.path.exists(path):
+    if name == "native/liboracle_native.so":
+        path = build_native()
+    elif not os.path.exists(path):
         build()
     lib = ctypes.CDLL(path)
     lib.oracle_last_error.restype = ctypes.c_char_p
@@ -55,6 +64,7 @@ def _load(name: str) -> ctypes.CDLL:
     lib.oracle_set_fp32_storage.argtypes = [ctypes.c_int]
     lib.oracle_set_component_mode.argtypes = [ctypes.c_int]
     lib.oracle_set_exact_sums.argtypes = [ctypes.c_int]
+    lib.oracle_set_keep_records.argtypes = [ctypes.c_int]
     lib.oracle_set_forced.argtypes = [_i64p, ctypes.c_size_t]
     lib.oracle_set_expected.argtypes = [_u8p, ctypes.c_size_t, _u8p, ctypes.c_size_t]
     lib.oracle_set_expected.restype = None
@@ -69,8 +79,11 @@ def _load(name: str) -> ctypes.CDLL:
 _LIBS: dict[str, ctypes.CDLL] = {}
 
 
-def lib(fast: bool = False) -> ctypes.CDLL:
-    name = "liboracle_fast.so" if fast else "liboracle.so"
+def lib(fast=False) -> ctypes.CDLL:
+    """fast = False: the parity build; True: the portable timing build;
+    "native": the timing build compiled for this host."""
+    name = ("native/liboracle_native.so" if fast == "native" else
+            "liboracle_fast.so" if fast else "liboracle.so")
     if name not in _LIBS:
         _LIBS[name] = _load(name)
     return _LIBS[name]
@@ -102,21 +115,29 @@ def _cols(n: int, packed: bool) -> int:
 
 
 def sample(blob: np.ndarray, shots: int, seed: int, *, chunk_shots: int = 0, batch_size: int = 0,
-           threads: int = 1, bit_packed: bool = True, observable: int = 0, fast: bool = False) -> dict:
-    """Reference-stream sampling; chunk/batch 0 pick the reference defaults."""
+           threads: int = 1, bit_packed: bool = True, observable: int = 0, fast=False,
+           keep_records: bool = True) -> dict:
+    """Reference-stream sampling; chunk/batch 0 pick the reference defaults.
+    keep_records = False samples counts only (no rows)."""
     inf = info(blob)
-    meas = np.zeros((shots, _cols(inf["nrecords"], bit_packed)), dtype=np.uint8)
-    det = np.zeros((shots, _cols(inf["ndetectors"], bit_packed)), dtype=np.uint8)
-    obs = np.zeros((shots, _cols(inf["nobservables"], bit_packed)), dtype=np.uint8)
+    n = shots if keep_records else 0
+    meas = np.zeros((n, _cols(inf["nrecords"], bit_packed)), dtype=np.uint8)
+    det = np.zeros((n, _cols(inf["ndetectors"], bit_packed)), dtype=np.uint8)
+    obs = np.zeros((n, _cols(inf["nobservables"], bit_packed)), dtype=np.uint8)
     counts = np.zeros(4, dtype=np.uint64)
     ones = np.zeros(max(1, inf["nobservables"]), dtype=np.uint64)
     timing = np.zeros(1, dtype=np.float64)
-    ev = np.zeros((shots, inf["nexpvals"]), dtype=np.float64)
-    _check(lib(fast).oracle_sample(_ptr(blob, _u64p), blob.size, shots, seed, chunk_shots, batch_size,
-                                   threads, int(bit_packed), observable, _ptr(meas, _u8p), _ptr(det, _u8p),
-                                   _ptr(obs, _u8p), _ptr(counts, _u64p), _ptr(ones, _u64p),
-                                   _ptr(timing, _f64p), _ptr(ev, _f64p)), fast)
-    rows = int(counts[2])  # postselection keeps the accepted shots' rows
+    ev = np.zeros((n, inf["nexpvals"]), dtype=np.float64)
+    L = lib(fast)
+    L.oracle_set_keep_records(int(keep_records))
+    try:
+        _check(L.oracle_sample(_ptr(blob, _u64p), blob.size, shots, seed, chunk_shots, batch_size,
+                               threads, int(bit_packed), observable, _ptr(meas, _u8p), _ptr(det, _u8p),
+                               _ptr(obs, _u8p), _ptr(counts, _u64p), _ptr(ones, _u64p),
+                               _ptr(timing, _f64p), _ptr(ev, _f64p)), fast)
+    finally:
+        L.oracle_set_keep_records(1)
+    rows = int(counts[2]) if keep_records else 0  # postselection keeps the accepted shots' rows
     return dict(measurements=meas[:rows], detectors=det[:rows], observables=obs[:rows], counts=counts,
                 observable_ones=ones[:inf["nobservables"]], sample_s=float(timing[0]), exp_vals=ev[:rows])
 
@@ -179,8 +200,9 @@ def set_exact_sums(on: bool) -> None:
 def set_component_mode(mode: int) -> None:
     """-1: run as the blob's use_active_components flag says; 0: dense;
     1: tensor components."""
-    for fast in (False, True):
-        lib(fast).oracle_set_component_mode(int(mode))
+    for name, kind in (("liboracle.so", False), ("liboracle_fast.so", True), ("native/liboracle_native.so", "native")):
+        if name in _LIBS or name != "native/liboracle_native.so":
+            lib(kind).oracle_set_component_mode(int(mode))
 
 
 STEP_KINDS = {0: "none", 1: "ignored_global_phase", 2: "rotation", 3: "promotion", 4: "measurement"}
@@ -264,9 +286,9 @@ def set_pins(blob: np.ndarray | None, constraints=None) -> list:
     for instr, c, conds in forced:
         spec += [instr, int(c), len(conds), *conds]
     arr = np.ascontiguousarray(spec or [0], dtype=np.int64)
-    for name in ("liboracle.so", "liboracle_fast.so"):
+    for name, kind in (("liboracle.so", False), ("liboracle_fast.so", True), ("native/liboracle_native.so", "native")):
         if name in _LIBS or name == "liboracle.so":
-            lib(name == "liboracle_fast.so").oracle_set_forced(_ptr(arr, _i64p), len(spec))
+            lib(kind).oracle_set_forced(_ptr(arr, _i64p), len(spec))
     return forced
 
 
@@ -278,9 +300,9 @@ def set_expected(detectors=None, observables=None) -> None:
     det = np.ascontiguousarray(dl or [0], dtype=np.uint8)
     obs = np.ascontiguousarray(ol or [0], dtype=np.uint8)
     nd, no = len(dl), len(ol)
-    for name in ("liboracle.so", "liboracle_fast.so"):
+    for name, kind in (("liboracle.so", False), ("liboracle_fast.so", True), ("native/liboracle_native.so", "native")):
         if name in _LIBS or name == "liboracle.so":
-            lib(name == "liboracle_fast.so").oracle_set_expected(_ptr(det, _u8p), nd, _ptr(obs, _u8p), no)
+            lib(kind).oracle_set_expected(_ptr(det, _u8p), nd, _ptr(obs, _u8p), no)
 
 
 def reference_sample(blob: np.ndarray) -> tuple[np.ndarray, np.ndarray]:

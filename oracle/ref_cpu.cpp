@@ -40,6 +40,10 @@ thread_local std::string g_err;
 thread_local bool g_fp32_storage = false;
 int g_component_mode = -1;  // read by worker threads: not thread-local
 bool g_exact_sums = false;  // Neumaier-compensated probability sums (accuracy studies)
+// keep_records (src/sampler/prepared.rs:104-142): false samples counts only,
+// emitting no rows (the reference's counts-only benchmark mode). Read by
+// worker threads: not thread-local.
+bool g_keep_records = true;
 // Pinned branches (src/sampler/pinning.rs ForcedBranch): instruction index ->
 // the XOR plan its branch takes instead of a draw.
 struct Forced {
@@ -562,7 +566,20 @@ struct CompSum {
   double value() const { return s + c; }
 };
 
-double prob_true(const double* re, const double* im, const MeasK& k) {
+double prob_sum(const double* re, const double* im, const MeasK& k);
+
+double prob_true(const double* re, const double* im, const MeasK& k) { return clamp01(prob_sum(re, im, k)); }
+
+// FP32 storage emulation of the device's FP32 path: P(true) relative to the
+// stored state's own norm (sum of |a|^2 over the 2 * out_dim amplitudes of
+// the measured body, in index order).
+double stored_norm(const double* re, const double* im, const MeasK& k) {
+  double n = 0.0;
+  for (size_t i = 0; i < 2 * k.out_dim; ++i) n += re[i] * re[i] + im[i] * im[i];
+  return n;
+}
+
+double prob_sum(const double* re, const double* im, const MeasK& k) {
   if (g_exact_sums) {
     CompSum acc;
     for (size_t idx = 0; idx < k.out_dim; ++idx) {
@@ -579,7 +596,7 @@ double prob_true(const double* re, const double* im, const MeasK& k) {
         acc.add(ar * ar + ai * ai);
       }
     }
-    return clamp01(acc.value());
+    return acc.value();
   }
   double p = 0.0;
   if (k.diagonal) {
@@ -587,7 +604,7 @@ double prob_true(const double* re, const double* im, const MeasK& k) {
       size_t s = diagonal_source(k, idx, true);
       p += re[s] * re[s] + im[s] * im[s];
     }
-    return clamp01(p);
+    return p;
   }
   for (size_t idx = 0; idx < k.out_dim; ++idx) {
     size_t s0 = insert_zero_bit(idx, k.pivot), s1 = s0 ^ size_t(k.xmask);
@@ -598,7 +615,7 @@ double prob_true(const double* re, const double* im, const MeasK& k) {
     double ai = kInvSqrt2 * im[s0] + c1r * im[s1] + c1i * re[s1];
     p += ar * ar + ai * ai;
   }
-  return clamp01(p);
+  return p;
 }
 
 // src/contiguous.rs:1617-1733, 2013-2044
@@ -1034,17 +1051,28 @@ struct Runner {
       return v & live_mask(shots, w);
     };
 
+    // Blocks start on a word boundary of the chunk's rows unless the batch
+    // is not a multiple of 64 shots; then a row slice is a plain word range.
+    const bool aligned = sh == 0 && w0 + words <= exo.words;
     auto eval = [&](const Expr& e) {
-      for (size_t w = 0; w < words; ++w) bits[w] = e.constant ? live_mask(shots, w) : 0;
+      uint64_t* out = bits.data();
+      for (size_t w = 0; w < words; ++w) out[w] = e.constant ? ~0ull : 0;
       for (int32_t c : e.conds) {
         if (c <= 0 || c > p.nsymbols) throw OracleError("symbolic condition exceeds batch executor symbol table");
         if (exogenous[c]) {
-          for (size_t w = 0; w < words; ++w) bits[w] ^= exo_word(c, w);
+          if (aligned) {
+            const uint64_t* row = &exo.vals[size_t(c - 1) * exo.words + w0];
+            for (size_t w = 0; w < words; ++w) out[w] ^= row[w];
+          } else {
+            for (size_t w = 0; w < words; ++w) out[w] ^= exo_word(c, w);
+          }
         } else {
           if (!assigned[c]) throw OracleError("symbolic condition expression has no concrete batch value");
-          for (size_t w = 0; w < words; ++w) bits[w] ^= sym[size_t(c - 1) * words + w];
+          const uint64_t* row = &sym[size_t(c - 1) * words];
+          for (size_t w = 0; w < words; ++w) out[w] ^= row[w];
         }
       }
+      for (size_t w = 0; w < words; ++w) out[w] &= live_mask(shots, w);
     };
     auto assign = [&](int32_t c, const std::vector<uint64_t>& v) {
       if (c <= 0 || c > p.nsymbols) throw OracleError("symbolic condition exceeds batch executor symbol table");
@@ -1183,7 +1211,9 @@ struct Runner {
             size_t e = exp_slot(in.exp_val);
             eval(in.expr);
             for (size_t s = 0; s < shots; ++s) {
-              double pt = clamp01(prob_true(&re[s * stride], &im[s * stride], meas[i]));
+              const double* r = &re[s * stride];
+              const double* m = &im[s * stride];
+              double pt = clamp01(prob_sum(r, m, meas[i]) / (fp32 ? stored_norm(r, m, meas[i]) : 1.0));
               if (probs_out) probs_out[measure_ordinal[i] * src.shots_total + src.first_shot + s] = pt;
               double sign = bit(bits, s) ? -1.0 : 1.0;
               ev[s * nexp + e] = sign * (1.0 - 2.0 * pt);
@@ -1207,7 +1237,8 @@ struct Runner {
           for (size_t s = 0; s < shots; ++s) {
             double* r = &re[s * stride + coff];
             double* m = &im[s * stride + coff];
-            double pt = clamp01(prob_true(r, m, mk));
+            const double nrm = fp32 ? stored_norm(r, m, mk) : 1.0;
+            double pt = clamp01(prob_sum(r, m, mk) / nrm);
             if (probs_out) probs_out[measure_ordinal[i] * src.shots_total + src.first_shot + s] = pt;
             bool b;
             if (forced != g_forced.end()) {
@@ -1231,7 +1262,7 @@ struct Runner {
             if (b) br[s >> 6] |= 1ull << (s & 63);
             double ps = b ? pt : 1.0 - pt;
             if (ps <= 0.0) throw OracleError("sampled an impossible active measurement branch");
-            project(r, m, sre.data(), sim.data(), mk, b, 1.0 / std::sqrt(ps));
+            project(r, m, sre.data(), sim.data(), mk, b, 1.0 / std::sqrt(ps * nrm));
             round_state(s * stride + coff, mk.out_dim);
           }
           if (comp) {
@@ -1323,6 +1354,7 @@ struct Runner {
       out.obs_ones[o] += ones;
       if (int(o) == selected_observable) out.logical += ones;
     }
+    if (!g_keep_records) return;
     append_rows(out.meas, rec, size_t(p.nrecords), words, shots, keep);
     append_rows(out.det, det, size_t(p.ndetectors), words, shots, keep);
     append_rows(out.obs, ow, nobs, words, shots, keep);
@@ -1401,6 +1433,10 @@ void oracle_set_component_mode(int mode) { g_component_mode = mode < 0 ? -1 : (m
 // 1: probability sums with Neumaier compensation instead of the reference's
 // plain scalar order, to measure the summation error of either side.
 void oracle_set_exact_sums(int on) { g_exact_sums = on != 0; }
+
+// keep_records = false: counts only, no rows written (the row buffers of
+// oracle_sample are then not touched).
+void oracle_set_keep_records(int on) { g_keep_records = on != 0; }
 
 // Pinned branches for the following runs: spec = repeated (instruction,
 // constant, ncond, conds...); n = 0 clears.

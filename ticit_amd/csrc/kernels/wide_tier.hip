@@ -16,6 +16,7 @@
 // 1582-1606, 2013-2044).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
 #include <vector>
 
 #include "device_common.hpp"
@@ -408,13 +409,23 @@ __global__ __launch_bounds__(kWideThreads) void wide_merge_kernel(DevProgram P, 
   }
 }
 
-__device__ __forceinline__ double block_sum(double v, double* red) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+// Two block sums at once (their reduction slots are separate).
+__device__ __forceinline__ void block_sum2(double& a, double& b, double (*red)[kWideThreads / 64]) {
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off, 64);
+    b += __shfl_xor(b, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = a;
+    red[1][threadIdx.x >> 6] = b;
+  }
   __syncthreads();
-  double t = 0.0;
-  for (int w = 0; w < kWideThreads / 64; ++w) t += red[w];
-  return t;
+  a = 0.0;
+  b = 0.0;
+  for (int w = 0; w < kWideThreads / 64; ++w) {
+    a += red[0][w];
+    b += red[1][w];
+  }
 }
 
 // Whether a measurement's outputs (idx, idx + 1), idx even, read adjacent,
@@ -426,12 +437,15 @@ __device__ __forceinline__ bool pairs_ok(const DevInstr& in, size_t out_dim, siz
   return !(in.op & kFlagDiagonal) || !(in.zwop & 1);
 }
 
-// FP64 partial sums of P(branch = 1) per (tile, shot).
+// FP64 partial sums of P(branch = 1) per (tile, shot). With `npart` (the
+// FP32 path) also the partial sums of the component's norm, over every
+// amplitude it holds, so the decision renormalizes by the stored state's
+// own norm instead of assuming it is 1.
 template <class T>
 __global__ __launch_bounds__(kWideThreads) void wide_prob_kernel(DevProgram P, int instr, const WideCtl* ctl,
                                                                  const T* re, const T* im, size_t dim, int k,
-                                                                 double* part, uint32_t max_tiles) {
-  __shared__ double red[kWideThreads / 64];
+                                                                 double* part, double* npart, uint32_t max_tiles) {
+  __shared__ double red[2][kWideThreads / 64];
   const uint32_t j = blockIdx.y;
   if (ctl[j].dead) return;
   const DevInstr in = P.instrs[instr];
@@ -439,7 +453,9 @@ __global__ __launch_bounds__(kWideThreads) void wide_prob_kernel(DevProgram P, i
   const T* I = im + size_t(j) * dim + in.comp_off;
   const size_t out_dim = (size_t(1) << k) >> 1;
   const bool diag = in.op & kFlagDiagonal;
-  double acc = 0.0;
+  const bool norm = npart != nullptr;
+  double acc = 0.0, nacc = 0.0;
+  auto sq = [](double r, double m) { return r * r + m * m; };
   if (pairs_ok(in, out_dim, dim)) {
 #pragma unroll
     for (int u = 0; u < kWidePairs; ++u) {
@@ -450,9 +466,18 @@ __global__ __launch_bounds__(kWideThreads) void wide_prob_kernel(DevProgram P, i
         const auto r = ld2(R + s0), m = ld2(I + s0);
         acc += double(r.x) * double(r.x) + double(m.x) * double(m.x);
         acc += double(r.y) * double(r.y) + double(m.y) * double(m.y);
+        if (norm) {  // the other half: the pivot bit flipped (pivot >= 1 keeps the pair)
+          const size_t c0 = s0 ^ (size_t(1) << in.pivot);
+          const auto rc = ld2(R + c0), mc = ld2(I + c0);
+          nacc += sq(double(r.x), double(m.x)) + sq(double(r.y), double(m.y)) + sq(double(rc.x), double(mc.x)) +
+                  sq(double(rc.y), double(mc.y));
+        }
       } else {
         const size_t s0 = insert_zero_bit(idx, in.pivot), s1 = (s0 ^ size_t(in.xmask)) & ~size_t(1);
         const auto a0 = ld2(R + s0), b0 = ld2(I + s0), a1 = ld2(R + s1), b1 = ld2(I + s1);
+        if (norm)  // s0 and s0 ^ xmask cover every amplitude once
+          nacc += sq(double(a0.x), double(b0.x)) + sq(double(a0.y), double(b0.y)) + sq(double(a1.x), double(b1.x)) +
+                  sq(double(a1.y), double(b1.y));
         const bool swap = in.xmask & 1;
         const double r0[2] = {double(a0.x), double(a0.y)}, m0[2] = {double(b0.x), double(b0.y)};
         const double r1[2] = {double(swap ? a1.y : a1.x), double(swap ? a1.x : a1.y)};
@@ -467,8 +492,11 @@ __global__ __launch_bounds__(kWideThreads) void wide_prob_kernel(DevProgram P, i
         }
       }
     }
-    const double t = block_sum(acc, red);
-    if (threadIdx.x == 0) part[size_t(j) * max_tiles + blockIdx.x] = t;
+    block_sum2(acc, nacc, red);
+    if (threadIdx.x == 0) {
+      part[size_t(j) * max_tiles + blockIdx.x] = acc;
+      if (norm) npart[size_t(j) * max_tiles + blockIdx.x] = nacc;
+    }
     return;
   }
 #pragma unroll
@@ -479,6 +507,10 @@ __global__ __launch_bounds__(kWideThreads) void wide_prob_kernel(DevProgram P, i
       size_t s = diagonal_source(in, idx, true);
       double r = double(R[s]), m = double(I[s]);
       acc += r * r + m * m;
+      if (norm) {
+        const size_t c = diagonal_source(in, idx, false);
+        nacc += sq(r, m) + sq(double(R[c]), double(I[c]));
+      }
     } else {
       size_t s0 = insert_zero_bit(idx, in.pivot), s1 = s0 ^ size_t(in.xmask);
       double dir = parity64(uint64_t(s0) & in.zmask) ? 1.0 : -1.0;
@@ -486,18 +518,28 @@ __global__ __launch_bounds__(kWideThreads) void wide_prob_kernel(DevProgram P, i
       double ar = kInvSqrt2 * double(R[s0]) + c1r * double(R[s1]) - c1i * double(I[s1]);
       double ai = kInvSqrt2 * double(I[s0]) + c1r * double(I[s1]) + c1i * double(R[s1]);
       acc += ar * ar + ai * ai;
+      if (norm) nacc += sq(double(R[s0]), double(I[s0])) + sq(double(R[s1]), double(I[s1]));
     }
   }
-  double t = block_sum(acc, red);
-  if (threadIdx.x == 0) part[size_t(j) * max_tiles + blockIdx.x] = t;
+  block_sum2(acc, nacc, red);
+  if (threadIdx.x == 0) {
+    part[size_t(j) * max_tiles + blockIdx.x] = acc;
+    if (norm) npart[size_t(j) * max_tiles + blockIdx.x] = nacc;
+  }
 }
 
-// Per shot, given P(branch = 1) before the clamp: clamp, draw, 1/sqrt,
-// branch symbol, outcome, record (or the expectation of a probe).
+// Per shot, given the summed P(branch = 1) and the measured component's
+// norm `nrm`: P(true) = pt / nrm clamped, draw, 1/sqrt(P(branch) * nrm),
+// branch symbol, outcome, record (or the expectation of a probe). The FP64
+// path passes nrm = 1, which leaves pt and P(branch) bit for bit as the
+// reference forms them; the FP32 path passes the stored state's norm, so
+// the collapse renormalizes the state to 1 and rounding accumulated in the
+// stored amplitudes does not grow through the 1 / P(branch) scaling.
 __device__ void wide_decide_shot(const DevProgram& P, const DevLaunch& L, const WideLayout& Y, uint32_t j,
-                                 uint64_t local, const DevInstr& in, double pt, uint64_t* bits, WideCtl* ctl) {
+                                 uint64_t local, const DevInstr& in, double pt, double nrm, uint64_t* bits,
+                                 WideCtl* ctl) {
   if (ctl[j].dead) return;
-  pt = clamp01(pt);
+  pt = clamp01(pt / nrm);
   if (L.want_probs) L.probs[uint64_t(in.ordinal) * L.shots + local] = pt;
   if (in.op & kFlagExpVal) {
     // src/sampler/batch/active.rs:818-850: +-(1 - 2 P(true)), no draw, no
@@ -528,7 +570,7 @@ __device__ void wide_decide_shot(const DevProgram& P, const DevLaunch& L, const 
     c.err |= kDevErrImpossibleBranch;
     ps = 1.0;
   }
-  c.invnorm = 1.0 / sqrt(ps);
+  c.invnorm = 1.0 / sqrt(ps * nrm);
   c.branch = branch;
   ctl[j] = c;
   WideShot w = wide_shot(bits, Y, j);
@@ -542,14 +584,16 @@ __device__ void wide_decide_shot(const DevProgram& P, const DevLaunch& L, const 
 __global__ __launch_bounds__(kWideThreads) void wide_decide_kernel(DevProgram P, DevLaunch L, WideLayout Y,
                                                                    uint64_t batch_start, uint32_t nshots,
                                                                    int instr, uint64_t* bits, WideCtl* ctl,
-                                                                   const double* part, uint32_t ntiles,
-                                                                   uint32_t max_tiles) {
+                                                                   const double* part, const double* npart,
+                                                                   uint32_t ntiles, uint32_t max_tiles) {
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nshots) return;
   const DevInstr in = P.instrs[instr];
-  double pt = 0.0;
+  double pt = 0.0, nrm = npart ? 0.0 : 1.0;
   for (uint32_t t = 0; t < ntiles; ++t) pt += part[size_t(j) * max_tiles + t];
-  wide_decide_shot(P, L, Y, j, batch_start + j, in, pt, bits, ctl);
+  if (npart)
+    for (uint32_t t = 0; t < ntiles; ++t) nrm += npart[size_t(j) * max_tiles + t];
+  wide_decide_shot(P, L, Y, j, batch_start + j, in, pt, nrm, bits, ctl);
 }
 
 // Collapse into the partner buffer (the renormalising multiply in FP64).
@@ -635,8 +679,8 @@ __global__ __launch_bounds__(kWideThreads) void wide_project_measure_kernel(DevP
                                                                             const WideCtl* ctl, const T* re,
                                                                             const T* im, T* ore, T* oim,
                                                                             size_t dim, int k, double* part,
-                                                                            uint32_t max_tiles) {
-  __shared__ double red[kWideThreads / 64];
+                                                                            double* npart, uint32_t max_tiles) {
+  __shared__ double red[2][kWideThreads / 64];
   const uint32_t j = blockIdx.y;
   if (ctl[j].dead) return;
   const DevInstr in = P.instrs[instr];
@@ -666,7 +710,7 @@ __global__ __launch_bounds__(kWideThreads) void wide_project_measure_kernel(DevP
     }
   };
   const size_t pairs = ((size_t(1) << k) >> 1) >> 1;  // next's output half
-  double acc = 0.0;
+  double acc = 0.0, nacc = 0.0;
 #pragma unroll
   for (int u = 0; u < kWidePairs; ++u) {
     const size_t q = size_t(blockIdx.x) * (kWideTile / 2) + size_t(u) * kWideThreads + threadIdx.x;
@@ -686,6 +730,10 @@ __global__ __launch_bounds__(kWideThreads) void wide_project_measure_kernel(DevP
     OI[a] = ma;
     OR[b] = rb;
     OI[b] = mb;
+    // a and b over every q cover the collapsed state once: its norm, from
+    // the stored values
+    if (npart) nacc += double(ra) * double(ra) + double(ma) * double(ma) + double(rb) * double(rb) +
+                       double(mb) * double(mb);
     if (ndiag) {
       acc += double(ra) * double(ra) + double(ma) * double(ma);
     } else {
@@ -696,8 +744,11 @@ __global__ __launch_bounds__(kWideThreads) void wide_project_measure_kernel(DevP
       acc += ar * ar + ai * ai;
     }
   }
-  const double t = block_sum(acc, red);
-  if (threadIdx.x == 0) part[size_t(j) * max_tiles + blockIdx.x] = t;
+  block_sum2(acc, nacc, red);
+  if (threadIdx.x == 0) {
+    part[size_t(j) * max_tiles + blockIdx.x] = acc;
+    if (npart) npart[size_t(j) * max_tiles + blockIdx.x] = nacc;
+  }
 }
 
 // ---- Factored components -------------------------------------------------
@@ -853,7 +904,8 @@ __global__ __launch_bounds__(kWideThreads) void wide_decide_factored_kernel(DevP
                                                                             WideLayout Y, uint64_t batch_start,
                                                                             uint32_t nshots, int instr,
                                                                             uint64_t* bits, WideCtl* ctl,
-                                                                            const double* part, FactorTiles ft) {
+                                                                            const double* part, FactorTiles ft,
+                                                                            int normalize) {
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nshots) return;
   const DevInstr in = P.instrs[instr];
@@ -883,7 +935,8 @@ __global__ __launch_bounds__(kWideThreads) void wide_decide_factored_kernel(DevP
     ei = in.c1_im > 0.35 ? 1.0 : (in.c1_im < -0.35 ? -1.0 : 0.0);
   }
   const double pt = 0.5 * (n - (er * tr - ei * ti));
-  wide_decide_shot(P, L, Y, j, batch_start + j, in, pt, bits, ctl);
+  // the FP32 path renormalizes by the factors' product norm n
+  wide_decide_shot(P, L, Y, j, batch_start + j, in, pt, normalize ? n : 1.0, bits, ctl);
 }
 
 // The collapse from the factors into the output buffer (as wide_project_kernel).
@@ -1074,7 +1127,8 @@ static hipError_t wide_workspace_reserve(WideWorkspace& ws, size_t batch, size_t
     ws.amp_bytes = amp;
   }
   hipError_t e = hipSuccess;
-  const size_t bb = batch * Y.words * 8 + 8, cb = batch * sizeof(WideCtl), pb = batch * Y.max_tiles * 8 + 8;
+  // probability partials, then (FP32 path) norm partials
+  const size_t bb = batch * Y.words * 8 + 8, cb = batch * sizeof(WideCtl), pb = 2 * batch * Y.max_tiles * 8 + 8;
   if (bb > ws.bit_bytes || cb > ws.ctl_bytes || pb > ws.part_bytes) {
     if ((e = hipEventSynchronize(ws.done)) != hipSuccess) return e;
     if ((e = grow(reinterpret_cast<void**>(&ws.bits), ws.bit_bytes, bb)) != hipSuccess) return e;
@@ -1096,7 +1150,7 @@ uint32_t wide_partials_per_shot(int max_local_k) {
 size_t wide_bytes_per_shot(uint32_t state_dim, int fp32, const WideLayout& y) {
   size_t dim = state_dim;
   size_t t = fp32 ? 4 : 8;
-  return 4 * dim * t + size_t(y.words) * 8 + sizeof(WideCtl) + size_t(y.max_tiles) * 8;
+  return 4 * dim * t + size_t(y.words) * 8 + sizeof(WideCtl) + 2 * size_t(y.max_tiles) * 8;
 }
 
 // Enqueues the whole run on `stream` without synchronizing; `instrs` is the
@@ -1118,6 +1172,9 @@ static hipError_t run_wide(const DevProgram& P, const DevInstr* instrs, const De
     }                      \
   } while (0)
   const unsigned per_shot_blocks = 256;
+  // FP32 amplitudes: every measurement also sums the measured component's
+  // norm (after the probability partials) and renormalizes by it
+  double* npart = std::is_same<T, float>::value ? ws.part + size_t(batch) * Y.max_tiles : nullptr;
   for (uint64_t start = 0; start < L.shots; start += batch) {
     uint32_t n = uint32_t(std::min<uint64_t>(batch, L.shots - start));
     dim3 sg((n + per_shot_blocks - 1) / per_shot_blocks), sb(per_shot_blocks);
@@ -1201,7 +1258,7 @@ static hipError_t run_wide(const DevProgram& P, const DevInstr* instrs, const De
                              stream, P, e2, F, ws.ctl, dim, ws.part, n);
           WCK(hipGetLastError());
           hipLaunchKernelGGL(wide_decide_factored_kernel, sg, sb, 0, stream, P, L, Y, start, n, e2, ws.bits, ws.ctl,
-                             ws.part, ft);
+                             ws.part, ft, int(npart != nullptr));
           WCK(hipGetLastError());
           size_t out_dim = (size_t(1) << W) >> 1;
           unsigned tiles = unsigned((out_dim + kWideTile - 1) / kWideTile);
@@ -1238,11 +1295,11 @@ static hipError_t run_wide(const DevProgram& P, const DevInstr* instrs, const De
           fused_tiles = 0;
         } else {
           hipLaunchKernelGGL(wide_prob_kernel<T>, g, dim3(kWideThreads), 0, stream, P, i, ws.ctl, R, I, dim, w,
-                             ws.part, Y.max_tiles);
+                             ws.part, npart, Y.max_tiles);
           WCK(hipGetLastError());
         }
         hipLaunchKernelGGL(wide_decide_kernel, sg, sb, 0, stream, P, L, Y, start, n, i, ws.bits, ws.ctl, ws.part,
-                           tiles, Y.max_tiles);
+                           npart, tiles, Y.max_tiles);
         WCK(hipGetLastError());
         if (in.op & kFlagExpVal) {  // a probe reads the state and leaves it
           ++i;
@@ -1254,7 +1311,7 @@ static hipError_t run_wide(const DevProgram& P, const DevInstr* instrs, const De
         if (nx && (nx->op & 0xff) == kOpMeasure && !(nx->op & kFlagExpVal) && nx->comp == in.comp &&
             nx->nq == w - 1 && pairs > 0 && ftiles <= Y.max_tiles) {
           hipLaunchKernelGGL(wide_project_measure_kernel<T>, dim3(ftiles, n), dim3(kWideThreads), 0, stream, P, i,
-                             i + 1, ws.ctl, R, I, RE(cb ^ 1), IM(cb ^ 1), dim, w, ws.part, Y.max_tiles);
+                             i + 1, ws.ctl, R, I, RE(cb ^ 1), IM(cb ^ 1), dim, w, ws.part, npart, Y.max_tiles);
           fused_tiles = ftiles;
         } else {
           hipLaunchKernelGGL(wide_project_kernel<T>, g, dim3(kWideThreads), 0, stream, P, i, ws.ctl, R, I,
