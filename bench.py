@@ -13,8 +13,13 @@ record stays in LDS. achieved = 35 B * shots / kernel time, measured with HIP
 events on the stream the kernel runs on; peak = 8 TB/s HBM3E.
 
 CPU baseline: the oracle restatement of ticit's CPU sampler (reference
-chunking and batch defaults) on a bounded sample of the same circuit, using
-every host core this process may run on (at most 16).
+chunking and batch defaults, rows kept, bit-packed), built on this host with
+-march=native, on a bounded sample of the same circuit: every core this
+process may use (affinity, capped by a cgroup CPU quota) and one thread, each
+the median of 3 runs. A sanity anchor times the same build on the
+reference's published per-core workload (surface_d7_r7 p=1e-3, counts only,
+one thread) and reports its ratio to the published 7.79 Mshot/s, so every
+HIP/CPU ratio can be read against how fast this restatement is.
 
 Unless --headline-only is given, the same JSON line carries a "configs"
 object with the other workloads: C0 (random Clifford+T, group tier), C2
@@ -109,6 +114,7 @@ def main() -> int:
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(circuit.blob, args.cpu_seconds)
+        cpu["hip_over_cpu"] = value / cpu["value"]
 
     line = {
         "metric": "shots/s per circuit on 1 MI355X + % HBM roofline",
@@ -147,12 +153,134 @@ def main() -> int:
     return 0
 
 
-def _cores() -> int:
+def _cgroup_cpu_quota():
+    """CPUs allowed by a cgroup quota (v2 cpu.max or v1 cfs), or None."""
     try:
-        cores = len(os.sched_getaffinity(0))
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            quota, period = f.read().split()[:2]
+        if quota != "max":
+            return float(quota) / float(period)
+    except (OSError, ValueError):
+        pass
+    try:
+        with open("/sys/fs/cgroup/cpu/cpu.cfs_quota_us") as f:
+            quota = float(f.read())
+        with open("/sys/fs/cgroup/cpu/cpu.cfs_period_us") as f:
+            period = float(f.read())
+        if quota > 0:
+            return quota / period
+    except (OSError, ValueError):
+        pass
+    return None
+
+
+def host_cpu() -> tuple[int, dict]:
+    """(threads the CPU legs use, what the host reports): every CPU in this
+    process's affinity set, capped by a cgroup CPU quota when one is set."""
+    try:
+        affinity = len(os.sched_getaffinity(0))
     except AttributeError:
-        cores = os.cpu_count() or 1
-    return max(1, min(cores, 16))
+        affinity = os.cpu_count() or 1
+    quota = _cgroup_cpu_quota()
+    threads = max(1, min(affinity, int(quota))) if quota else affinity
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return threads, {"nproc": os.cpu_count(), "affinity": affinity, "cgroup_cpu_quota": quota, "model": model}
+
+
+_HOST = None
+
+
+def _host():
+    global _HOST
+    if _HOST is None:
+        _HOST = host_cpu()
+    return _HOST
+
+
+def _cores() -> int:
+    return _host()[0]
+
+
+def worker_chunk(shots: int, threads: int) -> int:
+    """sample_chunk_shots that gives every worker thread a chunk: the
+    reference's default 2048-shot chunks (src/sampler/prepared.rs:692-720)
+    unless the sample has fewer than 2048 shots per thread, then an even
+    split (0 = the default)."""
+    return 0 if shots >= 2048 * threads else max(1, -(-shots // threads))
+
+
+def cpu_rate(blob, shots: int, seed: int, threads: int, reps: int = 3, **kw) -> float:
+    """Median over `reps` runs of the oracle's timing build (native to this
+    host) sampling `shots` shots on `threads` workers: shots per second of
+    the sample call."""
+    import statistics
+    from oracle import oracle as O
+    kw.setdefault("chunk_shots", worker_chunk(shots, threads))
+    rates = []
+    for r in range(reps):
+        out = O.sample(blob, shots, seed + r, threads=threads, fast="native", **kw)
+        rates.append(shots / max(out["sample_s"], 1e-9))
+    return float(statistics.median(rates))
+
+
+def cpu_rows_multi(items, seed: int, sample: str) -> dict:
+    """cpu_rows over several programs: items = (blob, shots on every core,
+    shots on one thread); each rate is total shots over total sample time,
+    the median of 3 runs."""
+    import statistics
+    from oracle import oracle as O
+    threads, _ = _host()
+
+    def rate(nthreads, col):
+        rs = []
+        for r in range(3):
+            n, t = 0, 0.0
+            for it in items:
+                out = O.sample(it[0], it[col], seed + r, threads=nthreads, fast="native",
+                               chunk_shots=worker_chunk(it[col], nthreads))
+                n += it[col]
+                t += out["sample_s"]
+            rs.append(n / max(t, 1e-9))
+        return float(statistics.median(rs))
+
+    return {"value": rate(threads, 1), "unit": "shots/s", "cores": threads, "kind": "port",
+            "one_thread": rate(1, 2), "sample": sample + " (median of 3 runs each; a program with fewer than "
+                                                        "2048 shots per thread is split evenly over the threads)"}
+
+
+def cpu_rows(blob, shots_all: int, shots_one: int, seed: int, sample: str, **kw) -> dict:
+    """The reported CPU baseline of one workload: every usable core and one
+    thread, each the median of 3 runs."""
+    threads, host = _host()
+    chunk = worker_chunk(shots_all, threads)
+    return {"value": cpu_rate(blob, shots_all, seed, threads, **kw), "unit": "shots/s", "cores": threads,
+            "kind": "port", "one_thread": cpu_rate(blob, shots_one, seed, 1, **kw),
+            "sample": sample + f" ({shots_all} shots on {threads} threads"
+                               + (f" in chunks of {chunk}" if chunk else "") + f", {shots_one} on 1 thread, "
+                               "median of 3 runs each)"}
+
+
+ANCHOR_PUBLISHED = 7.79e6  # shots/s per core, surface_d7_r7 p=1e-3, counts only (ticit docs/benchmark.md:81)
+
+
+def cpu_anchor() -> dict:
+    """The restatement on the reference's published per-core workload."""
+    import ticit_amd
+    path = os.path.join(ROOT, "tests", "golden", "circuits", "pure_surface_d7_r7_p1e-3.stim")
+    with open(path) as f:
+        blob = ticit_amd.plan_text(f.read())
+    rate = cpu_rate(blob, 400_000, 11, 1, keep_records=False)
+    return {"workload": "pure_surface_d7_r7_p1e-3 (the reference's testdata), counts only, 1 thread",
+            "shots_per_s": rate, "published_shots_per_s": ANCHOR_PUBLISHED,
+            "published_hardware": "i5-14600KF (one core)", "ratio_to_published": rate / ANCHOR_PUBLISHED}
 
 
 def _device_buffers(torch, s, n):
@@ -196,9 +324,8 @@ def other_configs(ticit_amd, w, torch, np, stream, use_cpu=True) -> dict:
     out["C0"] = {"workload": "random Clifford+T, 16 qubits, 40 layers, 12 T, M all; 1e5 shots, FP64",
                  "tier": s0.tier, "shots_per_s": n0 / t, "ms": t * 1e3}
     if use_cpu:
-        r = O.sample(c0.blob, 20000, 3, threads=cores, fast=True)
-        out["C0"]["cpu_baseline"] = {"value": 20000 / r["sample_s"], "unit": "shots/s", "cores": cores,
-                                     "kind": "port", "sample": "20000 shots of C0"}
+        out["C0"]["cpu_baseline"] = cpu_rows(c0.blob, 60000, 6000, 3, "C0")
+        out["C0"]["hip_over_cpu"] = out["C0"]["shots_per_s"] / out["C0"]["cpu_baseline"]["value"]
 
     # C2
     batch = w.config2_batch()
@@ -238,14 +365,10 @@ def other_configs(ticit_amd, w, torch, np, stream, use_cpu=True) -> dict:
     bt.close()
     if use_cpu:
         sel = list(range(0, 256, 16))
-        n_cpu, t_cpu = 0, 0.0
-        for i in sel:
-            n = min(shots[i], 4000)
-            r = O.sample(samplers[i].blob, n, 5, threads=cores, fast=True)
-            n_cpu += n
-            t_cpu += r["sample_s"]
-        out["C2"]["cpu_baseline"] = {"value": n_cpu / t_cpu, "unit": "shots/s", "cores": cores, "kind": "port",
-                                     "sample": f"circuits {sel[0]}..{sel[-1]} step 16, up to 4000 shots each"}
+        out["C2"]["cpu_baseline"] = cpu_rows_multi(
+            [(samplers[i].blob, min(shots[i], 4000), min(shots[i], 400)) for i in sel], 5,
+            f"circuits {sel[0]}..{sel[-1]} step 16, up to 4000 shots each on every core, 400 on one thread")
+        out["C2"]["hip_over_cpu"] = out["C2"]["batched_learned_shots_per_s"] / out["C2"]["cpu_baseline"]["value"]
     del bufs
 
     # C3
@@ -271,6 +394,32 @@ def other_configs(ticit_amd, w, torch, np, stream, use_cpu=True) -> dict:
         ox = O.sample_injected(c3.blob, q, None, uni, None, want_probs=True)
     finally:
         O.set_exact_sums(False)
+    # precision per result: maximum amplitude error of the width-20 state
+    # (C3 without its final M, dense) and maximum probability error of the
+    # forced-dense width-20 run, both against the FP64 oracle
+    c3n = ticit_amd.Circuit("\n".join(l for l in w.config3_circuit().splitlines()
+                                       if not (l.startswith("M ") and len(l.split()) == 31)) + "\n")
+    O.set_component_mode(0)
+    try:
+        ref = O.sample_injected(c3n.blob, 1, None, None, None, want_final_state=True)["final_state"]
+        qd = 4
+        unid = np.random.default_rng(5).random((s64.num_measure_instructions, qd))
+        od = O.sample_injected(c3.blob, qd, None, unid, None, want_probs=True)
+    finally:
+        O.set_component_mode(-1)
+    amp_err = {p: float(np.max(np.abs(c3n.compile(tier="wide", precision=p, components="dense").final_state(1) - ref)))
+               for p in ("fp64", "fp32")}
+    dense = {}
+    nd = 2000
+    for prec in ("fp64", "fp32"):
+        sd = c3.compile(precision=prec, components="dense")
+        gd = sd.sample_injected(qd, None, unid, None, want_probs=True)
+        bd = _device_buffers(torch, sd, nd)
+        t = _time(torch, stream, lambda r: sd.sample_device(nd, 30 + r, bd, stream=st), reps=1)
+        del bd
+        gbps = sd.algorithmic_bytes_per_shot * nd / t / 1e9
+        dense[prec] = {"shots_per_s": nd / t, "algorithmic_GBps": gbps, "hbm_frac": gbps / HBM_PEAK_GBS,
+                       "max_prob_error": float(np.max(np.abs(gd.probabilities - od["probs"])))}
     out["C3"] = {"workload": "30 qubits, 60 layers, 24 T, M on 8 qubits after layers 15/30/45, final M; 1e5 shots",
                  "layout": "components" if s64.components else "dense", "max_local_k": s64.max_local_k,
                  "fp64_shots_per_s": n3 / t64, "fp32_shots_per_s": n3 / t32, "fp32_over_fp64": t64 / t32,
@@ -281,11 +430,18 @@ def other_configs(ticit_amd, w, torch, np, stream, use_cpu=True) -> dict:
                  "max_prob_error_fp32": float(np.max(np.abs(g32.probabilities - o["probs"]))),
                  "max_prob_error_fp64_vs_compensated": float(np.max(np.abs(g64.probabilities - ox["probs"]))),
                  "max_prob_error_oracle_scalar_vs_compensated": float(np.max(np.abs(o["probs"] - ox["probs"]))),
-                 "error_sample": f"{q} shots, injected uniforms, against the FP64 oracle"}
+                 "error_sample": f"{q} shots, injected uniforms, against the FP64 oracle",
+                 "max_amp_error_fp64": amp_err["fp64"], "max_amp_error_fp32": amp_err["fp32"],
+                 "amp_error_sample": "the width-20 state of C3 without its final M (dense, 2^20 amplitudes), "
+                                     "against the FP64 oracle",
+                 "dense_width20": dict(dense, shots=nd, max_k=c3.max_active_qubits,
+                                       note=f"forced dense layout ({nd} shots timed; probability errors on "
+                                            f"{qd} injected shots); the generator plans to 2^20, not the "
+                                            "2^24 the configuration names")}
     if use_cpu:
-        r = O.sample(c3.blob, 32, 4, threads=cores, fast=True)
-        out["C3"]["cpu_baseline"] = {"value": 32 / r["sample_s"], "unit": "shots/s", "cores": cores,
-                                     "kind": "port", "sample": "32 shots of C3"}
+        out["C3"]["cpu_baseline"] = cpu_rows(c3.blob, 128 * cores, 128, 4, "C3, FP64 (components)")
+        out["C3"]["hip_over_cpu_fp64"] = out["C3"]["fp64_shots_per_s"] / out["C3"]["cpu_baseline"]["value"]
+        out["C3"]["hip_over_cpu_fp32"] = out["C3"]["fp32_shots_per_s"] / out["C3"]["cpu_baseline"]["value"]
 
     # C4
     from ticit_amd import planner as PL
@@ -321,13 +477,10 @@ def other_configs(ticit_amd, w, torch, np, stream, use_cpu=True) -> dict:
                  "train_programs": int(m["n_train"])}
     if use_cpu:
         sel = list(range(0, len(blobs), 32))
-        n_cpu, t_cpu = 0, 0.0
-        for i in sel:
-            r = O.sample(blobs[i], 2000, 5, threads=cores, fast=True)
-            n_cpu += 2000
-            t_cpu += r["sample_s"]
-        out["C4"]["cpu_baseline"] = {"value": n_cpu / t_cpu, "unit": "shots/s", "cores": cores, "kind": "port",
-                                     "sample": f"held-out programs {sel[0]}..{sel[-1]} step 32, 2000 shots each"}
+        out["C4"]["cpu_baseline"] = cpu_rows_multi(
+            [(blobs[i], 2000, 200) for i in sel], 5,
+            f"held-out programs {sel[0]}..{sel[-1]} step 32, 2000 shots each on every core, 200 on one thread")
+        out["C4"]["hip_over_cpu"] = out["C4"]["batched_learned_shots_per_s"] / out["C4"]["cpu_baseline"]["value"]
 
     # Postselection: the cultivation fixture as the reference's benchmark
     # driver runs it (normalize_syndromes, a postselection mask), every
@@ -347,16 +500,14 @@ def other_configs(ticit_amd, w, torch, np, stream, use_cpu=True) -> dict:
                             "postselected, normalized syndromes; 1e6 shots, survivors compacted on the device",
                 "tier": sp.tier, "shots_per_s": npost / t, "ms": t * 1e3, "accepted_fraction": kept / npost}
     if use_cpu:
-        n_cpu = 20000
-        O.lib(fast=True)  # loaded before the expected bits are set in both builds
+        O.lib(fast="native")  # loaded before the expected bits are set in every build
         O.set_expected(*O.reference_sample(sp.blob))
         try:
-            r = O.sample(sp.blob, n_cpu, 3, batch_size=1, threads=cores, fast=True)
+            out["P"]["cpu_baseline"] = cpu_rows(sp.blob, 40000, 4000, 3, "the same postselected program, one "
+                                                "shot per block", batch_size=1)
         finally:
             O.set_expected(None, None)
-        out["P"]["cpu_baseline"] = {"value": n_cpu / r["sample_s"], "unit": "shots/s", "cores": cores,
-                                    "kind": "port", "sample": f"{n_cpu} shots of the same postselected program, "
-                                                              "one shot per block"}
+        out["P"]["hip_over_cpu"] = out["P"]["shots_per_s"] / out["P"]["cpu_baseline"]["value"]
     return out
 
 
@@ -365,26 +516,21 @@ def ctypes_stream(stream) -> int:
 
 
 def cpu_baseline(blob, seconds: float) -> dict:
-    """Times the CPU oracle (reference defaults: 2048-shot chunks, default
-    batch) on a bounded sample sized to about `seconds` of work."""
+    """The oracle's timing build (reference defaults: 2048-shot chunks,
+    default batch, rows kept) on a bounded sample sized to about `seconds`
+    of work over the 3 all-core runs, plus a one-thread row and the anchor."""
     from oracle import oracle as O
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
-    probe = O.sample(blob, 20480, 1, threads=cores, fast=True)
+    threads, host = _host()
+    probe = O.sample(blob, 20480, 1, threads=threads, fast="native")
     rate = 20480 / max(probe["sample_s"], 1e-6)
-    shots = int(min(max(rate * seconds, 20480), 20_000_000))
-    r = O.sample(blob, shots, 2, threads=cores, fast=True)
-    return {
-        "value": shots / r["sample_s"],
-        "unit": "shots/s",
-        "cores": cores,
-        "kind": "port",
-        "sample": f"{shots} shots of the same circuit, seed 2, bit-packed rows kept, "
-                  f"{cores} worker threads, chunk 2048 / default batch",
-    }
+    shots = int(min(max(rate * seconds / 3, 20480), 20_000_000))
+    one = int(min(max(rate / threads * seconds / 6, 4096), 2_000_000))
+    out = cpu_rows(blob, shots, one, 2, f"the same circuit, seed 2+, bit-packed rows kept, chunk 2048 / "
+                                         f"default batch")
+    out["host"] = host
+    out["build"] = "g++ -O3 -march=native (built on this host), no FP contraction"
+    out["anchor"] = cpu_anchor()
+    return out
 
 
 if __name__ == "__main__":
