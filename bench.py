@@ -55,11 +55,15 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the surface-code line, without the other configurations")
+    ap.add_argument("--profile", choices=sorted(PROFILE_WORKLOADS),
+                    help="run one workload's launches for a rocprofv3 capture instead of the benchmark")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported "
               "(shots shard by chunk index across GPUs; see the design notes)", file=sys.stderr)
         return 2
+    if args.profile:
+        return profile_workload(args.profile)
 
     import numpy as np
     import torch
@@ -509,6 +513,68 @@ def other_configs(ticit_amd, w, torch, np, stream, use_cpu=True) -> dict:
             O.set_expected(None, None)
         out["P"]["hip_over_cpu"] = out["P"]["shots_per_s"] / out["P"]["cpu_baseline"]["value"]
     return out
+
+
+PROFILE_WORKLOADS = {
+    "c1": "surface code d=5, 5 rounds (lane tier): two launches of 1e6 shots",
+    "c0": "random Clifford+T n=16, 12 T (group tier): two launches of 1e5 shots",
+    "c3": "dense stress n=30, 24 T (wide tier, components, FP64): a warm and a measured launch of 4096 shots",
+    "c2": "the 256-circuit batch as one batched graph launch (learned layouts)",
+    "p": "cultivation d=3, every detector postselected, normalized: two launches of 1e6 shots, survivors "
+         "compacted on the device",
+}
+
+
+def profile_workload(which: str) -> int:
+    """--profile: one device copy of a known size (256 MiB read, 256 MiB
+    written), which calibrates the FETCH_SIZE / WRITE_SIZE counters, then
+    the workload's sampling launches into device memory, for
+    `rocprofv3 --kernel-trace --stats` or one `--pmc` counter per run."""
+    import torch
+
+    import ticit_amd
+    from ticit_amd import workloads as w
+
+    print(f"profile {which}: {PROFILE_WORKLOADS[which]}", flush=True)
+    n = 256 << 20
+    a = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+    if which in ("c1", "c0", "c3"):
+        text = {"c1": w.config1_circuit, "c0": w.config0_circuit, "c3": w.config3_circuit}[which]()
+        s = ticit_amd.Circuit(text).compile()
+        shots = {"c1": 1_000_000, "c0": 100_000, "c3": 4096}[which]
+        out = _device_buffers(torch, s, shots)
+        for seed in (1, 2):
+            s.sample_device(shots, seed, out)
+        torch.cuda.synchronize()
+    elif which == "c2":
+        batch = w.config2_batch()
+        samplers = [ticit_amd.CompiledSampler(ticit_amd.plan_text(t), planner="learned") for t, _ in batch]
+        shots = [k for _, k in batch]
+        outs = [_device_buffers(torch, s, k) for s, k in zip(samplers, shots)]
+        bt = ticit_amd.Batch(samplers, shots, [1 + i for i in range(len(batch))], outs, autotune=True,
+                             autotune_shots=4096)
+        bt.launch(0)
+        bt.wait()
+        torch.cuda.synchronize()
+    else:
+        path = os.path.join(ROOT, "tests", "golden", "circuits", "msc_d3_inject_cultivate_p1e-3.stim")
+        with open(path) as f:
+            c = ticit_amd.Circuit(f.read())
+        s = c.compile([1] * c.num_detectors, normalize_syndromes=True)
+        shots = 1_000_000
+        out = _device_buffers(torch, s, shots)
+        acc = torch.zeros(1, dtype=torch.int64, device="cuda")
+        for seed in (1, 2):
+            s.sample_device(shots, seed, out, accepted=acc)
+        torch.cuda.synchronize()
+        print(f"accepted {int(acc.item())} of {shots}", flush=True)
+    print(f"profile {which} done", flush=True)
+    return 0
 
 
 def ctypes_stream(stream) -> int:

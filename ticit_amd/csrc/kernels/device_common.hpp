@@ -15,6 +15,31 @@
 
 namespace ticit {
 
+// Program tables never change while a kernel runs. Read through the
+// constant address space, a load at a wave-uniform index becomes a scalar
+// load (SGPRs, the scalar cache) instead of one vector load per lane.
+template <class T>
+using ConstPtr = const __attribute__((address_space(4))) T*;
+template <class T>
+__device__ __forceinline__ ConstPtr<T> konst(const T* p) {
+  return (ConstPtr<T>)p;
+}
+
+// A whole table entry from constant memory, word by word (scalar loads at a
+// uniform address), into registers.
+template <class T>
+__device__ __forceinline__ T kload(ConstPtr<T> p) {
+  static_assert(sizeof(T) % 4 == 0, "table entries are whole words");
+  constexpr int N = int(sizeof(T) / 4);
+  const auto w = (const __attribute__((address_space(4))) uint32_t*)p;
+  uint32_t tmp[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) tmp[k] = w[k];
+  T out;
+  __builtin_memcpy(&out, tmp, sizeof(T));
+  return out;
+}
+
 constexpr uint64_t kExoSeedBase = 0x7eed0000ull;
 constexpr uint64_t kBranchSeedBase = 0x5eed1234ull;
 constexpr double kInvSqrt2 = 0.70710678118654752440;

@@ -41,9 +41,56 @@ constexpr int kProjNarrow = 2;
 // first barrier, and shared.
 constexpr int group_bit_copies(int tps) { return tps < 64 ? 1 : tps / 64; }
 
-__device__ __forceinline__ double group_wave_sum(double v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+// One DPP lane move of a double (both halves the same pattern).
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double read_lane(double v, int lane) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                          __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+
+// Sum over each group of N consecutive lanes (N = 8, 16, 32 or 64), left in
+// every lane of the group, in a fixed order: pairs of lanes combined by DPP
+// moves inside rows of 16 (xor 1, xor 2, mirror of 8, mirror of 16), then
+// rows by a shuffle (N = 32) or, for a whole wave, the four row sums read
+// as scalars and added as (r0 + r1) + (r2 + r3). IEEE addition commutes, so
+// both lanes of a pair hold the same value at every step.
+template <int N>
+__device__ __forceinline__ double group_lane_sum(double v) {
+  v += dpp_move<0xB1>(v);   // quad_perm(1, 0, 3, 2)
+  v += dpp_move<0x4E>(v);   // quad_perm(2, 3, 0, 1)
+  v += dpp_move<0x141>(v);  // row_half_mirror
+  if constexpr (N >= 16) v += dpp_move<0x140>(v);  // row_mirror
+  if constexpr (N == 32) v += __shfl_xor(v, 16, 64);
+  if constexpr (N == 64) v = (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
   return v;
+}
+
+// Index arithmetic of a group-tier state (at most 2^13 amplitudes per
+// component): 32-bit forms of insert_zero_bit and parity.
+__device__ __forceinline__ uint32_t ins0(uint32_t v, int bit) {
+  const uint32_t low = (1u << bit) - 1u;
+  return (v & low) | ((v & ~low) << 1);
+}
+__device__ __forceinline__ bool par32(uint32_t v) { return __popc(v) & 1; }
+
+// Orders the shot group's LDS accesses. A group of at most 64 threads is
+// one wave: its lanes run in lockstep and its LDS operations complete in
+// order, so only the compiler must not move accesses across this point.
+template <int TPS>
+__device__ __forceinline__ void group_sync() {
+  if constexpr (TPS <= 64) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
 }
 
 // Bytes of LDS one shot needs.
@@ -86,7 +133,7 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
     re0[a] = a == 0 ? 1.0 : 0.0;
     im0[a] = 0.0;
   }
-  __syncthreads();
+  group_sync<TPS>();
   if (t == 0 && valid) {
     auto toggle = [&](int32_t c) {
       for (uint32_t j = P.sym_off[c - 1]; j < P.sym_off[c]; ++j) {
@@ -105,17 +152,25 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
                        [&](uint32_t part) { pbits[part >> 6] ^= 1ull << (part & 63); });
     }
   }
-  __syncthreads();
+  group_sync<TPS>();
 
   auto bit_of = [&](const uint64_t* tab, uint32_t idx) -> bool { return (tab[idx >> 6] >> (idx & 63)) & 1; };
+  // an LDS atomic without return: no read-modify-write round trip
   auto set_bit = [&](uint64_t* tab, uint32_t idx, bool v) {
-    uint64_t m = 1ull << (idx & 63);
-    tab[idx >> 6] = v ? (tab[idx >> 6] | m) : (tab[idx >> 6] & ~m);
+    const unsigned long long m = 1ull << (idx & 63);
+    unsigned long long* w = reinterpret_cast<unsigned long long*>(tab + (idx >> 6));
+    if (v) atomicOr(w, m);
+    else atomicAnd(w, ~m);
   };
+  // program tables, read with scalar loads where the index is uniform
+  const auto steps = konst(P.steps);
+  const auto instrs = konst(P.instrs);
+  const auto resid = konst(P.resid);
+  const auto det_recs = konst(P.det_recs);
   auto eval = [&](const DevStep& st) -> bool {
     bool b = (st.op & kFlagConstant) != 0;
     if (st.partial >= 0) b ^= bit_of(pbits, uint32_t(st.partial));
-    for (uint32_t j = 0; j < st.resid_len; ++j) b ^= bit_of(sbits, P.resid[st.resid_off + j]);
+    for (uint32_t j = 0; j < st.resid_len; ++j) b ^= bit_of(sbits, resid[st.resid_off + j]);
     return b;
   };
 
@@ -127,9 +182,9 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
   double* expv = (L.expv && valid && t == 0) ? L.expv + local * uint64_t(P.nexp) : nullptr;
   for (int i = 0; i < P.ninstr; ++i) {
     // the compact step drives the loop; amplitude work reads the full record
-    const DevStep st = P.steps[i];
+    const DevStep st = kload(steps + i);
     const uint32_t op = st.op & 0xff;
-    const DevInstr& in = P.instrs[op == kOpCoin ? 0 : st.full];  // fields read where used
+    const DevInstr in = kload(instrs + (op == kOpCoin ? 0 : st.full));
     // every amplitude instruction acts on one component: 2^nq amplitudes at
     // comp_off (the whole state, offset 0, when the state is dense)
     double* re = re0 + in.comp_off;
@@ -138,9 +193,10 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
       const bool sign = eval(st);
       const int dim = 1 << in.nq;
       const double c = in.c;
+      const uint32_t xm = uint32_t(in.xmask), zm = uint32_t(in.zmask);
       if (in.op & kFlagDiagonal) {
         for (int b = t; b < dim; b += TPS) {
-          bool flip = sign != parity64(uint64_t(b) & in.zmask);
+          bool flip = sign != par32(uint32_t(b) & zm);
           double fr = c + (flip ? -in.me_re : in.me_re), fi = flip ? -in.me_im : in.me_im;
           double r = re[b], m = im[b];
           re[b] = fr * r - fi * m;
@@ -151,7 +207,7 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
         const bool xz_odd = in.op & kFlagXzOdd;
         const double q = sign ? -in.me_im : in.me_im;
         for (int idx = t; idx < (dim >> 1); idx += TPS) {
-          int i0 = int(insert_zero_bit(size_t(idx), in.pivot)), i1 = i0 ^ int(in.xmask);
+          const uint32_t i0 = ins0(uint32_t(idx), in.pivot), i1 = i0 ^ xm;
           double r0 = re[i0], m0 = im[i0], r1 = re[i1], m1 = im[i1];
           if (uni) {
             re[i0] = c * r0 - q * m1;
@@ -159,7 +215,7 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
             re[i1] = c * r1 - q * m0;
             im[i1] = c * m1 + q * r0;
           } else {
-            bool lodd = parity64(uint64_t(i0) & in.zmask);
+            bool lodd = par32(i0 & zm);
             double ld = (sign != lodd) ? -1.0 : 1.0;
             double rd = xz_odd ? -ld : ld;
             double lre = ld * in.me_re, lim = ld * in.me_im, rre = rd * in.me_re, rim = rd * in.me_im;
@@ -170,7 +226,7 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
           }
         }
       }
-      __syncthreads();
+      group_sync<TPS>();
     } else if (op == kOpPromote) {
       const bool sign = eval(st);
       const double c = in.c, q = sign ? in.s : -in.s;
@@ -193,7 +249,7 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
           im[dim + b] = q * r;
         }
       }
-      __syncthreads();
+      group_sync<TPS>();
     } else if (op == kOpMerge) {
       // src/sampler/batch/mod.rs:597-680: target (x) source into the target's
       // storage. Outputs with a non-zero source index lie above the target's
@@ -208,18 +264,18 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
         re[o] = a * cr - b * ci;
         im[o] = a * ci + b * cr;
       }
-      __syncthreads();
+      group_sync<TPS>();
       for (int o = t; o < td; o += TPS) {
         const double a = re[o], b = im[o], cr = sr[0], ci = si[0];
         re[o] = a * cr - b * ci;
         im[o] = a * ci + b * cr;
       }
-      __syncthreads();
+      group_sync<TPS>();
     } else if (op == kOpRecord || op == kOpDetector) {
       bool b = false;
       if (op == kOpDetector && (st.op & kFlagDetRecords)) {
         b = (st.op & kFlagConstant) != 0;
-        for (int32_t j = 0; j < st.c; ++j) b ^= bit_of(rbits, P.det_recs[st.b + j]);
+        for (int32_t j = 0; j < st.c; ++j) b ^= bit_of(rbits, det_recs[st.b + j]);
       } else {
         b = eval(st);
       }
@@ -251,14 +307,22 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
       if (op == kOpMeasure) {
         const int out_dim = (1 << in.nq) >> 1;
         const bool diag = in.op & kFlagDiagonal;
+        const int piv = in.pivot;
+        const uint32_t xm = uint32_t(in.xmask), zm = uint32_t(in.zmask), zw = uint32_t(in.zwop);
+        const uint32_t dph = (in.op & kFlagDiagPhase) ? 1u : 0u;
+        // diagonal_source (src/active.rs:347-356) in 32 bits
+        auto dsrc = [&](uint32_t idx, bool br) {
+          const uint32_t wo = ins0(idx, piv);
+          return wo | ((dph ^ uint32_t(par32(wo & zw)) ^ uint32_t(br)) << piv);
+        };
         double part = 0.0;
         for (int idx = t; idx < out_dim; idx += TPS) {
           if (diag) {
-            int s = int(diagonal_source(in, size_t(idx), true));
+            const uint32_t s = dsrc(uint32_t(idx), true);
             part += re[s] * re[s] + im[s] * im[s];
           } else {
-            int s0 = int(insert_zero_bit(size_t(idx), in.pivot)), s1 = s0 ^ int(in.xmask);
-            double dir = parity64(uint64_t(s0) & in.zmask) ? 1.0 : -1.0;
+            const uint32_t s0 = ins0(uint32_t(idx), piv), s1 = s0 ^ xm;
+            double dir = par32(s0 & zm) ? 1.0 : -1.0;
             double c1r = dir * in.c1_re, c1i = dir * in.c1_im;
             double ar = kInvSqrt2 * re[s0] + c1r * re[s1] - c1i * im[s1];
             double ai = kInvSqrt2 * im[s0] + c1r * im[s1] + c1i * re[s1];
@@ -266,14 +330,12 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
           }
         }
         double pt = 0.0;
-        if constexpr (WPG == 0) {
-          // lanes [g * TPS, (g + 1) * TPS) of the wave: a butterfly inside
-          // the group leaves the sum in each of them
-          for (int off = TPS / 2; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
-          pt = part;
-          __syncthreads();
+        if constexpr (TPS <= 64) {
+          // lanes [g * TPS, (g + 1) * TPS) of one wave: the group's sum in
+          // each of them
+          pt = group_lane_sum<TPS>(part);
         } else {
-          part = group_wave_sum(part);
+          part = group_lane_sum<64>(part);
           if ((tid & 63) == 0) red[tid >> 6] = part;
           __syncthreads();
           for (int w = 0; w < WPG; ++w) pt += red[g * WPG + w];
@@ -283,7 +345,7 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
           // src/sampler/batch/active.rs:818-850: +-(1 - 2 P(true)), no collapse
           if (valid && L.want_probs && t == 0) L.probs[uint64_t(in.ordinal) * L.shots + local] = pt;
           if (expv) expv[st.a] = (eval(st) ? -1.0 : 1.0) * (1.0 - 2.0 * pt);
-          __syncthreads();  // every thread has read `red` before the next reduction
+          group_sync<TPS>();  // every thread has read `red` before the next reduction
           continue;
         }
         if (valid) {
@@ -291,7 +353,7 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
           if (in.op & kFlagForced) {
             // pinned: the XOR of earlier branches, no draw; only a fair coin
             bool f = (in.op & kFlagForcedOne) != 0;
-            for (uint32_t j = 0; j < in.forced_len; ++j) f ^= bit_of(sbits, P.resid[in.forced_off + j]);
+            for (uint32_t j = 0; j < in.forced_len; ++j) f ^= bit_of(sbits, resid[in.forced_off + j]);
             branch = f;
             if (fabs(pt - 0.5) > kFairBranchTolerance) err |= kDevErrUnfairPin;
           } else if (L.injected) {
@@ -309,20 +371,23 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
           ps = 1.0;
         }
         const double inv = 1.0 / sqrt(ps);
+        // staged outputs j = 0, 1, ... while j * TPS < out_dim (a uniform
+        // bound: the loops stop at the state's size, not at PROJ)
         double nr[PROJ], ni[PROJ];
 #pragma unroll
         for (int j = 0; j < PROJ; ++j) {
-          int idx = t + j * TPS;
+          if (j * TPS >= out_dim) break;
+          const int idx = t + j * TPS;
           nr[j] = 0.0;
           ni[j] = 0.0;
           if (idx < out_dim) {
             if (diag) {
-              int s = int(diagonal_source(in, size_t(idx), branch));
+              const uint32_t s = dsrc(uint32_t(idx), branch);
               nr[j] = re[s] * inv;
               ni[j] = im[s] * inv;
             } else {
-              int s0 = int(insert_zero_bit(size_t(idx), in.pivot)), s1 = s0 ^ int(in.xmask);
-              bool odd = parity64(uint64_t(s0) & in.zmask);
+              const uint32_t s0 = ins0(uint32_t(idx), piv), s1 = s0 ^ xm;
+              bool odd = par32(s0 & zm);
               double dir = (branch != odd) ? -1.0 : 1.0;
               double c1r = dir * in.c1_re, c1i = dir * in.c1_im;
               double ar = kInvSqrt2 * re[s0] + c1r * re[s1] - c1i * im[s1];
@@ -332,10 +397,11 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
             }
           }
         }
-        __syncthreads();
+        group_sync<TPS>();
 #pragma unroll
         for (int j = 0; j < PROJ; ++j) {
-          int idx = t + j * TPS;
+          if (j * TPS >= out_dim) break;
+          const int idx = t + j * TPS;
           if (idx < out_dim) {
             re[idx] = nr[j];
             im[idx] = ni[j];
@@ -343,9 +409,10 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
         }
       } else if (valid) {
         if (st.op & kFlagForced) {  // pinned: the XOR of earlier branches, no draw
-          const DevInstr& fin = P.instrs[st.full];
-          bool f = (fin.op & kFlagForcedOne) != 0;
-          for (uint32_t j = 0; j < fin.forced_len; ++j) f ^= bit_of(sbits, P.resid[fin.forced_off + j]);
+          const uint32_t fop = instrs[st.full].op, foff = instrs[st.full].forced_off;
+          const uint32_t flen = instrs[st.full].forced_len;
+          bool f = (fop & kFlagForcedOne) != 0;
+          for (uint32_t j = 0; j < flen; ++j) f ^= bit_of(sbits, resid[foff + j]);
           branch = f;
         } else if (L.injected) {
           const uint64_t words = (L.shots + 63) / 64;
@@ -362,7 +429,7 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
         if (st.a >= 0) set_bit(rbits, uint32_t(st.a), b);
         if (st.b >= 0) set_bit(sbits, uint32_t(st.b), b);
       }
-      if (op == kOpMeasure) __syncthreads();  // the collapsed state before its next reader
+      if (op == kOpMeasure) group_sync<TPS>();  // the collapsed state before its next reader
     }
   }
   if (!valid) return;
