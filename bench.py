@@ -26,7 +26,9 @@ object with the other workloads: C0 (random Clifford+T, group tier), C2
 (256 circuits: one-at-a-time versus one batched graph launch), C3 (dense
 stress: FP64 versus FP32 amplitudes, probability error against the FP64
 oracle), C4 (held-out planner batch: heuristic versus learned layouts,
-one-at-a-time versus batched) and P (a cultivation circuit with every
+one-at-a-time versus batched), CCZ (the reference's own CCZ d05/d07
+benchmark fixtures, run as its benchmark table runs them, next to its
+published CPU and GPU figures) and P (a cultivation circuit with every
 detector postselected and normalized syndromes), each with a bounded
 CPU-oracle sample.
 """
@@ -485,6 +487,38 @@ def other_configs(ticit_amd, w, torch, np, stream, use_cpu=True) -> dict:
             [(blobs[i], 2000, 200) for i in sel], 5,
             f"held-out programs {sel[0]}..{sel[-1]} step 32, 2000 shots each on every core, 200 on one thread")
         out["C4"]["hip_over_cpu"] = out["C4"]["batched_learned_shots_per_s"] / out["C4"]["cpu_baseline"]["value"]
+
+    # The reference's own CCZ benchmark fixtures (testdata/circuits/ccz),
+    # run as its benchmark table runs them (docs/benchmark.md:3-43): E(0.125)
+    # markers rewritten to E(0), counts only (no rows kept), detectors and
+    # observables normalized to the reference sample, no postselection.
+    published = {"d05": (42.6e3, 118e3), "d07": (15.0e3, 23.5e3)}
+    out["CCZ"] = {"workload": "CCZ direct-T fixtures d05/d07 p=1e-3 (834/2254 qubits, MPP, 28 EXP_VAL probes), "
+                              "E(0.125) -> E(0), keep_records=false, normalized syndromes",
+                  "published_source": "ticit docs/benchmark.md:19-20 (CPU, 1 thread, Xeon Platinum 8260), "
+                                      ":36-43 (GPU, RTX 4090 D)"}
+    for name, (pub_cpu, pub_gpu) in published.items():
+        path = os.path.join(ROOT, "tests", "golden", "circuits", f"ccz_{name}_p1e-3.clifft")
+        with open(path) as f:
+            cz = ticit_amd.Circuit(f.read().replace("E(0.125)", "E(0)"))
+        sz = cz.compile(normalize_syndromes=True)
+        nz = 100_000
+        bz = (torch.empty((1, 1), dtype=torch.uint8, device="cuda:0"),) * 3 + \
+             (torch.zeros(max(1, sz.num_observables), dtype=torch.int64, device="cuda:0"),)
+        t = _time(torch, stream, lambda r: sz.sample_device(nz, 40 + r, bz, stream=st, keep_records=False), reps=3)
+        row = {"tier": sz.tier, "max_k": sz.max_active_qubits, "shots_per_s": nz / t,
+               "published_cpu_1thread": pub_cpu, "published_gpu_rtx4090d": pub_gpu,
+               "over_published_gpu": nz / t / pub_gpu, "over_published_cpu": nz / t / pub_cpu}
+        if use_cpu:
+            O.lib(fast="native")
+            O.set_expected(*O.reference_sample(cz.blob))
+            try:
+                row["cpu_baseline"] = cpu_rows(cz.blob, 16384 if name == "d05" else 4096, 2048 if name == "d05" else 512,
+                                               9, f"the {name} fixture, counts only", keep_records=False)
+            finally:
+                O.set_expected(None, None)
+            row["hip_over_cpu"] = row["shots_per_s"] / row["cpu_baseline"]["value"]
+        out["CCZ"][name] = row
 
     # Postselection: the cultivation fixture as the reference's benchmark
     # driver runs it (normalize_syndromes, a postselection mask), every

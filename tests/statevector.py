@@ -8,6 +8,8 @@ a few Pauli-noise sites as an exact mixture over noise and branch outcomes.
 """
 from __future__ import annotations
 
+import math
+
 import numpy as np
 
 _H = np.array([[1, 1], [1, -1]], dtype=complex) / np.sqrt(2)
@@ -212,33 +214,92 @@ def _pauli_terms(op: str, p: float, targets: list[int]) -> list[list[tuple[float
     return sites
 
 
+def _product(tokens, n):
+    """[(qubit, axis)] and sign of a Pauli product written as tokens such as
+    ``X0*!Z1`` (one token) or ``X0 Z1`` (one factor per token)."""
+    ops, sign = [], 1.0
+    for tok in tokens:
+        for f in tok.split("*"):
+            while f.startswith("!"):
+                sign = -sign
+                f = f[1:]
+            ops.append((int(f[1:]), f[0].upper()))
+    return ops, sign
+
+
+def _apply_ops(psi, n, ops):
+    for q, a in ops:
+        psi = apply1(psi, n, q, _PAULI[a])
+    return psi
+
+
+def _pauli_channel_terms(k: int, probs: list, qubits: list) -> list:
+    """PAULI_CHANNEL_k per target group: probabilities in the order of the
+    axis codes (I, X, Y, Z per target), the first target's code the most
+    significant."""
+    axes = [None, "X", "Y", "Z"]
+    sites = []
+    for i in range(0, len(qubits), k):
+        grp = qubits[i:i + k]
+        terms = [(1.0 - sum(probs), [])]
+        for code in range(1, 4 ** k):
+            ops, rem = [], code
+            for pos in reversed(range(k)):
+                if axes[rem & 3]:
+                    ops.append((grp[pos], axes[rem & 3]))
+                rem >>= 2
+            terms.append((probs[code - 1], ops))
+        sites.append(terms)
+    return sites
+
+
 def record_distribution(text: str, max_noise_sites: int = 3) -> np.ndarray:
     """Exact probability of every measurement-record value (bit j = record j)
-    of a small circuit with mid-circuit M, R, MR, Pauli-error channels
-    (X/Y/Z_ERROR, DEPOLARIZE1, DEPOLARIZE2) and unitary gates, as a mixture:
-    every noise site splits each branch by its Pauli terms, every
-    measurement by its Born probabilities. Detector and observable lines are
-    ignored. At most `max_noise_sites` noise sites (one per target, or per
-    target pair for DEPOLARIZE2)."""
+    of a small circuit, as a mixture: every noise site splits each branch by
+    its Pauli terms, every measurement by its Born probabilities.
+
+    Supported: one-qubit gates and CX / CZ; M, MR, R; Pauli-product
+    measurements MPP, MXX, MYY, MZZ; Pauli rotations exp(-i a P) for SPP
+    (a = pi/4), SPP_DAG (-pi/4) and R_PAULI (half turns * pi/2); MPAD;
+    X/Y/Z_ERROR, DEPOLARIZE1 (p/3 per Pauli), DEPOLARIZE2 (p/15 per pair),
+    PAULI_CHANNEL_1/2, E / ELSE_CORRELATED_ERROR chains (alternative i fires
+    with p_i times the product of (1 - p_j) over earlier ones), and
+    HERALDED_ERASE / HERALDED_PAULI_CHANNEL_1 (a herald record, then I, X,
+    Y or Z). Detector and observable lines are ignored. At most
+    `max_noise_sites` noise sites (one per target, target group or chain)."""
     lines = [l.split("#")[0].strip() for l in text.splitlines()]
     lines = [l for l in lines if l and l != "TICK"]
     n = 1
     for l in lines:
         head = l.split()[0]
-        if head.startswith(("DETECTOR", "OBSERVABLE_INCLUDE")):
+        if head.startswith(("DETECTOR", "OBSERVABLE_INCLUDE", "MPAD")):
             continue
         for tok in l.split()[1:]:
-            if tok.isdigit():
-                n = max(n, int(tok) + 1)
+            for f in tok.split("*"):
+                f = f.lstrip("!")
+                if f[:1].upper() in ("X", "Y", "Z"):
+                    f = f[1:]
+                if f.isdigit():
+                    n = max(n, int(f) + 1)
     psi0 = np.zeros(2 ** n, dtype=complex)
     psi0[0] = 1.0
-    branches = [(1.0, psi0, 0, 0)]  # (weight, normalized state, record bits, record count)
-    nsites = 0
+    state = {"branches": [(1.0, psi0, 0, 0)], "sites": 0}  # (weight, normalized state, records, count)
     idx = np.arange(2 ** n)
 
-    def measure(brs, q, reset):
+    def site():
+        state["sites"] += 1
+        if state["sites"] > max_noise_sites:
+            raise ValueError("too many noise sites for an exact mixture")
+
+    def noise(sites_terms):
+        for terms in sites_terms:
+            site()
+            state["branches"] = [(w * pt, _apply_ops(psi, n, ops), rec, nr)
+                                 for w, psi, rec, nr in state["branches"] for pt, ops in terms if pt > 0.0]
+
+    def measure_z(q, reset, record=True):
         out = []
-        for w, psi, rec, nr in brs:
+        for w, psi, rec, nr in state["branches"]:
             one = ((idx >> q) & 1) == 1
             p1 = float(np.sum(np.abs(psi[one]) ** 2))
             for bit, pb in ((0, 1.0 - p1), (1, p1)):
@@ -247,49 +308,103 @@ def record_distribution(text: str, max_noise_sites: int = 3) -> np.ndarray:
                 phi = np.where(one if bit else ~one, psi, 0) / np.sqrt(pb)
                 if reset and bit:
                     phi = apply1(phi, n, q, GATES1["X"])
-                out.append((w * pb, phi, rec | (bit << nr), nr + 1))
-        return out
+                out.append((w * pb, phi, rec | (bit << nr) if record else rec, nr + 1 if record else nr))
+        state["branches"] = out
+
+    def measure_product(ops, sign):
+        out = []
+        for w, psi, rec, nr in state["branches"]:
+            ppsi = sign * _apply_ops(psi, n, ops)
+            for bit in (0, 1):
+                proj = 0.5 * (psi + (ppsi if bit == 0 else -ppsi))
+                pb = float(np.vdot(proj, proj).real)
+                if pb <= 1e-15:
+                    continue
+                out.append((w * pb, proj / np.sqrt(pb), rec | (bit << nr), nr + 1))
+        state["branches"] = out
+
+    chain = []  # open correlated-error alternatives: (p, ops)
+
+    def flush_chain():
+        if chain:
+            terms, rem = [], 1.0
+            for p, ops in chain:
+                terms.append((rem * p, ops))
+                rem *= 1.0 - p
+            terms.append((rem, []))
+            chain.clear()
+            noise([terms])
 
     for l in lines:
         parts = l.split()
         head = parts[0]
+        op, _, arg = head.partition("(")
+        args = [float(v) for v in arg.rstrip(")").split(",")] if arg else []
+        if op in ("E", "CORRELATED_ERROR", "ELSE_CORRELATED_ERROR"):
+            if op != "ELSE_CORRELATED_ERROR":
+                flush_chain()
+            chain.append((args[0], _product(parts[1:], n)[0]))
+            continue
+        flush_chain()
         if head.startswith(("DETECTOR", "OBSERVABLE_INCLUDE")):
             continue
-        op, _, arg = head.partition("(")
-        targets = [int(t) for t in parts[1:]]
-        if op in ("M", "MR"):
-            for q in targets:
-                branches = measure(branches, q, op == "MR")
+        if op in ("MPP", "MXX", "MYY", "MZZ"):
+            if op == "MPP":
+                prods = [_product([t], n) for t in parts[1:]]
+            else:
+                prods = []
+                for a, b in zip(parts[1::2], parts[2::2]):
+                    ops, _ = _product([op[1] + a.lstrip("!"), op[1] + b.lstrip("!")], n)
+                    prods.append((ops, -1.0 if a.startswith("!") != b.startswith("!") else 1.0))
+            for ops, sign in prods:
+                measure_product(ops, sign)
+        elif op in ("SPP", "SPP_DAG", "R_PAULI"):
+            ang = math.pi / 4 if op == "SPP" else -math.pi / 4 if op == "SPP_DAG" else args[0] * math.pi / 2
+            for t in parts[1:]:
+                ops, sign = _product([t], n)
+                state["branches"] = [(w, math.cos(ang) * psi - 1j * math.sin(ang) * sign * _apply_ops(psi, n, ops),
+                                      rec, nr) for w, psi, rec, nr in state["branches"]]
+        elif op == "MPAD":
+            # each literal (or its inversion) a record, flipped with the
+            # optional probability
+            pf = args[0] if args else 0.0
+            for t in parts[1:]:
+                bit = int(t.lstrip("!")) ^ int(t.startswith("!"))
+                if pf > 0.0:
+                    site()
+                state["branches"] = [(w * pb, psi, rec | ((bit ^ f) << nr), nr + 1)
+                                     for w, psi, rec, nr in state["branches"]
+                                     for f, pb in ((0, 1.0 - pf), (1, pf)) if pb > 0.0]
+        elif op in ("PAULI_CHANNEL_1", "PAULI_CHANNEL_2"):
+            noise(_pauli_channel_terms(1 if op == "PAULI_CHANNEL_1" else 2, args, [int(t) for t in parts[1:]]))
+        elif op in ("HERALDED_ERASE", "HERALDED_PAULI_CHANNEL_1"):
+            probs = [args[0] / 4.0] * 4 if op == "HERALDED_ERASE" else args
+            for t in parts[1:]:
+                q = int(t)
+                site()
+                out = []
+                for w, psi, rec, nr in state["branches"]:
+                    if 1.0 - sum(probs) > 0.0:
+                        out.append((w * (1.0 - sum(probs)), psi, rec, nr + 1))
+                    for pb, ax in zip(probs, [None, "X", "Y", "Z"]):
+                        if pb > 0.0:
+                            out.append((w * pb, _apply_ops(psi, n, [(q, ax)] if ax else []), rec | (1 << nr), nr + 1))
+                state["branches"] = out
+        elif op in ("M", "MR"):
+            for t in parts[1:]:
+                measure_z(int(t), op == "MR")
         elif op == "R":
-            for q in targets:
-                # reset = measure without a record, then flip a 1 back to 0
-                out = []
-                for w, psi, rec, nr in branches:
-                    for w2, phi, _, _ in measure([(w, psi, 0, 0)], q, True):
-                        out.append((w2, phi, rec, nr))
-                branches = out
+            for t in parts[1:]:
+                measure_z(int(t), True, record=False)
         elif op in _NOISE1 or op in ("DEPOLARIZE1", "DEPOLARIZE2"):
-            p = float(arg.rstrip(")"))
-            for terms in _pauli_terms(op, p, targets):
-                nsites += 1
-                if nsites > max_noise_sites:
-                    raise ValueError("too many noise sites for an exact mixture")
-                out = []
-                for w, psi, rec, nr in branches:
-                    for pt, ops in terms:
-                        if pt <= 0.0:
-                            continue
-                        phi = psi
-                        for q, a in ops:
-                            phi = apply1(phi, n, q, _PAULI[a])
-                        out.append((w * pt, phi, rec, nr))
-                branches = out
-        elif op in GATES1:
-            branches = [(w, psi_apply(psi, n, op, targets), rec, nr) for w, psi, rec, nr in branches]
-        elif op in ("CX", "CNOT", "CZ"):
-            branches = [(w, psi_apply(psi, n, op, targets), rec, nr) for w, psi, rec, nr in branches]
+            noise(_pauli_terms(op, args[0], [int(t) for t in parts[1:]]))
+        elif op in GATES1 or op in ("CX", "CNOT", "CZ"):
+            state["branches"] = [(w, psi_apply(psi, n, op, [int(t) for t in parts[1:]]), rec, nr)
+                                 for w, psi, rec, nr in state["branches"]]
         else:
             raise ValueError(f"unsupported instruction {op}")
+    flush_chain()
+    branches = state["branches"]
     nrec = branches[0][3] if branches else 0
     dist = np.zeros(2 ** nrec)
     for w, _, rec, nr in branches:

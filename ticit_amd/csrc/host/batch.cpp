@@ -132,7 +132,7 @@ constexpr uint64_t kBatchGroupMaxShots = 50000;
 
 bool group_in_batch(const ticit_hip_program* h, uint64_t shots, int lds_limit) {
   const HostTables& t = h->tab;
-  return h->opt.tier == 0 && h->opt.precision == 0 && shots <= kBatchGroupMaxShots &&
+  return h->opt.tier == 0 && h->opt.precision == 0 && !h->bit_heavy && shots <= kBatchGroupMaxShots &&
          t.max_local_k <= group_max_local_k(256) &&
          group_tier_lds_bytes(t.state_dim, h->group, 256) <= size_t(lds_limit);
 }

@@ -179,21 +179,102 @@ struct Clifford {
     }
     return out;
   }
-  // Decomposition of p over the rows: preimage(result) == p exactly.
-  Pauli coordinates(const Pauli& p) const {
-    Pauli out(n);
-    for (int q = 0; q < n; ++q) {
-      if (anticommutes(p, rows[q])) out.setz(q, true);
-      if (anticommutes(p, rows[n + q])) out.setx(q, true);
+  // Rows whose body is not the identity tableau's (X_q, Z_q). For the others
+  // the anticommutation tests of coordinates() are p's own bits, so a frame
+  // that moves few rows costs few tests per Pauli.
+  mutable std::vector<int> moved;
+  // the moved rows' nonzero words (word, x, z), moved_off[i] .. [i + 1]:
+  // anticommutation tests read only those
+  struct RowWord {
+    uint32_t w;
+    uint64_t x, z;
+  };
+  mutable std::vector<RowWord> moved_words;
+  mutable std::vector<uint32_t> moved_off;
+  // qubits whose X or Z row differs from the identity's in body or phase
+  mutable std::vector<uint64_t> touched;
+  // touched qubits and the supports of the moved rows: coordinates() leaves
+  // every Pauli outside them unchanged
+  mutable std::vector<uint64_t> footprint;
+  mutable bool moved_ready = false;
+  bool anticommutes_moved(const Pauli& p, size_t i) const {
+    uint32_t c = 0;
+    for (uint32_t j = moved_off[i]; j < moved_off[i + 1]; ++j) {
+      const RowWord& rw = moved_words[j];
+      c += uint32_t(__builtin_popcountll(p.x[rw.w] & rw.z) + __builtin_popcountll(p.z[rw.w] & rw.x));
     }
+    return c & 1;
+  }
+  void index_moved() const {
+    moved.clear();
+    moved_words.clear();
+    moved_off.assign(1, 0);
+    touched.assign(size_t((n + 63) / 64), 0);
+    for (int r = 0; r < 2 * n; ++r) {
+      const Pauli& row = rows[size_t(r)];
+      const int q = r < n ? r : r - n;
+      bool trivial = true;
+      for (size_t w = 0; w < row.x.size() && trivial; ++w) {
+        const uint64_t bit = (size_t(q) >> 6) == w ? (1ull << (q & 63)) : 0ull;
+        trivial = row.x[w] == (r < n ? bit : 0ull) && row.z[w] == (r < n ? 0ull : bit);
+      }
+      if (!trivial) {
+        moved.push_back(r);
+        for (size_t w = 0; w < row.x.size(); ++w)
+          if (row.x[w] | row.z[w]) moved_words.push_back(RowWord{uint32_t(w), row.x[w], row.z[w]});
+        moved_off.push_back(uint32_t(moved_words.size()));
+      }
+      if (!trivial || row.phase != 0) touched[size_t(q) >> 6] |= 1ull << (q & 63);
+    }
+    footprint = touched;
+    for (const RowWord& rw : moved_words) footprint[rw.w] |= rw.x | rw.z;
+    moved_ready = true;
+  }
+  // Decomposition of p over the rows: preimage(result) == p exactly. Bit q
+  // of the result's z (x) part is whether p anticommutes with row q (n + q).
+  Pauli coordinates(const Pauli& p) const {
+    if (!moved_ready) index_moved();
+    Pauli out(n);
+    out.x = p.x;  // anticommutes(p, Z_q) = x_q(p)
+    out.z = p.z;  // anticommutes(p, X_q) = z_q(p)
+    for (size_t i = 0; i < moved.size(); ++i) {
+      const int r = moved[i];
+      const bool a = anticommutes_moved(p, i);
+      if (r < n) out.setz(r, a);
+      else out.setx(r - n, a);
+    }
+    // Unchanged bits over qubits whose rows are all the identity's: the
+    // preimage is p itself, phase included (X_q Z_q multiply to the XZ form
+    // with phase 0).
+    bool same = true;
+    for (size_t w = 0; w < out.x.size() && same; ++w)
+      same = out.x[w] == p.x[w] && out.z[w] == p.z[w] && ((p.x[w] | p.z[w]) & touched[w]) == 0;
+    if (same) return p;
     Pauli rec = preimage(out);
     if (!rec.same_body(p)) throw TicitError(4, "frame rows do not span the Pauli body");
     out.set_phase(p.phase - rec.phase);
     return out;
   }
-  void swap_rows(int a, int b) { std::swap(rows[a], rows[b]); }
+  // p <- coordinates(p), without a copy when p comes out unchanged (no
+  // moved row flips one of its bits and its support meets no moved row).
+  void coordinates_into(Pauli& p) const {
+    if (!moved_ready) index_moved();
+    bool change = false;
+    for (size_t w = 0; w < p.x.size() && !change; ++w) change = ((p.x[w] | p.z[w]) & touched[w]) != 0;
+    for (size_t i = 0; i < moved.size() && !change; ++i) {
+      const int r = moved[i];
+      const bool cur = r < n ? p.zb(r) : p.xb(r - n);
+      change = anticommutes_moved(p, i) != cur;
+    }
+    if (change) p = coordinates(p);
+  }
+  void swap_rows(int a, int b) {
+    moved_ready = false;
+    std::swap(rows[a], rows[b]);
+  }
   void add_phase(int r, int d) { rows[r].set_phase(rows[r].phase + d); }
   void mul_rows(int dst, int lhs, int rhs, int extra) {
+    moved_ready = false;
     Pauli o = pmul(rows[lhs], rows[rhs]);
     o.set_phase(o.phase + extra);
     rows[dst] = std::move(o);
@@ -316,6 +397,10 @@ enum class K {
   Tick, H, S, SDag, SqrtX, SqrtXDag, SqrtY, SqrtYDag, X, Y, Z, CX, CY, CZ, Swap,
   T, TDag, Rot, MZ, MX, MY, MRZ, MRX, MRY, RZ, RX, RY, MPP, XErr, YErr, ZErr, Dep1, Dep2,
   FeedbackX, FeedbackY, FeedbackZ, ExpVal,
+  PauliChannel,    // PAULI_CHANNEL_1/2/3 and DEPOLARIZE3: `arity` targets per call, `probs`
+  ProductChannel,  // a closed E / ELSE_CORRELATED_ERROR chain: absolute `probs` per product
+  Herald,          // HERALDED_ERASE / HERALDED_PAULI_CHANNEL_1: probs (I, X, Y, Z) after a herald
+  MPad,            // MPAD: mtargets hold the literal pad values
 };
 
 struct MTarget {
@@ -338,6 +423,8 @@ struct CInstr {
   double angle = 0.0;
   int feedback_record = 0;  // classically controlled Pauli: record id
   int exp_base = 0;         // EXP_VAL: slot of the first product
+  int arity = 1;            // PauliChannel: targets per channel call
+  std::vector<double> probs;  // channel probabilities (see K)
 };
 
 struct CDetector {
@@ -376,6 +463,10 @@ std::string upper(std::string s) {
   for (auto& c : s) c = char(std::toupper(static_cast<unsigned char>(c)));
   return s;
 }
+
+// Marker line that closes an open correlated-error chain (no instruction
+// name can contain a space).
+const char* const kFlushChain = "END OF BLOCK";
 
 // Reads lines into a flat list, expanding REPEAT blocks.
 void read_block(const std::vector<std::string>& lines, size_t& pos, int depth,
@@ -435,7 +526,16 @@ void read_block(const std::vector<std::string>& lines, size_t& pos, int depth,
       std::vector<RawLine> body;
       read_block(lines, pos, depth + 1, body);
       (void)body_start;
-      for (long r = 0; r < count; ++r) out.insert(out.end(), body.begin(), body.end());
+      // a correlated-error chain never straddles a REPEAT block or one of
+      // its repetitions (src/circuit/parser.rs:1590-1610)
+      RawLine flush;
+      flush.name = kFlushChain;
+      flush.line = lineno;
+      out.push_back(flush);
+      for (long r = 0; r < count; ++r) {
+        out.insert(out.end(), body.begin(), body.end());
+        out.push_back(flush);
+      }
       continue;
     }
     // MPP-style products may be written "X0 * Z1": glue '*' to neighbours.
@@ -487,7 +587,7 @@ Circuit parse_circuit(const std::string& text) {
   int maxq = -1;
   for (const auto& rl : raw) {
     if (rl.name == "DETECTOR" || rl.name == "OBSERVABLE_INCLUDE" || rl.name == "DISCARD" ||
-        rl.name == "SHIFT_COORDS" || rl.name == "TICK")
+        rl.name == "SHIFT_COORDS" || rl.name == "TICK" || rl.name == "MPAD" || rl.name == kFlushChain)
       continue;
     for (const auto& t : rl.targets) {
       std::string body = t;
@@ -575,10 +675,69 @@ Circuit parse_circuit(const std::string& text) {
       {"CZ", K::CZ}, {"ZCZ", K::CZ}, {"SWAP", K::Swap},
   };
 
+  // E / ELSE_CORRELATED_ERROR chains (src/circuit/parser.rs:987-1021): the
+  // alternatives accumulate with absolute probabilities p_i * prod_{j<i}
+  // (1 - p_j) and close into one channel at the next other instruction.
+  std::vector<Product> chain_products;
+  std::vector<double> chain_probs;
+  double chain_remaining = 1.0;
+  auto flush_chain = [&](int line) {
+    chain_remaining = 1.0;
+    if (chain_products.empty()) return;
+    CInstr e;
+    e.kind = K::ProductChannel;
+    e.line = line;
+    e.products = std::move(chain_products);
+    e.probs = std::move(chain_probs);
+    chain_products.clear();
+    chain_probs.clear();
+    c.instrs.push_back(std::move(e));
+  };
+  auto required_prob = [](const RawLine& rl) {
+    if (rl.args.size() != 1) parse_error(rl.line, "operation expects one probability argument");
+    double p = rl.args[0];
+    if (!(p >= 0.0 && p <= 1.0)) parse_error(rl.line, "probability must be between 0 and 1");
+    return p;
+  };
+  auto check_prob_list = [](const RawLine& rl, const std::string& msg) {
+    double total = 0.0;
+    for (double p : rl.args) {
+      if (!(p >= 0.0 && p <= 1.0)) parse_error(rl.line, "probability must be between 0 and 1");
+      total += p;
+    }
+    if (total > 1.0 + 1e-12) parse_error(rl.line, msg);
+  };
+
   for (const auto& rl : raw) {
     const std::string& op = rl.name;
     CInstr ci;
     ci.line = rl.line;
+    if (op == "E" || op == "CORRELATED_ERROR" || op == "ELSE_CORRELATED_ERROR") {
+      if (op != "ELSE_CORRELATED_ERROR") {
+        flush_chain(rl.line);
+      } else if (chain_products.empty()) {
+        parse_error(rl.line, "ELSE_CORRELATED_ERROR must follow CORRELATED_ERROR or ELSE_CORRELATED_ERROR");
+      }
+      const double p = required_prob(rl);
+      // the implicit product of the targets, one Pauli factor each
+      Product pr;
+      pr.pauli = Pauli(n);
+      for (const auto& t : rl.targets) {
+        if (t.find('*') != std::string::npos)
+          parse_error(rl.line, "implicit Pauli products do not use combiner targets");
+        RawLine one = rl;
+        one.targets = {t};
+        Product f = products_of(one)[0];
+        mul_into(pr.pauli, f.pauli);
+        pr.inverted = pr.inverted != f.inverted;
+      }
+      chain_products.push_back(pr);
+      chain_probs.push_back(chain_remaining * p);
+      chain_remaining *= 1.0 - p;
+      continue;
+    }
+    flush_chain(rl.line);
+    if (op == kFlushChain) continue;
     if (op == "TICK") {
       ++c.ticks;
       continue;
@@ -676,6 +835,85 @@ Circuit parse_circuit(const std::string& text) {
       ci.kind = op == "X_ERROR" ? K::XErr : op == "Y_ERROR" ? K::YErr : K::ZErr;
       ci.probability = one_prob(rl);
       ci.qubits = qubits_of(rl);
+    } else if (op == "MXX" || op == "MYY" || op == "MZZ") {
+      // src/circuit/parser.rs:1488-1497,741-790: consecutive target pairs,
+      // the product's sign the XOR of the pair's '!' flags
+      ci.kind = K::MPP;
+      ci.probability = opt_prob(rl);
+      if (rl.targets.size() % 2) parse_error(rl.line, "pair measurement requires paired targets");
+      for (size_t i = 0; i < rl.targets.size(); i += 2) {
+        std::string a = rl.targets[i], b = rl.targets[i + 1];
+        bool ia = false, ib = false;
+        if (!a.empty() && a[0] == '!') { ia = true; a = a.substr(1); }
+        if (!b.empty() && b[0] == '!') { ib = true; b = b.substr(1); }
+        const int qa = qubit_target(rl, a), qb = qubit_target(rl, b);
+        if (qa == qb) parse_error(rl.line, "two-qubit Pauli operation requires distinct qubits");
+        auto axis = [&](int q) { return op == "MXX" ? pX(n, q) : op == "MYY" ? pY(n, q) : pZ(n, q); };
+        Product pr;
+        pr.pauli = axis(qa);
+        mul_into(pr.pauli, axis(qb));
+        pr.inverted = ia != ib;
+        ci.products.push_back(pr);
+      }
+      c.nrecords += int(ci.products.size());
+    } else if (op == "SPP" || op == "SPP_DAG") {
+      // src/circuit/parser.rs:1498-1502: kernel angle +-pi/4
+      need_no_args(rl);
+      ci.kind = K::Rot;
+      ci.angle = op == "SPP" ? M_PI / 4.0 : -M_PI / 4.0;
+      ci.products = products_of(rl);
+    } else if (op == "R_PAULI") {
+      if (rl.args.size() != 1) parse_error(rl.line, "R_PAULI expects one angle argument");
+      ci.kind = K::Rot;
+      ci.angle = rl.args[0] * M_PI / 2.0;  // half turns -> kernel angle
+      ci.products = products_of(rl);
+    } else if (op == "I_ERROR" || op == "II_ERROR") {
+      // validated, then nothing happens (src/circuit/parser.rs:1509-1521)
+      check_prob_list(rl, op + " probabilities must be disjoint and sum to at most 1");
+      if (op == "II_ERROR" && qubits_of(rl).size() % 2) parse_error(rl.line, "II_ERROR requires paired targets");
+      (void)qubits_of(rl);
+      continue;
+    } else if (op == "PAULI_CHANNEL_1" || op == "PAULI_CHANNEL_2" || op == "PAULI_CHANNEL_3") {
+      // src/circuit/parser.rs:1529-1549: 4^n - 1 probabilities
+      const int arity = op == "PAULI_CHANNEL_1" ? 1 : op == "PAULI_CHANNEL_2" ? 2 : 3;
+      if (int(rl.args.size()) != (1 << (2 * arity)) - 1)
+        parse_error(rl.line, "PAULI_CHANNEL probability count does not match gate arity");
+      ci.kind = K::PauliChannel;
+      ci.arity = arity;
+      ci.probs = rl.args;
+      ci.qubits = qubits_of(rl);
+    } else if (op == "DEPOLARIZE3") {
+      ci.kind = K::PauliChannel;
+      ci.arity = 3;
+      const double p = one_prob(rl);
+      ci.probs.assign(63, p / 63.0);
+      ci.qubits = qubits_of(rl);
+    } else if (op == "HERALDED_ERASE" || op == "HERALDED_PAULI_CHANNEL_1") {
+      // src/circuit/parser.rs:1551-1572; one herald record per target
+      ci.kind = K::Herald;
+      if (op == "HERALDED_ERASE") {
+        const double p = required_prob(rl) / 4.0;
+        ci.probs.assign(4, p);
+      } else {
+        if (rl.args.size() != 4) parse_error(rl.line, "HERALDED_PAULI_CHANNEL_1 expects four probabilities");
+        for (double p : rl.args)
+          if (!(p >= 0.0 && p <= 1.0)) parse_error(rl.line, "probability must be between 0 and 1");
+        ci.probs = rl.args;
+      }
+      ci.qubits = qubits_of(rl);
+      c.nrecords += int(ci.qubits.size());
+    } else if (op == "MPAD") {
+      ci.kind = K::MPad;
+      ci.probability = opt_prob(rl);
+      for (const auto& t : rl.targets) {
+        MTarget mt;
+        std::string s = t;
+        if (!s.empty() && s[0] == '!') { mt.inverted = true; s = s.substr(1); }
+        mt.qubit = qubit_target(rl, s);
+        if (mt.qubit != 0 && mt.qubit != 1) parse_error(rl.line, "MPAD targets must be 0 or 1");
+        ci.mtargets.push_back(mt);
+      }
+      c.nrecords += int(ci.mtargets.size());
     } else if (op == "DEPOLARIZE1") {
       ci.kind = K::Dep1;
       ci.probability = one_prob(rl);
@@ -706,6 +944,7 @@ Circuit parse_circuit(const std::string& text) {
     }
     c.instrs.push_back(std::move(ci));
   }
+  flush_chain(raw.empty() ? 0 : raw.back().line);
   return c;
 }
 
@@ -787,6 +1026,54 @@ struct Lowering {
     op.sign = pr.second;
     op.exp_val = slot;
     pending.push_back(std::move(op));
+  }
+  // src/factored.rs:624-640: a record whose value is a known expression
+  void classical_record(const SymBool& outcome, int32_t record) {
+    ctx.bump(outcome.max_condition());
+    PendingOp op;
+    op.kind = 2;
+    op.outcome = outcome;
+    op.record = record;
+    pending.push_back(std::move(op));
+  }
+  // src/circuit/lowering.rs:120-128: axis code 1 = X, 2 = Y, 3 = Z
+  static void axis_bits(int code, bool& x, bool& z) {
+    x = code == 1 || code == 2;
+    z = code == 2 || code == 3;
+  }
+  // PAULI_CHANNEL_n on `qs` (src/circuit/lowering.rs:195-242): a 4^n-way
+  // categorical over (x, z) bit pairs, target i's bits at 2i, 2i + 1, its
+  // axis code read from the high end of the case number.
+  void pauli_channel(const std::vector<int>& qs, const std::vector<double>& probs) {
+    for (size_t i = 0; i < qs.size(); ++i)
+      for (size_t j = i + 1; j < qs.size(); ++j)
+        if (qs[i] == qs[j]) throw TicitError(1, "Pauli channel targets must be distinct");
+    const int nq = int(qs.size());
+    const size_t cases = size_t(1) << (2 * nq);
+    if (probs.size() != cases - 1) throw TicitError(1, "Pauli channel probability count does not match target arity");
+    double total = 0.0;
+    for (double p : probs) total += Context::check_p(p);
+    if (total > 1.0 + 1e-12) throw TicitError(1, "Pauli channel probabilities sum to more than 1");
+    std::vector<uint64_t> assign{0};
+    std::vector<double> dist{std::max(0.0, 1.0 - total)};
+    for (size_t code = 1; code < cases; ++code) {
+      uint64_t a = 0;
+      size_t rem = code;
+      for (int pos = nq - 1; pos >= 0; --pos) {
+        bool x, z;
+        axis_bits(int(rem & 3), x, z);
+        rem >>= 2;
+        if (x) a |= uint64_t(1) << (2 * pos);
+        if (z) a |= uint64_t(1) << (2 * pos + 1);
+      }
+      assign.push_back(a);
+      dist.push_back(probs[code - 1]);
+    }
+    auto bits = ctx.fresh_cat(2 * nq, assign, dist);
+    for (int i = 0; i < nq; ++i) {
+      apply_pauli_symbolic(pX(n, qs[size_t(i)]), sym_var(bits[size_t(2 * i)]));
+      apply_pauli_symbolic(pZ(n, qs[size_t(i)]), sym_var(bits[size_t(2 * i + 1)]));
+    }
   }
   SymBool readout_sign(bool inverted, double p) {
     SymBool s;
@@ -901,6 +1188,64 @@ struct Lowering {
         apply_pauli_symbolic(p, records[ci.feedback_record - 1]);
         return;
       }
+      case K::PauliChannel: {
+        if (ci.qubits.size() % size_t(ci.arity))
+          throw TicitError(1, ci.arity == 2 ? "PAULI_CHANNEL_2 requires paired targets"
+                                            : "Pauli channel requires whole groups of targets");
+        for (size_t i = 0; i < ci.qubits.size(); i += size_t(ci.arity))
+          pauli_channel(std::vector<int>(ci.qubits.begin() + long(i), ci.qubits.begin() + long(i) + ci.arity),
+                        ci.probs);
+        return;
+      }
+      case K::ProductChannel: {
+        // src/circuit/lowering.rs:254-292: one alternative per product
+        const size_t m = ci.products.size();
+        double total = 0.0;
+        for (double p : ci.probs) total += Context::check_p(p);
+        if (total > 1.0 + 1e-12) throw TicitError(1, "Pauli product channel probabilities sum to more than 1");
+        if (m > 64) throw TicitError(3, "correlated-error chains of more than 64 alternatives are not supported");
+        std::vector<uint64_t> assign{0};
+        std::vector<double> dist{std::max(0.0, 1.0 - total)};
+        for (size_t i = 0; i < m; ++i) {
+          assign.push_back(uint64_t(1) << i);
+          dist.push_back(ci.probs[i]);
+        }
+        auto bits = ctx.fresh_cat(int(m), assign, dist);
+        for (size_t i = 0; i < m; ++i) {
+          Pauli pp = ci.products[i].pauli;
+          if (ci.products[i].inverted) pp.set_phase(pp.phase + 2);
+          apply_pauli_symbolic(pp, sym_var(bits[i]));
+        }
+        return;
+      }
+      case K::Herald: {
+        // src/circuit/lowering.rs:294-340: (herald, x, z) rows 000, 100,
+        // 110, 111, 101 = no event, herald + I, + X, + Y, + Z; the herald
+        // bit is a record
+        double total = 0.0;
+        for (double p : ci.probs) total += Context::check_p(p);
+        if (total > 1.0 + 1e-12) throw TicitError(1, "heralded channel probabilities sum to more than 1");
+        for (int q : ci.qubits) {
+          auto bits = ctx.fresh_cat(3, {0, 1, 3, 7, 5},
+                                    {1.0 - total, ci.probs[0], ci.probs[1], ci.probs[2], ci.probs[3]});
+          records.push_back(sym_var(bits[0]));
+          classical_record(sym_var(bits[0]), int32_t(records.size()));
+          apply_pauli_symbolic(pX(n, q), sym_var(bits[1]));
+          apply_pauli_symbolic(pZ(n, q), sym_var(bits[2]));
+        }
+        return;
+      }
+      case K::MPad:
+        // src/circuit/lowering.rs:343-366: the literal pad value, optionally
+        // inverted and noisy
+        for (const auto& t : ci.mtargets) {
+          SymBool v;
+          v.constant = (t.qubit != 0) != t.inverted;
+          if (ci.probability != 0.0) v = sym_xor(v, sym_var(ctx.fresh_bern(Context::check_p(ci.probability))));
+          records.push_back(v);
+          classical_record(v, int32_t(records.size()));
+        }
+        return;
       case K::Dep1:
         // src/circuit/lowering.rs:143-159: bits (x, z), rows I, Z, X, Y
         for (int q : ci.qubits) {
@@ -1324,9 +1669,54 @@ struct Planner {
     rel_words.push_back(words);
   }
 
+  // Queued operations by the qubits their Pauli acts on. A frame or a
+  // pushed Pauli can only change operations that share a qubit with it, so
+  // only those are visited. Entries go stale when an operation's Pauli
+  // moves (it is indexed again under its new support) or leaves the queue;
+  // visits re-check the operation itself.
+  std::vector<std::vector<uint32_t>> by_qubit;
+  std::vector<uint32_t> seen;
+  uint32_t visit = 0;
+  std::vector<uint32_t> hits;
+
+  void index_op(uint32_t i) {
+    const Pauli& p = queue[i].pauli;
+    for (size_t w = 0; w < p.x.size(); ++w)
+      for (uint64_t b = p.x[w] | p.z[w]; b; b &= b - 1) by_qubit[w * 64 + size_t(__builtin_ctzll(b))].push_back(i);
+  }
+  void build_index() {
+    by_qubit.assign(size_t(n), {});
+    seen.assign(queue.size(), 0);
+    for (size_t i = head; i < queue.size(); ++i)
+      if (queue[i].kind != 2) index_op(uint32_t(i));
+  }
+  // Queued (not yet planned) operations indexed under a qubit of `mask`,
+  // each once, in ascending queue order.
+  const std::vector<uint32_t>& ops_on(const std::vector<uint64_t>& mask) {
+    ++visit;
+    hits.clear();
+    for (size_t w = 0; w < mask.size(); ++w)
+      for (uint64_t b = mask[w]; b; b &= b - 1) {
+        auto& list = by_qubit[w * 64 + size_t(__builtin_ctzll(b))];
+        size_t keep = 0;
+        for (uint32_t i : list) {
+          if (i < head) continue;  // planned: dropped from the index
+          list[keep++] = i;
+          if (seen[i] == visit) continue;
+          seen[i] = visit;
+          hits.push_back(i);
+        }
+        list.resize(keep);
+      }
+    std::sort(hits.begin(), hits.end());
+    return hits;
+  }
+
   void push_through(const Pauli& p, const SymBool& sign) {
     ctx.bump(sign.max_condition());
-    for (size_t i = head; i < queue.size(); ++i) {
+    std::vector<uint64_t> support(p.x.size());
+    for (size_t w = 0; w < p.x.size(); ++w) support[w] = p.x[w] | p.z[w];
+    for (uint32_t i : ops_on(support)) {
       PendingOp& op = queue[i];
       if (op.kind == 2) continue;
       if (anticommutes(p, op.pauli)) op.sign = sym_xor(op.sign, sign);
@@ -1334,10 +1724,17 @@ struct Planner {
   }
 
   void transform_queue(const Clifford& f) {
-    for (size_t i = head; i < queue.size(); ++i) {
+    if (!f.moved_ready) f.index_moved();
+    const std::vector<uint32_t> cand = ops_on(f.footprint);
+    for (uint32_t i : cand) {
       PendingOp& op = queue[i];
       if (op.kind == 2) continue;
-      op.pauli = f.coordinates(op.pauli);
+      const Pauli before = op.pauli;
+      f.coordinates_into(op.pauli);
+      bool moved_support = false;
+      for (size_t w = 0; w < before.x.size() && !moved_support; ++w)
+        moved_support = ((op.pauli.x[w] | op.pauli.z[w]) & ~(before.x[w] | before.z[w])) != 0;
+      if (moved_support) index_op(i);
     }
   }
 
@@ -1701,6 +2098,7 @@ Program compile_circuit_text(const std::string& text, CompileStats* stats_out,
   pl.n = c.nqubits;
   pl.ctx = low.ctx;
   pl.queue = std::move(low.pending);
+  pl.build_index();
   pl.run();
   Program planned = freeze(pl.n, 0, pl.max_k, std::move(pl.instrs), pl.ctx);
 

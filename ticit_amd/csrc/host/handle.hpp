@@ -69,6 +69,7 @@ struct ticit_hip_program {
   ticit::GroupLayout group{};
   ticit::WideLayout wide{};
   int group_tps = 256;  // group tier threads per shot of the plain launch
+  bool bit_heavy = false;  // record/detector/coin steps dominate: automatic tier wide
   uint32_t wide_batch = 0;
   int32_t planner_source = 0;
   float planner_p = -1.f;
