@@ -117,6 +117,8 @@ def main() -> int:
         if prof.get("shots") == shots:
             traffic = prof.get("hbm_bytes_per_launch")
 
+    host_rows = host_rows_rate(lib, sampler, shots, value)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(circuit.blob, args.cpu_seconds)
@@ -152,11 +154,39 @@ def main() -> int:
             "traffic": traffic,
         },
         "cpu_baseline": cpu,
+        "host_rows": host_rows,
     }
     if not args.headline_only:
         line["configs"] = other_configs(ticit_amd, w, torch, np, stream, use_cpu=not args.no_cpu)
     print(json.dumps(line))
     return 0
+
+
+def host_rows_rate(lib, sampler, shots: int, device_value: float, reps: int = 5) -> dict:
+    """The drop-in call, ticit_hip_sample (Sampler::sample_with_seed,
+    src/sampler/prepared.rs:663-670): the headline's shots with bit-packed
+    rows returned in host memory, timed around the C call (median of `reps`
+    after one warm call, each result freed outside the timed region)."""
+    import ctypes
+    import statistics
+
+    from ticit_amd import _capi
+    times = []
+    for i in range(reps + 1):
+        res = _capi.Result()
+        t0 = time.perf_counter()
+        _capi.check(lib.ticit_hip_sample(sampler.handle, shots, 7000 + i, 1, 1, ctypes.byref(res)))
+        dt = time.perf_counter() - t0
+        rows = int(res.record_rows)
+        lib.ticit_hip_result_free(ctypes.byref(res))
+        if rows != shots:
+            raise RuntimeError(f"ticit_hip_sample returned {rows} rows for {shots} shots")
+        if i:
+            times.append(dt)
+    med = statistics.median(times)
+    return {"value": shots / med, "unit": "shots/s", "ms_per_call": med * 1e3,
+            "fraction_of_device_resident": shots / med / device_value,
+            "call": "ticit_hip_sample, bit-packed rows kept, host memory, median of 5 calls"}
 
 
 def _cgroup_cpu_quota():

@@ -117,3 +117,54 @@ def test_learned_decision_and_fallback(hip):
         assert info.planner_source == 0 and bool(info.components) == bool(int(blob[9]) & 1)
     finally:
         PL.uninstall()
+
+
+def test_held_out_batch_runs_the_shipped_model_on_the_device(hip):
+    """The C4 workload: the 512 held-out programs created with
+    planner="learned". The device inference's P(components) equals the
+    oracle's forward pass of the shipped model (1e-4), the layout each
+    program runs on is the decision rule's (learned choice, or the
+    heuristic's when |p - 1/2| is under the margin), and the number of
+    fallbacks is the stored evaluation's; a sample of the programs then
+    samples the same rows batched as one at a time."""
+    import torch
+
+    from oracle import mlp
+    from ticit_amd import planner as PL
+    from ticit_amd import workloads as W
+
+    m = PL.load_model()
+    x = (m["ho_x"] - m["mean"]) / m["scale"]
+    p_ref = PL.softmax_p_components(mlp.predict(m["params"], x))
+    want, confident = PL.decide(p_ref, m["ho_heuristic"], float(m["margin"]))
+    samplers = []
+    for j, idx in enumerate(m["ho_idx"]):
+        s = hip.CompiledSampler(hip.plan_text(W.planner_corpus_circuit(int(idx))), planner="learned")
+        samplers.append(s)
+        assert abs(s.planner_p_components - float(p_ref[j])) <= 1e-4, j
+        assert s.components == bool(want[j]), j
+        assert (s.planner_source == "learned") == bool(confident[j]), j
+    fallbacks = sum(s.planner_source != "learned" for s in samplers)
+    print("held-out programs", len(samplers), "components", sum(s.components for s in samplers),
+          "fallbacks", fallbacks, flush=True)
+    assert fallbacks == int((~confident).sum())
+    pick = samplers[::32]
+    shots = [3000] * len(pick)
+    row = lambda n: max(1, (n + 7) // 8)
+    outs, singles = [], []
+    for s, n in zip(pick, shots):
+        mk = lambda: (torch.zeros((n, row(s.num_measurements)), dtype=torch.uint8, device="cuda"),
+                      torch.zeros((n, row(s.num_detectors)), dtype=torch.uint8, device="cuda"),
+                      torch.zeros((n, row(s.num_observables)), dtype=torch.uint8, device="cuda"),
+                      torch.zeros(max(1, s.num_observables), dtype=torch.int64, device="cuda"))
+        outs.append(mk())
+        singles.append(mk())
+    b = hip.Batch(pick, shots, list(range(len(pick))), outs, autotune=True, autotune_shots=512)
+    b.launch(0)
+    b.wait()
+    for i, (s, n) in enumerate(zip(pick, shots)):
+        s.sample_device(n, i, singles[i])
+    torch.cuda.synchronize()
+    for a, c in zip(outs, singles):
+        assert all(torch.equal(u, v) for u, v in zip(a, c))
+    b.close()

@@ -25,6 +25,7 @@ SOURCES = [
     "csrc/host/pinning.cpp",
     "csrc/host/device_program.cpp",
     "csrc/host/capi.cpp",
+    "csrc/host/host_copy.cpp",
     "csrc/host/batch.cpp",
     "csrc/host/planner.cpp",
     "csrc/kernels/sampler.hip",

@@ -5,17 +5,20 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <chrono>
 #include <map>
 #include <mutex>
 #include <cstring>
 #include <string>
+#include <sys/mman.h>
 #include <vector>
 
 #include "../kernels/compact.hpp"
 #include "../kernels/static_scratch.hpp"
 #include "component_plan.hpp"
 #include "handle.hpp"
+#include "host_copy.hpp"
 #include "pinning.hpp"
 
 using namespace ticit;
@@ -284,20 +287,205 @@ hipError_t launch(ticit_hip_program* h, const DevLaunch& L, hipStream_t stream, 
                           dump_re, dump_im, dump_k);
 }
 
-struct DeviceBuffers {
-  std::vector<void*> ptrs;
-  ~DeviceBuffers() {
-    for (void* p : ptrs) (void)hipFree(p);
-  }
+// One call's device buffers, carved from the program's arena: sizes are
+// planned first, then one reservation (the arena grows only when a call
+// needs more than any before it), then the pointers are assigned.
+struct ArenaPlan {
+  struct Slot {
+    void** out;
+    size_t bytes;
+  };
+  std::vector<Slot> slots;
   template <class T>
-  hipError_t alloc(T** out, size_t count) {
-    void* p = nullptr;
-    hipError_t e = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
-    if (e == hipSuccess) ptrs.push_back(p);
-    *out = static_cast<T*>(p);
-    return e;
+  void add(T** out, size_t count) {
+    slots.push_back(Slot{reinterpret_cast<void**>(out), std::max<size_t>(count, 1) * sizeof(T)});
+  }
+  static size_t up(size_t b) { return (b + 255) / 256 * 256; }
+  size_t total() const {
+    size_t t = 0;
+    for (const Slot& s : slots) t += up(s.bytes);
+    return t;
   }
 };
+
+hipError_t arena_assign(ticit_hip_program* h, ArenaPlan& plan) {
+  const size_t need = plan.total();
+  if (need > h->arena_bytes) {
+    // every earlier host-result call has finished with the arena
+    if (h->arena) (void)hipFree(h->arena);
+    h->arena = nullptr;
+    h->arena_bytes = 0;
+    hipError_t e = hipMalloc(&h->arena, need);
+    if (e != hipSuccess) return e;
+    h->arena_bytes = need;
+  }
+  uint8_t* p = static_cast<uint8_t*>(h->arena);
+  for (const ArenaPlan::Slot& s : plan.slots) {
+    *s.out = p;
+    p += ArenaPlan::up(s.bytes);
+  }
+  return hipSuccess;
+}
+
+hipError_t pinned_reserve(ticit_hip_program* h, size_t bytes) {
+  if (bytes <= h->pinned_bytes) return hipSuccess;
+  if (h->pinned) (void)hipHostFree(h->pinned);
+  h->pinned = nullptr;
+  h->pinned_bytes = 0;
+  hipError_t e = hipHostMalloc(&h->pinned, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) return e;
+  h->pinned_bytes = bytes;
+  return hipSuccess;
+}
+
+// Copies of device results into host buffers through the pinned staging:
+// every piece is queued device-to-pinned, one wait, then each is copied to
+// its destination.
+struct CopyBack {
+  struct Piece {
+    void* dst;
+    const void* src;
+    size_t bytes;
+  };
+  std::vector<Piece> pieces;
+  void add(void* dst, const void* src, size_t bytes) {
+    if (bytes) pieces.push_back(Piece{dst, src, bytes});
+  }
+  hipError_t run(ticit_hip_program* h) {
+    size_t total = 0;
+    for (const Piece& p : pieces) total += ArenaPlan::up(p.bytes);
+    if (!total) return hipSuccess;
+    hipError_t e = pinned_reserve(h, total);
+    if (e != hipSuccess) return e;
+    uint8_t* stage = static_cast<uint8_t*>(h->pinned);
+    size_t off = 0;
+    for (const Piece& p : pieces) {
+      if ((e = hipMemcpyAsync(stage + off, p.src, p.bytes, hipMemcpyDeviceToHost, nullptr)) != hipSuccess) return e;
+      off += ArenaPlan::up(p.bytes);
+    }
+    if ((e = hipStreamSynchronize(nullptr)) != hipSuccess) return e;
+    std::vector<HostCopy> out;
+    off = 0;
+    for (const Piece& p : pieces) {
+      out.push_back(HostCopy{p.dst, stage + off, p.bytes});
+      off += ArenaPlan::up(p.bytes);
+    }
+    host_copy(out.data(), out.size());
+    return hipSuccess;
+  }
+};
+
+constexpr int kDevErrAny = kDevErrImpossibleBranch | kDevErrBadProbability | kDevErrUnfairPin;
+
+// Host result rows (freed with free() by ticit_hip_result_free): large ones
+// start on a 2 MiB boundary with transparent huge pages requested, so the
+// first touch of a row buffer faults once per 2 MiB rather than per 4 KiB.
+void* host_rows(size_t bytes) {
+  constexpr size_t kHuge = size_t(2) << 20;
+  bytes = std::max<size_t>(bytes, 1);
+  if (bytes >= 2 * kHuge) {
+    void* p = nullptr;
+    if (posix_memalign(&p, kHuge, bytes) == 0) {
+      (void)madvise(p, bytes & ~size_t(4095), MADV_HUGEPAGE);
+      return p;
+    }
+  }
+  return malloc(bytes);
+}
+
+// Row bytes per chunk of a chunked host-result call (all row kinds
+// together): small enough that the copies overlap the next chunks' kernels
+// and the host copy of the previous chunk, large enough that a chunk fills
+// the device.
+constexpr size_t kChunkBytes = size_t(8) << 20;
+constexpr uint64_t kMinChunkShots = 65536;
+
+// Samples L in chunks and returns every chunk's rows into dst (full-size
+// host buffers; dst[i] null = no such rows): kernels on the null stream,
+// one event per chunk, device-to-pinned copies on the program's copy
+// stream into two slots used in turn, host copies out of a slot once its
+// copy event has fired. Ends with every queued operation finished, on
+// error too.
+hipError_t chunked_rows(ticit_hip_program* h, const DevLaunch& L, const uint8_t* const src[4],
+                        const uint32_t rbs[4], uint8_t* const dst[4], uint64_t* observable_ones, int* derr,
+                        float* ms) {
+  const uint64_t shots = L.shots;
+  size_t row_all = 0;
+  for (int i = 0; i < 4; ++i)
+    if (dst[i]) row_all += rbs[i];
+  uint64_t chunk = std::max<uint64_t>(kChunkBytes / std::max<size_t>(row_all, 1), kMinChunkShots);
+  chunk = std::min<uint64_t>((chunk + 63) / 64 * 64, shots);
+  const uint64_t nchunks = (shots + chunk - 1) / chunk;
+  size_t slot = 0;
+  for (int i = 0; i < 4; ++i)
+    if (dst[i]) slot += ArenaPlan::up(size_t(chunk) * rbs[i]);
+  hipError_t e = pinned_reserve(h, 2 * slot);
+  if (e != hipSuccess) return e;
+  if (!h->copy_stream && (e = hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking)) != hipSuccess)
+    return e;
+  for (hipEvent_t& ev : h->slot_done)
+    if (!ev && (e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return e;
+  while (h->chunk_done.size() < nchunks) {
+    hipEvent_t ev = nullptr;
+    if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return e;
+    h->chunk_done.push_back(ev);
+  }
+  uint8_t* const stage = static_cast<uint8_t*>(h->pinned);
+  auto queue_copy = [&](uint64_t c) -> hipError_t {
+    const uint64_t first = c * chunk, n = std::min<uint64_t>(chunk, shots - first);
+    hipError_t r = hipStreamWaitEvent(h->copy_stream, h->chunk_done[c], 0);
+    size_t off = (c & 1) * slot;
+    for (int i = 0; i < 4 && r == hipSuccess; ++i) {
+      if (!dst[i]) continue;
+      if (n * rbs[i])
+        r = hipMemcpyAsync(stage + off, src[i] + first * rbs[i], n * rbs[i], hipMemcpyDeviceToHost, h->copy_stream);
+      off += ArenaPlan::up(size_t(chunk) * rbs[i]);
+    }
+    return r == hipSuccess ? hipEventRecord(h->slot_done[c & 1], h->copy_stream) : r;
+  };
+  auto body = [&]() -> hipError_t {
+    hipError_t r = hipEventRecord(h->ev_start, nullptr);
+    for (uint64_t c = 0; c < nchunks && r == hipSuccess; ++c) {
+      DevLaunch Lc = L;
+      Lc.first_shot = L.first_shot + c * chunk;
+      Lc.shots = std::min<uint64_t>(chunk, shots - c * chunk);
+      Lc.meas = L.meas + c * chunk * L.meas_row;
+      Lc.det = L.det + c * chunk * L.det_row;
+      Lc.obs = L.obs + c * chunk * L.obs_row;
+      if (L.expv) Lc.expv = L.expv + c * chunk * (rbs[3] / 8);
+      r = launch(h, Lc, nullptr);
+      if (r == hipSuccess) r = hipEventRecord(h->chunk_done[c], nullptr);
+    }
+    if (r == hipSuccess) r = hipEventRecord(h->ev_stop, nullptr);
+    for (uint64_t c = 0; c < std::min<uint64_t>(2, nchunks) && r == hipSuccess; ++c) r = queue_copy(c);
+    for (uint64_t c = 0; c < nchunks && r == hipSuccess; ++c) {
+      if ((r = hipEventSynchronize(h->slot_done[c & 1])) != hipSuccess) break;
+      const uint64_t first = c * chunk, n = std::min<uint64_t>(chunk, shots - first);
+      size_t off = (c & 1) * slot;
+      HostCopy pieces[4];
+      size_t np = 0;
+      for (int i = 0; i < 4; ++i) {
+        if (!dst[i]) continue;
+        pieces[np++] = HostCopy{dst[i] + first * rbs[i], stage + off, n * rbs[i]};
+        off += ArenaPlan::up(size_t(chunk) * rbs[i]);
+      }
+      host_copy(pieces, np);
+      if (c + 2 < nchunks) r = queue_copy(c + 2);
+    }
+    if (r != hipSuccess) return r;
+    if ((r = hipEventSynchronize(h->ev_stop)) != hipSuccess) return r;
+    (void)hipEventElapsedTime(ms, h->ev_start, h->ev_stop);
+    if ((r = hipMemcpy(derr, h->d_error, sizeof(int), hipMemcpyDeviceToHost)) != hipSuccess) return r;
+    return hipMemcpy(observable_ones, L.obs_ones, size_t(h->tab.nobs) * 8, hipMemcpyDeviceToHost);
+  };
+  e = body();
+  if (e != hipSuccess) {
+    // nothing queued may outlive the call: the next one reuses the buffers
+    (void)hipStreamSynchronize(h->copy_stream);
+    (void)hipStreamSynchronize(nullptr);
+  }
+  return e;
+}
 
 ticit_hip_status run_host(ticit_hip_program* h, uint64_t shots, uint64_t seed, int bit_packed, int keep_records,
                           bool injected, const uint64_t* exo, const double* uniforms, const uint64_t* coins,
@@ -328,161 +516,180 @@ ticit_hip_status run_host(ticit_hip_program* h, uint64_t shots, uint64_t seed, i
     return TICIT_HIP_OK;
   }
   if (h->tier == kTierGroup && shots > 0x7fffffffull) return fail(TICIT_HIP_INVALID_INPUT, "too many shots for one call");
-  DeviceBuffers buf;
+  std::lock_guard<std::mutex> lock(h->host_mu);
   hipError_t e = hipSuccess;
-  const size_t words = (shots + 63) / 64;
-  uint64_t* d_exo = nullptr;
-  double* d_uni = nullptr;
-  uint64_t* d_coins = nullptr;
-  double* d_probs = nullptr;
-  double* d_expv = nullptr;
-  uint8_t* d_keep = nullptr;
-#define CK(expr, what)                        \
-  do {                                        \
-    e = (expr);                               \
+#define CK(expr, what)                             \
+  do {                                             \
+    e = (expr);                                    \
     if (e != hipSuccess) return from_hip(e, what); \
   } while (0)
-  if (keep_records) {
-    CK(buf.alloc(&L.meas, size_t(shots) * L.meas_row), "allocating measurement rows");
-    CK(buf.alloc(&L.det, size_t(shots) * L.det_row), "allocating detector rows");
-    CK(buf.alloc(&L.obs, size_t(shots) * L.obs_row), "allocating observable rows");
-    if (t.nexp > 0) {
-      CK(buf.alloc(&d_expv, size_t(shots) * size_t(t.nexp)), "allocating expectation values");
-      L.expv = d_expv;
-    }
-  }
-  CK(buf.alloc(&L.obs_ones, size_t(std::max(1, t.nobs))), "allocating observable counts");
-  if (t.npost > 0 && !reference) {  // the reference shot is never discarded
-    CK(buf.alloc(&d_keep, size_t(shots)), "allocating postselection flags");
-    L.keep = d_keep;
-  }
-  CK(hipMemset(L.obs_ones, 0, size_t(std::max(1, t.nobs)) * 8), "clearing observable counts");
-  CK(hipMemset(h->d_error, 0, sizeof(int)), "clearing the error flag");
-  L.error = h->d_error;
-  if (injected) {
-    CK(buf.alloc(&d_exo, size_t(std::max(1, t.nsym)) * words), "allocating exogenous planes");
-    CK(buf.alloc(&d_uni, size_t(std::max(1, t.nmeasure)) * shots), "allocating uniforms");
-    CK(buf.alloc(&d_coins, size_t(std::max(1, t.ncoin)) * words), "allocating coins");
-    if (exo && t.nsym) CK(hipMemcpy(d_exo, exo, size_t(t.nsym) * words * 8, hipMemcpyHostToDevice), "copying exo");
-    else CK(hipMemset(d_exo, 0, size_t(std::max(1, t.nsym)) * words * 8), "clearing exo");
-    if (uniforms && t.nmeasure)
-      CK(hipMemcpy(d_uni, uniforms, size_t(t.nmeasure) * shots * 8, hipMemcpyHostToDevice), "copying uniforms");
-    else CK(hipMemset(d_uni, 0, size_t(std::max(1, t.nmeasure)) * shots * 8), "clearing uniforms");
-    if (coins && t.ncoin) CK(hipMemcpy(d_coins, coins, size_t(t.ncoin) * words * 8, hipMemcpyHostToDevice), "copying coins");
-    else CK(hipMemset(d_coins, 0, size_t(std::max(1, t.ncoin)) * words * 8), "clearing coins");
-    L.exo = d_exo;
-    L.uniforms = d_uni;
-    L.coins = d_coins;
-  }
-  if (probs) {
-    CK(buf.alloc(&d_probs, size_t(std::max(1, t.nmeasure)) * shots), "allocating probabilities");
-    L.probs = d_probs;
-  }
-  double* d_state = nullptr;
+  const size_t words = (shots + 63) / 64;
+  const size_t nobs1 = size_t(std::max(1, t.nobs));
+  const bool postselect = t.npost > 0 && !reference;  // the reference shot is never discarded
   if (state_re) {
     if (h->tier != kTierWide) return fail(TICIT_HIP_UNSUPPORTED, "state dumps are served by the wide tier");
     if (h->tab.components)
       return fail(TICIT_HIP_UNSUPPORTED, "state dumps need the dense layout (components = 1)");
-    CK(buf.alloc(&d_state, (size_t(2) << h->tab.max_k)), "allocating the state dump");
   }
-  hipEvent_t ev0, ev1;
-  CK(hipEventCreate(&ev0), "creating events");
-  CK(hipEventCreate(&ev1), "creating events");
-  CK(hipEventRecord(ev0, nullptr), "recording event");
-  int dump_k = 0;
-  e = launch(h, L, nullptr, d_state, d_state ? d_state + (size_t(1) << h->tab.max_k) : nullptr, &dump_k);
-  if (e != hipSuccess) {
-    (void)hipEventDestroy(ev0);
-    (void)hipEventDestroy(ev1);
-    return from_hip(e, "launching the sampling kernel");
+  // every device buffer of the call, carved from the program's arena
+  double* d_expv = nullptr;
+  uint8_t* d_keep = nullptr;
+  uint64_t* d_exo = nullptr;
+  double* d_uni = nullptr;
+  uint64_t* d_coins = nullptr;
+  double* d_probs = nullptr;
+  double* d_state = nullptr;
+  uint64_t* tile_off = nullptr;
+  unsigned long long* d_acc = nullptr;
+  uint8_t* survivors[4] = {nullptr, nullptr, nullptr, nullptr};
+  const uint32_t rbs[4] = {L.meas_row, L.det_row, L.obs_row, uint32_t(t.nexp) * 8u};
+  ArenaPlan plan;
+  if (keep_records) {
+    plan.add(&L.meas, size_t(shots) * L.meas_row);
+    plan.add(&L.det, size_t(shots) * L.det_row);
+    plan.add(&L.obs, size_t(shots) * L.obs_row);
+    if (t.nexp > 0) plan.add(&d_expv, size_t(shots) * size_t(t.nexp));
   }
-  CK(hipEventRecord(ev1, nullptr), "recording event");
-  CK(hipEventSynchronize(ev1), "running the sampling kernel");
-  float ms = 0.f;
-  (void)hipEventElapsedTime(&ms, ev0, ev1);
-  (void)hipEventDestroy(ev0);
-  (void)hipEventDestroy(ev1);
-  int derr = 0;
-  CK(hipMemcpy(&derr, h->d_error, sizeof(int), hipMemcpyDeviceToHost), "reading the error flag");
-  if (derr & kDevErrBadProbability) return fail(TICIT_HIP_INVALID_INPUT, "probability must be between 0 and 1");
-  if (derr & kDevErrUnfairPin)
-    return fail(TICIT_HIP_INVALID_INPUT,
-                "pinned measurement branch has a probability other than one half (not one half within 1e-9); "
-                "pinning it would bias sampling");
-  if (derr & kDevErrImpossibleBranch)
-    return fail(TICIT_HIP_INVALID_INPUT, "sampled an impossible active measurement branch");
+  plan.add(&L.obs_ones, nobs1);
+  if (postselect) {
+    plan.add(&d_keep, size_t(shots));
+    plan.add(&tile_off, compaction_scratch_words(shots));
+    plan.add(&d_acc, 1);
+    if (keep_records)
+      for (int i = 0; i < 4; ++i)
+        if (rbs[i]) plan.add(&survivors[i], size_t(shots) * rbs[i]);
+  }
+  if (injected) {
+    plan.add(&d_exo, size_t(std::max(1, t.nsym)) * words);
+    plan.add(&d_uni, size_t(std::max(1, t.nmeasure)) * shots);
+    plan.add(&d_coins, size_t(std::max(1, t.ncoin)) * words);
+  }
+  if (probs) plan.add(&d_probs, size_t(std::max(1, t.nmeasure)) * shots);
+  if (state_re) plan.add(&d_state, size_t(2) << h->tab.max_k);
+  CK(arena_assign(h, plan), "allocating device buffers");
+  L.expv = d_expv;
+  L.keep = d_keep;
+  L.error = h->d_error;
+  CK(hipMemsetAsync(L.obs_ones, 0, nobs1 * 8, nullptr), "clearing observable counts");
+  CK(hipMemsetAsync(h->d_error, 0, sizeof(int), nullptr), "clearing the error flag");
+  if (injected) {
+    if (exo && t.nsym) CK(hipMemcpy(d_exo, exo, size_t(t.nsym) * words * 8, hipMemcpyHostToDevice), "copying exo");
+    else CK(hipMemsetAsync(d_exo, 0, size_t(std::max(1, t.nsym)) * words * 8, nullptr), "clearing exo");
+    if (uniforms && t.nmeasure)
+      CK(hipMemcpy(d_uni, uniforms, size_t(t.nmeasure) * shots * 8, hipMemcpyHostToDevice), "copying uniforms");
+    else CK(hipMemsetAsync(d_uni, 0, size_t(std::max(1, t.nmeasure)) * shots * 8, nullptr), "clearing uniforms");
+    if (coins && t.ncoin) CK(hipMemcpy(d_coins, coins, size_t(t.ncoin) * words * 8, hipMemcpyHostToDevice), "copying coins");
+    else CK(hipMemsetAsync(d_coins, 0, size_t(std::max(1, t.ncoin)) * words * 8, nullptr), "clearing coins");
+    L.exo = d_exo;
+    L.uniforms = d_uni;
+    L.coins = d_coins;
+  }
+  L.probs = d_probs;
+  if (!h->ev_start) CK(hipEventCreate(&h->ev_start), "creating events");
+  if (!h->ev_stop) CK(hipEventCreate(&h->ev_stop), "creating events");
 
   out->shots = shots;
   out->accepted = shots;
   out->nexpvals = uint64_t(t.nexp);
-  // postselection: the kept shots' rows move to the front on the device, in
-  // shot order (src/sampler/prepared.rs:881-1028 appends only survivors),
-  // and only those are copied back
-  if (d_keep) {
-    uint64_t* tile_off = nullptr;
-    unsigned long long* d_acc = nullptr;
-    CK(buf.alloc(&tile_off, compaction_scratch_words(shots)), "allocating compaction offsets");
-    CK(buf.alloc(&d_acc, 1), "allocating the survivor count");
-    CompactRows sets[4] = {};
-    int nsets = 0;
-    if (keep_records) {
-      uint8_t* srcs[4] = {L.meas, L.det, L.obs, reinterpret_cast<uint8_t*>(d_expv)};
-      uint32_t rbs[4] = {L.meas_row, L.det_row, L.obs_row, uint32_t(t.nexp) * 8u};
-      for (int i = 0; i < 4; ++i) {
-        if (!srcs[i] || rbs[i] == 0) continue;
-        uint8_t* dst = nullptr;
-        CK(buf.alloc(&dst, size_t(shots) * rbs[i]), "allocating survivor rows");
-        sets[nsets++] = CompactRows{srcs[i], dst, rbs[i], 0};
-      }
-    }
-    CK(launch_compaction(d_keep, shots, tile_off, d_acc, sets, nsets, nullptr), "compacting survivor rows");
-    unsigned long long acc = 0;
-    CK(hipMemcpy(&acc, d_acc, 8, hipMemcpyDeviceToHost), "reading the survivor count");
-    out->accepted = acc;
-    out->discarded = shots - acc;
-    // the compacted sets replace the full ones
-    for (int i = 0; i < nsets; ++i) {
-      if (sets[i].src == L.meas) L.meas = sets[i].dst;
-      else if (sets[i].src == L.det) L.det = sets[i].dst;
-      else if (sets[i].src == L.obs) L.obs = sets[i].dst;
-      else d_expv = reinterpret_cast<double*>(sets[i].dst);
-    }
-  }
-  const uint64_t rows = out->accepted;
   out->bit_packed = bit_packed != 0;
   out->nmeasurements = uint64_t(t.nrec);
   out->ndetectors = uint64_t(t.ndet);
   out->nobservables = uint64_t(t.nobs);
-  out->observable_ones = static_cast<uint64_t*>(calloc(std::max(1, t.nobs), 8));
-  CK(hipMemcpy(out->observable_ones, L.obs_ones, size_t(t.nobs) * 8, hipMemcpyDeviceToHost), "reading counts");
-  if (t.nobs > 0 && h->opt.observable < uint32_t(t.nobs)) out->logical_errors = out->observable_ones[h->opt.observable];
-  if (keep_records) {
-    out->record_rows = rows;
-    out->measurements = static_cast<uint8_t*>(malloc(std::max<size_t>(1, size_t(rows) * L.meas_row)));
-    out->detectors = static_cast<uint8_t*>(malloc(std::max<size_t>(1, size_t(rows) * L.det_row)));
-    out->observables = static_cast<uint8_t*>(malloc(std::max<size_t>(1, size_t(rows) * L.obs_row)));
-    if (rows && L.meas_row)
-      CK(hipMemcpy(out->measurements, L.meas, size_t(rows) * L.meas_row, hipMemcpyDeviceToHost), "reading rows");
-    if (rows && L.det_row)
-      CK(hipMemcpy(out->detectors, L.det, size_t(rows) * L.det_row, hipMemcpyDeviceToHost), "reading rows");
-    if (rows && L.obs_row)
-      CK(hipMemcpy(out->observables, L.obs, size_t(rows) * L.obs_row, hipMemcpyDeviceToHost), "reading rows");
-    if (t.nexp > 0) {
-      out->exp_vals = static_cast<double*>(malloc(std::max<size_t>(1, size_t(rows) * size_t(t.nexp)) * sizeof(double)));
-      if (rows)
-        CK(hipMemcpy(out->exp_vals, d_expv, size_t(rows) * size_t(t.nexp) * 8, hipMemcpyDeviceToHost),
-           "reading expectation values");
+  int derr = 0;
+  float ms = 0.f;
+  // Lane- and group-tier rows come back in chunks: every chunk's kernel is
+  // queued at once, and chunk i is copied to pinned staging on the copy
+  // stream as soon as its kernel ends, while the host moves chunk i-1 into
+  // the result rows.
+  const bool chunked = keep_records && !postselect && !injected && !probs && !state_re && h->tier != kTierWide;
+  if (chunked) {
+    uint8_t* dst[4] = {nullptr, nullptr, nullptr, nullptr};
+    const uint8_t* src[4] = {L.meas, L.det, L.obs, reinterpret_cast<const uint8_t*>(d_expv)};
+    for (int i = 0; i < 4; ++i)
+      if (i < 3 || t.nexp > 0) dst[i] = static_cast<uint8_t*>(host_rows(size_t(shots) * rbs[i]));
+    out->measurements = dst[0];
+    out->detectors = dst[1];
+    out->observables = dst[2];
+    out->exp_vals = reinterpret_cast<double*>(dst[3]);
+    out->record_rows = shots;
+    out->observable_ones = static_cast<uint64_t*>(calloc(nobs1, 8));
+    e = chunked_rows(h, L, src, rbs, dst, out->observable_ones, &derr, &ms);
+    if (e != hipSuccess) {
+      ticit_hip_result_free(out);
+      return from_hip(e, "sampling in chunks");
+    }
+  } else {
+    CK(hipEventRecord(h->ev_start, nullptr), "recording event");
+    int dump_k = 0;
+    CK(launch(h, L, nullptr, d_state, d_state ? d_state + (size_t(1) << h->tab.max_k) : nullptr, &dump_k),
+       "launching the sampling kernel");
+    CK(hipEventRecord(h->ev_stop, nullptr), "recording event");
+    CK(hipEventSynchronize(h->ev_stop), "running the sampling kernel");
+    (void)hipEventElapsedTime(&ms, h->ev_start, h->ev_stop);
+    CK(hipMemcpy(&derr, h->d_error, sizeof(int), hipMemcpyDeviceToHost), "reading the error flag");
+    if (!(derr & kDevErrAny)) {
+      // postselection: the kept shots' rows move to the front on the
+      // device, in shot order (src/sampler/prepared.rs:881-1028 appends
+      // only survivors), and only those are copied back
+      uint8_t* src[4] = {L.meas, L.det, L.obs, reinterpret_cast<uint8_t*>(d_expv)};
+      if (postselect) {
+        CompactRows sets[4] = {};
+        int nsets = 0, which[4] = {0, 0, 0, 0};
+        for (int i = 0; i < 4; ++i) {
+          if (!survivors[i]) continue;
+          which[nsets] = i;
+          sets[nsets++] = CompactRows{src[i], survivors[i], rbs[i], 0};
+        }
+        CK(launch_compaction(d_keep, shots, tile_off, d_acc, sets, nsets, nullptr), "compacting survivor rows");
+        unsigned long long acc = 0;
+        CK(hipMemcpy(&acc, d_acc, 8, hipMemcpyDeviceToHost), "reading the survivor count");
+        out->accepted = acc;
+        out->discarded = shots - acc;
+        for (int j = 0; j < nsets; ++j) src[which[j]] = survivors[which[j]];
+      }
+      const uint64_t rows = out->accepted;
+      CopyBack back;
+      out->observable_ones = static_cast<uint64_t*>(calloc(nobs1, 8));
+      back.add(out->observable_ones, L.obs_ones, size_t(t.nobs) * 8);
+      if (keep_records) {
+        out->record_rows = rows;
+        uint8_t* dst[4] = {nullptr, nullptr, nullptr, nullptr};
+        for (int i = 0; i < 4; ++i) {
+          if (i == 3 && t.nexp == 0) continue;
+          dst[i] = static_cast<uint8_t*>(host_rows(size_t(rows) * rbs[i]));
+          back.add(dst[i], src[i], size_t(rows) * rbs[i]);
+        }
+        out->measurements = dst[0];
+        out->detectors = dst[1];
+        out->observables = dst[2];
+        out->exp_vals = reinterpret_cast<double*>(dst[3]);
+      }
+      if (probs) back.add(probs, d_probs, size_t(t.nmeasure) * shots * 8);
+      if (state_re) {
+        const size_t nd = size_t(1) << dump_k;
+        back.add(state_re, d_state, nd * 8);
+        back.add(state_im, d_state + (size_t(1) << h->tab.max_k), nd * 8);
+        *state_k = dump_k;
+      }
+      e = back.run(h);
+      if (e != hipSuccess) {
+        ticit_hip_result_free(out);
+        return from_hip(e, "reading results");
+      }
     }
   }
-  if (probs && t.nmeasure)
-    CK(hipMemcpy(probs, d_probs, size_t(t.nmeasure) * shots * 8, hipMemcpyDeviceToHost), "reading probabilities");
-  if (state_re) {
-    size_t nd = size_t(1) << dump_k;
-    CK(hipMemcpy(state_re, d_state, nd * 8, hipMemcpyDeviceToHost), "reading the state");
-    CK(hipMemcpy(state_im, d_state + (size_t(1) << h->tab.max_k), nd * 8, hipMemcpyDeviceToHost), "reading the state");
-    *state_k = dump_k;
-  }
 #undef CK
+  if (derr & kDevErrAny) {
+    ticit_hip_result_free(out);
+    std::memset(out, 0, sizeof(*out));
+    if (derr & kDevErrBadProbability) return fail(TICIT_HIP_INVALID_INPUT, "probability must be between 0 and 1");
+    if (derr & kDevErrUnfairPin)
+      return fail(TICIT_HIP_INVALID_INPUT,
+                  "pinned measurement branch has a probability other than one half (not one half within 1e-9); "
+                  "pinning it would bias sampling");
+    return fail(TICIT_HIP_INVALID_INPUT, "sampled an impossible active measurement branch");
+  }
+  if (t.nobs > 0 && h->opt.observable < uint32_t(t.nobs)) out->logical_errors = out->observable_ones[h->opt.observable];
   out->kernel_s = double(ms) * 1e-3;
   out->sample_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return TICIT_HIP_OK;

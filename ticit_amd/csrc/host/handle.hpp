@@ -88,12 +88,40 @@ struct ticit_hip_program {
   bool post_pending = false;
   std::mutex post_mu;  // host threads sharing the program take turns on the staging
 
+  // Device memory of the synchronous host-result calls (ticit_hip_sample,
+  // the injected and reference runs): rows, counts, flags, injected inputs
+  // and compaction scratch, grown to the largest call and reused, since a
+  // call has finished with it before it returns. Rows come back through
+  // pinned host staging, also kept.
+  void* arena = nullptr;
+  size_t arena_bytes = 0;
+  void* pinned = nullptr;
+  size_t pinned_bytes = 0;
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  // chunked row return: copies run on their own stream behind per-chunk
+  // kernel events, into two pinned slots used in turn
+  hipStream_t copy_stream = nullptr;
+  std::vector<hipEvent_t> chunk_done;
+  hipEvent_t slot_done[2] = {nullptr, nullptr};
+  std::mutex host_mu;  // host threads sharing the program take turns on these
+
   ~ticit_hip_program() {
     if (post_done) {
       (void)hipEventSynchronize(post_done);
       (void)hipEventDestroy(post_done);
     }
     if (post_buf) (void)hipFree(post_buf);
+    if (arena) (void)hipFree(arena);
+    if (pinned) (void)hipHostFree(pinned);
+    if (copy_stream) {
+      (void)hipStreamSynchronize(copy_stream);
+      (void)hipStreamDestroy(copy_stream);
+    }
+    if (ev_start) (void)hipEventDestroy(ev_start);
+    if (ev_stop) (void)hipEventDestroy(ev_stop);
+    for (hipEvent_t ev : chunk_done) (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : slot_done)
+      if (ev) (void)hipEventDestroy(ev);
     for (void* p : allocations) (void)hipFree(p);
   }
 
