@@ -44,15 +44,20 @@ typedef struct ticit_hip_options {
   uint32_t observable;   /* observable counted as a logical error */
   uint32_t precision;    /* 0: FP64 amplitudes; 1: FP32 amplitudes, FP64 reductions */
   uint32_t planner_mode; /* 0: heuristic tier choice; 1: learned planner */
-  uint32_t tier;         /* 0: automatic (lane tier when it fits, group tier up to active
-                            width 10, wide tier above); 1: lane; 2: group; 3: wide */
+  uint32_t tier;         /* 0: automatic (lane tier when its tables fit LDS, and for programs
+                            dominated by record/detector/coin steps up to dense width 8 with
+                            its tables in device memory; group tier up to active width 10;
+                            wide tier above); 1: lane; 2: group; 3: wide */
   uint32_t wide_batch;   /* cap on wide-tier shots in flight (0 = memory budget) */
   uint32_t components;   /* 0: as planned (use_active_components, src/planner.rs:1482-1484);
                             1: dense active state; 2: tensor components
                             (src/component_plan.rs, src/sampler/batch/runtime.rs:388-552) */
   uint32_t group_threads; /* group tier threads per shot: 8, 16, 32, 64, 128, 256 or 512 (0 = by the
                              widest component: 64 up to width 7, 128 up to 10, 256 above) */
-  uint32_t reserved;
+  uint32_t lane_store;    /* lane tier only: 0 automatic (LDS when the tables fit, else device
+                             memory); 1 all tables in LDS; 2 bit tables and row staging in device
+                             memory, amplitudes in LDS (width <= 5); 3 everything in device
+                             memory (dense widths 6 to 8) */
 } ticit_hip_options;
 
 /* Mirrors SampleResult (src/sampler/prepared.rs:197-219). Buffers are owned
@@ -105,6 +110,12 @@ typedef struct ticit_hip_program_info {
   uint32_t table_words[4];       /* per-shot bit-table words: partials, slots, records, detectors */
   uint64_t expectation_values;   /* EXP_VAL slots per shot (src/circuit/mod.rs:122) */
   uint64_t postselected_detectors; /* detectors whose firing discards a shot (src/circuit/mod.rs:329-333) */
+  /* Lane tier: where a wave's tables live. 0 all in LDS; 1 the bit tables
+   * and row staging in device memory, amplitudes in LDS; 2 all of them in
+   * device memory (states of 2^6 to 2^8 amplitudes, bit tables too large
+   * for LDS). */
+  uint32_t lane_store;
+  uint32_t lane_region_bytes; /* device memory per resident wave (0 in LDS), bit-packed rows */
 } ticit_hip_program_info;
 
 /* Thread-local message of the last failing call on this thread. */

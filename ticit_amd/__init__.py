@@ -23,6 +23,8 @@ __all__ = ["Batch", "Circuit", "CompiledSampler", "InvalidInputError", "ParseErr
            "TicitError", "plan_features", "plan_text"]
 
 TIERS = {"auto": 0, "lane": 1, "group": 2, "wide": 3}
+# lane tier table placement (ticit_hip_options.lane_store)
+LANE_STORES = {"auto": 0, "lds": 1, "bits_global": 2, "all_global": 3}
 PRECISIONS = {"fp64": 0, "fp32": 1}
 PLANNERS = {"heuristic": 0, "learned": 1}
 COMPONENTS = {"planned": 0, "dense": 1, "components": 2}
@@ -135,7 +137,8 @@ class CompiledSampler:
     def __init__(self, blob: np.ndarray, observable: int = 0, precision: str = "fp64",
                  tier: str = "auto", planner: str = "heuristic", wide_batch: int = 0,
                  components: str = "planned", group_threads: int = 0, pin_measurements=None,
-                 normalize_syndromes: bool = False, expected_detectors=None, expected_observables=None):
+                 normalize_syndromes: bool = False, expected_detectors=None, expected_observables=None,
+                 lane_store: str = "auto"):
         lib = _capi.load()
         self._lib = lib
         self.blob = np.ascontiguousarray(blob, dtype=np.uint64)
@@ -150,6 +153,7 @@ class CompiledSampler:
         opt.wide_batch = wide_batch
         opt.components = COMPONENTS[components]
         opt.group_threads = group_threads
+        opt.lane_store = LANE_STORES[lane_store]
         handle = ctypes.c_void_p()
         reference = None
         if normalize_syndromes or expected_detectors is not None or expected_observables is not None:
@@ -210,6 +214,10 @@ class CompiledSampler:
         self.num_expectation_values = int(info.expectation_values)
         self.num_postselected_detectors = int(info.postselected_detectors)
         self.has_postselection = self.num_postselected_detectors > 0
+        # lane tier: where a wave's tables live ("lds", "bits_global": bit
+        # tables in device memory, "all_global": amplitudes there too)
+        self.lane_store = {0: "lds", 1: "bits_global", 2: "all_global"}[int(info.lane_store)]
+        self.lane_region_bytes = int(info.lane_region_bytes)
 
     def reference_sample(self) -> "ReferenceSample":
         """The noiseless reference sample of this program, obeying its pins
@@ -352,7 +360,7 @@ class Circuit:
                 threads: int | None = None, wide_batch: int = 0,
                 components: str = "planned", group_threads: int = 0, pin_measurements=None,
                 normalize_syndromes: bool = False, expected_detectors=None,
-                expected_observables=None) -> CompiledSampler:
+                expected_observables=None, lane_store: str = "auto") -> CompiledSampler:
         """Circuit::compile with ticit_py's ``backend=`` selector; only the
         ``"hip"`` backend lives in this package."""
         if backend != "hip":
@@ -364,7 +372,7 @@ class Circuit:
                                planner=planner, wide_batch=wide_batch, components=components,
                                group_threads=group_threads, pin_measurements=pin_measurements,
                                normalize_syndromes=normalize_syndromes, expected_detectors=expected_detectors,
-                               expected_observables=expected_observables)
+                               expected_observables=expected_observables, lane_store=lane_store)
 
     def reference_sample(self) -> ReferenceSample:
         """Circuit::reference_sample (src/circuit/mod.rs:89-91): the

@@ -19,7 +19,7 @@ class Options(ctypes.Structure):
     _fields_ = [("observable", ctypes.c_uint32), ("precision", ctypes.c_uint32),
                 ("planner_mode", ctypes.c_uint32), ("tier", ctypes.c_uint32),
                 ("wide_batch", ctypes.c_uint32), ("components", ctypes.c_uint32),
-                ("group_threads", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("group_threads", ctypes.c_uint32), ("lane_store", ctypes.c_uint32)]
 
 
 class Result(ctypes.Structure):
@@ -46,7 +46,8 @@ class ProgramInfo(ctypes.Structure):
                 ("max_local_k", ctypes.c_int32), ("planner_source", ctypes.c_int32),
                 ("planner_p_components", ctypes.c_double), ("lds_bytes_per_block", ctypes.c_uint32),
                 ("blocks_per_cu", ctypes.c_uint32), ("table_words", ctypes.c_uint32 * 4),
-                ("expectation_values", ctypes.c_uint64), ("postselected_detectors", ctypes.c_uint64)]
+                ("expectation_values", ctypes.c_uint64), ("postselected_detectors", ctypes.c_uint64),
+                ("lane_store", ctypes.c_uint32), ("lane_region_bytes", ctypes.c_uint32)]
 
 
 class Parity(ctypes.Structure):

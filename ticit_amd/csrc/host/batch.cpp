@@ -69,6 +69,9 @@ struct ticit_hip_batch {
   uint64_t* d_seed_offset = nullptr;
   int* d_error = nullptr;
   WideWorkspace* ws = nullptr;
+  // lane items with tables in device memory share these regions: they run
+  // one after another on `main`
+  LaneWorkspace lane_ws{};
   hipStream_t main = nullptr, side = nullptr;
   hipEvent_t fork = nullptr, join = nullptr, join_main = nullptr;
   hipGraph_t graph = nullptr;        // lane and group work
@@ -430,6 +433,24 @@ ticit_hip_status ticit_hip_batch_create(const ticit_hip_batch_item* items, size_
     if ((e = b->upload(&k.d_unit_off, off)) != hipSuccess) return bail_hip(e, "uploading batch tables");
   }
 
+  // Lane items with tables in device memory share one set of regions,
+  // sized for the largest before capture.
+  {
+    size_t need = 0;
+    for (uint32_t i : b->lane_items) {
+      uint64_t slots = 0;
+      size_t slot = 0;
+      if ((e = lane_region_plan(b->progs[i], b->launches[i], &slots, &slot)) != hipSuccess)
+        return bail_hip(e, "planning lane regions");
+      need = std::max(need, size_t(slots) * slot);
+      b->lane_ws.max_slots = std::max(b->lane_ws.max_slots, slots);
+    }
+    if (need) {
+      if ((e = b->alloc(&b->lane_ws.base, need)) != hipSuccess) return bail_hip(e, "allocating lane regions");
+      b->lane_ws.bytes = need;
+    }
+  }
+
   // Wide items share one workspace, sized before capture.
   if (!b->wide_items.empty()) {
     b->ws = wide_workspace_new();
@@ -456,7 +477,7 @@ ticit_hip_status ticit_hip_batch_create(const ticit_hip_batch_item* items, size_
   }
   for (uint32_t i : b->lane_items) {
     ticit_hip_program* h = b->progs[i];
-    step(launch_lane_tier(h->tab.max_k, h->dev, b->launches[i], h->lane, b->main));
+    step(launch_lane_tier(h->tab.max_k, h->dev, b->launches[i], h->lane, b->main, &b->lane_ws));
   }
   auto compact_item = [&](uint32_t i, hipStream_t st) {
     auto it = b->post.find(i);

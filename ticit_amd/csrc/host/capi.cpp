@@ -37,12 +37,39 @@ ticit_hip_status from_hip(hipError_t e, const char* what) {
   return fail(TICIT_HIP_DEVICE_ERROR, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// The lane tier's device memory regions for launch L: their count (one per
+// resident wave, at most the launch's waves and 4 GiB in all; the kernel
+// strides its wave blocks over them) and bytes each. Zero when the tables
+// live in LDS.
+hipError_t lane_region_plan(ticit_hip_program* h, const DevLaunch& L, uint64_t* slots, size_t* slot_bytes) {
+  *slots = 0;
+  *slot_bytes = 0;
+  if (h->tier != kTierLane || h->lane.store == kLaneLds) return hipSuccess;
+  if (h->lane_ws.max_slots == 0) {
+    uint32_t lds = 0, blocks = 0;
+    lane_tier_occupancy(h->tab.max_k, h->lane, 0, &lds, &blocks);
+    int cus = 0;
+    hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device);
+    if (e != hipSuccess) return e;
+    h->lane_ws.max_slots = std::max<uint64_t>(1, uint64_t(std::max<uint32_t>(blocks, 1)) * uint64_t(cus));
+  }
+  const size_t slot = lane_region_bytes(h->tab.max_k, lane_launch_layout(h->tab.max_k, h->lane, L));
+  const uint64_t waves = std::max<uint64_t>(1, (L.shots + 63) / 64);
+  uint64_t n = std::min<uint64_t>(waves, h->lane_ws.max_slots);
+  n = std::max<uint64_t>(1, std::min<uint64_t>(n, (size_t(4) << 30) / slot));
+  *slots = n;
+  *slot_bytes = slot;
+  return hipSuccess;
+}
+
 }  // namespace ticit
 
 namespace {
 
 // Widest active state the automatic tier choice gives the group tier.
 constexpr int kGroupAutoMaxK = 10;
+// Widest dense state the lane tier runs with its tables in device memory.
+constexpr int kLaneGlobalMaxK = 8;
 
 // Bit steps (records, detectors, coins) at least 256 and 16 times the
 // amplitude steps.
@@ -205,6 +232,15 @@ ticit_hip_status create_from_program(Program prog, const ticit_hip_options* opti
                        h->lane.pw + h->lane.sw + h->lane.rw + h->lane.dw,
                        wide_partials_per_shot(t.max_local_k)};
   const bool lane_ok = !t.components && t.max_k <= 5 && lane_tier_lds_bytes(t.max_k, h->lane) <= size_t(lds_limit);
+  // Tables too large for LDS: the lane tier keeps its bit tables (and, above
+  // width 5, the amplitudes) in per-wave device memory regions, with the
+  // same [item][lane] layout, so record and detector steps stay one word per
+  // 64 shots. Dense states up to width 8 only: a lane walks its 2^k
+  // amplitudes one by one.
+  uint32_t lane_store = kLaneLds;
+  if (!lane_ok && !t.components && t.max_k <= kLaneGlobalMaxK)
+    lane_store = t.max_k <= 5 ? kLaneBitsGlobal : kLaneAllGlobal;
+  const bool lane_global_ok = !lane_ok && !t.components && t.max_k <= kLaneGlobalMaxK;
   auto group_fits = [&](int tps) {
     return t.max_local_k <= group_max_local_k(tps) &&
            group_tier_lds_bytes(t.state_dim, h->group, tps) <= size_t(lds_limit);
@@ -232,17 +268,47 @@ ticit_hip_status create_from_program(Program prog, const ticit_hip_options* opti
   const bool group_ok = group_fits(tps);
   uint32_t want = h->opt.tier;
   if (want == 0 && h->opt.precision == 1) want = kTierWide;  // the FP32 path lives in the wide tier
-  // A program dominated by record, detector and coin steps runs them one
-  // thread per shot in the wide tier, not a whole workgroup per shot as in
-  // the group tier: on the CCZ d5 and d7 fixtures (k = 8; 22.9k and 61k
-  // steps, almost all bit steps) the wide tier ran 7.6x and 12.5x faster.
+  // A program dominated by record, detector and coin steps (the CCZ
+  // fixtures: k = 8, 22.9k and 61k steps, almost all of them bit steps)
+  // runs in the lane tier with its tables in device memory, where a step is
+  // one word operation for 64 shots; the group tier would spend a whole
+  // workgroup per shot on it, the wide tier a thread per shot.
   h->bit_heavy = bit_heavy_program(t);
-  if (want == 0)
-    want = lane_ok ? kTierLane
-                   : (group_ok && t.max_local_k <= kGroupAutoMaxK && !h->bit_heavy ? kTierGroup : kTierWide);
-  if (want == kTierLane && !lane_ok) {
+  if (want == 0) {
+    if (lane_ok)
+      want = kTierLane;
+    else if (h->bit_heavy && lane_global_ok)
+      want = kTierLane;
+    else
+      want = group_ok && t.max_local_k <= kGroupAutoMaxK && !h->bit_heavy ? kTierGroup : kTierWide;
+  }
+  if (want == kTierLane && !lane_ok && !lane_global_ok) {
     delete h;
-    return fail(TICIT_HIP_UNSUPPORTED, "lane tier needs a dense max_k <= 5 and its tables in LDS");
+    return fail(TICIT_HIP_UNSUPPORTED, "lane tier needs a dense max_k <= " + std::to_string(kLaneGlobalMaxK));
+  }
+  if (want == kTierLane) {
+    h->lane.store = lane_ok ? kLaneLds : lane_store;
+    // a caller's placement: 1 LDS, 2 bit tables in device memory, 3 all
+    const uint32_t asked = h->opt.lane_store;
+    if (asked > 3) {
+      delete h;
+      return fail(TICIT_HIP_INVALID_INPUT, "lane_store must be 0, 1, 2 or 3");
+    }
+    if (asked == 1 && !lane_ok) {
+      delete h;
+      return fail(TICIT_HIP_UNSUPPORTED, "the lane tier's tables do not fit LDS");
+    }
+    LaneLayout amps_only{};
+    amps_only.store = kLaneBitsGlobal;
+    if (asked == 2 && !(t.max_k <= 5 && !t.components && lane_tier_lds_bytes(t.max_k, amps_only) <= size_t(lds_limit))) {
+      delete h;
+      return fail(TICIT_HIP_UNSUPPORTED, "bit tables in device memory keep the amplitudes in LDS: width 5 at most");
+    }
+    if (asked == 3 && !(t.max_k > 5 && t.max_k <= kLaneGlobalMaxK && !t.components)) {
+      delete h;
+      return fail(TICIT_HIP_UNSUPPORTED, "all lane tables in device memory: dense widths 6 to 8");
+    }
+    if (asked) h->lane.store = asked - 1;
   }
   if (want == kTierGroup && !group_ok) {
     delete h;
@@ -269,9 +335,39 @@ ticit_hip_status create_from_program(Program prog, const ticit_hip_options* opti
   return TICIT_HIP_OK;
 }
 
+// Grows the program's lane regions for launch L, after their previous user
+// has finished. Nothing is allocated when they are large enough already.
+hipError_t lane_workspace_prepare(ticit_hip_program* h, const DevLaunch& L) {
+  uint64_t slots = 0;
+  size_t slot = 0;
+  hipError_t e = lane_region_plan(h, L, &slots, &slot);
+  if (e != hipSuccess || slots == 0) return e;
+  if (!h->lane_ws_done && (e = hipEventCreateWithFlags(&h->lane_ws_done, hipEventDisableTiming)) != hipSuccess)
+    return e;
+  const size_t need = size_t(slots) * slot;
+  if (need <= h->lane_ws.bytes) return hipSuccess;
+  if ((e = hipEventSynchronize(h->lane_ws_done)) != hipSuccess) return e;
+  if (h->lane_ws.base) (void)hipFree(h->lane_ws.base);
+  h->lane_ws.base = nullptr;
+  h->lane_ws.bytes = 0;
+  void* p = nullptr;
+  if ((e = hipMalloc(&p, need)) != hipSuccess) return e;
+  h->lane_ws.base = static_cast<unsigned char*>(p);
+  h->lane_ws.bytes = need;
+  return hipSuccess;
+}
+
 hipError_t launch(ticit_hip_program* h, const DevLaunch& L, hipStream_t stream, double* dump_re = nullptr,
                   double* dump_im = nullptr, int* dump_k = nullptr) {
-  if (h->tier == kTierLane) return launch_lane_tier(h->tab.max_k, h->dev, L, h->lane, stream);
+  if (h->tier == kTierLane) {
+    if (h->lane.store == kLaneLds) return launch_lane_tier(h->tab.max_k, h->dev, L, h->lane, stream);
+    hipError_t e = lane_workspace_prepare(h, L);
+    if (e != hipSuccess) return e;
+    // calls sharing the regions may come on different streams: order them
+    if ((e = hipStreamWaitEvent(stream, h->lane_ws_done, 0)) != hipSuccess) return e;
+    if ((e = launch_lane_tier(h->tab.max_k, h->dev, L, h->lane, stream, &h->lane_ws)) != hipSuccess) return e;
+    return hipEventRecord(h->lane_ws_done, stream);
+  }
   if (h->tier == kTierGroup)
     return launch_group_tier(h->dev, L, h->group, stream, h->group_tps);
   uint32_t batch = uint32_t(std::min<uint64_t>(h->wide_batch, L.shots));
@@ -984,6 +1080,10 @@ ticit_hip_status ticit_hip_program_info_get(const ticit_hip_program* h, ticit_hi
     uint32_t row = std::max({h->row_bytes(uint64_t(h->tab.nrec), true), h->row_bytes(uint64_t(h->tab.ndet), true),
                              h->row_bytes(uint64_t(h->tab.nobs), true)});
     lane_tier_occupancy(h->tab.max_k, h->lane, row, &out->lds_bytes_per_block, &out->blocks_per_cu);
+    out->lane_store = h->lane.store;
+    LaneLayout y = h->lane;
+    y.stage_bytes = 64 * ((row + 7) / 8 * 8);
+    out->lane_region_bytes = uint32_t(lane_region_bytes(h->tab.max_k, y));
   } else if (h->tier == kTierGroup) {
     out->lds_bytes_per_block = uint32_t(group_tier_lds_bytes(h->tab.state_dim, h->group, h->group_tps));
   }
@@ -1111,6 +1211,10 @@ ticit_hip_status ticit_hip_sample_device_postselected_noalloc(ticit_hip_program*
     return fail(TICIT_HIP_UNSUPPORTED,
                 "the wide tier keeps its per-shot states in a workspace the library allocates; "
                 "ticit_hip_sample_device_postselected_noalloc serves lane- and group-tier programs");
+  if (h->tier == kTierLane && h->lane.store != kLaneLds)
+    return fail(TICIT_HIP_UNSUPPORTED,
+                "this lane-tier program keeps its tables in device memory the library allocates; "
+                "ticit_hip_sample_device_postselected_noalloc serves programs whose tables fit LDS");
   int current = -1;
   hipError_t e = hipGetDevice(&current);
   if (e != hipSuccess) return from_hip(e, "reading the current device");

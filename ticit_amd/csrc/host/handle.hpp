@@ -17,6 +17,8 @@
 namespace ticit {
 
 size_t lane_tier_lds_bytes(int kmax, const LaneLayout& y);
+size_t lane_region_bytes(int kmax, const LaneLayout& y);
+LaneLayout lane_launch_layout(int kmax, const LaneLayout& Yp, const DevLaunch& L);
 void lane_tier_occupancy(int kmax, const LaneLayout& Y, uint32_t row_bytes, uint32_t* lds, uint32_t* blocks);
 size_t group_tier_lds_bytes(uint32_t state_dim, const GroupLayout& y, int tps);
 size_t group_region_bytes(uint32_t state_dim, const GroupLayout& y, int tps);
@@ -24,8 +26,9 @@ int group_max_local_k(int tps);
 int group_shots_per_block(int tps);
 int group_shots_per_unit(int tps);
 int group_batch_occupancy(int tps, uint32_t region_bytes, int max_local_k);
+// `ws`: the device memory regions when Y.store is not kLaneLds.
 hipError_t launch_lane_tier(int kmax, const DevProgram& P, const DevLaunch& L, const LaneLayout& Y,
-                            hipStream_t stream);
+                            hipStream_t stream, const LaneWorkspace* ws = nullptr);
 hipError_t launch_group_tier(const DevProgram& P, const DevLaunch& L, const GroupLayout& Y, hipStream_t stream,
                              int tps);
 hipError_t launch_group_batch(const DevProgram* progs, const DevLaunch* launches, const GroupLayout* layouts,
@@ -48,6 +51,16 @@ hipError_t launch_wide_tier(const DevProgram& P, const DevInstr* host_instrs, co
 
 constexpr uint32_t kTierLane = 1, kTierGroup = 2, kTierWide = 3;
 
+}  // namespace ticit
+
+struct ticit_hip_program;
+
+namespace ticit {
+
+// Lane-tier programs with tables in device memory: the regions a launch
+// uses (count and bytes each; zero otherwise).
+hipError_t lane_region_plan(ticit_hip_program* h, const DevLaunch& L, uint64_t* slots, size_t* slot_bytes);
+
 // Learned layout decision (planner.cpp); false = keep the heuristic's.
 bool learned_layout(const Program& p, int device, bool* use_components, float* p_components);
 
@@ -69,7 +82,11 @@ struct ticit_hip_program {
   ticit::GroupLayout group{};
   ticit::WideLayout wide{};
   int group_tps = 256;  // group tier threads per shot of the plain launch
-  bool bit_heavy = false;  // record/detector/coin steps dominate: automatic tier wide
+  bool bit_heavy = false;  // record/detector/coin steps dominate the program
+  // lane tier with tables in device memory: the wave regions, grown to the
+  // largest launch and reused in stream order through `lane_ws_done`
+  ticit::LaneWorkspace lane_ws{};
+  hipEvent_t lane_ws_done = nullptr;
   uint32_t wide_batch = 0;
   int32_t planner_source = 0;
   float planner_p = -1.f;
@@ -111,6 +128,11 @@ struct ticit_hip_program {
       (void)hipEventDestroy(post_done);
     }
     if (post_buf) (void)hipFree(post_buf);
+    if (lane_ws_done) {
+      (void)hipEventSynchronize(lane_ws_done);
+      (void)hipEventDestroy(lane_ws_done);
+    }
+    if (lane_ws.base) (void)hipFree(lane_ws.base);
     if (arena) (void)hipFree(arena);
     if (pinned) (void)hipHostFree(pinned);
     if (copy_stream) {

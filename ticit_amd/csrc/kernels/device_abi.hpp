@@ -241,6 +241,21 @@ struct LaneLayout {
   // its table may share the amplitudes' LDS, dead by then
   uint32_t dets_once;
   uint32_t dets_in_amps;    // per launch: the detector table lives there
+  uint32_t store;           // kLaneLds, kLaneBitsGlobal or kLaneAllGlobal
+};
+
+// Where a lane-tier program keeps its per-wave tables: all in LDS; the bit
+// tables and row staging in device memory with the amplitudes in LDS; or
+// everything in device memory (states of 2^6 to 2^8 amplitudes per lane).
+enum : uint32_t { kLaneLds = 0, kLaneBitsGlobal = 1, kLaneAllGlobal = 2 };
+
+// Device memory of the lane tier's tables outside LDS: `bytes` split into
+// regions of one wave each, at most `max_slots` of them in use (the
+// resident waves of the chip).
+struct LaneWorkspace {
+  unsigned char* base;
+  size_t bytes;
+  uint64_t max_slots;
 };
 
 struct GroupLayout {

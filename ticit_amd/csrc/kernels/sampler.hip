@@ -38,37 +38,51 @@ __device__ __forceinline__ uint64_t transpose64(uint64_t x, int lane) {
   return x;
 }
 
-template <int KMAX>
-__global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L, LaneLayout Y) {
-  extern __shared__ __align__(16) unsigned char smem[];
+// Runs the 64 shots [wb * 64, wb * 64 + 64) of launch L on one wave.
+// STORE (kLaneLds, kLaneBitsGlobal, kLaneAllGlobal) says where the tables
+// live: all in LDS; the bit tables and row staging in the wave's device
+// memory region `region`, the amplitudes in LDS; or everything in the
+// region. The region layout is the LDS layout (tables [item][lane]), so
+// every uniform-index access is one coalesced 64-lane transaction there as
+// well, and the wave's memory operations on one address complete in issue
+// order in both spaces.
+template <int KMAX, int STORE>
+__device__ __forceinline__ void lane_wave(const DevProgram& P, const DevLaunch& L, const LaneLayout& Y, uint64_t wb,
+                                          unsigned char* smem, unsigned char* region) {
   constexpr int DIM = 1 << KMAX;
   constexpr int SDIM = KMAX > 0 ? DIM / 2 : 0;
   // the collapse's output half: registers up to four amplitudes, LDS above
   constexpr bool kRegScratch = KMAX <= 3;
   constexpr int SDIM_LDS = kRegScratch ? 0 : SDIM;
   const int lane = threadIdx.x;
-  const uint64_t local = uint64_t(blockIdx.x) * 64 + lane;
+  const uint64_t local = wb * 64 + lane;
   const bool live = local < L.shots;
   const uint64_t shot = L.first_shot + local;
 
-  double2* amp = reinterpret_cast<double2*>(smem);  // [DIM][64]
-  double2* scr = amp + DIM * 64;                    // [SDIM_LDS][64]
+  unsigned char* const abase = STORE == kLaneAllGlobal ? region : smem;
+  double2* amp = reinterpret_cast<double2*>(abase);  // [DIM][64]
+  double2* scr = amp + DIM * 64;                     // [SDIM_LDS][64]
   // Bit-sliced tables: word i holds item i of all 64 shots (bit l = lane l),
   // so a record, detector or expression update is one uniform word
   // operation for the whole wave.
-  uint64_t* Pw = reinterpret_cast<uint64_t*>(scr + SDIM_LDS * 64);  // exogenous partials
-  uint64_t* Sw = Pw + Y.np;                                     // runtime slots
-  uint64_t* Rw = Sw + Y.ns;                                     // records
+  uint64_t* Pw = STORE == kLaneLds ? reinterpret_cast<uint64_t*>(scr + SDIM_LDS * 64)
+                                   : reinterpret_cast<uint64_t*>(region + (STORE == kLaneAllGlobal
+                                                                               ? size_t(DIM + SDIM_LDS) * 64 * 16
+                                                                               : 0));  // exogenous partials
+  uint64_t* Sw = Pw + Y.np;  // runtime slots
+  uint64_t* Rw = Sw + Y.ns;  // records
   // The amplitudes are dead once the schedule's last step (the detectors)
-  // runs: the detector table and the row staging share their LDS when they
-  // fit (the host decides, lane_tier_lds_bytes counts the same way).
+  // runs: in LDS the detector table and the row staging share their space
+  // when they fit (the host decides, lane_tier_lds_bytes counts the same
+  // way). Outside LDS both follow the records.
   const uint32_t amp_bytes = uint32_t((DIM + SDIM_LDS) * 64 * 16);
-  const bool stage_in_amps = Y.stage_bytes <= amp_bytes;
-  uint64_t* Dw = Y.dets_in_amps ? reinterpret_cast<uint64_t*>(smem + (stage_in_amps ? Y.stage_bytes : 0u))
-                                : Rw + Y.nr;  // detectors
-  uint8_t* stage = stage_in_amps ? smem : reinterpret_cast<uint8_t*>(Rw + Y.nr + (Y.dets_in_amps ? 0u : Y.nd));
+  const bool stage_in_amps = STORE == kLaneLds && Y.stage_bytes <= amp_bytes;
+  const bool dets_in_amps = STORE == kLaneLds && Y.dets_in_amps;
+  uint64_t* Dw = dets_in_amps ? reinterpret_cast<uint64_t*>(smem + (stage_in_amps ? Y.stage_bytes : 0u))
+                              : Rw + Y.nr;  // detectors
+  uint8_t* stage = stage_in_amps ? smem : reinterpret_cast<uint8_t*>(Rw + Y.nr + (dets_in_amps ? 0u : Y.nd));
 
-  const uint32_t nzero = Y.np + Y.ns + Y.nr + (Y.dets_in_amps ? 0u : Y.nd);
+  const uint32_t nzero = Y.np + Y.ns + Y.nr + (dets_in_amps ? 0u : Y.nd);
   for (uint32_t w = lane; w < nzero; w += 64) Pw[w] = 0;
   __syncthreads();
 
@@ -77,7 +91,7 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
     const uint64_t words = (L.shots + 63) / 64;
     for (int i = 0; i < P.nexo; ++i) {
       const int32_t c = P.exo_syms[i];
-      const uint64_t wv = L.exo[uint64_t(c - 1) * words + blockIdx.x];
+      const uint64_t wv = L.exo[uint64_t(c - 1) * words + wb];
       if (wv)
         for (uint32_t j = P.sym_off[c - 1]; j < P.sym_off[c]; ++j) Pw[P.sym_part[j]] ^= wv;  // same value in every lane
       __syncthreads();
@@ -124,7 +138,7 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
         uint64_t coins;
         if (L.injected) {
           const uint64_t words = (L.shots + 63) / 64;
-          coins = L.coins[uint64_t(q.full) * words + blockIdx.x];
+          coins = L.coins[uint64_t(q.full) * words + wb];
         } else {
           coins = __ballot(sm_next(brng) & 1);
         }
@@ -188,7 +202,7 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
         for (uint32_t j = 0; j < fin.forced_len; ++j) coins ^= Sw[P.resid[fin.forced_off + j]];
       } else if (L.injected) {
         const uint64_t words = (L.shots + 63) / 64;
-        coins = L.coins[uint64_t(st.full) * words + blockIdx.x];
+        coins = L.coins[uint64_t(st.full) * words + wb];
       } else {
         coins = __ballot(sm_next(brng) & 1);  // one word per shot's stream
       }
@@ -338,7 +352,7 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
   // Observables (src/sampler/prepared.rs:881-907) and their popcounts;
   // shots a postselected detector fired on are discarded
   // (src/sampler/batch/postselect.rs) and not counted.
-  const uint32_t nlive = uint32_t(min<uint64_t>(64, L.shots - uint64_t(blockIdx.x) * 64));
+  const uint32_t nlive = uint32_t(min<uint64_t>(64, L.shots - wb * 64));
   uint64_t live_mask = nlive >= 64 ? ~0ull : ((1ull << nlive) - 1);
   if (P.npost > 0) {
     uint64_t dead = 0;
@@ -378,7 +392,7 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
     }
     __syncthreads();
     const size_t total = size_t(nlive) * row_bytes;
-    uint8_t* dst = out + (uint64_t(blockIdx.x) * 64) * row_bytes;
+    uint8_t* dst = out + (wb * 64) * row_bytes;
     if (!(total & 7) && !(reinterpret_cast<uintptr_t>(dst) & 7)) {
       const uint2* src8 = reinterpret_cast<const uint2*>(stage);
       uint2* dst8 = reinterpret_cast<uint2*>(dst);
@@ -391,6 +405,27 @@ __global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L
   emit(L.meas, L.meas_row, [&](uint32_t c) { return Rw[c]; }, uint32_t(P.nrec));
   emit(L.det, L.det_row, [&](uint32_t c) { return Dw[c]; }, uint32_t(P.ndet));
   emit(L.obs, L.obs_row, obs_word, uint32_t(P.nobs));
+}
+
+// One wave per 64 shots. With the tables in LDS the grid covers the launch;
+// with tables in device memory the grid is the workspace's slot count, and
+// each workgroup runs the wave blocks blockIdx.x, blockIdx.x + gridDim.x,
+// ... in its own region (every wave leaves the loop after the launch's last
+// block).
+template <int KMAX, int STORE>
+__global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L, LaneLayout Y, unsigned char* ws,
+                                                       size_t slot_bytes) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  if constexpr (STORE == kLaneLds) {
+    lane_wave<KMAX, STORE>(P, L, Y, blockIdx.x, smem, nullptr);
+  } else {
+    unsigned char* region = ws + size_t(blockIdx.x) * slot_bytes;
+    const uint64_t nwb = (L.shots + 63) / 64;
+    for (uint64_t wb = blockIdx.x; wb < nwb; wb += gridDim.x) {
+      lane_wave<KMAX, STORE>(P, L, Y, wb, smem, region);
+      __syncthreads();  // the region's last readers before the next block clears it
+    }
+  }
 }
 
 // --------------------------------------------------------------------------
@@ -411,67 +446,121 @@ static uint32_t lane_dets_in_amps(int kmax, const LaneLayout& y) {
 }
 
 size_t lane_tier_lds_bytes(int kmax, const LaneLayout& y) {
+  if (y.store == kLaneBitsGlobal) return lane_amp_bytes(kmax);
+  if (y.store == kLaneAllGlobal) return 0;
   const size_t amps = lane_amp_bytes(kmax);
   const size_t dets = lane_dets_in_amps(kmax, y) ? 0 : size_t(y.nd) * 8;
   return amps + size_t(y.np + y.ns + y.nr) * 8 + dets + (y.stage_bytes <= amps ? 0 : y.stage_bytes);
 }
 
-template <int K>
-static hipError_t launch_lane_k(const DevProgram& P, const DevLaunch& L, const LaneLayout& Yp, hipStream_t stream) {
-  // Row staging sized for this launch's rows (packed rows are 8x smaller
-  // than the program's worst case): less LDS per wave, more waves per CU.
+// Device memory of one wave's region (tables outside LDS): the bit tables,
+// the row staging and, for kLaneAllGlobal, the amplitudes ahead of them.
+size_t lane_region_bytes(int kmax, const LaneLayout& y) {
+  if (y.store == kLaneLds) return 0;
+  size_t b = size_t(y.np + y.ns + y.nr + y.nd) * 8 + y.stage_bytes;
+  if (y.store == kLaneAllGlobal) b += lane_amp_bytes(kmax);
+  return (b + 255) / 256 * 256;
+}
+
+// The layout of one launch: row staging sized for its rows (packed rows are
+// 8x smaller than the program's worst case), so in LDS a wave takes less
+// space and more waves fit a CU.
+LaneLayout lane_launch_layout(int kmax, const LaneLayout& Yp, const DevLaunch& L) {
   LaneLayout Y = Yp;
   uint32_t row = L.keep_records ? (L.meas_row > L.det_row ? L.meas_row : L.det_row) : 0;
   row = row > L.obs_row ? row : L.obs_row;
   Y.stage_bytes = 64 * ((row + 7) / 8 * 8);
-  Y.dets_in_amps = lane_dets_in_amps(K, Y);
-  size_t lds = lane_tier_lds_bytes(K, Y);
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&lane_tier_kernel<K>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+  Y.dets_in_amps = Y.store == kLaneLds ? lane_dets_in_amps(kmax, Y) : 0;
+  return Y;
+}
+
+template <int K, int STORE>
+static hipError_t launch_lane_k(const DevProgram& P, const DevLaunch& L, const LaneLayout& Yp,
+                                const LaneWorkspace* ws, hipStream_t stream) {
+  const LaneLayout Y = lane_launch_layout(K, Yp, L);
+  const size_t lds = lane_tier_lds_bytes(K, Y);
+  const void* fn = reinterpret_cast<const void*>(&lane_tier_kernel<K, STORE>);
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
   if (e != hipSuccess) return e;
-  dim3 grid(unsigned((L.shots + 63) / 64));
-  hipLaunchKernelGGL(lane_tier_kernel<K>, grid, dim3(64), lds, stream, P, L, Y);
+  const uint64_t nwb = (L.shots + 63) / 64;
+  if constexpr (STORE == kLaneLds) {
+    hipLaunchKernelGGL((lane_tier_kernel<K, STORE>), dim3(unsigned(nwb)), dim3(64), lds, stream, P, L, Y, nullptr,
+                       size_t(0));
+  } else {
+    const size_t slot = lane_region_bytes(K, Y);
+    if (!ws || !ws->base || slot == 0) return hipErrorInvalidValue;
+    uint64_t slots = ws->bytes / slot;
+    if (ws->max_slots && slots > ws->max_slots) slots = ws->max_slots;
+    if (slots > nwb) slots = nwb;
+    if (slots == 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((lane_tier_kernel<K, STORE>), dim3(unsigned(slots)), dim3(64), lds, stream, P, L, Y,
+                       ws->base, slot);
+  }
   return hipGetLastError();
 }
 
 // LDS bytes and resident workgroups per CU of a lane-tier launch with
 // packed rows of `row_bytes`.
-template <int K>
+template <int K, int STORE>
 static void lane_occupancy_k(const LaneLayout& Yp, uint32_t row_bytes, uint32_t* lds, uint32_t* blocks) {
   LaneLayout Y = Yp;
   Y.stage_bytes = 64 * ((row_bytes + 7) / 8 * 8);
-  Y.dets_in_amps = lane_dets_in_amps(K, Y);
+  Y.dets_in_amps = Y.store == kLaneLds ? lane_dets_in_amps(K, Y) : 0;
   *lds = uint32_t(lane_tier_lds_bytes(K, Y));
   int b = 0;
-  if (hipFuncSetAttribute(reinterpret_cast<const void*>(&lane_tier_kernel<K>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, int(*lds)) == hipSuccess)
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, reinterpret_cast<const void*>(&lane_tier_kernel<K>), 64,
-                                                       *lds);
+  const void* fn = reinterpret_cast<const void*>(&lane_tier_kernel<K, STORE>);
+  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(*lds)) == hipSuccess)
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fn, 64, *lds);
   *blocks = uint32_t(b);
 }
 
-void lane_tier_occupancy(int kmax, const LaneLayout& Y, uint32_t row_bytes, uint32_t* lds, uint32_t* blocks) {
-  *lds = *blocks = 0;
+template <int STORE>
+static void lane_occupancy_store(int kmax, const LaneLayout& Y, uint32_t row_bytes, uint32_t* lds, uint32_t* blocks) {
   switch (kmax) {
-    case 0: return lane_occupancy_k<0>(Y, row_bytes, lds, blocks);
-    case 1: return lane_occupancy_k<1>(Y, row_bytes, lds, blocks);
-    case 2: return lane_occupancy_k<2>(Y, row_bytes, lds, blocks);
-    case 3: return lane_occupancy_k<3>(Y, row_bytes, lds, blocks);
-    case 4: return lane_occupancy_k<4>(Y, row_bytes, lds, blocks);
-    case 5: return lane_occupancy_k<5>(Y, row_bytes, lds, blocks);
+    case 0: return lane_occupancy_k<0, STORE>(Y, row_bytes, lds, blocks);
+    case 1: return lane_occupancy_k<1, STORE>(Y, row_bytes, lds, blocks);
+    case 2: return lane_occupancy_k<2, STORE>(Y, row_bytes, lds, blocks);
+    case 3: return lane_occupancy_k<3, STORE>(Y, row_bytes, lds, blocks);
+    case 4: return lane_occupancy_k<4, STORE>(Y, row_bytes, lds, blocks);
+    case 5: return lane_occupancy_k<5, STORE>(Y, row_bytes, lds, blocks);
     default: return;
   }
 }
 
-hipError_t launch_lane_tier(int kmax, const DevProgram& P, const DevLaunch& L, const LaneLayout& Y,
-                            hipStream_t stream) {
+void lane_tier_occupancy(int kmax, const LaneLayout& Y, uint32_t row_bytes, uint32_t* lds, uint32_t* blocks) {
+  *lds = *blocks = 0;
+  if (Y.store == kLaneLds) return lane_occupancy_store<kLaneLds>(kmax, Y, row_bytes, lds, blocks);
+  if (Y.store == kLaneBitsGlobal) return lane_occupancy_store<kLaneBitsGlobal>(kmax, Y, row_bytes, lds, blocks);
   switch (kmax) {
-    case 0: return launch_lane_k<0>(P, L, Y, stream);
-    case 1: return launch_lane_k<1>(P, L, Y, stream);
-    case 2: return launch_lane_k<2>(P, L, Y, stream);
-    case 3: return launch_lane_k<3>(P, L, Y, stream);
-    case 4: return launch_lane_k<4>(P, L, Y, stream);
-    case 5: return launch_lane_k<5>(P, L, Y, stream);
+    case 6: return lane_occupancy_k<6, kLaneAllGlobal>(Y, row_bytes, lds, blocks);
+    case 7: return lane_occupancy_k<7, kLaneAllGlobal>(Y, row_bytes, lds, blocks);
+    case 8: return lane_occupancy_k<8, kLaneAllGlobal>(Y, row_bytes, lds, blocks);
+    default: return;
+  }
+}
+
+template <int STORE>
+static hipError_t launch_lane_store(int kmax, const DevProgram& P, const DevLaunch& L, const LaneLayout& Y,
+                                    const LaneWorkspace* ws, hipStream_t stream) {
+  switch (kmax) {
+    case 0: return launch_lane_k<0, STORE>(P, L, Y, ws, stream);
+    case 1: return launch_lane_k<1, STORE>(P, L, Y, ws, stream);
+    case 2: return launch_lane_k<2, STORE>(P, L, Y, ws, stream);
+    case 3: return launch_lane_k<3, STORE>(P, L, Y, ws, stream);
+    case 4: return launch_lane_k<4, STORE>(P, L, Y, ws, stream);
+    case 5: return launch_lane_k<5, STORE>(P, L, Y, ws, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_lane_tier(int kmax, const DevProgram& P, const DevLaunch& L, const LaneLayout& Y,
+                            hipStream_t stream, const LaneWorkspace* ws) {
+  if (Y.store == kLaneLds) return launch_lane_store<kLaneLds>(kmax, P, L, Y, nullptr, stream);
+  if (Y.store == kLaneBitsGlobal) return launch_lane_store<kLaneBitsGlobal>(kmax, P, L, Y, ws, stream);
+  switch (kmax) {
+    case 6: return launch_lane_k<6, kLaneAllGlobal>(P, L, Y, ws, stream);
+    case 7: return launch_lane_k<7, kLaneAllGlobal>(P, L, Y, ws, stream);
+    case 8: return launch_lane_k<8, kLaneAllGlobal>(P, L, Y, ws, stream);
     default: return hipErrorInvalidValue;
   }
 }
