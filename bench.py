@@ -532,7 +532,7 @@ def other_configs(ticit_amd, w, torch, np, stream, use_cpu=True) -> dict:
         with open(path) as f:
             cz = ticit_amd.Circuit(f.read().replace("E(0.125)", "E(0)"))
         sz = cz.compile(normalize_syndromes=True)
-        nz = 100_000
+        nz = 1_000_000
         bz = (torch.empty((1, 1), dtype=torch.uint8, device="cuda:0"),) * 3 + \
              (torch.zeros(max(1, sz.num_observables), dtype=torch.int64, device="cuda:0"),)
         t = _time(torch, stream, lambda r: sz.sample_device(nz, 40 + r, bz, stream=st, keep_records=False), reps=3)
@@ -586,6 +586,8 @@ PROFILE_WORKLOADS = {
     "c2": "the 256-circuit batch as one batched graph launch (learned layouts)",
     "p": "cultivation d=3, every detector postselected, normalized: two launches of 1e6 shots, survivors "
          "compacted on the device",
+    "ccz": "the reference's CCZ d05 fixture (lane tier, tables in device memory), normalized, counts only: "
+           "two launches of 1e6 shots",
 }
 
 
@@ -624,6 +626,16 @@ def profile_workload(which: str) -> int:
                              autotune_shots=4096)
         bt.launch(0)
         bt.wait()
+        torch.cuda.synchronize()
+    elif which == "ccz":
+        with open(os.path.join(ROOT, "tests", "golden", "circuits", "ccz_d05_p1e-3.clifft")) as f:
+            cz = ticit_amd.Circuit(f.read().replace("E(0.125)", "E(0)"))
+        s = cz.compile(normalize_syndromes=True)
+        print(f"tier {s.tier} tables {s.lane_store} region {s.lane_region_bytes} B per wave", flush=True)
+        bz = (torch.empty((1, 1), dtype=torch.uint8, device="cuda:0"),) * 3 + \
+             (torch.zeros(max(1, s.num_observables), dtype=torch.int64, device="cuda:0"),)
+        for seed in (1, 2):
+            s.sample_device(1_000_000, seed, bz, keep_records=False)
         torch.cuda.synchronize()
     else:
         path = os.path.join(ROOT, "tests", "golden", "circuits", "msc_d3_inject_cultivate_p1e-3.stim")
