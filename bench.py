@@ -44,6 +44,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+SIMD_CLOCK_HZ = 2.4e9  # MI355X peak engine clock
+
+
+def props_simds() -> int:
+    """SIMDs of device 0: four per CU."""
+    import ticit_amd
+    p = ticit_amd.device_properties()
+    return int(p.get("compute_units", 0)) * 4
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "lane_tier_surface_d5_traffic.json")
 
 
@@ -111,11 +119,23 @@ def main() -> int:
     bytes_per_shot = row(sampler.num_measurements) + row(sampler.num_detectors) + row(sampler.num_observables)
     achieved = bytes_per_shot * shots / avg_kernel_s / 1e9
     traffic = None
+    issue = None
     if os.path.exists(PROFILE_TRAFFIC):
         with open(PROFILE_TRAFFIC) as f:
             prof = json.load(f)
         if prof.get("shots") == shots:
             traffic = prof.get("hbm_bytes_per_launch")
+            # The kernel writes only its rows, so HBM is not what bounds it:
+            # the instruction counts of its profile over this run's kernel
+            # time give the share of the SIMDs' vector issue it uses (a wave64
+            # VALU instruction holds a 16-lane SIMD for 4 cycles).
+            simds = props_simds()
+            if prof.get("valu_insts_per_launch") and simds:
+                cycles = simds * SIMD_CLOCK_HZ * avg_kernel_s
+                issue = {"valu_busy_frac": prof["valu_insts_per_launch"] * 4 / cycles,
+                         "salu_busy_frac": prof["salu_insts_per_launch"] / cycles,
+                         "simds": simds, "clock_hz": SIMD_CLOCK_HZ,
+                         "counters": "SQ_INSTS_VALU / SQ_INSTS_SALU per launch from the kernel's profile"}
 
     host_rows = host_rows_rate(lib, sampler, shots, value)
 
@@ -152,6 +172,7 @@ def main() -> int:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
+            "issue": issue,
         },
         "cpu_baseline": cpu,
         "host_rows": host_rows,
@@ -340,6 +361,28 @@ def _time(torch, stream, fn, reps=2):
     return best
 
 
+def fp32_streaming(ticit_amd, np, samplers, probe=8, q=4):
+    """The same programs and layouts with FP32 amplitude storage where the
+    state streams from device memory (the wide tier; the resident lane and
+    group tiers keep FP64), and the largest probability difference from the
+    FP64 samplers on up to `probe` of them (q injected shots each)."""
+    out, err, probed = [], 0.0, 0
+    rng = np.random.default_rng(11)
+    for s in samplers:
+        if s.tier != "wide":
+            out.append(s)
+            continue
+        s32 = ticit_amd.CompiledSampler(s.blob, planner="learned", precision="fp32")
+        out.append(s32)
+        if probed < probe and s.num_measure_instructions:
+            uni = rng.random((s.num_measure_instructions, q))
+            p64 = s.sample_injected(q, None, uni, None, want_probs=True).probabilities
+            p32 = s32.sample_injected(q, None, uni, None, want_probs=True).probabilities
+            err = max(err, float(np.max(np.abs(p32 - p64))))
+            probed += 1
+    return out, err, probed
+
+
 def other_configs(ticit_amd, w, torch, np, stream, use_cpu=True) -> dict:
     """The other benchmark configurations: C0 (random Clifford+T, group
     tier), C2 (256 circuits, plain versus one batched launch), C3 (dense
@@ -384,6 +427,10 @@ def other_configs(ticit_amd, w, torch, np, stream, use_cpu=True) -> dict:
     setup = time.perf_counter() - t0
     tb = _time(torch, stream, lambda r: bt.launch(100 * (r + 2), st), reps=3)
     info = bt.info()
+    learned32, err32, probed32 = fp32_streaming(ticit_amd, np, learned)
+    bt32 = ticit_amd.Batch(learned32, shots, seeds, bufs, autotune=True, autotune_shots=4096)
+    tb32 = _time(torch, stream, lambda r: bt32.launch(100 * (r + 2), st), reps=3)
+    bt32.close()
     total = sum(shots)
     out["C2"] = {"workload": "256 random Clifford+T circuits, 8-40 qubits, 4-20 T, DEPOLARIZE1 5e-4, "
                              "log-uniform 1e4-1e6 shots",
@@ -392,6 +439,13 @@ def other_configs(ticit_amd, w, torch, np, stream, use_cpu=True) -> dict:
                  "batched_learned_shots_per_s": total / tb,
                  "batched_learned_over_plain": tp / tb, "batched_heuristic_over_plain": tp / tbh,
                  "plain_s": tp, "batched_learned_s": tb, "batch_setup_s": setup,
+                 "batched_learned_fp32_streaming_shots_per_s": total / tb32,
+                 "batched_learned_fp32_streaming_over_plain": tp / tb32,
+                 "fp32_streaming": {"programs": int(sum(x.tier == "wide" for x in learned)),
+                                    "max_prob_error_vs_fp64": err32,
+                                    "error_sample": f"{probed32} wide programs, 4 injected shots each",
+                                    "note": "FP32 amplitude storage in the wide tier (FP64 reductions and "
+                                            "collapse); lane and group programs stay FP64"},
                  "timing": "each leg: one warm run, then the best of 3 timed runs",
                  "components_chosen": {"heuristic": int(sum(s.components for s in samplers)),
                                        "learned": int(sum(s.components for s in learned))},
@@ -499,12 +553,22 @@ def other_configs(ticit_amd, w, torch, np, stream, use_cpu=True) -> dict:
                          autotune_shots=4096)
     tbl = _time(torch, stream, lambda r: b4.launch(1000 * (r + 2), st), reps=3)
     b4.close()
+    lear32, err4, probed4 = fp32_streaming(ticit_amd, np, lear)
+    b432 = ticit_amd.Batch(lear32, [n4] * len(lear32), [7 + i for i in range(len(lear32))], bufs4, autotune=True,
+                           autotune_shots=4096)
+    tbl32 = _time(torch, stream, lambda r: b432.launch(1000 * (r + 2), st), reps=3)
+    b432.close()
     ev = PL.evaluate(m["ho_p_components"], m["ho_heuristic"], m["ho_td"], m["ho_tc"], float(m["margin"]))
     total4 = n4 * len(blobs)
     out["C4"] = {"workload": f"{len(blobs)} held-out planner-corpus programs (indices >= 100000), {n4} shots each",
                  "plain_heuristic_shots_per_s": total4 / th, "plain_learned_shots_per_s": total4 / tl,
                  "batched_learned_shots_per_s": total4 / tbl, "learned_over_heuristic": th / tl,
                  "batched_learned_over_plain_heuristic": th / tbl,
+                 "batched_learned_fp32_streaming_shots_per_s": total4 / tbl32,
+                 "batched_learned_fp32_streaming_over_plain_heuristic": th / tbl32,
+                 "fp32_streaming": {"programs": int(sum(x.tier == "wide" for x in lear)),
+                                    "max_prob_error_vs_fp64": err4,
+                                    "error_sample": f"{probed4} wide programs, 4 injected shots each"},
                  "components_chosen": {"heuristic": int(sum(s.components for s in heur)),
                                        "learned": int(sum(s.components for s in lear))},
                  "learned_fallbacks": int(sum(s.planner_source != "learned" for s in lear)),
