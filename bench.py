@@ -650,6 +650,7 @@ PROFILE_WORKLOADS = {
     "c2": "the 256-circuit batch as one batched graph launch (learned layouts)",
     "p": "cultivation d=3, every detector postselected, normalized: two launches of 1e6 shots, survivors "
          "compacted on the device",
+    "c4": "the 512 held-out planner programs (learned layouts, 2e4 shots each) as one batched graph launch",
     "ccz": "the reference's CCZ d05 fixture (lane tier, tables in device memory), normalized, counts only: "
            "two launches of 1e6 shots",
 }
@@ -688,6 +689,18 @@ def profile_workload(which: str) -> int:
         outs = [_device_buffers(torch, s, k) for s, k in zip(samplers, shots)]
         bt = ticit_amd.Batch(samplers, shots, [1 + i for i in range(len(batch))], outs, autotune=True,
                              autotune_shots=4096)
+        bt.launch(0)
+        bt.wait()
+        torch.cuda.synchronize()
+    elif which == "c4":
+        from ticit_amd import planner as PL
+        m = PL.load_model()
+        samplers = [ticit_amd.CompiledSampler(ticit_amd.plan_text(w.planner_corpus_circuit(int(i))), planner="learned")
+                    for i in m["ho_idx"]]
+        outs = [_device_buffers(torch, s, 20_000) for s in samplers]
+        bt = ticit_amd.Batch(samplers, [20_000] * len(samplers), [7 + i for i in range(len(samplers))], outs,
+                             autotune=True, autotune_shots=4096)
+        print("tiers", {t: sum(s.tier == t for s in samplers) for t in ("lane", "group", "wide")}, flush=True)
         bt.launch(0)
         bt.wait()
         torch.cuda.synchronize()
