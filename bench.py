@@ -400,11 +400,23 @@ def other_configs(ticit_amd, w, torch, np, stream, use_cpu=True) -> dict:
     n0 = 100_000
     b0 = _device_buffers(torch, s0, n0)
     t = _time(torch, stream, lambda r: s0.sample_device(n0, 10 + r, b0, stream=st), reps=5)
+    # the same program through the batched/learned path: the learned
+    # planner's layout and the autotuner's threads per shot
+    l0 = ticit_amd.CompiledSampler(c0.blob, planner="learned")
+    bl0 = ticit_amd.Batch([l0], [n0], [10], [b0], autotune=True, autotune_shots=4096)
+    tl0 = _time(torch, stream, lambda r: bl0.launch(r + 1, st), reps=5)
+    info0 = bl0.info()
+    bl0.close()
     out["C0"] = {"workload": "random Clifford+T, 16 qubits, 40 layers, 12 T, M all; 1e5 shots, FP64",
-                 "tier": s0.tier, "shots_per_s": n0 / t, "ms": t * 1e3}
+                 "tier": s0.tier, "shots_per_s": n0 / t, "ms": t * 1e3,
+                 "batched_learned": {"shots_per_s": n0 / tl0, "ms": tl0 * 1e3,
+                                     "layout": "components" if l0.components else "dense",
+                                     "threads_per_shot": [bk["threads_per_shot"] for bk in info0["buckets"]],
+                                     "over_plain": t / tl0}}
     if use_cpu:
         out["C0"]["cpu_baseline"] = cpu_rows(c0.blob, 60000, 6000, 3, "C0")
         out["C0"]["hip_over_cpu"] = out["C0"]["shots_per_s"] / out["C0"]["cpu_baseline"]["value"]
+        out["C0"]["batched_learned"]["hip_over_cpu"] = n0 / tl0 / out["C0"]["cpu_baseline"]["value"]
 
     # C2
     batch = w.config2_batch()

@@ -350,6 +350,57 @@ __global__ __launch_bounds__(kWideThreads) void wide_promote_kernel(DevProgram P
   }
 }
 
+// Consecutive promotions of one component, widths k0, k0 + 1, ...,
+// k0 + m - 1 (m <= kWideMaxPromoteRun), in one pass into the partner
+// buffer. Promotion j extends the state by the one-qubit factor
+// (c_j, i q_j), so output b takes input b mod 2^k0 and, for j = 0, 1, ...
+// in turn, the factor's arithmetic of wide_promote_kernel in T (c_j times
+// both parts on bit k0 + j clear; (-q_j m, q_j r) on it set): the values of
+// the promotions run one after another, with one read of the old state and
+// one write of the new instead of a read and two writes per promotion.
+constexpr int kWideMaxPromoteRun = 16;
+
+template <class T>
+__global__ __launch_bounds__(kWideThreads) void wide_promote_run_kernel(DevProgram P, WideLayout Y, int instr, int m,
+                                                                        uint64_t* bits, const WideCtl* ctl,
+                                                                        const T* re, const T* im, T* ore, T* oim,
+                                                                        size_t dim, int k0) {
+  const uint32_t j = blockIdx.y;
+  if (ctl[j].dead) return;
+  __shared__ T cs[kWideMaxPromoteRun], qs[kWideMaxPromoteRun];
+  if (int(threadIdx.x) < m) {
+    const DevInstr in = P.instrs[instr + int(threadIdx.x)];
+    const bool sign = wide_eval(P, in, wide_shot(bits, Y, j));
+    cs[threadIdx.x] = T(in.c);
+    qs[threadIdx.x] = T(sign ? in.s : -in.s);
+  }
+  __syncthreads();
+  const size_t off = size_t(j) * dim + P.instrs[instr].comp_off;
+  const T* R = re + off;
+  const T* I = im + off;
+  T* OR = ore + off;
+  T* OI = oim + off;
+  const size_t n0 = size_t(1) << k0, total = n0 << m;
+#pragma unroll
+  for (int u = 0; u < kWidePer; ++u) {
+    const size_t b = size_t(blockIdx.x) * kWideTile + size_t(u) * kWideThreads + threadIdx.x;
+    if (b >= total) break;
+    T r = R[b & (n0 - 1)], mi = I[b & (n0 - 1)];
+    for (int q = 0; q < m; ++q) {
+      if ((b >> (k0 + q)) & 1) {
+        const T nr = -qs[q] * mi, ni = qs[q] * r;
+        r = nr;
+        mi = ni;
+      } else {
+        r = cs[q] * r;
+        mi = cs[q] * mi;
+      }
+    }
+    OR[b] = r;
+    OI[b] = mi;
+  }
+}
+
 // A promotion that starts a new one-qubit component, or an initial |0>
 // component (src/sampler/batch/runtime.rs:511-550, batch/mod.rs:376-402).
 template <class T>
@@ -1214,8 +1265,28 @@ static hipError_t run_wide(const DevProgram& P, const DevInstr* instrs, const De
                            dim, w);
         WCK(hipGetLastError());
       } else if (op == kOpPromote) {
+        // consecutive promotions of this component at widths w, w + 1, ...
+        int run = 1;
+        while (!(in.op & kFlagFresh) && i + run < P.ninstr && run < kWideMaxPromoteRun) {
+          const DevInstr& nx = instrs[i + run];
+          if ((nx.op & 0xff) != kOpPromote || (nx.op & kFlagFresh) || nx.comp != in.comp ||
+              nx.comp_off != in.comp_off || nx.nq != w + run)
+            break;
+          ++run;
+        }
         if (in.op & kFlagFresh) {
           hipLaunchKernelGGL(wide_fresh_kernel<T>, sg, sb, 0, stream, P, Y, i, n, ws.bits, ws.ctl, R, I, dim);
+        } else if (run > 1) {
+          const size_t total = size_t(1) << (w + run);
+          dim3 g(unsigned((total + kWideTile - 1) / kWideTile), n);
+          hipLaunchKernelGGL(wide_promote_run_kernel<T>, g, dim3(kWideThreads), 0, stream, P, Y, i, run, ws.bits,
+                             ws.ctl, static_cast<const T*>(R), static_cast<const T*>(I), RE(cb ^ 1), IM(cb ^ 1),
+                             dim, w);
+          WCK(hipGetLastError());
+          cur[size_t(in.comp)] ^= 1;
+          k += run;
+          i += run;
+          continue;
         } else {
           dim3 g(unsigned(((size_t(1) << w) + kWideTile - 1) / kWideTile), n);
           hipLaunchKernelGGL(wide_promote_kernel<T>, g, dim3(kWideThreads), 0, stream, P, Y, i, ws.bits, ws.ctl, R,
