@@ -165,9 +165,9 @@ def test_device_entry_point_split_launches(hip):
 @pytest.mark.parametrize("tps", [8, 16, 32, 64, 128, 256, 512])
 def test_group_threads_per_shot_give_identical_rows(hip, tps):
     """Packing 8, 4 or 2 shots per wave, or one shot per workgroup of 64 to
-    512 threads, changes the launch shape and the reduction tree of a shot's
-    probability sums (last-bit differences), not the rows: they equal the
-    oracle's, and P(branch = 1) agrees to 1e-12."""
+    512 threads, changes the launch shape, not the rows: they equal the
+    oracle's, and P(branch = 1) agrees to 1e-12 (the tier's pairwise tree
+    against the oracle's scalar order)."""
     c = hip.Circuit(w.config2_batch()[94][0])  # component layout, local k 6
     s = c.compile(tier="group", group_threads=tps)
     shots = 333  # not a multiple of any shots-per-workgroup
@@ -182,6 +182,30 @@ def test_group_threads_per_shot_give_identical_rows(hip, tps):
     oi = O.sample_injected(c.blob, shots, None, uni, coins, want_probs=True)
     assert np.array_equal(ri.measurements, oi["measurements"])
     assert np.max(np.abs(ri.probabilities - oi["probs"])) <= PROB_TOL
+
+
+def test_group_tier_sums_do_not_depend_on_threads_per_shot(hip):
+    """The group tier sums a probability pairwise in index order at every
+    threads-per-shot, so P(branch = 1) is bit-identical across launch shapes
+    and so are seeded rows, also where a deterministic measurement's sum
+    lands on exactly 1 in one order and one ulp below in another (the draw
+    of a uniform depends on it). Planner-corpus program 100321 (components,
+    local k 8) has such measurements: with per-thread sums in strided order,
+    16 threads per shot gave 2 of 3000 seeded rows unlike 8, 32 or 64."""
+    c = hip.Circuit(w.planner_corpus_circuit(100321))
+    shots = 3000
+    rows, probs = {}, {}
+    rng = np.random.default_rng(3)
+    uni = None
+    for tps in (8, 16, 32, 64, 128, 256):
+        s = c.compile(tier="group", group_threads=tps, components="components")
+        if uni is None:
+            uni = rng.random((s.num_measure_instructions, shots))
+        rows[tps] = s.sample(shots, seed=6, bit_packed=True).measurements
+        probs[tps] = s.sample_injected(shots, uniforms=uni, want_probs=True).probabilities
+    for tps in rows:
+        assert np.array_equal(rows[tps], rows[8]), tps
+        assert np.array_equal(probs[tps], probs[8]), tps
 
 
 def test_group_tier_holds_a_width_13_state(hip):

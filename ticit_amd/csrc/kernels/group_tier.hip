@@ -17,8 +17,8 @@
 // batch of programs from an atomic counter.
 //
 // Arithmetic follows ticit's contiguous kernels (src/contiguous.rs:93-177,
-// 1088-1113, 1582-1606, 2013-2044); probabilities are reduced with a fixed
-// tree (thread partials in index order, then waves in order).
+// 1088-1113, 1582-1606, 2013-2044); a probability is the pairwise sum of
+// its terms in index order, the same tree at every TPS.
 #include <hip/hip_runtime.h>
 
 #include "device_common.hpp"
@@ -315,19 +315,40 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
           const uint32_t wo = ins0(idx, piv);
           return wo | ((dph ^ uint32_t(par32(wo & zw)) ^ uint32_t(br)) << piv);
         };
+        // P(true) is the pairwise sum of the out_dim terms in index order:
+        // thread t sums its block of `per` consecutive terms pairwise, then
+        // the group's lanes combine pairwise (group_lane_sum), then its
+        // waves. The tree is the same at every TPS, so a shot's rows do not
+        // depend on the launch shape (an exact 0 or 1 decides whether a
+        // uniform is drawn, and the rounding of the sum decides that).
         double part = 0.0;
-        for (int idx = t; idx < out_dim; idx += TPS) {
-          if (diag) {
-            const uint32_t s = dsrc(uint32_t(idx), true);
-            part += re[s] * re[s] + im[s] * im[s];
-          } else {
+        {
+          auto term = [&](int idx) -> double {
+            if (diag) {
+              const uint32_t s = dsrc(uint32_t(idx), true);
+              return re[s] * re[s] + im[s] * im[s];
+            }
             const uint32_t s0 = ins0(uint32_t(idx), piv), s1 = s0 ^ xm;
             double dir = par32(s0 & zm) ? 1.0 : -1.0;
             double c1r = dir * in.c1_re, c1i = dir * in.c1_im;
             double ar = kInvSqrt2 * re[s0] + c1r * re[s1] - c1i * im[s1];
             double ai = kInvSqrt2 * im[s0] + c1r * im[s1] + c1i * re[s1];
-            part += ar * ar + ai * ai;
-          }
+            return ar * ar + ai * ai;
+          };
+          // out_dim is a power of two of at most PROJ * TPS (the launch's
+          // staging bound), so `per` is a power of two of at most PROJ
+          const int per = out_dim > TPS ? out_dim / TPS : 1;
+          const int base = t * per;
+          double v[PROJ];
+#pragma unroll
+          for (int j = 0; j < PROJ; ++j) v[j] = (j < per && base + j < out_dim) ? term(base + j) : 0.0;
+          // absent terms are +0.0, and x + 0.0 is x: the tree over `per`
+          // terms is unchanged by the padding
+#pragma unroll
+          for (int w = 1; w < PROJ; w <<= 1)
+#pragma unroll
+            for (int j = 0; j + w < PROJ; j += 2 * w) v[j] = v[j] + v[j + w];
+          part = v[0];
         }
         double pt = 0.0;
         if constexpr (TPS <= 64) {
@@ -338,7 +359,14 @@ __device__ void group_shots(const DevProgram& P, const DevLaunch& L, const Group
           part = group_lane_sum<64>(part);
           if ((tid & 63) == 0) red[tid >> 6] = part;
           __syncthreads();
-          for (int w = 0; w < WPG; ++w) pt += red[g * WPG + w];
+          double ws[WPG > 0 ? WPG : 1];
+#pragma unroll
+          for (int w = 0; w < WPG; ++w) ws[w] = red[g * WPG + w];
+#pragma unroll
+          for (int s = 1; s < WPG; s <<= 1)
+#pragma unroll
+            for (int w = 0; w + s < WPG; w += 2 * s) ws[w] = ws[w] + ws[w + s];
+          pt = ws[0];
         }
         pt = clamp01(pt);
         if (st.op & kFlagExpVal) {
