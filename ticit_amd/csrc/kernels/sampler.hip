@@ -38,6 +38,34 @@ __device__ __forceinline__ uint64_t transpose64(uint64_t x, int lane) {
   return x;
 }
 
+// A pair of amplitudes of one wave's shot: indices i0 (pivot bit clear)
+// and i1 = i0 ^ xmask.
+struct AmpPair {
+  double2 v0, v1;
+  int i0, i1;
+};
+
+// Runs use(i, load(i)) for i in [0, n), n a power of two, in index order.
+// With B > 1 and n >= B it issues the loads of B consecutive indices before
+// their uses, so one memory latency covers B elements; use(i) must not
+// write what load(j) reads for a later j of the same batch. No guards inside
+// a batch, so the batch stays straight-line code.
+template <int B, class Load, class Use>
+__device__ __forceinline__ void amp_loop(int n, Load load, Use use) {
+  if (B > 1 && n >= B) {
+#pragma nounroll
+    for (int i0 = 0; i0 < n; i0 += B) {
+      decltype(load(0)) v[B];
+#pragma unroll
+      for (int u = 0; u < B; ++u) v[u] = load(i0 + u);
+#pragma unroll
+      for (int u = 0; u < B; ++u) use(i0 + u, v[u]);
+    }
+  } else {
+    for (int i = 0; i < n; ++i) use(i, load(i));
+  }
+}
+
 // Runs the 64 shots [wb * 64, wb * 64 + 64) of launch L on one wave.
 // STORE (kLaneLds, kLaneBitsGlobal, kLaneAllGlobal) says where the tables
 // live: all in LDS; the bit tables and row staging in the wave's device
@@ -54,6 +82,12 @@ __device__ __forceinline__ void lane_wave(const DevProgram& P, const DevLaunch& 
   // the collapse's output half: registers up to four amplitudes, LDS above
   constexpr bool kRegScratch = KMAX <= 3;
   constexpr int SDIM_LDS = kRegScratch ? 0 : SDIM;
+  // With the amplitudes in device memory each amplitude loop issues a batch
+  // of independent loads before it computes and stores, so one memory
+  // latency covers the batch; in LDS the loops stay one element at a time.
+  constexpr bool kBatchAmps = STORE == kLaneAllGlobal;
+  constexpr int kAmpBatch = kBatchAmps ? 8 : 1, kPairBatch = kBatchAmps ? 4 : 1;
+  static_assert(!kBatchAmps || !kRegScratch, "the batched collapse stages through the scratch half");
   const int lane = threadIdx.x;
   const uint64_t local = wb * 64 + lane;
   const bool live = local < L.shots;
@@ -210,77 +244,93 @@ __device__ __forceinline__ void lane_wave(const DevProgram& P, const DevLaunch& 
       write_outcome(st, eval(st));
       continue;
     }
-    const DevInstr& in = P.instrs[st.full];
+    // With the amplitudes in device memory the compiler cannot tell their
+    // stores from the instruction table, and would re-read the instruction
+    // (and wait for every store in flight) inside each amplitude loop: take
+    // a private copy there.
+    const DevInstr& in_table = P.instrs[st.full];
+    DevInstr in_copy;
+    if constexpr (kBatchAmps) in_copy = in_table;
+    const DevInstr& in = kBatchAmps ? in_copy : in_table;
+    auto load_pair = [&](int idx) {
+      AmpPair a;
+      a.i0 = int(insert_zero_bit(size_t(idx), in.pivot));
+      a.i1 = a.i0 ^ int(in.xmask);
+      a.v0 = amp[a.i0 * 64 + lane];
+      a.v1 = amp[a.i1 * 64 + lane];
+      return a;
+    };
     if (op == kOpRotate) {
       const bool sign = (eval(st) >> lane) & 1;
       const int dim = 1 << k;
       const double c = in.c;
       if (in.op & kFlagDiagonal) {
-        for (int b = 0; b < dim; ++b) {
+        amp_loop<kAmpBatch>(dim, [&](int b) { return amp[b * 64 + lane]; }, [&](int b, double2 v) {
           bool flip = sign != parity64(uint64_t(b) & in.zmask);
           double cr = flip ? -in.me_re : in.me_re;
           double ci = flip ? -in.me_im : in.me_im;
           double fr = c + cr, fi = ci;
-          double2 v = amp[b * 64 + lane];
           amp[b * 64 + lane] = make_double2(fr * v.x - fi * v.y, fr * v.y + fi * v.x);
-        }
+        });
       } else if (in.op & kFlagUniformImag) {
         const double q = sign ? -in.me_im : in.me_im;
-        const int sel = 1 << in.pivot, step = sel << 1;
-        for (int block = 0; block < dim; block += step) {
-          for (int off = 0; off < sel; ++off) {
-            int i0 = block + off, i1 = i0 ^ int(in.xmask);
-            double2 a0 = amp[i0 * 64 + lane], a1 = amp[i1 * 64 + lane];
-            amp[i0 * 64 + lane] = make_double2(c * a0.x - q * a1.y, c * a0.y + q * a1.x);
-            amp[i1 * 64 + lane] = make_double2(c * a1.x - q * a0.y, c * a1.y + q * a0.x);
+        if constexpr (kBatchAmps) {
+          // the pairs are disjoint: a batch of them is loaded, then written
+          amp_loop<kPairBatch>(dim >> 1, load_pair, [&](int, AmpPair a) {
+            amp[a.i0 * 64 + lane] = make_double2(c * a.v0.x - q * a.v1.y, c * a.v0.y + q * a.v1.x);
+            amp[a.i1 * 64 + lane] = make_double2(c * a.v1.x - q * a.v0.y, c * a.v1.y + q * a.v0.x);
+          });
+        } else {
+          const int sel = 1 << in.pivot, step = sel << 1;
+          for (int block = 0; block < dim; block += step) {
+            for (int off = 0; off < sel; ++off) {
+              int i0 = block + off, i1 = i0 ^ int(in.xmask);
+              double2 a0 = amp[i0 * 64 + lane], a1 = amp[i1 * 64 + lane];
+              amp[i0 * 64 + lane] = make_double2(c * a0.x - q * a1.y, c * a0.y + q * a1.x);
+              amp[i1 * 64 + lane] = make_double2(c * a1.x - q * a0.y, c * a1.y + q * a0.x);
+            }
           }
         }
       } else {
-        const int pairs = dim >> 1;
         const bool xz_odd = in.op & kFlagXzOdd;
-        for (int idx = 0; idx < pairs; ++idx) {
-          int left = int(insert_zero_bit(size_t(idx), in.pivot)), right = left ^ int(in.xmask);
-          bool lodd = parity64(uint64_t(left) & in.zmask);
+        amp_loop<kPairBatch>(dim >> 1, load_pair, [&](int, AmpPair a) {
+          bool lodd = parity64(uint64_t(a.i0) & in.zmask);
           double ld = (sign != lodd) ? -1.0 : 1.0;
           double rd = xz_odd ? -ld : ld;
           double lre = ld * in.me_re, lim = ld * in.me_im;
           double rre = rd * in.me_re, rim = rd * in.me_im;
-          double2 a0 = amp[left * 64 + lane], a1 = amp[right * 64 + lane];
-          amp[left * 64 + lane] =
-              make_double2(c * a0.x + rre * a1.x - rim * a1.y, c * a0.y + rre * a1.y + rim * a1.x);
-          amp[right * 64 + lane] =
-              make_double2(c * a1.x + lre * a0.x - lim * a0.y, c * a1.y + lre * a0.y + lim * a0.x);
-        }
+          amp[a.i0 * 64 + lane] = make_double2(c * a.v0.x + rre * a.v1.x - rim * a.v1.y,
+                                               c * a.v0.y + rre * a.v1.y + rim * a.v1.x);
+          amp[a.i1 * 64 + lane] = make_double2(c * a.v1.x + lre * a.v0.x - lim * a.v0.y,
+                                               c * a.v1.y + lre * a.v0.y + lim * a.v0.x);
+        });
       }
     } else if (op == kOpPromote) {
       const bool sign = (eval(st) >> lane) & 1;
       const int dim = 1 << k;
       const double c = in.c, q = sign ? in.s : -in.s;
-      for (int b = 0; b < dim; ++b) {
-        double2 v = amp[b * 64 + lane];
+      // the new half [dim, 2 dim) is written, never read, by the loop
+      amp_loop<kAmpBatch>(dim, [&](int b) { return amp[b * 64 + lane]; }, [&](int b, double2 v) {
         amp[b * 64 + lane] = make_double2(c * v.x, c * v.y);
         amp[(dim + b) * 64 + lane] = make_double2(-q * v.y, q * v.x);
-      }
+      });
       ++k;
     } else if (op == kOpMeasure) {
       const int out_dim = (1 << k) >> 1;
       const bool diag = in.op & kFlagDiagonal;
-      double pt = 0.0;
+      double pt = 0.0;  // P(true), accumulated in index order
       if (diag) {
-        for (int idx = 0; idx < out_dim; ++idx) {
-          double2 v = amp[int(diagonal_source(in, size_t(idx), true)) * 64 + lane];
-          pt += v.x * v.x + v.y * v.y;
-        }
+        amp_loop<kAmpBatch>(
+            out_dim, [&](int i) { return amp[int(diagonal_source(in, size_t(i), true)) * 64 + lane]; },
+            [&](int, double2 v) { pt += v.x * v.x + v.y * v.y; });
       } else {
-        for (int idx = 0; idx < out_dim; ++idx) {
-          int s0 = int(insert_zero_bit(size_t(idx), in.pivot)), s1 = s0 ^ int(in.xmask);
-          double dir = parity64(uint64_t(s0) & in.zmask) ? 1.0 : -1.0;  // branch = true
+        amp_loop<kPairBatch>(out_dim, load_pair, [&](int, AmpPair a) {
+          double dir = parity64(uint64_t(a.i0) & in.zmask) ? 1.0 : -1.0;  // branch = true
           double c1r = dir * in.c1_re, c1i = dir * in.c1_im;
-          double2 v0 = amp[s0 * 64 + lane], v1 = amp[s1 * 64 + lane];
-          double ar = kInvSqrt2 * v0.x + c1r * v1.x - c1i * v1.y;
-          double ai = kInvSqrt2 * v0.y + c1r * v1.y + c1i * v1.x;
+          double ar = kInvSqrt2 * a.v0.x + c1r * a.v1.x - c1i * a.v1.y;
+          double ai = kInvSqrt2 * a.v0.y + c1r * a.v1.y + c1i * a.v1.x;
           pt += ar * ar + ai * ai;
-        }
+        });
       }
       pt = clamp01(pt);
       if (L.want_probs && live) L.probs[uint64_t(in.ordinal) * L.shots + local] = pt;
@@ -314,22 +364,29 @@ __device__ __forceinline__ void lane_wave(const DevProgram& P, const DevLaunch& 
       }
       const double inv = 1.0 / sqrt(ps);
       if (diag) {
-        for (int idx = 0; idx < out_dim; ++idx) {
-          double2 v = amp[int(diagonal_source(in, size_t(idx), branch)) * 64 + lane];
-          amp[idx * 64 + lane] = make_double2(v.x * inv, v.y * inv);
-        }
+        // in place: the source of output i is at or above i, and a batch
+        // loads all its sources before it writes
+        amp_loop<kAmpBatch>(
+            out_dim, [&](int i) { return amp[int(diagonal_source(in, size_t(i), branch)) * 64 + lane]; },
+            [&](int i, double2 v) { amp[i * 64 + lane] = make_double2(v.x * inv, v.y * inv); });
       } else {
-        auto mixed = [&](int idx) {
-          int s0 = int(insert_zero_bit(size_t(idx), in.pivot)), s1 = s0 ^ int(in.xmask);
-          bool odd = parity64(uint64_t(s0) & in.zmask);
+        auto mixed_of = [&](const AmpPair& a) {
+          bool odd = parity64(uint64_t(a.i0) & in.zmask);
           double dir = (branch != odd) ? -1.0 : 1.0;
           double c1r = dir * in.c1_re, c1i = dir * in.c1_im;
-          double2 v0 = amp[s0 * 64 + lane], v1 = amp[s1 * 64 + lane];
-          double ar = kInvSqrt2 * v0.x + c1r * v1.x - c1i * v1.y;
-          double ai = kInvSqrt2 * v0.y + c1r * v1.y + c1i * v1.x;
+          double ar = kInvSqrt2 * a.v0.x + c1r * a.v1.x - c1i * a.v1.y;
+          double ai = kInvSqrt2 * a.v0.y + c1r * a.v1.y + c1i * a.v1.x;
           return make_double2(ar * inv, ai * inv);
         };
-        if constexpr (kRegScratch) {
+        auto mixed = [&](int idx) { return mixed_of(load_pair(idx)); };
+        if constexpr (kBatchAmps) {
+          // an output's partner may lie below it: outputs go to the scratch
+          // half, then are copied down
+          amp_loop<kPairBatch>(out_dim, load_pair,
+                               [&](int i, AmpPair a) { scr[i * 64 + lane] = mixed_of(a); });
+          amp_loop<kAmpBatch>(out_dim, [&](int i) { return scr[i * 64 + lane]; },
+                              [&](int i, double2 v) { amp[i * 64 + lane] = v; });
+        } else if constexpr (kRegScratch) {
           double2 out[SDIM > 0 ? SDIM : 1];
 #pragma unroll
           for (int idx = 0; idx < SDIM; ++idx)
