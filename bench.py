@@ -658,6 +658,8 @@ def other_configs(ticit_amd, w, torch, np, stream, use_cpu=True) -> dict:
 PROFILE_WORKLOADS = {
     "c1": "surface code d=5, 5 rounds (lane tier): two launches of 1e6 shots",
     "c0": "random Clifford+T n=16, 12 T (group tier): two launches of 1e5 shots",
+    "c0b": "the same program through the batched/learned path (learned layout, autotuned threads per shot): "
+           "two replays of 1e5 shots",
     "c3": "dense stress n=30, 24 T (wide tier, components, FP64): a warm and a measured launch of 4096 shots",
     "c2": "the 256-circuit batch as one batched graph launch (learned layouts)",
     "p": "cultivation d=3, every detector postselected, normalized: two launches of 1e6 shots, survivors "
@@ -693,6 +695,18 @@ def profile_workload(which: str) -> int:
         out = _device_buffers(torch, s, shots)
         for seed in (1, 2):
             s.sample_device(shots, seed, out)
+        torch.cuda.synchronize()
+    elif which == "c0b":
+        c0 = ticit_amd.Circuit(w.config0_circuit())
+        l0 = ticit_amd.CompiledSampler(c0.blob, planner="learned")
+        out = _device_buffers(torch, l0, 100_000)
+        bt = ticit_amd.Batch([l0], [100_000], [10], [out], autotune=True, autotune_shots=4096)
+        info = bt.info()
+        print("layout", "components" if l0.components else "dense", "threads per shot",
+              [bk["threads_per_shot"] for bk in info["buckets"]], flush=True)
+        for r in (1, 2):
+            bt.launch(r)
+        bt.wait()
         torch.cuda.synchronize()
     elif which == "c2":
         batch = w.config2_batch()

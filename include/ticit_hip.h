@@ -52,8 +52,8 @@ typedef struct ticit_hip_options {
   uint32_t components;   /* 0: as planned (use_active_components, src/planner.rs:1482-1484);
                             1: dense active state; 2: tensor components
                             (src/component_plan.rs, src/sampler/batch/runtime.rs:388-552) */
-  uint32_t group_threads; /* group tier threads per shot: 8, 16, 32, 64, 128, 256 or 512 (0 = by the
-                             widest component: 64 up to width 7, 128 up to 10, 256 above) */
+  uint32_t group_threads; /* group tier threads per shot: 8, 16, 32, 64, 128, 256 or 512 (0 = the
+                             fewest that hold the widest component and fit the LDS) */
   uint32_t lane_store;    /* lane tier only: 0 automatic (LDS when the tables fit, else device
                              memory); 1 all tables in LDS; 2 bit tables and row staging in device
                              memory, amplitudes in LDS (width <= 5); 3 everything in device
