@@ -124,20 +124,24 @@ int tps_slot(int tps) {
   return s;
 }
 
-// Shots up to which a batch runs a width-11/12 program in the group tier
-// when the automatic choice gave it the wide tier. A lone launch of such a
-// program is faster wide, but a small one in the batch's wide sequence is
-// tail- and launch-bound, while its shots pack into a persistent group
-// bucket: on 256 random circuits (14 of them moved) the batch took 0.97x
-// the time, on 512 corpus programs of 2e4 shots (88 moved) 0.94-0.97x.
+// Shots up to which a batch runs a width-11 to 13 program in the group
+// tier when the automatic choice gave it the wide tier. A lone launch of
+// such a program is faster wide, but the batch's wide sequence is its
+// critical path while the group stream idles, and a small program's shots
+// pack into a persistent group bucket: on 256 random circuits (14 of them
+// moved) the batch took 0.97x the time, on 512 corpus programs of 2e4 shots
+// (88 moved) 0.94-0.97x; adding width 13 (512 threads per shot) took the
+// latter to 0.94x again (28 more moved) and left the former unchanged.
 // With every width-11/12 program moved, the first took 1.14x.
 constexpr uint64_t kBatchGroupMaxShots = 50000;
 
 bool group_in_batch(const ticit_hip_program* h, uint64_t shots, int lds_limit) {
   const HostTables& t = h->tab;
+  auto fits = [&](int tps) {
+    return t.max_local_k <= group_max_local_k(tps) && group_tier_lds_bytes(t.state_dim, h->group, tps) <= size_t(lds_limit);
+  };
   return h->opt.tier == 0 && h->opt.precision == 0 && !h->bit_heavy && shots <= kBatchGroupMaxShots &&
-         t.max_local_k <= group_max_local_k(256) &&
-         group_tier_lds_bytes(t.state_dim, h->group, 256) <= size_t(lds_limit);
+         (fits(256) || fits(512));
 }
 
 // Tuned threads-per-shot by bucket signature, kept for the process.
@@ -381,7 +385,8 @@ ticit_hip_status ticit_hip_batch_create(const ticit_hip_batch_item* items, size_
       // Buckets also split by the threads per shot a width needs, so every
       // item of a bucket fits the shape its widest item forces: an item
       // fits at the shape it was created for (or, moved from the wide
-      // tier, at 256), and a narrower shape only shrinks its region.
+      // tier, at 256 or, width 13, 512), and a narrower shape only shrinks its
+      // region.
       const int wcls = t.max_local_k > group_max_local_k(256) ? 2 : (t.max_local_k > group_max_local_k(64) ? 1 : 0);
       auto it = std::find_if(b->buckets.begin(), b->buckets.end(),
                              [&](const ticit_hip_batch::Bucket& k) {
