@@ -132,8 +132,11 @@ def main() -> int:
             simds = props_simds()
             if prof.get("valu_insts_per_launch") and simds:
                 cycles = simds * SIMD_CLOCK_HZ * avg_kernel_s
+                # The sequencer serves one SIMD per cycle, so each SIMD issues at
+                # most one scalar instruction every 4 cycles as well (the
+                # CU's scalar unit takes one per cycle).
                 issue = {"valu_busy_frac": prof["valu_insts_per_launch"] * 4 / cycles,
-                         "salu_busy_frac": prof["salu_insts_per_launch"] / cycles,
+                         "salu_busy_frac": prof["salu_insts_per_launch"] * 4 / cycles,
                          "simds": simds, "clock_hz": SIMD_CLOCK_HZ,
                          "counters": "SQ_INSTS_VALU / SQ_INSTS_SALU per launch from the kernel's profile"}
 
