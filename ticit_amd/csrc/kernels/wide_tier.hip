@@ -917,16 +917,68 @@ __global__ __launch_bounds__(kWideThreads) void wide_expect_factored_kernel(DevP
   const T* R = F.re[f] + shot;
   const T* I = F.im[f] + shot;
   double n = 0.0, tr = 0.0, ti = 0.0;
+  if (fmask < 64 * kWidePer) {
+    // A factor of at most 512 entries is one wave's work: the other waves
+    // leave at once (the branch is the same in the whole block) and the sums
+    // reduce by shuffles alone, with no barrier.
+    if (threadIdx.x >= 64) return;
 #pragma unroll
-  for (int u = 0; u < kWidePer; ++u) {
-    const size_t b = base + size_t(u) * kWideThreads + threadIdx.x;
-    if (b > fmask) break;
-    const double r = double(R[b]), m = double(I[b]);
-    const double r2 = double(R[b ^ xf]), m2 = double(I[b ^ xf]);
-    const double sg = parity64(uint64_t(b) & zf) ? -1.0 : 1.0;
-    n += r * r + m * m;
-    tr += sg * (r * r2 + m * m2);
-    ti += sg * (r * m2 - m * r2);
+    for (int u = 0; u < kWidePer; ++u) {
+      const size_t b = size_t(u) * 64 + threadIdx.x;
+      if (b > fmask) break;
+      const double r = double(R[b]), m = double(I[b]);
+      const double r2 = double(R[b ^ xf]), m2 = double(I[b ^ xf]);
+      const double sg = parity64(uint64_t(b) & zf) ? -1.0 : 1.0;
+      n += r * r + m * m;
+      tr += sg * (r * r2 + m * m2);
+      ti += sg * (r * m2 - m * r2);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      n += __shfl_xor(n, off, 64);
+      tr += __shfl_xor(tr, off, 64);
+      ti += __shfl_xor(ti, off, 64);
+    }
+    if (threadIdx.x == 0) {
+      part[(size_t(3) * blockIdx.x) * nshots + j] = n;
+      part[(size_t(3) * blockIdx.x + 1) * nshots + j] = tr;
+      part[(size_t(3) * blockIdx.x + 2) * nshots + j] = ti;
+    }
+    return;
+  }
+  if (xf == 0) {
+#pragma unroll
+    for (int u = 0; u < kWidePer; ++u) {
+      const size_t b = base + size_t(u) * kWideThreads + threadIdx.x;
+      if (b > fmask) break;
+      const double r = double(R[b]), m = double(I[b]);
+      const double sg = parity64(uint64_t(b) & zf) ? -1.0 : 1.0;
+      n += r * r + m * m;
+      tr += sg * (r * r + m * m);
+      ti += sg * (r * m - m * r);
+    }
+  } else {
+    // The entries pair up as (b, b ^ x_f), b with the lowest bit of x_f
+    // clear: one thread reads both once and adds both terms, so the pass
+    // reads each amplitude once (a thread per entry read each twice). The
+    // block's tile of kWideTile entries is kWideTile / 2 pairs.
+    const int low = __builtin_ctzll(xf);
+    const size_t pbase = base >> 1, npairs = (fmask + 1) >> 1;
+#pragma unroll
+    for (int u = 0; u < kWidePer / 2; ++u) {
+      const size_t q = pbase + size_t(u) * kWideThreads + threadIdx.x;
+      if (q >= npairs) break;
+      const size_t b0 = insert_zero_bit(q, low), b1 = b0 ^ xf;
+      const double r0 = double(R[b0]), m0 = double(I[b0]);
+      const double r1 = double(R[b1]), m1 = double(I[b1]);
+      const double s0 = parity64(uint64_t(b0) & zf) ? -1.0 : 1.0;
+      const double s1 = parity64(uint64_t(b1) & zf) ? -1.0 : 1.0;
+      n += r0 * r0 + m0 * m0;
+      n += r1 * r1 + m1 * m1;
+      tr += s0 * (r0 * r1 + m0 * m1);
+      tr += s1 * (r1 * r0 + m1 * m0);
+      ti += s0 * (r0 * m1 - m0 * r1);
+      ti += s1 * (r1 * m0 - m1 * r0);
+    }
   }
   for (int off = 32; off > 0; off >>= 1) {
     n += __shfl_xor(n, off, 64);
