@@ -1117,6 +1117,87 @@ __global__ __launch_bounds__(kWideThreads) void wide_project_factored_kernel(Dev
   }
 }
 
+// The collapse from the factors (wide_project_factored_kernel's values) fused
+// with the probability of the next measurement of the same component, as
+// wide_project_measure_kernel fuses them for a stored state: each thread
+// writes a pair (a, b) of the next measurement's sources and sums that
+// measurement's terms from the stored values.
+template <class T>
+__global__ __launch_bounds__(kWideThreads) void wide_project_factored_measure_kernel(
+    DevProgram P, int instr, int next, const WideCtl* ctl, Factors<T> F, T* ore, T* oim, size_t dim, int k,
+    double* part, double* npart, uint32_t max_tiles) {
+  __shared__ double red[2][kWideThreads / 64];
+  const uint32_t j = blockIdx.y;
+  if (ctl[j].dead) return;
+  const size_t shot = size_t(j) * dim;
+  const DevInstr in = P.instrs[instr];
+  const DevInstr nx = P.instrs[next];
+  const bool branch = ctl[j].branch;
+  const double inv = ctl[j].invnorm;
+  T* OR = ore + shot + in.comp_off;
+  T* OI = oim + shot + in.comp_off;
+  const bool diag = in.op & kFlagDiagonal, ndiag = nx.op & kFlagDiagonal;
+  // collapsed amplitude idx of `in` from the factors, rounded as stored
+  auto collapsed = [&](size_t idx, T& r, T& m) {
+    double rr, mm;
+    if (diag) {
+      factor_amp(F, shot, diagonal_source(in, idx, branch), rr, mm);
+      r = T(rr * inv);
+      m = T(mm * inv);
+    } else {
+      const size_t s0 = insert_zero_bit(idx, in.pivot), s1 = s0 ^ size_t(in.xmask);
+      double r1, m1;
+      factor_amp(F, shot, s0, rr, mm);
+      factor_amp(F, shot, s1, r1, m1);
+      const bool odd = parity64(uint64_t(s0) & in.zmask);
+      const double dir = (branch != odd) ? -1.0 : 1.0;
+      const double c1r = dir * in.c1_re, c1i = dir * in.c1_im;
+      const double ar = kInvSqrt2 * rr + c1r * r1 - c1i * m1;
+      const double ai = kInvSqrt2 * mm + c1r * m1 + c1i * r1;
+      r = T(ar * inv);
+      m = T(ai * inv);
+    }
+  };
+  const size_t pairs = ((size_t(1) << k) >> 1) >> 1;  // next's output half
+  double acc = 0.0, nacc = 0.0;
+#pragma unroll
+  for (int u = 0; u < kWidePairs; ++u) {
+    const size_t q = size_t(blockIdx.x) * (kWideTile / 2) + size_t(u) * kWideThreads + threadIdx.x;
+    if (q >= pairs) break;
+    size_t a, b;  // a: the amplitude next's P(true) reads (nondiagonal: with b)
+    if (ndiag) {
+      a = diagonal_source(nx, q, true);
+      b = diagonal_source(nx, q, false);
+    } else {
+      a = insert_zero_bit(q, nx.pivot);
+      b = a ^ size_t(nx.xmask);
+    }
+    T ra, ma, rb, mb;
+    collapsed(a, ra, ma);
+    collapsed(b, rb, mb);
+    OR[a] = ra;
+    OI[a] = ma;
+    OR[b] = rb;
+    OI[b] = mb;
+    if (npart) nacc += double(ra) * double(ra) + double(ma) * double(ma) + double(rb) * double(rb) +
+                       double(mb) * double(mb);
+    if (ndiag) {
+      acc += double(ra) * double(ra) + double(ma) * double(ma);
+    } else {
+      const double dir = parity64(uint64_t(a) & nx.zmask) ? 1.0 : -1.0;
+      const double c1r = dir * nx.c1_re, c1i = dir * nx.c1_im;
+      const double ar = kInvSqrt2 * double(ra) + c1r * double(rb) - c1i * double(mb);
+      const double ai = kInvSqrt2 * double(ma) + c1r * double(mb) + c1i * double(rb);
+      acc += ar * ar + ai * ai;
+    }
+  }
+  block_sum2(acc, nacc, red);
+  if (threadIdx.x == 0) {
+    part[size_t(j) * max_tiles + blockIdx.x] = acc;
+    if (npart) npart[size_t(j) * max_tiles + blockIdx.x] = nacc;
+  }
+}
+
 // Observables, counts, error flags and packed rows.
 __global__ __launch_bounds__(kWideThreads) void wide_finish_kernel(DevProgram P, DevLaunch L, WideLayout Y,
                                                                    uint64_t batch_start, uint32_t nshots,
@@ -1385,9 +1466,22 @@ static hipError_t run_wide(const DevProgram& P, const DevInstr* instrs, const De
           WCK(hipGetLastError());
           size_t out_dim = (size_t(1) << W) >> 1;
           unsigned tiles = unsigned((out_dim + kWideTile - 1) / kWideTile);
-          dim3 g(tiles, n);
-          hipLaunchKernelGGL(wide_project_factored_kernel<T>, g, dim3(kWideThreads), 0, stream, P, e2, ws.ctl, F,
-                             RE(cb ^ 1), IM(cb ^ 1), dim, W);
+          // the next measurement of the merged component, if it follows at
+          // once, has its probability summed by the collapse
+          const DevInstr* after = e2 + 1 < P.ninstr ? &instrs[e2 + 1] : nullptr;
+          const size_t fpairs = out_dim >> 1;
+          const unsigned ftiles = unsigned((fpairs + kWideTile / 2 - 1) / (kWideTile / 2));
+          if (after && (after->op & 0xff) == kOpMeasure && !(after->op & kFlagExpVal) && after->comp == in.comp &&
+              after->nq == W - 1 && fpairs > 0 && ftiles <= Y.max_tiles) {
+            hipLaunchKernelGGL(wide_project_factored_measure_kernel<T>, dim3(ftiles, n), dim3(kWideThreads), 0,
+                               stream, P, e2, e2 + 1, ws.ctl, F, RE(cb ^ 1), IM(cb ^ 1), dim, W, ws.part, npart,
+                               Y.max_tiles);
+            fused_tiles = ftiles;
+          } else {
+            dim3 g(tiles, n);
+            hipLaunchKernelGGL(wide_project_factored_kernel<T>, g, dim3(kWideThreads), 0, stream, P, e2, ws.ctl, F,
+                               RE(cb ^ 1), IM(cb ^ 1), dim, W);
+          }
           WCK(hipGetLastError());
           cur[size_t(in.comp)] ^= 1;
           i = e2 + 1;
