@@ -1371,6 +1371,17 @@ static hipError_t run_wide(const DevProgram& P, const DevInstr* instrs, const De
     WCK(hipGetLastError());
     int k = P.initial_k;  // dense width, for the final-state dump
     unsigned fused_tiles = 0;  // > 0: the next measurement's partials are in ws.part
+    // The first instruction at or after e that is not a record, detector or
+    // coin: bit steps leave the amplitudes and the partials alone, so a
+    // collapse can sum the probability of a measurement behind them.
+    auto next_amplitude_step = [&](int e) -> const DevInstr* {
+      while (e < P.ninstr) {
+        const uint32_t o = instrs[e].op & 0xff;
+        if (o != kOpRecord && o != kOpDetector && o != kOpCoin) return &instrs[e];
+        ++e;
+      }
+      return nullptr;
+    };
     int i = 0;
     while (i < P.ninstr) {
       const DevInstr& in = instrs[i];
@@ -1468,14 +1479,14 @@ static hipError_t run_wide(const DevProgram& P, const DevInstr* instrs, const De
           unsigned tiles = unsigned((out_dim + kWideTile - 1) / kWideTile);
           // the next measurement of the merged component, if it follows at
           // once, has its probability summed by the collapse
-          const DevInstr* after = e2 + 1 < P.ninstr ? &instrs[e2 + 1] : nullptr;
+          const DevInstr* after = next_amplitude_step(e2 + 1);
           const size_t fpairs = out_dim >> 1;
           const unsigned ftiles = unsigned((fpairs + kWideTile / 2 - 1) / (kWideTile / 2));
           if (after && (after->op & 0xff) == kOpMeasure && !(after->op & kFlagExpVal) && after->comp == in.comp &&
               after->nq == W - 1 && fpairs > 0 && ftiles <= Y.max_tiles) {
             hipLaunchKernelGGL(wide_project_factored_measure_kernel<T>, dim3(ftiles, n), dim3(kWideThreads), 0,
-                               stream, P, e2, e2 + 1, ws.ctl, F, RE(cb ^ 1), IM(cb ^ 1), dim, W, ws.part, npart,
-                               Y.max_tiles);
+                               stream, P, e2, int(after - instrs), ws.ctl, F, RE(cb ^ 1), IM(cb ^ 1), dim, W, ws.part,
+                               npart, Y.max_tiles);
             fused_tiles = ftiles;
           } else {
             dim3 g(tiles, n);
@@ -1522,13 +1533,14 @@ static hipError_t run_wide(const DevProgram& P, const DevInstr* instrs, const De
           ++i;
           continue;
         }
-        const DevInstr* nx = i + 1 < P.ninstr ? &instrs[i + 1] : nullptr;
+        const DevInstr* nx = next_amplitude_step(i + 1);
         const size_t pairs = out_dim >> 1;
         const unsigned ftiles = unsigned((pairs + kWideTile / 2 - 1) / (kWideTile / 2));
         if (nx && (nx->op & 0xff) == kOpMeasure && !(nx->op & kFlagExpVal) && nx->comp == in.comp &&
             nx->nq == w - 1 && pairs > 0 && ftiles <= Y.max_tiles) {
           hipLaunchKernelGGL(wide_project_measure_kernel<T>, dim3(ftiles, n), dim3(kWideThreads), 0, stream, P, i,
-                             i + 1, ws.ctl, R, I, RE(cb ^ 1), IM(cb ^ 1), dim, w, ws.part, npart, Y.max_tiles);
+                             int(nx - instrs), ws.ctl, R, I, RE(cb ^ 1), IM(cb ^ 1), dim, w, ws.part, npart,
+                             Y.max_tiles);
           fused_tiles = ftiles;
         } else {
           hipLaunchKernelGGL(wide_project_kernel<T>, g, dim3(kWideThreads), 0, stream, P, i, ws.ctl, R, I,
