@@ -4,8 +4,8 @@
 // barriers never wait on another shot); TPS of 8, 16 or 32 packs 8, 4 or 2
 // shots into one wave, reduced with shuffles inside each lane group, which
 // spreads the interpreter's per-instruction work over several shots for
-// small states. TPS = 256 is the default launch; 512 holds states of 2^13
-// (128 KiB FP64).
+// small states. A plain launch takes the fewest threads that hold its
+// widest component; 512 holds states of 2^13 (128 KiB FP64).
 // Projections go through registers, PROJ outputs per thread (kProjPer at
 // most), so a shot group of TPS threads handles local widths up to
 // log2(2 * PROJ * TPS). Each launch picks the smallest PROJ that covers its
