@@ -153,3 +153,26 @@ def test_moved_and_explicit_wide_group_shapes_share_a_batch(hip):
         for a, r in zip(out, ref):
             assert torch.equal(a, r), s.max_local_k
     b.close()
+
+
+def test_wide_items_in_two_sequences_equal_single_runs(hip):
+    """Wide items of a batch are split over two captured sequences on two
+    streams (by estimated bytes), each with its own workspace; every item's
+    rows and counts equal its single-program run, also with FP32 amplitude
+    storage."""
+    batch = W.config2_batch()
+    for precision in ("fp64", "fp32"):
+        samplers = [hip.Circuit(batch[i][0]).compile(tier="wide", precision=precision) for i in (3, 9, 27, 212, 250)]
+        shots = [700, 1300, 500, 900, 1100]
+        seeds = [5 + i for i in range(len(samplers))]
+        outs = [buffers(s, n) for s, n in zip(samplers, shots)]
+        b = hip.Batch(samplers, shots, seeds, outs, autotune=False)
+        assert b.info()["wide_items"] == len(samplers)
+        b.launch(3)
+        b.wait()
+        for s, n, sd, out in zip(samplers, shots, seeds, outs):
+            ref = buffers(s, n)
+            s.sample_device(n, sd + 3, ref)
+            torch.cuda.synchronize()
+            for a, r in zip(out, ref):
+                assert torch.equal(a, r), (precision, s.max_active_qubits)

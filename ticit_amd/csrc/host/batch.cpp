@@ -21,6 +21,7 @@
 #include <mutex>
 #include <string>
 #include <tuple>
+#include <utility>
 #include <vector>
 
 #include "../kernels/compact.hpp"
@@ -69,15 +70,22 @@ struct ticit_hip_batch {
   uint64_t* d_seed_offset = nullptr;
   int* d_error = nullptr;
   WideWorkspace* ws = nullptr;
+  // Wide items run in two sequences on two streams, split by their
+  // estimated HBM bytes, each with its own workspace: one sequence's small,
+  // latency-bound kernels then overlap the other's (empty: one sequence).
+  std::vector<uint32_t> wide_items2;
+  WideWorkspace* ws2 = nullptr;
   // lane items with tables in device memory share these regions: they run
   // one after another on `main`
   LaneWorkspace lane_ws{};
-  hipStream_t main = nullptr, side = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr, join_main = nullptr;
+  hipStream_t main = nullptr, side = nullptr, side2 = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr, join_main = nullptr, join2 = nullptr;
   hipGraph_t graph = nullptr;        // lane and group work
   hipGraphExec_t exec = nullptr;
   hipGraph_t wide_graph = nullptr;   // wide work, replayed on `side` beside `graph`
   hipGraphExec_t wide_exec = nullptr;
+  hipGraph_t wide_graph2 = nullptr;  // the second wide sequence, on `side2`
+  hipGraphExec_t wide_exec2 = nullptr;
   hipStream_t last = nullptr;
   bool split = false;  // main/side are CU-masked halves of the device
   int group_cus = 0;   // CUs the group/lane stream may use when split
@@ -85,16 +93,22 @@ struct ticit_hip_batch {
   ~ticit_hip_batch() {
     if (last) (void)hipStreamSynchronize(last);
     if (side) (void)hipStreamSynchronize(side);
+    if (side2) (void)hipStreamSynchronize(side2);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (wide_exec) (void)hipGraphExecDestroy(wide_exec);
     if (wide_graph) (void)hipGraphDestroy(wide_graph);
+    if (wide_exec2) (void)hipGraphExecDestroy(wide_exec2);
+    if (wide_graph2) (void)hipGraphDestroy(wide_graph2);
+    if (join2) (void)hipEventDestroy(join2);
     if (fork) (void)hipEventDestroy(fork);
     if (join) (void)hipEventDestroy(join);
     if (join_main) (void)hipEventDestroy(join_main);
     if (main) (void)hipStreamDestroy(main);
     if (side) (void)hipStreamDestroy(side);
+    if (side2) (void)hipStreamDestroy(side2);
     wide_workspace_free(ws);
+    wide_workspace_free(ws2);
     for (void* p : allocs) (void)hipFree(p);
   }
 
@@ -479,15 +493,55 @@ ticit_hip_status ticit_hip_batch_create(const ticit_hip_batch_item* items, size_
     }
   }
 
-  // Wide items share one workspace, sized before capture.
-  if (!b->wide_items.empty()) {
-    b->ws = wide_workspace_new();
-    for (uint32_t i : b->wide_items) {
+  // Wide items: two sequences balanced by their estimated HBM bytes
+  // (largest first, each to the lighter sequence), each with a workspace
+  // sized before capture. Where the second workspace cannot be allocated
+  // every item runs in the first sequence.
+  auto prepare = [&](WideWorkspace* w, const std::vector<uint32_t>& items) {
+    for (uint32_t i : items) {
       ticit_hip_program* h = b->progs[i];
       uint32_t batch = uint32_t(std::min<uint64_t>(h->wide_batch, b->launches[i].shots));
-      if ((e = wide_workspace_prepare(b->ws, batch, h->tab.state_dim, h->opt.precision == 1, h->wide)) != hipSuccess)
-        return bail_hip(e, "allocating the wide workspace");
+      hipError_t r = wide_workspace_prepare(w, batch, h->tab.state_dim, h->opt.precision == 1, h->wide);
+      if (r != hipSuccess) return r;
     }
+    return hipSuccess;
+  };
+  if (!b->split && b->wide_items.size() >= 2) {
+    std::vector<std::pair<double, uint32_t>> cost;
+    for (uint32_t i : b->wide_items) {
+      const ticit_hip_program* h = b->progs[i];
+      cost.push_back({h->tab.amp_traffic * (h->opt.precision == 1 ? 8.0 : 16.0) * double(b->launches[i].shots), i});
+    }
+    std::stable_sort(cost.begin(), cost.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+    std::vector<uint32_t> one, two;
+    double c1 = 0.0, c2 = 0.0;
+    for (const auto& [c, i] : cost) {
+      if (c1 <= c2) {
+        one.push_back(i);
+        c1 += c;
+      } else {
+        two.push_back(i);
+        c2 += c;
+      }
+    }
+    // program order within each sequence
+    std::sort(one.begin(), one.end());
+    std::sort(two.begin(), two.end());
+    b->ws2 = wide_workspace_new();
+    if (prepare(b->ws2, two) == hipSuccess &&
+        hipStreamCreateWithFlags(&b->side2, hipStreamNonBlocking) == hipSuccess &&
+        hipEventCreateWithFlags(&b->join2, hipEventDisableTiming) == hipSuccess) {
+      b->wide_items = one;
+      b->wide_items2 = two;
+    } else {
+      (void)hipGetLastError();  // the failed allocation's error, not a sticky one
+      wide_workspace_free(b->ws2);
+      b->ws2 = nullptr;
+    }
+  }
+  if (!b->wide_items.empty()) {
+    b->ws = wide_workspace_new();
+    if ((e = prepare(b->ws, b->wide_items)) != hipSuccess) return bail_hip(e, "allocating the wide workspace");
   }
 
   // Capture: the lane and group work into one graph, the wide work into
@@ -537,6 +591,22 @@ ticit_hip_status ticit_hip_batch_create(const ticit_hip_batch_item* items, size_
     if ((e = hipGraphInstantiate(&b->wide_exec, b->wide_graph, nullptr, nullptr, 0)) != hipSuccess)
       return bail_hip(e, "instantiating the batch graph");
   }
+  if (!b->wide_items2.empty()) {
+    if ((e = hipStreamBeginCapture(b->side2, hipStreamCaptureModeRelaxed)) != hipSuccess)
+      return bail_hip(e, "beginning graph capture");
+    for (uint32_t i : b->wide_items2) {
+      ticit_hip_program* h = b->progs[i];
+      uint32_t batch = uint32_t(std::min<uint64_t>(h->wide_batch, b->launches[i].shots));
+      step(launch_wide_tier(h->dev, h->tab.instrs.data(), b->launches[i], h->wide, batch, h->opt.precision == 1,
+                            b->ws2, b->side2, nullptr, nullptr, nullptr, false));
+      compact_item(i, b->side2);
+    }
+    e = hipStreamEndCapture(b->side2, &b->wide_graph2);
+    if (ce != hipSuccess) return bail_hip(ce, "capturing the batch");
+    if (e != hipSuccess) return bail_hip(e, "ending graph capture");
+    if ((e = hipGraphInstantiate(&b->wide_exec2, b->wide_graph2, nullptr, nullptr, 0)) != hipSuccess)
+      return bail_hip(e, "instantiating the batch graph");
+  }
   *out = b;
   return TICIT_HIP_OK;
 }
@@ -555,6 +625,12 @@ ticit_hip_status ticit_hip_batch_launch(ticit_hip_batch* b, uint64_t seed_offset
     if ((e = hipGraphLaunch(b->wide_exec, b->side)) != hipSuccess) return from_hip(e, "launching the batch graph");
     if ((e = hipEventRecord(b->join, b->side)) != hipSuccess) return from_hip(e, "joining the batch");
   }
+  if (b->wide_exec2) {
+    if (!b->wide_exec && (e = hipEventRecord(b->fork, s)) != hipSuccess) return from_hip(e, "forking the batch");
+    if ((e = hipStreamWaitEvent(b->side2, b->fork, 0)) != hipSuccess) return from_hip(e, "forking the batch");
+    if ((e = hipGraphLaunch(b->wide_exec2, b->side2)) != hipSuccess) return from_hip(e, "launching the batch graph");
+    if ((e = hipEventRecord(b->join2, b->side2)) != hipSuccess) return from_hip(e, "joining the batch");
+  }
   if (b->split) {
     // the group and lane graph runs on its CU-masked stream beside the wide one
     if ((e = hipStreamWaitEvent(b->main, b->fork, 0)) != hipSuccess) return from_hip(e, "forking the batch");
@@ -565,6 +641,7 @@ ticit_hip_status ticit_hip_batch_launch(ticit_hip_batch* b, uint64_t seed_offset
     return from_hip(e, "launching the batch graph");
   }
   if (b->wide_exec && (e = hipStreamWaitEvent(s, b->join, 0)) != hipSuccess) return from_hip(e, "joining the batch");
+  if (b->wide_exec2 && (e = hipStreamWaitEvent(s, b->join2, 0)) != hipSuccess) return from_hip(e, "joining the batch");
   b->last = s;
   return TICIT_HIP_OK;
 }
@@ -588,7 +665,7 @@ ticit_hip_status ticit_hip_batch_info_get(const ticit_hip_batch* b, ticit_hip_ba
   *out = ticit_hip_batch_info{};
   out->items = b->progs.size();
   out->lane_items = b->lane_items.size();
-  out->wide_items = b->wide_items.size();
+  out->wide_items = b->wide_items.size() + b->wide_items2.size();
   out->group_buckets = b->buckets.size();
   size_t n = std::min<size_t>(b->buckets.size(), sizeof(out->bucket_tps) / sizeof(out->bucket_tps[0]));
   for (size_t j = 0; j < n; ++j) {
