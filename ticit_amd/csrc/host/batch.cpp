@@ -70,22 +70,28 @@ struct ticit_hip_batch {
   uint64_t* d_seed_offset = nullptr;
   int* d_error = nullptr;
   WideWorkspace* ws = nullptr;
-  // Wide items run in two sequences on two streams, split by their
-  // estimated HBM bytes, each with its own workspace: one sequence's small,
-  // latency-bound kernels then overlap the other's (empty: one sequence).
-  std::vector<uint32_t> wide_items2;
-  WideWorkspace* ws2 = nullptr;
+  // Wide items run in kWideSequences sequences on as many streams, split
+  // by their estimated HBM bytes, each with its own workspace: one
+  // sequence's small, latency-bound kernels then overlap another's. The
+  // first is wide_items on `side`; these are the others.
+  struct WideSeq {
+    std::vector<uint32_t> items;
+    WideWorkspace* ws = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t join = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<WideSeq> more;
   // lane items with tables in device memory share these regions: they run
   // one after another on `main`
   LaneWorkspace lane_ws{};
-  hipStream_t main = nullptr, side = nullptr, side2 = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr, join_main = nullptr, join2 = nullptr;
+  hipStream_t main = nullptr, side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr, join_main = nullptr;
   hipGraph_t graph = nullptr;        // lane and group work
   hipGraphExec_t exec = nullptr;
   hipGraph_t wide_graph = nullptr;   // wide work, replayed on `side` beside `graph`
   hipGraphExec_t wide_exec = nullptr;
-  hipGraph_t wide_graph2 = nullptr;  // the second wide sequence, on `side2`
-  hipGraphExec_t wide_exec2 = nullptr;
   hipStream_t last = nullptr;
   bool split = false;  // main/side are CU-masked halves of the device
   int group_cus = 0;   // CUs the group/lane stream may use when split
@@ -93,22 +99,25 @@ struct ticit_hip_batch {
   ~ticit_hip_batch() {
     if (last) (void)hipStreamSynchronize(last);
     if (side) (void)hipStreamSynchronize(side);
-    if (side2) (void)hipStreamSynchronize(side2);
+    for (auto& q : more)
+      if (q.stream) (void)hipStreamSynchronize(q.stream);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (wide_exec) (void)hipGraphExecDestroy(wide_exec);
     if (wide_graph) (void)hipGraphDestroy(wide_graph);
-    if (wide_exec2) (void)hipGraphExecDestroy(wide_exec2);
-    if (wide_graph2) (void)hipGraphDestroy(wide_graph2);
-    if (join2) (void)hipEventDestroy(join2);
+    for (auto& q : more) {
+      if (q.exec) (void)hipGraphExecDestroy(q.exec);
+      if (q.graph) (void)hipGraphDestroy(q.graph);
+      if (q.join) (void)hipEventDestroy(q.join);
+      if (q.stream) (void)hipStreamDestroy(q.stream);
+      wide_workspace_free(q.ws);
+    }
     if (fork) (void)hipEventDestroy(fork);
     if (join) (void)hipEventDestroy(join);
     if (join_main) (void)hipEventDestroy(join_main);
     if (main) (void)hipStreamDestroy(main);
     if (side) (void)hipStreamDestroy(side);
-    if (side2) (void)hipStreamDestroy(side2);
     wide_workspace_free(ws);
-    wide_workspace_free(ws2);
     for (void* p : allocs) (void)hipFree(p);
   }
 
@@ -148,6 +157,10 @@ int tps_slot(int tps) {
 // latter to 0.94x again (28 more moved) and left the former unchanged.
 // With every width-11/12 program moved, the first took 1.14x.
 constexpr uint64_t kBatchGroupMaxShots = 50000;
+
+// Wide sequences of a batch, each on its own stream (C2 and C4 with two:
+// 0.93x and 0.94x the time of one).
+constexpr size_t kWideSequences = 2;
 
 bool group_in_batch(const ticit_hip_program* h, uint64_t shots, int lds_limit) {
   const HostTables& t = h->tab;
@@ -513,30 +526,30 @@ ticit_hip_status ticit_hip_batch_create(const ticit_hip_batch_item* items, size_
       cost.push_back({h->tab.amp_traffic * (h->opt.precision == 1 ? 8.0 : 16.0) * double(b->launches[i].shots), i});
     }
     std::stable_sort(cost.begin(), cost.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
-    std::vector<uint32_t> one, two;
-    double c1 = 0.0, c2 = 0.0;
+    const size_t nseq = std::min<size_t>(kWideSequences, cost.size());
+    std::vector<std::vector<uint32_t>> seq(nseq);
+    std::vector<double> load(nseq, 0.0);
     for (const auto& [c, i] : cost) {
-      if (c1 <= c2) {
-        one.push_back(i);
-        c1 += c;
-      } else {
-        two.push_back(i);
-        c2 += c;
-      }
+      const size_t t = size_t(std::min_element(load.begin(), load.end()) - load.begin());
+      seq[t].push_back(i);
+      load[t] += c;
     }
-    // program order within each sequence
-    std::sort(one.begin(), one.end());
-    std::sort(two.begin(), two.end());
-    b->ws2 = wide_workspace_new();
-    if (prepare(b->ws2, two) == hipSuccess &&
-        hipStreamCreateWithFlags(&b->side2, hipStreamNonBlocking) == hipSuccess &&
-        hipEventCreateWithFlags(&b->join2, hipEventDisableTiming) == hipSuccess) {
-      b->wide_items = one;
-      b->wide_items2 = two;
-    } else {
-      (void)hipGetLastError();  // the failed allocation's error, not a sticky one
-      wide_workspace_free(b->ws2);
-      b->ws2 = nullptr;
+    for (auto& q : seq) std::sort(q.begin(), q.end());  // program order within a sequence
+    b->wide_items = seq[0];
+    for (size_t t = 1; t < nseq; ++t) {
+      ticit_hip_batch::WideSeq q;
+      q.ws = wide_workspace_new();
+      if (prepare(q.ws, seq[t]) == hipSuccess && hipStreamCreateWithFlags(&q.stream, hipStreamNonBlocking) == hipSuccess &&
+          hipEventCreateWithFlags(&q.join, hipEventDisableTiming) == hipSuccess) {
+        q.items = seq[t];
+        b->more.push_back(q);
+      } else {
+        (void)hipGetLastError();  // the failed allocation's error, not a sticky one
+        if (q.stream) (void)hipStreamDestroy(q.stream);
+        wide_workspace_free(q.ws);
+        b->wide_items.insert(b->wide_items.end(), seq[t].begin(), seq[t].end());
+        std::sort(b->wide_items.begin(), b->wide_items.end());
+      }
     }
   }
   if (!b->wide_items.empty()) {
@@ -591,20 +604,20 @@ ticit_hip_status ticit_hip_batch_create(const ticit_hip_batch_item* items, size_
     if ((e = hipGraphInstantiate(&b->wide_exec, b->wide_graph, nullptr, nullptr, 0)) != hipSuccess)
       return bail_hip(e, "instantiating the batch graph");
   }
-  if (!b->wide_items2.empty()) {
-    if ((e = hipStreamBeginCapture(b->side2, hipStreamCaptureModeRelaxed)) != hipSuccess)
+  for (auto& q : b->more) {
+    if ((e = hipStreamBeginCapture(q.stream, hipStreamCaptureModeRelaxed)) != hipSuccess)
       return bail_hip(e, "beginning graph capture");
-    for (uint32_t i : b->wide_items2) {
+    for (uint32_t i : q.items) {
       ticit_hip_program* h = b->progs[i];
       uint32_t batch = uint32_t(std::min<uint64_t>(h->wide_batch, b->launches[i].shots));
       step(launch_wide_tier(h->dev, h->tab.instrs.data(), b->launches[i], h->wide, batch, h->opt.precision == 1,
-                            b->ws2, b->side2, nullptr, nullptr, nullptr, false));
-      compact_item(i, b->side2);
+                            q.ws, q.stream, nullptr, nullptr, nullptr, false));
+      compact_item(i, q.stream);
     }
-    e = hipStreamEndCapture(b->side2, &b->wide_graph2);
+    e = hipStreamEndCapture(q.stream, &q.graph);
     if (ce != hipSuccess) return bail_hip(ce, "capturing the batch");
     if (e != hipSuccess) return bail_hip(e, "ending graph capture");
-    if ((e = hipGraphInstantiate(&b->wide_exec2, b->wide_graph2, nullptr, nullptr, 0)) != hipSuccess)
+    if ((e = hipGraphInstantiate(&q.exec, q.graph, nullptr, nullptr, 0)) != hipSuccess)
       return bail_hip(e, "instantiating the batch graph");
   }
   *out = b;
@@ -625,11 +638,10 @@ ticit_hip_status ticit_hip_batch_launch(ticit_hip_batch* b, uint64_t seed_offset
     if ((e = hipGraphLaunch(b->wide_exec, b->side)) != hipSuccess) return from_hip(e, "launching the batch graph");
     if ((e = hipEventRecord(b->join, b->side)) != hipSuccess) return from_hip(e, "joining the batch");
   }
-  if (b->wide_exec2) {
-    if (!b->wide_exec && (e = hipEventRecord(b->fork, s)) != hipSuccess) return from_hip(e, "forking the batch");
-    if ((e = hipStreamWaitEvent(b->side2, b->fork, 0)) != hipSuccess) return from_hip(e, "forking the batch");
-    if ((e = hipGraphLaunch(b->wide_exec2, b->side2)) != hipSuccess) return from_hip(e, "launching the batch graph");
-    if ((e = hipEventRecord(b->join2, b->side2)) != hipSuccess) return from_hip(e, "joining the batch");
+  for (auto& q : b->more) {  // the first sequence exists whenever these do, and recorded `fork`
+    if ((e = hipStreamWaitEvent(q.stream, b->fork, 0)) != hipSuccess) return from_hip(e, "forking the batch");
+    if ((e = hipGraphLaunch(q.exec, q.stream)) != hipSuccess) return from_hip(e, "launching the batch graph");
+    if ((e = hipEventRecord(q.join, q.stream)) != hipSuccess) return from_hip(e, "joining the batch");
   }
   if (b->split) {
     // the group and lane graph runs on its CU-masked stream beside the wide one
@@ -641,7 +653,8 @@ ticit_hip_status ticit_hip_batch_launch(ticit_hip_batch* b, uint64_t seed_offset
     return from_hip(e, "launching the batch graph");
   }
   if (b->wide_exec && (e = hipStreamWaitEvent(s, b->join, 0)) != hipSuccess) return from_hip(e, "joining the batch");
-  if (b->wide_exec2 && (e = hipStreamWaitEvent(s, b->join2, 0)) != hipSuccess) return from_hip(e, "joining the batch");
+  for (auto& q : b->more)
+    if ((e = hipStreamWaitEvent(s, q.join, 0)) != hipSuccess) return from_hip(e, "joining the batch");
   b->last = s;
   return TICIT_HIP_OK;
 }
@@ -665,7 +678,8 @@ ticit_hip_status ticit_hip_batch_info_get(const ticit_hip_batch* b, ticit_hip_ba
   *out = ticit_hip_batch_info{};
   out->items = b->progs.size();
   out->lane_items = b->lane_items.size();
-  out->wide_items = b->wide_items.size() + b->wide_items2.size();
+  out->wide_items = b->wide_items.size();
+  for (const auto& q : b->more) out->wide_items += q.items.size();
   out->group_buckets = b->buckets.size();
   size_t n = std::min<size_t>(b->buckets.size(), sizeof(out->bucket_tps) / sizeof(out->bucket_tps[0]));
   for (size_t j = 0; j < n; ++j) {
