@@ -47,7 +47,8 @@ hipError_t wide_workspace_prepare(WideWorkspace* ws, uint32_t batch, uint32_t st
 // event (off inside graph capture, where the graph orders the work).
 hipError_t launch_wide_tier(const DevProgram& P, const DevInstr* host_instrs, const DevLaunch& L,
                             const WideLayout& Y, uint32_t batch, int fp32, WideWorkspace* ws, hipStream_t stream,
-                            double* dump_re, double* dump_im, int* dump_k, bool ordered = true);
+                            double* dump_re, double* dump_im, int* dump_k, bool ordered = true,
+                            const WideSecond* second = nullptr);
 
 constexpr uint32_t kTierLane = 1, kTierGroup = 2, kTierWide = 3;
 

@@ -8,6 +8,8 @@
 // bit-identical cos/sin values.
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 
 namespace ticit {
@@ -256,6 +258,15 @@ struct LaneWorkspace {
   unsigned char* base;
   size_t bytes;
   uint64_t max_slots;
+};
+
+// A second workspace and stream for a plain wide launch's odd shot batches,
+// and the event that forks that stream from the caller's (host side).
+struct WideWorkspace;
+struct WideSecond {
+  WideWorkspace* ws;
+  hipStream_t stream;
+  hipEvent_t fork;
 };
 
 struct GroupLayout {

@@ -1341,13 +1341,11 @@ size_t wide_bytes_per_shot(uint32_t state_dim, int fp32, const WideLayout& y) {
 // host copy of P.instrs.
 template <class T>
 static hipError_t run_wide(const DevProgram& P, const DevInstr* instrs, const DevLaunch& L, const WideLayout& Y,
-                           uint32_t batch, WideWorkspace& ws, hipStream_t stream, double* dump_re, double* dump_im,
-                           int* dump_k, bool ordered) {
+                           uint32_t batch, WideWorkspace& ws0, hipStream_t stream0, double* dump_re, double* dump_im,
+                           int* dump_k, bool ordered, const WideSecond* second) {
   const size_t dim = P.state_dim;
-  hipError_t e = wide_workspace_reserve(ws, batch, dim, sizeof(T), Y);
+  hipError_t e = wide_workspace_reserve(ws0, batch, dim, sizeof(T), Y);
   if (e != hipSuccess) return e;
-  // calls sharing the workspace may come on different streams: order them
-  if (ordered && (e = hipStreamWaitEvent(stream, ws.done, 0)) != hipSuccess) return e;
 #define WCK(x)             \
   do {                     \
     e = (x);               \
@@ -1355,11 +1353,32 @@ static hipError_t run_wide(const DevProgram& P, const DevInstr* instrs, const De
       return e;            \
     }                      \
   } while (0)
+  // Shot batches alternate between two workspaces on two streams when a
+  // second pair is given and the launch has more than one batch: one
+  // batch's small, latency-bound kernels then overlap the other's streaming
+  // ones. Where the second workspace cannot grow, every batch stays on the
+  // first.
+  bool two = second && second->ws && second->stream && L.shots > batch && !dump_re;
+  if (two && wide_workspace_reserve(*second->ws, batch, dim, sizeof(T), Y) != hipSuccess) {
+    (void)hipGetLastError();  // the failed allocation's error, not a sticky one
+    two = false;
+  }
+  // calls sharing a workspace may come on different streams: order them
+  if (ordered) WCK(hipStreamWaitEvent(stream0, ws0.done, 0));
+  if (two) {
+    // the second stream starts after the caller's earlier work
+    WCK(hipEventRecord(second->fork, stream0));
+    WCK(hipStreamWaitEvent(second->stream, second->fork, 0));
+    if (ordered) WCK(hipStreamWaitEvent(second->stream, second->ws->done, 0));
+  }
   const unsigned per_shot_blocks = 256;
-  // FP32 amplitudes: every measurement also sums the measured component's
-  // norm (after the probability partials) and renormalizes by it
-  double* npart = std::is_same<T, float>::value ? ws.part + size_t(batch) * Y.max_tiles : nullptr;
-  for (uint64_t start = 0; start < L.shots; start += batch) {
+  uint64_t bi = 0;
+  for (uint64_t start = 0; start < L.shots; start += batch, ++bi) {
+    WideWorkspace& ws = (two && (bi & 1)) ? *second->ws : ws0;
+    const hipStream_t stream = (two && (bi & 1)) ? second->stream : stream0;
+    // FP32 amplitudes: every measurement also sums the measured component's
+    // norm (after the probability partials) and renormalizes by it
+    double* npart = std::is_same<T, float>::value ? ws.part + size_t(batch) * Y.max_tiles : nullptr;
     uint32_t n = uint32_t(std::min<uint64_t>(batch, L.shots - start));
     dim3 sg((n + per_shot_blocks - 1) / per_shot_blocks), sb(per_shot_blocks);
     // which of the two buffers holds each component's live amplitudes
@@ -1564,17 +1583,23 @@ static hipError_t run_wide(const DevProgram& P, const DevInstr* instrs, const De
       *dump_k = k;
     }
   }
-  if (ordered) WCK(hipEventRecord(ws.done, stream));
+  if (two) {
+    // join: the caller's stream waits for the second stream's batches
+    WCK(hipEventRecord(second->ws->done, second->stream));
+    WCK(hipStreamWaitEvent(stream0, second->ws->done, 0));
+  }
+  if (ordered) WCK(hipEventRecord(ws0.done, stream0));
 #undef WCK
   return hipSuccess;
 }
 
 hipError_t launch_wide_tier(const DevProgram& P, const DevInstr* host_instrs, const DevLaunch& L,
                             const WideLayout& Y, uint32_t batch, int fp32, WideWorkspace* ws, hipStream_t stream,
-                            double* dump_re, double* dump_im, int* dump_k, bool ordered) {
+                            double* dump_re, double* dump_im, int* dump_k, bool ordered, const WideSecond* second) {
   if (!ws) return hipErrorInvalidValue;
-  if (fp32) return run_wide<float>(P, host_instrs, L, Y, batch, *ws, stream, dump_re, dump_im, dump_k, ordered);
-  return run_wide<double>(P, host_instrs, L, Y, batch, *ws, stream, dump_re, dump_im, dump_k, ordered);
+  if (fp32)
+    return run_wide<float>(P, host_instrs, L, Y, batch, *ws, stream, dump_re, dump_im, dump_k, ordered, second);
+  return run_wide<double>(P, host_instrs, L, Y, batch, *ws, stream, dump_re, dump_im, dump_k, ordered, second);
 }
 
 // Grows the workspace for `batch` shots of a program ahead of a graph
