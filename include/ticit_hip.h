@@ -217,7 +217,10 @@ ticit_hip_status ticit_hip_sample(ticit_hip_program* program, uint64_t shots, ui
  * meas/det/obs are [shots][row] bytes, observable_ones is [nobservables]
  * uint64 accumulated (not cleared) by the call, exp_vals (NULL to skip) is
  * [shots][expectation_values] doubles written with the rows. first_shot
- * offsets the per-shot streams so a large job can be split into launches. */
+ * offsets the per-shot streams so a large job can be split into launches.
+ * A wide-tier program whose shots take more than one batch of its workspace
+ * runs every other batch on a stream of the library's, forked from and
+ * joined to `stream` by events: the call stays ordered on `stream`. */
 ticit_hip_status ticit_hip_sample_device(ticit_hip_program* program, uint64_t shots, uint64_t seed,
                                          uint64_t first_shot, int bit_packed, int keep_records, uint8_t* meas,
                                          uint8_t* det, uint8_t* obs, uint64_t* observable_ones, double* exp_vals,
