@@ -1,5 +1,5 @@
-// Batched sampling of many programs (ticit_hip_batch_*): one hipGraph whose
-// launch runs every program of the batch.
+// Batched sampling of many programs (ticit_hip_batch_*): hipGraphs captured
+// once whose replay runs every program of the batch.
 //
 // Programs are grouped by the tier they run on:
 //   * group tier: buckets of programs with similar per-shot LDS footprints,
@@ -9,9 +9,9 @@
 //     per bucket by timing each candidate on a sample of the bucket's
 //     shots;
 //   * lane tier: one launch per program;
-//   * wide tier: each program's kernel sequence, in a second graph replayed
-//     on a second stream.
-// The graph is captured once at creation and replayed by every launch; a
+//   * wide tier: each program's kernel sequence, split over two more graphs
+//     replayed on two more streams (balanced by estimated HBM bytes).
+// The graphs are captured once at creation and replayed by every launch; a
 // device word added to every program's seed lets replays draw fresh shots.
 #include <hip/hip_runtime.h>
 
