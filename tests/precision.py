@@ -44,6 +44,6 @@ def fp64_sum_bound(probs: np.ndarray, uniforms: np.ndarray, width: int) -> np.nd
 
 
 # The FP32 amplitude path's bar against the FP64 oracle: probabilities (the
-# north-star target) and amplitudes.
+# 1e-6 target) and amplitudes.
 FP32_PROB_TOL = 1e-6
 FP32_AMP_TOL = 1e-6

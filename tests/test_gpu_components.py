@@ -160,7 +160,7 @@ def test_component_fp32_path(hip):
     print(f"components fp32: vs fp64 {d64:.3e} (storage emulation vs fp64 {e32:.3e}), "
           f"vs emulation {d32:.3e}", flush=True)
     # FP32 storage with every measurement renormalized by the stored norm:
-    # within the north-star 1e-6 of the FP64 oracle, rows equal
+    # within the 1e-6 target of the FP64 oracle, rows equal
     assert np.array_equal(r.measurements, o["measurements"])
     assert d64 <= FP32_PROB_TOL
 
