@@ -86,7 +86,7 @@ __device__ __forceinline__ void lane_wave(const DevProgram& P, const DevLaunch& 
   // of independent loads before it computes and stores, so one memory
   // latency covers the batch; in LDS the loops stay one element at a time.
   constexpr bool kBatchAmps = STORE == kLaneAllGlobal;
-  constexpr int kAmpBatch = kBatchAmps ? 8 : 1, kPairBatch = kBatchAmps ? 4 : 1;
+  constexpr int kAmpBatch = kBatchAmps ? 4 : 1, kPairBatch = kBatchAmps ? 2 : 1;
   static_assert(!kBatchAmps || !kRegScratch, "the batched collapse stages through the scratch half");
   const int lane = threadIdx.x;
   const uint64_t local = wb * 64 + lane;
@@ -474,8 +474,12 @@ __device__ __forceinline__ void lane_wave(const DevProgram& P, const DevLaunch& 
 // ... in its own region (every wave leaves the loop after the launch's last
 // block).
 template <int KMAX, int STORE>
-__global__ __launch_bounds__(64) void lane_tier_kernel(DevProgram P, DevLaunch L, LaneLayout Y, unsigned char* ws,
-                                                       size_t slot_bytes) {
+// With the amplitudes in device memory the kernel is held to 128 VGPRs (four
+// waves per SIMD, without spills at four-element batches): its waves wait
+// on their regions, and a fourth wave per SIMD took the CCZ d05 fixture from
+// 3.28e7 to 3.49e7 shots/s against three waves at 165 VGPRs.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(STORE == kLaneAllGlobal ? 4 : 1))) void
+lane_tier_kernel(DevProgram P, DevLaunch L, LaneLayout Y, unsigned char* ws, size_t slot_bytes) {
   extern __shared__ __align__(16) unsigned char smem[];
   if constexpr (STORE == kLaneLds) {
     lane_wave<KMAX, STORE>(P, L, Y, blockIdx.x, smem, nullptr);
