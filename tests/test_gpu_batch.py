@@ -89,7 +89,7 @@ def test_sample_batch_host_results_match_the_oracle(hip):
 
 def test_small_width_11_program_joins_a_group_bucket(hip):
     """A dense width-11 program runs wide on its own; in a batch it joins a
-    group bucket when it has at most 5e4 shots and stays wide above. Both
+    group bucket when it has at most 4e5 shots and stays wide above. Both
     ways its rows equal the single-program run's and the oracle's."""
     n = 11
     lines = [f"H {q}\nT {q}" for q in range(n)] + [f"CX {q} {q + 1}" for q in range(n - 1)]
@@ -97,7 +97,7 @@ def test_small_width_11_program_joins_a_group_bucket(hip):
     c = hip.Circuit("\n".join(lines) + "\n")
     s = c.compile(components="dense")
     assert s.tier == "wide" and s.max_local_k == n
-    for shots, wide in ((3000, 0), (50001, 1)):
+    for shots, wide in ((3000, 0), (400001, 1)):
         out = buffers(s, shots)
         b = hip.Batch([s], [shots], [21], [out], autotune=True, autotune_shots=256)
         b.launch(0)

@@ -155,8 +155,11 @@ int tps_slot(int tps) {
 // moved) the batch took 0.97x the time, on 512 corpus programs of 2e4 shots
 // (88 moved) 0.94-0.97x; adding width 13 (512 threads per shot) took the
 // latter to 0.94x again (28 more moved) and left the former unchanged.
-// With every width-11/12 program moved, the first took 1.14x.
-constexpr uint64_t kBatchGroupMaxShots = 50000;
+// With the group tier's narrower default shapes the 256 circuits (1e4 to
+// 9.8e5 shots each) gain further up to 4e5 shots: 2.27-2.32 s at 5e4,
+// 2.23 s at 2e5, 2.17-2.20 s at 4e5 (13 more moved), and 2.88 s at 1e6,
+// where the group stream becomes the critical path.
+constexpr uint64_t kBatchGroupMaxShots = 400000;
 
 // Wide sequences of a batch, each on its own stream (C2 and C4 with two:
 // 0.93x and 0.94x the time of one).
